@@ -1,0 +1,195 @@
+#!/usr/bin/env python3
+"""Benchmark of the CG Poisson hot path on one MI355X.
+
+Workload (one "step"): one plain-FP64 Jacobi-preconditioned CG solve of
+100 iterations on a 32x32x32-element box at N=9 (nx1=10, 32,768,000 local
+points), brick geometry, homogeneous Dirichlet faces, RHS U[-1,1) seed 0
+-> dssum -> *c.  Inputs are generated on the device before the timed
+region.  value = local DOF x iterations / second (DOF = E*(N+1)^3, the
+reference's definition, src/prox_dssum.f:32).
+
+Also reported: the dominant kernel's roofline (element operator, HIP
+events on the library stream during the timed steps), time to a 1e-10
+relative residual on the deformed-geometry variant, and a CPU baseline
+(the compiled reference when oracle/_ref was built, else the oracle) on a
+bounded sample of the same workload.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import re
+import resource
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def cpu_baseline(nel, nx1, niter_sample):
+    """Time the CPU path on a bounded sample: the same mesh, fewer
+    iterations.  Returns the cpu_baseline record."""
+    n = nel[0] * nel[1] * nel[2] * nx1 ** 3
+    ref = os.path.join(ROOT, "oracle", "_ref", "c1", "refharness")
+    if os.path.exists(ref) and nx1 <= 10 and nel[0] * nel[1] * nel[2] <= 32768:
+        import numpy as np
+        tmp = os.path.join(ROOT, "gpurun_out", "cpu_baseline")
+        os.makedirs(tmp, exist_ok=True)
+        from oracle import oracle as O
+        O.rand_uniform(n, 0).tofile(os.path.join(tmp, "f.bin"))
+
+        def pre():
+            resource.setrlimit(resource.RLIMIT_STACK,
+                               (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+
+        inp = "bench\n%d %d %d %d %d 1\n%s\n%s\n" % (nel[0], nel[1], nel[2], nx1, niter_sample,
+                                                    os.path.join(tmp, "f.bin"),
+                                                    os.path.join(tmp, "o"))
+        r = subprocess.run([ref], input=inp.encode(), capture_output=True, preexec_fn=pre,
+                           timeout=900)
+        out = r.stdout.decode()
+        m = re.search(r"HK cg call\s+1 niter\s+(\d+) n\s+(\d+) seconds\s+(\S+)", out)
+        if r.returncode == 0 and m:
+            secs = float(m.group(3))
+            del np
+            return {"value": n * niter_sample / secs, "unit": "DOF*iter/s", "cores": 1,
+                    "kind": "reference",
+                    "sample": "flang-compiled reference cg (src/cg.f, identity M^-1, its own "
+                              "mask) on the full %dx%dx%d N=%d mesh, %d iterations, %.1f s" %
+                              (nel[0], nel[1], nel[2], nx1 - 1, niter_sample, secs)}
+    from oracle import oracle as O
+    Q = O.Problem(*nel, nx1)
+    Q.set_mask_mode(O.MASK_DIRICHLET)
+    Q.setup_jacobi()
+    f = Q.rhs_from_raw(O.rand_uniform(Q.n, 0))
+    t0 = time.perf_counter()
+    Q.cg(f, niter_sample, O.PRECOND_JACOBI)
+    secs = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": Q.n * niter_sample / secs, "unit": "DOF*iter/s", "cores": threads,
+            "kind": "port",
+            "sample": "oracle Jacobi CG on the full mesh, %d iterations, %.1f s" %
+                      (niter_sample, secs)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--niter", type=int, default=100)
+    ap.add_argument("--nel", type=int, default=32)
+    ap.add_argument("--nx1", type=int, default=10)
+    ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tol", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (one rank's share of the mesh); "
+              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import nekbone_amd as nb
+    nb.init(0)
+    nel = (args.nel, args.nel, args.nel)
+    nx1, niter = args.nx1, args.niter
+    P = nb.Problem(*nel, nx1)
+    n = P.n
+    P.set_mask_mode(nb.MASK_DIRICHLET)
+    P.setup_jacobi()
+    f, x, r = P.vec(), P.vec(), P.vec()
+    P.rand_uniform_d(f, 0)
+    P.rhs_from_raw_d(f)
+    nb.sync()
+    print("bench: setup done, n=%d" % n, file=sys.stderr, flush=True)
+
+    for _ in range(args.warmup):
+        P.cg_d(x, f, r, niter, nb.PRECOND_JACOBI)
+    nb.sync()
+
+    P.timing_enable(1 << nb.KCLASS_AX)
+    P.timing_reset()
+    nb.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        P.cg_d(x, f, r, niter, nb.PRECOND_JACOBI)
+    nb.sync()
+    t1 = time.perf_counter()
+    launches, ax_ms = P.timing_get(nb.KCLASS_AX)
+    P.timing_enable(0)
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    value = n * niter / ((t1 - t0) / args.steps)
+    print("bench: %.3f ms/solve" % ms_step, file=sys.stderr, flush=True)
+
+    # Dominant kernel: the element operator as launched inside CG, with the
+    # direction update fused in (reads p, r, dinv, g; writes p, w).
+    ax_avg_s = ax_ms / max(launches, 1) / 1e3
+    bytes_fused = 88.0 * n
+    flops_ax = (19.0 + 12.0 * nx1) * nx1 ** 3 * P.nelt
+    roofline = {"bound": "hbm", "achieved": bytes_fused / ax_avg_s / 1e9, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": bytes_fused / ax_avg_s / 1e9 / HBM_PEAK_GBS,
+                "traffic": None, "kernel": "k_ax_local<10,1,false,2> (ax_e + fused p update)",
+                "bytes_per_point": 88, "avg_us": ax_avg_s * 1e6, "launches": launches,
+                "gflops": flops_ax / ax_avg_s / 1e9}
+
+    # The plain operator w = A_L u alone (64 B/point: u, 6 factors, w).
+    P.timing_enable(1 << nb.KCLASS_AX)
+    P.timing_reset()
+    for _ in range(20):
+        P.ax_d(r, f)
+    nb.sync()
+    la, ms_a = P.timing_get(nb.KCLASS_AX)
+    P.timing_enable(0)
+    plain_s = ms_a / max(la, 1) / 1e3
+    ax_plain = {"avg_us": plain_s * 1e6, "bytes_per_point": 64,
+                "achieved_GBs": 64.0 * n / plain_s / 1e9,
+                "frac": 64.0 * n / plain_s / 1e9 / HBM_PEAK_GBS}
+
+    extra = {}
+    if not args.no_tol:
+        Q = nb.Problem(*nel, nx1)
+        Q.set_mask_mode(nb.MASK_DIRICHLET)
+        Q.set_geometry_deformed(0.1, 1)
+        Q.setup_jacobi()
+        fq, xq, rq = Q.vec(), Q.vec(), Q.vec()
+        Q.rand_uniform_d(fq, 0)
+        Q.rhs_from_raw_d(fq)
+        Q.cg_tol_d(xq, fq, rq, 1e-10, 5000, nb.PRECOND_JACOBI)
+        nb.sync()
+        ta = time.perf_counter()
+        iters, _ = Q.cg_tol_d(xq, fq, rq, 1e-10, 5000, nb.PRECOND_JACOBI)
+        nb.sync()
+        tb = time.perf_counter()
+        extra["time_to_1e-10"] = {"fp64_ms": (tb - ta) * 1e3, "fp64_iters": iters,
+                                  "geometry": "deformed +-10% vertices, seed 1"}
+        del Q, fq, xq, rq
+
+    cpu = None
+    if not args.no_cpu:
+        print("bench: cpu baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(nel, nx1, args.cpu_iters)
+
+    rec = {
+        "metric": "CG DOF*iter/s + time to 1e-10 rel. residual, mixed vs FP64, N=9, 32^3 elems",
+        "value": value, "unit": "DOF*iter/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "plain FP64 Jacobi CG, %d iterations, %dx%dx%d elements, N=%d, "
+                               "brick, Dirichlet, RHS U[-1,1) seed 0" %
+                               (niter, nel[0], nel[1], nel[2], nx1 - 1),
+                   "elements": P.nelt, "nx1": nx1, "local_points": n, "iterations": niter,
+                   "parallelism": "single GPU"},
+        "roofline": roofline, "ax_plain": ax_plain, "cpu_baseline": cpu,
+    }
+    rec.update(extra)
+    print(json.dumps(rec))
+
+
+if __name__ == "__main__":
+    main()

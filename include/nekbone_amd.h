@@ -1,0 +1,218 @@
+/* nekbone_amd: MI355X (gfx950) implementation of Nekbone's CG Poisson hot
+ * path behind a C ABI.
+ *
+ * Two layers:
+ *
+ *  1. Reference-named, Fortran-callable entry points (trailing underscore,
+ *     every argument by reference, REAL = double, INTEGER = int32,
+ *     glo_num = int64).  A Fortran host links these in place of the
+ *     reference's objects; they act on the problem bound by
+ *     nekbone_setup_() (the state the reference keeps in COMMON blocks).
+ *
+ *  2. Explicit nekbone_* entry points with an opaque problem handle, plain
+ *     pointers and sizes, returning an int status (0 = success; on failure
+ *     nekbone_last_error() describes it).  *_d variants take device
+ *     pointers; the others take host arrays and copy.
+ *
+ * Citations are to the reference sources (src/...).
+ */
+#ifndef NEKBONE_AMD_H
+#define NEKBONE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------- */
+/* Status, lifecycle                                                        */
+/* ---------------------------------------------------------------------- */
+enum {
+    NEKBONE_OK = 0,
+    NEKBONE_ERR_ARG = 1,      /* bad argument / size                    */
+    NEKBONE_ERR_HIP = 2,      /* HIP runtime error                      */
+    NEKBONE_ERR_STATE = 3,    /* call order (e.g. no Jacobi set up)     */
+    NEKBONE_ERR_NUMERIC = 4,  /* non-finite or non-positive CG scalar   */
+    NEKBONE_ERR_UNSUPPORTED = 5
+};
+
+enum { NEKBONE_PRECOND_IDENTITY = 0, NEKBONE_PRECOND_JACOBI = 1, NEKBONE_PRECOND_HSMG = 2 };
+enum { NEKBONE_MASK_REF = 0, NEKBONE_MASK_DIRICHLET = 1 };
+/* Arithmetic of the element operator and vector updates:
+ *   EXACT  FP64, the reference's evaluation order, no FMA contraction
+ *          (bit-identical ax/dssum; CG differs only through glsc3's
+ *          summation order).
+ *   FMA    FP64 with fused multiply-adds (same order of terms).           */
+enum { NEKBONE_ARITH_EXACT = 0, NEKBONE_ARITH_FMA = 1 };
+
+/* Last error message of the calling thread ("" if none). */
+const char *nekbone_last_error(void);
+/* Select the device (default 0) and create the library's stream. */
+int nekbone_init(int device);
+int nekbone_finalize(void);
+
+typedef struct nekbone_problem nekbone_problem;
+
+/* ---------------------------------------------------------------------- */
+/* Problem set-up                                                           */
+/* ---------------------------------------------------------------------- */
+
+/* Box of nelx*nely*nelz elements with nx1 GLL points per direction, set up
+ * as the reference driver does for one rank: element numbering
+ * (src/driver.f:117-194, lglel(e)=e), global numbering and gather-scatter
+ * map (src/prox_dssum.f:14-73, src/jl/gs.c:88-113,330-357), inverse
+ * multiplicity c (src/driver.f:274-291), D = dxm1 (src/prox_setup.f:2-25,
+ * src/semhat.f), brick geometric factors (src/prox_setup.f:27-52), mask
+ * lists (src/cg.f:214-368).  2 <= nx1 <= 16. */
+int nekbone_problem_create_box(nekbone_problem **P, int nelx, int nely, int nelz, int nx1);
+
+/* Drop-in set-up from the caller's arrays: the host has already run the
+ * reference's set-up.  dxm1(nx1,nx1) column-major; glo_num(n) int64 as
+ * set_vert_box produces; gxyz(6,n) AoS; c(n); lglel(nelt) 1-based global
+ * element numbers of the nelx x nely x nelz box (for the Dirichlet faces,
+ * src/cg.f:302-368). */
+int nekbone_problem_create(nekbone_problem **P, int nelt, int nx1, int nelx, int nely, int nelz,
+                           const int32_t *lglel, const double *dxm1, const int64_t *glo_num,
+                           const double *gxyz, const double *c);
+int nekbone_problem_destroy(nekbone_problem *P);
+
+int64_t nekbone_problem_npts(const nekbone_problem *P);
+int nekbone_problem_nx1(const nekbone_problem *P);
+int nekbone_problem_nelt(const nekbone_problem *P);
+
+/* Replace the geometric factors: from host AoS g(6,n), or by the deformed
+ * trilinear box (vertices perturbed by amp*2*U[-1,1), seeded; computed on
+ * the device).  The deformed map is not in the reference; it reduces
+ * exactly to src/prox_setup.f:27-52 for amp = 0. */
+int nekbone_set_geometry(nekbone_problem *P, const double *gxyz);
+int nekbone_set_geometry_deformed(nekbone_problem *P, double amp, uint64_t seed);
+int nekbone_get_geometry(nekbone_problem *P, double *gxyz);
+
+/* Host copies of set-up data (for checks). */
+int nekbone_get_dxm1(const nekbone_problem *P, double *dxm1);
+int nekbone_get_c(nekbone_problem *P, double *c);
+int nekbone_get_glo(const nekbone_problem *P, int64_t *glo);
+int64_t nekbone_gs_ngroups(const nekbone_problem *P);
+int64_t nekbone_gs_nshared(const nekbone_problem *P);
+
+/* Mask: REF reproduces src/cg.f:214-258 exactly (the first maskit after a
+ * reset zeroes every domain-boundary point and records element 1's
+ * positions; later calls zero only those).  DIRICHLET zeroes every
+ * domain-boundary point on every call.  reset = cmask(-1)=1
+ * (src/driver.f:125). */
+int nekbone_set_mask_mode(nekbone_problem *P, int mode);
+int nekbone_reset_mask(nekbone_problem *P);
+int nekbone_set_arith(nekbone_problem *P, int arith);
+
+/* Jacobi preconditioner (the reference has none): dinv = 1/(QQ^T
+ * diag(A_L) + 0.1), computed on the device. */
+int nekbone_setup_jacobi(nekbone_problem *P);
+int nekbone_get_dinv(nekbone_problem *P, double *dinv);
+
+/* ---------------------------------------------------------------------- */
+/* Operators (host arrays; *_d: device pointers on the library stream)     */
+/* ---------------------------------------------------------------------- */
+/* w = mask(QQ^T A_L u + 0.1 u): ax, src/cg.f:96-125 */
+int nekbone_ax(nekbone_problem *P, double *w, const double *u);
+int nekbone_ax_d(nekbone_problem *P, double *w, const double *u);
+/* w = A_L u element by element, no dssum/shift/mask: ax_e over all
+ * elements, src/cg.f:142-168 */
+int nekbone_ax_local(nekbone_problem *P, double *w, const double *u);
+/* u = QQ^T u: dssum, src/prox_dssum.f:2-12 */
+int nekbone_dssum(nekbone_problem *P, double *u);
+int nekbone_dssum_d(nekbone_problem *P, double *u);
+/* maskit on a host vector (advances the mask state like the reference) */
+int nekbone_maskit(nekbone_problem *P, double *w);
+/* sum a*b*m: glsc3, src/math.f:603-617 */
+int nekbone_glsc3(nekbone_problem *P, const double *a, const double *b, const double *m,
+                  double *result);
+
+/* ---------------------------------------------------------------------- */
+/* CG (src/cg.f:2-84)                                                       */
+/* ---------------------------------------------------------------------- */
+/* x = 0, r = mask(f), then niter PCG iterations.  hist (may be NULL)
+ * receives every glsc3 value in call order: rnorm^2, then (rtz1, pap, rtr)
+ * per iteration (1+3*niter doubles).  Host arrays; x, r may be NULL. */
+int nekbone_cg(nekbone_problem *P, double *x, const double *f, double *r, int niter,
+               int precond, double *hist);
+/* Same on device arrays already resident (x, f, r device pointers of n
+ * doubles; hist host, may be NULL). */
+int nekbone_cg_d(nekbone_problem *P, double *x, const double *f, double *r, int niter,
+                 int precond, double *hist);
+/* Run to a relative residual: stops after the first iteration with
+ * rtr <= tol^2 * rtr_0 (rtr_0 the initial glsc3(r,c,r)), or at maxiter.
+ * *iters receives the iteration count. */
+int nekbone_cg_tol_d(nekbone_problem *P, double *x, const double *f, double *r, double tol,
+                     int maxiter, int precond, int *iters, double *final_rtr);
+
+/* Device buffers owned by the problem (n doubles each), for *_d calls. */
+int nekbone_alloc_vec(nekbone_problem *P, double **dptr);
+int nekbone_free_vec(nekbone_problem *P, double *dptr);
+int nekbone_h2d(nekbone_problem *P, double *dst, const double *src, int64_t count);
+int nekbone_d2h(nekbone_problem *P, double *dst, const double *src, int64_t count);
+/* Synthetic inputs on the device: U[-1,1) counter-based splitmix64
+ * (value k = mix(seed + (k+1)*0x9E3779B97F4A7C15)), and the reference's
+ * set_f post-processing f = c * dssum(f) (src/driver.f:96-97). */
+int nekbone_rand_uniform_d(nekbone_problem *P, double *dst, int64_t count, uint64_t seed);
+int nekbone_rhs_from_raw_d(nekbone_problem *P, double *f);
+
+/* ---------------------------------------------------------------------- */
+/* Measurement                                                              */
+/* ---------------------------------------------------------------------- */
+/* Per-kernel HIP-event timing on the library stream.  Kernel classes:
+ * 0 = element operator (ax local), 1 = gather-scatter, 2 = post (shift +
+ * mask + pap), 3 = vector update (+ rtr/rtz), 4 = reductions.  mask bit k
+ * enables class k (0 disables).  Launches of enabled classes are
+ * bracketed by events without host synchronisation; nekbone_timing_get
+ * resolves them and returns the launch count and total milliseconds. */
+int nekbone_timing_enable(nekbone_problem *P, int mask);
+int nekbone_timing_get(nekbone_problem *P, int kclass, int64_t *launches, double *ms);
+int nekbone_timing_reset(nekbone_problem *P);
+int nekbone_sync(void);
+
+/* ---------------------------------------------------------------------- */
+/* Reference-named Fortran-callable entry points                           */
+/* ---------------------------------------------------------------------- */
+/* Bind the problem the legacy names act on.  A Fortran host calls this
+ * once per mesh after the reference's set-up, in place of the COMMON
+ * state (/dimn/, /dxyz/, /mymask/, /comm_handles/) the reference reads.
+ * gsh is the handle returned by gs_setup_. */
+void nekbone_setup_(const int *nelt, const int *nx1, const int *nelx, const int *nely,
+                    const int *nelz, const int32_t *lglel, const double *dxm1, const int *gsh);
+/* Select the preconditioner solvem_/cg_ apply: 0 = identity (the
+ * reference's default build, src/hsmg_dum.f:9-15), nonzero = Jacobi.
+ * No counterpart in the reference, which has no Jacobi option. */
+void nekbone_use_jacobi_(const int *on);
+/* src/cg.f:2-84 */
+void cg_(double *x, const double *f, const double *g, const double *c, double *r, double *w,
+         double *p, double *z, const int *n, const int *niter, double *flop_cg);
+/* src/cg.f:86-94 (identity, or Jacobi after nekbone_setup_jacobi) */
+void solvem_(double *z, double *r, const int *n);
+/* src/cg.f:96-125 */
+void ax_(double *w, const double *u, const double *gxyz, double *ur, double *us, double *ut,
+         double *wk, const int *n);
+/* src/cg.f:142-168: one element, g(6,nx1^3) */
+void ax_e_(double *w, const double *u, const double *g, double *ur, double *us, double *ut,
+           double *wk);
+/* src/cg.f:170-187, src/cg.f:189-212 (n = polynomial order) */
+void local_grad3_(double *ur, double *us, double *ut, const double *u, const int *n,
+                  const double *D, const double *Dt);
+void local_grad3_t_(double *u, const double *ur, const double *us, const double *ut, const int *N,
+                    const double *D, const double *Dt, double *w);
+/* src/prox_dssum.f:2-12 */
+void dssum_(double *f);
+/* src/cg.f:214-258 (pmask ignored: the mask state lives in the library) */
+void maskit_(double *w, double *pmask, const int *nx, const int *ny, const int *nz);
+/* src/math.f:603-617 */
+double glsc3_(const double *a, const double *b, const double *mult, const int *n);
+/* src/jl/gs.c:1160-1170, 1188-1193, 1237-1242 (one rank: comm, np
+ * ignored; dom must be 1 = double, op 1 = add) */
+void gs_setup_(int *handle, const int64_t *glo_num, const int *n, const int *comm, const int *np);
+void gs_op_(const int *handle, double *u, const int *dom, const int *op, const int *transpose);
+void gs_free_(const int *handle);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,16 @@
+"""nekbone_amd: MI355X-native implementation of Nekbone's CG Poisson hot path.
+
+The compute path is the HIP library libnekbone_amd.so (C ABI declared in
+include/nekbone_amd.h); this package is its host-side binding.
+"""
+from .core import (ARITH_EXACT, ARITH_FMA, KCLASS_AX, KCLASS_GS, KCLASS_POST, KCLASS_REDUCE,
+                   KCLASS_UPDATE, LIB_PATH, MASK_DIRICHLET, MASK_REF, PRECOND_HSMG,
+                   PRECOND_IDENTITY, PRECOND_JACOBI, DeviceVec, NekboneError, Problem, init, lib,
+                   sync)
+
+__all__ = [
+    "ARITH_EXACT", "ARITH_FMA", "KCLASS_AX", "KCLASS_GS", "KCLASS_POST", "KCLASS_REDUCE",
+    "KCLASS_UPDATE", "LIB_PATH", "MASK_DIRICHLET", "MASK_REF", "PRECOND_HSMG",
+    "PRECOND_IDENTITY", "PRECOND_JACOBI", "DeviceVec", "NekboneError", "Problem", "init", "lib",
+    "sync",
+]

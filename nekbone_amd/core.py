@@ -1,0 +1,324 @@
+"""ctypes binding of libnekbone_amd.so (the C ABI in include/nekbone_amd.h).
+
+The product path: every compute call goes to the HIP library.  There is no
+CPU fallback; if the library is not built, or no GPU is present, calls
+raise NekboneError.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libnekbone_amd.so")
+
+PRECOND_IDENTITY = 0
+PRECOND_JACOBI = 1
+PRECOND_HSMG = 2
+MASK_REF = 0
+MASK_DIRICHLET = 1
+ARITH_EXACT = 0
+ARITH_FMA = 1
+
+KCLASS_AX, KCLASS_GS, KCLASS_POST, KCLASS_UPDATE, KCLASS_REDUCE = range(5)
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_vp = ctypes.c_void_p
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "nekbone_last_error": (ctypes.c_char_p, []),
+    "nekbone_init": (ctypes.c_int, [ctypes.c_int]),
+    "nekbone_finalize": (ctypes.c_int, []),
+    "nekbone_sync": (ctypes.c_int, []),
+    "nekbone_problem_create_box": (ctypes.c_int, [ctypes.POINTER(_vp)] + [ctypes.c_int] * 4),
+    "nekbone_problem_create": (ctypes.c_int, [ctypes.POINTER(_vp)] + [ctypes.c_int] * 5 +
+                               [_i32p, _dp, _i64p, _dp, _dp]),
+    "nekbone_problem_destroy": (ctypes.c_int, [_vp]),
+    "nekbone_problem_npts": (ctypes.c_int64, [_vp]),
+    "nekbone_problem_nx1": (ctypes.c_int, [_vp]),
+    "nekbone_problem_nelt": (ctypes.c_int, [_vp]),
+    "nekbone_set_geometry": (ctypes.c_int, [_vp, _dp]),
+    "nekbone_set_geometry_deformed": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_uint64]),
+    "nekbone_get_geometry": (ctypes.c_int, [_vp, _dp]),
+    "nekbone_get_dxm1": (ctypes.c_int, [_vp, _dp]),
+    "nekbone_get_c": (ctypes.c_int, [_vp, _dp]),
+    "nekbone_get_glo": (ctypes.c_int, [_vp, _i64p]),
+    "nekbone_gs_ngroups": (ctypes.c_int64, [_vp]),
+    "nekbone_gs_nshared": (ctypes.c_int64, [_vp]),
+    "nekbone_set_mask_mode": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "nekbone_reset_mask": (ctypes.c_int, [_vp]),
+    "nekbone_set_arith": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "nekbone_setup_jacobi": (ctypes.c_int, [_vp]),
+    "nekbone_get_dinv": (ctypes.c_int, [_vp, _dp]),
+    "nekbone_ax": (ctypes.c_int, [_vp, _dp, _dp]),
+    "nekbone_ax_d": (ctypes.c_int, [_vp, _vp, _vp]),
+    "nekbone_ax_local": (ctypes.c_int, [_vp, _dp, _dp]),
+    "nekbone_dssum": (ctypes.c_int, [_vp, _dp]),
+    "nekbone_dssum_d": (ctypes.c_int, [_vp, _vp]),
+    "nekbone_maskit": (ctypes.c_int, [_vp, _dp]),
+    "nekbone_glsc3": (ctypes.c_int, [_vp, _dp, _dp, _dp, _dp]),
+    "nekbone_cg": (ctypes.c_int, [_vp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp]),
+    "nekbone_cg_d": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _dp]),
+    "nekbone_cg_tol_d": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int,
+                                        ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp]),
+    "nekbone_alloc_vec": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
+    "nekbone_free_vec": (ctypes.c_int, [_vp, _vp]),
+    "nekbone_h2d": (ctypes.c_int, [_vp, _vp, _dp, ctypes.c_int64]),
+    "nekbone_d2h": (ctypes.c_int, [_vp, _dp, _vp, ctypes.c_int64]),
+    "nekbone_rand_uniform_d": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_uint64]),
+    "nekbone_rhs_from_raw_d": (ctypes.c_int, [_vp, _vp]),
+    "nekbone_timing_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "nekbone_timing_get": (ctypes.c_int, [_vp, ctypes.c_int, _i64p, _dp]),
+    "nekbone_timing_reset": (ctypes.c_int, [_vp]),
+}
+
+_LIB = None
+
+
+class NekboneError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load the HIP library (fails loudly if it is not built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise NekboneError("libnekbone_amd.so is not built: run `python -m nekbone_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name, None)
+            if f is None:
+                continue
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _chk(rc):
+    if rc != 0:
+        msg = lib().nekbone_last_error().decode(errors="replace")
+        raise NekboneError("nekbone error %d: %s" % (rc, msg))
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if n is not None and a.size != n:
+        raise ValueError("expected %d values, got %d" % (n, a.size))
+    return a
+
+
+def init(device=0):
+    _chk(lib().nekbone_init(device))
+
+
+def sync():
+    _chk(lib().nekbone_sync())
+
+
+class DeviceVec:
+    """n doubles of device memory owned by a problem."""
+
+    def __init__(self, problem):
+        self.problem = problem
+        self.n = problem.n
+        h = _vp()
+        _chk(lib().nekbone_alloc_vec(problem.handle, ctypes.byref(h)))
+        self.ptr = h.value
+
+    def upload(self, a):
+        a = _f64(a, self.n)
+        _chk(lib().nekbone_h2d(self.problem.handle, self.ptr, _p(a), self.n))
+
+    def download(self):
+        out = np.empty(self.n)
+        _chk(lib().nekbone_d2h(self.problem.handle, _p(out), self.ptr, self.n))
+        return out
+
+    def free(self):
+        if self.ptr:
+            _chk(lib().nekbone_free_vec(self.problem.handle, self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Problem:
+    """One rank's share of a Nekbone box mesh, resident on the GPU.
+
+    Mirrors the reference's set-up for one rank (src/driver.f:117-194 and
+    the routines it calls).  Construct with a box size, or with
+    ``Problem.from_arrays`` to take the reference's own set-up arrays.
+    """
+
+    def __init__(self, nelx, nely, nelz, nx1, _handle=None):
+        self.handle = _handle
+        if self.handle is None:
+            h = _vp()
+            _chk(lib().nekbone_problem_create_box(ctypes.byref(h), nelx, nely, nelz, nx1))
+            self.handle = h.value
+        self.nelx, self.nely, self.nelz, self.nx1 = nelx, nely, nelz, nx1
+        self.nelt = lib().nekbone_problem_nelt(self.handle)
+        self.n = lib().nekbone_problem_npts(self.handle)
+
+    @classmethod
+    def from_arrays(cls, nelt, nx1, nelx, nely, nelz, lglel, dxm1, glo_num, gxyz=None, c=None):
+        lglel = np.ascontiguousarray(lglel, dtype=np.int32)
+        dxm1 = _f64(dxm1, nx1 * nx1)
+        glo = np.ascontiguousarray(glo_num, dtype=np.int64)
+        g = None if gxyz is None else _f64(gxyz)
+        cc = None if c is None else _f64(c)
+        h = _vp()
+        _chk(lib().nekbone_problem_create(
+            ctypes.byref(h), nelt, nx1, nelx, nely, nelz, lglel.ctypes.data_as(_i32p), _p(dxm1),
+            glo.ctypes.data_as(_i64p), None if g is None else _p(g), None if cc is None else _p(cc)))
+        return cls(nelx, nely, nelz, nx1, _handle=h.value)
+
+    def close(self):
+        if self.handle:
+            _chk(lib().nekbone_problem_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # set-up data -------------------------------------------------------
+    def dxm1(self):
+        d = np.empty(self.nx1 * self.nx1)
+        _chk(lib().nekbone_get_dxm1(self.handle, _p(d)))
+        return d
+
+    def c(self):
+        v = np.empty(self.n)
+        _chk(lib().nekbone_get_c(self.handle, _p(v)))
+        return v
+
+    def glo(self):
+        v = np.empty(self.n, dtype=np.int64)
+        _chk(lib().nekbone_get_glo(self.handle, v.ctypes.data_as(_i64p)))
+        return v
+
+    def gs_counts(self):
+        return (lib().nekbone_gs_ngroups(self.handle), lib().nekbone_gs_nshared(self.handle))
+
+    def geometry(self):
+        g = np.empty(6 * self.n)
+        _chk(lib().nekbone_get_geometry(self.handle, _p(g)))
+        return g
+
+    def set_geometry(self, g):
+        g = _f64(g, 6 * self.n)
+        _chk(lib().nekbone_set_geometry(self.handle, _p(g)))
+
+    def set_geometry_deformed(self, amp=0.1, seed=1):
+        _chk(lib().nekbone_set_geometry_deformed(self.handle, amp, seed))
+
+    def set_mask_mode(self, mode):
+        _chk(lib().nekbone_set_mask_mode(self.handle, mode))
+
+    def reset_mask(self):
+        _chk(lib().nekbone_reset_mask(self.handle))
+
+    def set_arith(self, arith):
+        _chk(lib().nekbone_set_arith(self.handle, arith))
+
+    def setup_jacobi(self):
+        _chk(lib().nekbone_setup_jacobi(self.handle))
+
+    def dinv(self):
+        v = np.empty(self.n)
+        _chk(lib().nekbone_get_dinv(self.handle, _p(v)))
+        return v
+
+    # operators (host arrays) -------------------------------------------
+    def ax(self, u):
+        u = _f64(u, self.n)
+        w = np.empty(self.n)
+        _chk(lib().nekbone_ax(self.handle, _p(w), _p(u)))
+        return w
+
+    def ax_local(self, u):
+        u = _f64(u, self.n)
+        w = np.empty(self.n)
+        _chk(lib().nekbone_ax_local(self.handle, _p(w), _p(u)))
+        return w
+
+    def dssum(self, u):
+        u = np.array(u, dtype=np.float64, copy=True)
+        _chk(lib().nekbone_dssum(self.handle, _p(u)))
+        return u
+
+    def maskit(self, u):
+        u = np.array(u, dtype=np.float64, copy=True)
+        _chk(lib().nekbone_maskit(self.handle, _p(u)))
+        return u
+
+    def glsc3(self, a, b, m):
+        a, b, m = (_f64(v, self.n) for v in (a, b, m))
+        out = ctypes.c_double()
+        _chk(lib().nekbone_glsc3(self.handle, _p(a), _p(b), _p(m), ctypes.byref(out)))
+        return out.value
+
+    def cg(self, f, niter, precond=PRECOND_IDENTITY):
+        """PCG from x=0 (src/cg.f:2-84).  Returns x, r and the glsc3
+        history (rnorm^2, then rtz1, pap, rtr per iteration)."""
+        f = _f64(f, self.n)
+        x = np.empty(self.n)
+        r = np.empty(self.n)
+        hist = np.empty(1 + 3 * niter)
+        _chk(lib().nekbone_cg(self.handle, _p(x), _p(f), _p(r), niter, precond, _p(hist)))
+        return x, r, hist
+
+    # device-resident operations ------------------------------------------
+    def vec(self):
+        return DeviceVec(self)
+
+    def rand_uniform_d(self, dv, seed):
+        _chk(lib().nekbone_rand_uniform_d(self.handle, dv.ptr, dv.n, seed))
+
+    def rhs_from_raw_d(self, dv):
+        _chk(lib().nekbone_rhs_from_raw_d(self.handle, dv.ptr))
+
+    def cg_d(self, x, f, r, niter, precond=PRECOND_IDENTITY, want_hist=False):
+        hist = np.empty(1 + 3 * niter) if want_hist else None
+        _chk(lib().nekbone_cg_d(self.handle, x.ptr, f.ptr, r.ptr, niter, precond,
+                                None if hist is None else _p(hist)))
+        return hist
+
+    def cg_tol_d(self, x, f, r, tol, maxiter, precond=PRECOND_IDENTITY):
+        it = ctypes.c_int()
+        rtr = ctypes.c_double()
+        _chk(lib().nekbone_cg_tol_d(self.handle, x.ptr, f.ptr, r.ptr, tol, maxiter, precond,
+                                    ctypes.byref(it), ctypes.byref(rtr)))
+        return it.value, rtr.value
+
+    def ax_d(self, w, u):
+        _chk(lib().nekbone_ax_d(self.handle, w.ptr, u.ptr))
+
+    # timing --------------------------------------------------------------
+    def timing_enable(self, mask):
+        _chk(lib().nekbone_timing_enable(self.handle, mask))
+
+    def timing_reset(self):
+        _chk(lib().nekbone_timing_reset(self.handle))
+
+    def timing_get(self, kclass):
+        n = ctypes.c_int64()
+        ms = ctypes.c_double()
+        _chk(lib().nekbone_timing_get(self.handle, kclass, ctypes.byref(n), ctypes.byref(ms)))
+        return n.value, ms.value

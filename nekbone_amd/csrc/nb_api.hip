@@ -1,0 +1,729 @@
+// C ABI of nekbone_amd: problem lifecycle, operators, CG drivers, timing,
+// and the reference-named Fortran-callable entry points.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+
+#include "nb_internal.h"
+
+namespace nb {
+
+static thread_local std::string t_err;
+void set_error(const std::string &msg) { t_err = msg; }
+int fail(int code, const std::string &msg)
+{
+    set_error(msg);
+    return code;
+}
+
+namespace {
+int g_device = -1;
+hipStream_t g_stream = nullptr;
+
+__global__ void k_fill(double *a, double v, int64_t n)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+__global__ void k_recip(double *a, int64_t n)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 1. / a[i];
+}
+int grid(int64_t n) { return int((n + 255) / 256); }
+
+int fill(double *a, double v, int64_t n)
+{
+    if (n <= 0) return NEKBONE_OK;
+    hipLaunchKernelGGL(k_fill, dim3(grid(n)), dim3(256), 0, stream(), a, v, n);
+    NB_HIP(hipGetLastError());
+    return NEKBONE_OK;
+}
+
+template <class T>
+int dalloc(T **p, int64_t count)
+{
+    *p = nullptr;
+    if (count <= 0) count = 1;
+    NB_HIP(hipMalloc(reinterpret_cast<void **>(p), sizeof(T) * size_t(count)));
+    return NEKBONE_OK;
+}
+
+template <class T>
+void dfree(T *&p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+int check(const nekbone_problem *P)
+{
+    if (!P) return fail(NEKBONE_ERR_ARG, "null problem");
+    return NEKBONE_OK;
+}
+
+}  // namespace
+
+hipStream_t stream() { return g_stream; }
+
+int ensure_init()
+{
+    if (g_stream) return NEKBONE_OK;
+    return nekbone_init(g_device < 0 ? 0 : g_device);
+}
+
+int next_mask_bit(nekbone_problem *P)
+{
+    if (P->mask_mode == NEKBONE_MASK_DIRICHLET) return 1;
+    if (!P->mask_built) {
+        P->mask_built = 1;
+        return 1;
+    }
+    return 2;
+}
+
+static int take_event(Timing &T, hipEvent_t *ev)
+{
+    if (!T.pool.empty()) {
+        *ev = T.pool.back();
+        T.pool.pop_back();
+        return NEKBONE_OK;
+    }
+    NB_HIP(hipEventCreate(ev));
+    return NEKBONE_OK;
+}
+
+int timed_begin(nekbone_problem *P, int kclass)
+{
+    Timing &T = P->timing;
+    if (!(T.mask >> kclass & 1)) return NEKBONE_OK;
+    if (int rc = take_event(T, &T.cur)) return rc;
+    NB_HIP(hipEventRecord(T.cur, stream()));
+    return NEKBONE_OK;
+}
+
+int timed_end(nekbone_problem *P, int kclass)
+{
+    Timing &T = P->timing;
+    if (!(T.mask >> kclass & 1) || !T.cur) return NEKBONE_OK;
+    hipEvent_t b;
+    if (int rc = take_event(T, &b)) return rc;
+    NB_HIP(hipEventRecord(b, stream()));
+    T.pending.push_back({T.cur, b, kclass});
+    T.cur = nullptr;
+    if (T.pending.size() >= 4096) return timing_resolve(P);
+    return NEKBONE_OK;
+}
+
+int timing_resolve(nekbone_problem *P)
+{
+    Timing &T = P->timing;
+    if (T.pending.empty()) return NEKBONE_OK;
+    NB_HIP(hipEventSynchronize(T.pending.back().b));
+    for (const Timing::Pending &q : T.pending) {
+        float ms = 0.f;
+        NB_HIP(hipEventElapsedTime(&ms, q.a, q.b));
+        T.launches[q.kclass] += 1;
+        T.ms[q.kclass] += double(ms);
+        T.pool.push_back(q.a);
+        T.pool.push_back(q.b);
+    }
+    T.pending.clear();
+    return NEKBONE_OK;
+}
+
+// Upload the set-up data common to both constructors and allocate the
+// work space.  Expects n, nelt, nx1, dxm1, lglel, glo, masks filled in.
+static int finish_problem(nekbone_problem *P, const double *c_host)
+{
+    const int64_t n = P->n;
+    std::vector<int32_t> off, idx;
+    gs_groups(P->glo.data(), n, off, idx);
+    P->gs.ngroups = int64_t(off.size()) - 1;
+    P->gs.nshared = int64_t(idx.size());
+    if (int rc = dalloc(&P->gs.d_off, int64_t(off.size()))) return rc;
+    if (int rc = dalloc(&P->gs.d_idx, int64_t(idx.size()))) return rc;
+    NB_HIP(hipMemcpy(P->gs.d_off, off.data(), sizeof(int32_t) * off.size(),
+                     hipMemcpyHostToDevice));
+    if (!idx.empty())
+        NB_HIP(hipMemcpy(P->gs.d_idx, idx.data(), sizeof(int32_t) * idx.size(),
+                         hipMemcpyHostToDevice));
+
+    if (int rc = dalloc(&P->d_D, int64_t(P->nx1) * P->nx1)) return rc;
+    NB_HIP(hipMemcpy(P->d_D, P->dxm1, sizeof(double) * P->nx1 * P->nx1, hipMemcpyHostToDevice));
+    if (int rc = dalloc(&P->d_g, 6 * n)) return rc;
+    if (int rc = dalloc(&P->d_c, n)) return rc;
+    for (double **v : {&P->d_x, &P->d_r, &P->d_p, &P->d_w, &P->d_f, &P->d_t})
+        if (int rc = dalloc(v, n)) return rc;
+    if (int rc = dalloc(&P->d_part, 2 * 1024)) return rc;
+    if (int rc = dalloc(&P->d_scal, 1)) return rc;
+    NB_HIP(hipMemset(P->d_scal, 0, sizeof(CgScalars)));
+    for (double *v : {P->d_x, P->d_r, P->d_p, P->d_w, P->d_f, P->d_t})
+        NB_HIP(hipMemset(v, 0, sizeof(double) * size_t(n)));
+
+    std::vector<uint8_t> flag(size_t(n), 0);
+    for (int32_t l : P->mfull) flag[size_t(l)] |= 1;
+    for (int32_t l : P->mref) flag[size_t(l)] |= 2;
+    if (int rc = dalloc(&P->d_flag, n)) return rc;
+    NB_HIP(hipMemcpy(P->d_flag, flag.data(), size_t(n), hipMemcpyHostToDevice));
+
+    if (c_host) {
+        NB_HIP(hipMemcpy(P->d_c, c_host, sizeof(double) * size_t(n), hipMemcpyHostToDevice));
+    } else {
+        // set_multiplicity, src/driver.f:274-291: c = 1 / QQ^T(1)
+        if (int rc = fill(P->d_c, 1.0, n)) return rc;
+        if (int rc = launch_gs(P, P->d_c, nullptr)) return rc;
+        hipLaunchKernelGGL(k_recip, dim3(grid(n)), dim3(256), 0, stream(), P->d_c, n);
+        NB_HIP(hipGetLastError());
+    }
+    NB_HIP(hipStreamSynchronize(stream()));
+    return NEKBONE_OK;
+}
+
+static int ensure_hist(nekbone_problem *P, int niter)
+{
+    const int need = 1 + 3 * std::max(niter, 0) + 3;
+    if (P->hist_cap >= need) return NEKBONE_OK;
+    dfree(P->d_hist);
+    if (int rc = dalloc(&P->d_hist, need)) return rc;
+    P->hist_cap = need;
+    return NEKBONE_OK;
+}
+
+// CG on device vectors (src/cg.f:2-84).
+static int run_cg(nekbone_problem *P, double *x, const double *f, double *r, int niter,
+                  int precond, double tol2)
+{
+    if (precond == NEKBONE_PRECOND_JACOBI && !P->d_dinv)
+        return fail(NEKBONE_ERR_STATE, "cg: Jacobi preconditioner not set up");
+    if (precond != NEKBONE_PRECOND_JACOBI && precond != NEKBONE_PRECOND_IDENTITY)
+        return fail(NEKBONE_ERR_UNSUPPORTED, "cg: preconditioner not available on this path");
+    if (niter < 0) return fail(NEKBONE_ERR_ARG, "cg: niter < 0");
+    if (int rc = ensure_hist(P, niter)) return rc;
+    const int mbit0 = next_mask_bit(P);  // maskit(r), src/cg.f:40
+    if (int rc = launch_cg_init(P, x, r, f, mbit0, precond, tol2)) return rc;
+    const int mbit = next_mask_bit(P);  // maskit in ax, src/cg.f:119 (state is now fixed)
+    for (int it = 0; it < niter; ++it)
+        if (int rc = launch_cg_iteration(P, x, r, precond, mbit)) return rc;
+    return NEKBONE_OK;
+}
+
+}  // namespace nb
+
+using namespace nb;
+
+// ======================================================================
+// Lifecycle
+// ======================================================================
+extern "C" const char *nekbone_last_error(void) { return t_err.c_str(); }
+
+extern "C" int nekbone_init(int device)
+{
+    if (g_stream && device == g_device) return NEKBONE_OK;
+    int count = 0;
+    NB_HIP(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count)
+        return fail(NEKBONE_ERR_ARG, "nekbone_init: no such device");
+    NB_HIP(hipSetDevice(device));
+    if (g_stream) NB_HIP(hipStreamDestroy(g_stream));
+    NB_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    g_device = device;
+    return NEKBONE_OK;
+}
+
+extern "C" int nekbone_finalize(void)
+{
+    if (g_stream) {
+        NB_HIP(hipStreamSynchronize(g_stream));
+        NB_HIP(hipStreamDestroy(g_stream));
+    }
+    g_stream = nullptr;
+    return NEKBONE_OK;
+}
+
+extern "C" int nekbone_sync(void)
+{
+    if (int rc = ensure_init()) return rc;
+    NB_HIP(hipStreamSynchronize(stream()));
+    return NEKBONE_OK;
+}
+
+// ======================================================================
+// Problem set-up
+// ======================================================================
+extern "C" int nekbone_problem_create_box(nekbone_problem **out, int nelx, int nely, int nelz,
+                                          int nx1)
+{
+    if (!out) return fail(NEKBONE_ERR_ARG, "null output");
+    *out = nullptr;
+    if (nx1 < 2 || nx1 > kMaxNx || nelx < 1 || nely < 1 || nelz < 1)
+        return fail(NEKBONE_ERR_ARG, "create_box: need 2 <= nx1 <= 16 and nel* >= 1");
+    const int64_t nelt64 = int64_t(nelx) * nely * nelz;
+    const int64_t n64 = nelt64 * nx1 * nx1 * nx1;
+    if (n64 >= (int64_t(1) << 31)) return fail(NEKBONE_ERR_ARG, "create_box: n >= 2^31");
+    if (int rc = ensure_init()) return rc;
+    auto P = std::make_unique<nekbone_problem>();
+    P->nelx = nelx;
+    P->nely = nely;
+    P->nelz = nelz;
+    P->nelt = int(nelt64);
+    P->nx1 = nx1;
+    P->n = n64;
+    P->lglel.resize(size_t(P->nelt));
+    for (int e = 0; e < P->nelt; ++e) P->lglel[size_t(e)] = e + 1;
+    double a[kMaxNx * kMaxNx], cc[kMaxNx * kMaxNx];
+    semhat(a, P->wgll, cc, P->dxm1, P->zgll, nx1 - 1);
+    box_numbering(P->glo, nx1, nelx, nely, nelz, P->lglel);
+    mask_lists(nx1, P->nelt, nelx, nely, nelz, P->lglel, P->mfull, P->mref);
+    if (int rc = finish_problem(P.get(), nullptr)) {
+        nekbone_problem_destroy(P.release());
+        return rc;
+    }
+    if (int rc = launch_brick_g(P.get())) {
+        nekbone_problem_destroy(P.release());
+        return rc;
+    }
+    NB_HIP(hipStreamSynchronize(stream()));
+    *out = P.release();
+    return NEKBONE_OK;
+}
+
+extern "C" int nekbone_problem_create(nekbone_problem **out, int nelt, int nx1, int nelx,
+                                      int nely, int nelz, const int32_t *lglel,
+                                      const double *dxm1, const int64_t *glo_num,
+                                      const double *gxyz, const double *c)
+{
+    if (!out) return fail(NEKBONE_ERR_ARG, "null output");
+    *out = nullptr;
+    if (nx1 < 2 || nx1 > kMaxNx || nelt < 1 || !lglel || !dxm1 || !glo_num)
+        return fail(NEKBONE_ERR_ARG, "create: bad arguments");
+    if (int64_t(nelx) * nely * nelz < 1) return fail(NEKBONE_ERR_ARG, "create: bad box");
+    if (int rc = ensure_init()) return rc;
+    auto P = std::make_unique<nekbone_problem>();
+    P->nelx = nelx;
+    P->nely = nely;
+    P->nelz = nelz;
+    P->nelt = nelt;
+    P->nx1 = nx1;
+    P->n = int64_t(nelt) * nx1 * nx1 * nx1;
+    P->lglel.assign(lglel, lglel + nelt);
+    std::memcpy(P->dxm1, dxm1, sizeof(double) * nx1 * nx1);
+    double a[kMaxNx * kMaxNx], cc[kMaxNx * kMaxNx], d[kMaxNx * kMaxNx];
+    semhat(a, P->wgll, cc, d, P->zgll, nx1 - 1);
+    P->glo.assign(glo_num, glo_num + P->n);
+    mask_lists(nx1, nelt, nelx, nely, nelz, P->lglel, P->mfull, P->mref);
+    if (int rc = finish_problem(P.get(), c)) {
+        nekbone_problem_destroy(P.release());
+        return rc;
+    }
+    if (gxyz) {
+        if (int rc = nekbone_set_geometry(P.get(), gxyz)) {
+            nekbone_problem_destroy(P.release());
+            return rc;
+        }
+    } else if (int rc = launch_brick_g(P.get())) {
+        nekbone_problem_destroy(P.release());
+        return rc;
+    }
+    NB_HIP(hipStreamSynchronize(stream()));
+    *out = P.release();
+    return NEKBONE_OK;
+}
+
+extern "C" int nekbone_problem_destroy(nekbone_problem *P)
+{
+    if (!P) return NEKBONE_OK;
+    if (g_stream) (void)hipStreamSynchronize(g_stream);
+    dfree(P->d_D);
+    dfree(P->d_g);
+    dfree(P->d_c);
+    dfree(P->d_dinv);
+    dfree(P->d_flag);
+    dfree(P->gs.d_off);
+    dfree(P->gs.d_idx);
+    for (double **v : {&P->d_x, &P->d_r, &P->d_p, &P->d_w, &P->d_f, &P->d_t, &P->d_part,
+                       &P->d_hist})
+        dfree(*v);
+    dfree(P->d_scal);
+    (void)timing_resolve(P);
+    for (hipEvent_t e : P->timing.pool) (void)hipEventDestroy(e);
+    if (P->timing.cur) (void)hipEventDestroy(P->timing.cur);
+    delete P;
+    return NEKBONE_OK;
+}
+
+extern "C" int64_t nekbone_problem_npts(const nekbone_problem *P) { return P ? P->n : -1; }
+extern "C" int nekbone_problem_nx1(const nekbone_problem *P) { return P ? P->nx1 : -1; }
+extern "C" int nekbone_problem_nelt(const nekbone_problem *P) { return P ? P->nelt : -1; }
+
+extern "C" int nekbone_set_geometry(nekbone_problem *P, const double *gxyz)
+{
+    if (int rc = check(P)) return rc;
+    if (!gxyz) return fail(NEKBONE_ERR_ARG, "set_geometry: null g");
+    double *tmp = nullptr;
+    if (int rc = dalloc(&tmp, 6 * P->n)) return rc;
+    NB_HIP(hipMemcpy(tmp, gxyz, sizeof(double) * 6 * size_t(P->n), hipMemcpyHostToDevice));
+    int rc = launch_aos_to_soa(P, tmp, P->d_g, 0);
+    (void)hipStreamSynchronize(stream());
+    dfree(tmp);
+    P->jacobi = false;
+    return rc;
+}
+
+extern "C" int nekbone_get_geometry(nekbone_problem *P, double *gxyz)
+{
+    if (int rc = check(P)) return rc;
+    double *tmp = nullptr;
+    if (int rc = dalloc(&tmp, 6 * P->n)) return rc;
+    int rc = launch_aos_to_soa(P, tmp, P->d_g, 1);
+    if (rc == NEKBONE_OK) {
+        NB_HIP(hipStreamSynchronize(stream()));
+        NB_HIP(hipMemcpy(gxyz, tmp, sizeof(double) * 6 * size_t(P->n), hipMemcpyDeviceToHost));
+    }
+    dfree(tmp);
+    return rc;
+}
+
+extern "C" int nekbone_set_geometry_deformed(nekbone_problem *P, double amp, uint64_t seed)
+{
+    if (int rc = check(P)) return rc;
+    if (!(amp >= 0.0) || amp >= 0.25)
+        return fail(NEKBONE_ERR_ARG, "deformed geometry: need 0 <= amp < 0.25");
+    const int vx = P->nelx + 1, vy = P->nely + 1, vz = P->nelz + 1;
+    const int64_t nv = int64_t(vx) * vy * vz;
+    double *d_v = nullptr;
+    if (int rc = dalloc(&d_v, 3 * nv)) return rc;
+    // Vertex k of the width-2 grid, coordinate d: 2*index + (2*amp)*U_{3k+d}.
+    std::vector<double> V(size_t(3 * nv));
+    if (int rc = launch_rand(d_v, 3 * nv, seed)) return rc;
+    NB_HIP(hipMemcpyAsync(V.data(), d_v, sizeof(double) * V.size(), hipMemcpyDeviceToHost,
+                          stream()));
+    NB_HIP(hipStreamSynchronize(stream()));
+    const double scale = amp * 2.0;
+    for (int c = 0; c < vz; ++c)
+        for (int b = 0; b < vy; ++b)
+            for (int a = 0; a < vx; ++a) {
+                const int64_t v = a + int64_t(vx) * (b + int64_t(vy) * c);
+                V[size_t(3 * v + 0)] = 2.0 * a + scale * V[size_t(3 * v + 0)];
+                V[size_t(3 * v + 1)] = 2.0 * b + scale * V[size_t(3 * v + 1)];
+                V[size_t(3 * v + 2)] = 2.0 * c + scale * V[size_t(3 * v + 2)];
+            }
+    NB_HIP(hipMemcpy(d_v, V.data(), sizeof(double) * V.size(), hipMemcpyHostToDevice));
+    int rc = launch_deformed_g(P, d_v);
+    (void)hipStreamSynchronize(stream());
+    dfree(d_v);
+    P->jacobi = false;
+    return rc;
+}
+
+extern "C" int nekbone_get_dxm1(const nekbone_problem *P, double *dxm1)
+{
+    if (int rc = check(P)) return rc;
+    std::memcpy(dxm1, P->dxm1, sizeof(double) * P->nx1 * P->nx1);
+    return NEKBONE_OK;
+}
+
+extern "C" int nekbone_get_c(nekbone_problem *P, double *c)
+{
+    if (int rc = check(P)) return rc;
+    NB_HIP(hipStreamSynchronize(stream()));
+    NB_HIP(hipMemcpy(c, P->d_c, sizeof(double) * size_t(P->n), hipMemcpyDeviceToHost));
+    return NEKBONE_OK;
+}
+
+extern "C" int nekbone_get_glo(const nekbone_problem *P, int64_t *glo)
+{
+    if (int rc = check(P)) return rc;
+    std::memcpy(glo, P->glo.data(), sizeof(int64_t) * size_t(P->n));
+    return NEKBONE_OK;
+}
+
+extern "C" int64_t nekbone_gs_ngroups(const nekbone_problem *P) { return P ? P->gs.ngroups : -1; }
+extern "C" int64_t nekbone_gs_nshared(const nekbone_problem *P) { return P ? P->gs.nshared : -1; }
+
+extern "C" int nekbone_set_mask_mode(nekbone_problem *P, int mode)
+{
+    if (int rc = check(P)) return rc;
+    if (mode != NEKBONE_MASK_REF && mode != NEKBONE_MASK_DIRICHLET)
+        return fail(NEKBONE_ERR_ARG, "bad mask mode");
+    P->mask_mode = mode;
+    return NEKBONE_OK;
+}
+
+extern "C" int nekbone_reset_mask(nekbone_problem *P)
+{
+    if (int rc = check(P)) return rc;
+    P->mask_built = 0;
+    return NEKBONE_OK;
+}
+
+extern "C" int nekbone_set_arith(nekbone_problem *P, int arith)
+{
+    if (int rc = check(P)) return rc;
+    if (arith != NEKBONE_ARITH_EXACT && arith != NEKBONE_ARITH_FMA)
+        return fail(NEKBONE_ERR_ARG, "bad arithmetic mode");
+    P->arith = arith;
+    return NEKBONE_OK;
+}
+
+extern "C" int nekbone_setup_jacobi(nekbone_problem *P)
+{
+    if (int rc = check(P)) return rc;
+    if (!P->d_dinv)
+        if (int rc = dalloc(&P->d_dinv, P->n)) return rc;
+    if (int rc = launch_jacobi_local(P, P->d_dinv)) return rc;
+    if (int rc = launch_gs(P, P->d_dinv, nullptr)) return rc;
+    if (int rc = launch_jacobi_finish(P, P->d_dinv)) return rc;
+    NB_HIP(hipStreamSynchronize(stream()));
+    P->jacobi = true;
+    return NEKBONE_OK;
+}
+
+extern "C" int nekbone_get_dinv(nekbone_problem *P, double *dinv)
+{
+    if (int rc = check(P)) return rc;
+    if (!P->d_dinv) return fail(NEKBONE_ERR_STATE, "Jacobi not set up");
+    NB_HIP(hipMemcpy(dinv, P->d_dinv, sizeof(double) * size_t(P->n), hipMemcpyDeviceToHost));
+    return NEKBONE_OK;
+}
+
+// ======================================================================
+// Operators
+// ======================================================================
+extern "C" int nekbone_ax_d(nekbone_problem *P, double *w, const double *u)
+{
+    if (int rc = check(P)) return rc;
+    if (int rc = launch_ax_local(P, w, u, 0, nullptr, nullptr)) return rc;
+    if (int rc = launch_gs(P, w, nullptr)) return rc;
+    return launch_post(P, w, u, next_mask_bit(P), false);
+}
+
+extern "C" int nekbone_ax(nekbone_problem *P, double *w, const double *u)
+{
+    if (int rc = check(P)) return rc;
+    NB_HIP(hipMemcpyAsync(P->d_t, u, sizeof(double) * size_t(P->n), hipMemcpyHostToDevice,
+                          stream()));
+    if (int rc = nekbone_ax_d(P, P->d_w, P->d_t)) return rc;
+    NB_HIP(hipMemcpyAsync(w, P->d_w, sizeof(double) * size_t(P->n), hipMemcpyDeviceToHost,
+                          stream()));
+    NB_HIP(hipStreamSynchronize(stream()));
+    return NEKBONE_OK;
+}
+
+extern "C" int nekbone_ax_local(nekbone_problem *P, double *w, const double *u)
+{
+    if (int rc = check(P)) return rc;
+    NB_HIP(hipMemcpyAsync(P->d_t, u, sizeof(double) * size_t(P->n), hipMemcpyHostToDevice,
+                          stream()));
+    if (int rc = launch_ax_local(P, P->d_w, P->d_t, 0, nullptr, nullptr)) return rc;
+    NB_HIP(hipMemcpyAsync(w, P->d_w, sizeof(double) * size_t(P->n), hipMemcpyDeviceToHost,
+                          stream()));
+    NB_HIP(hipStreamSynchronize(stream()));
+    return NEKBONE_OK;
+}
+
+extern "C" int nekbone_dssum_d(nekbone_problem *P, double *u)
+{
+    if (int rc = check(P)) return rc;
+    return launch_gs(P, u, nullptr);
+}
+
+extern "C" int nekbone_dssum(nekbone_problem *P, double *u)
+{
+    if (int rc = check(P)) return rc;
+    NB_HIP(hipMemcpyAsync(P->d_t, u, sizeof(double) * size_t(P->n), hipMemcpyHostToDevice,
+                          stream()));
+    if (int rc = launch_gs(P, P->d_t, nullptr)) return rc;
+    NB_HIP(hipMemcpyAsync(u, P->d_t, sizeof(double) * size_t(P->n), hipMemcpyDeviceToHost,
+                          stream()));
+    NB_HIP(hipStreamSynchronize(stream()));
+    return NEKBONE_OK;
+}
+
+extern "C" int nekbone_maskit(nekbone_problem *P, double *w)
+{
+    if (int rc = check(P)) return rc;
+    NB_HIP(hipMemcpyAsync(P->d_t, w, sizeof(double) * size_t(P->n), hipMemcpyHostToDevice,
+                          stream()));
+    if (int rc = launch_mask(P, P->d_t, next_mask_bit(P))) return rc;
+    NB_HIP(hipMemcpyAsync(w, P->d_t, sizeof(double) * size_t(P->n), hipMemcpyDeviceToHost,
+                          stream()));
+    NB_HIP(hipStreamSynchronize(stream()));
+    return NEKBONE_OK;
+}
+
+extern "C" int nekbone_glsc3(nekbone_problem *P, const double *a, const double *b,
+                             const double *m, double *result)
+{
+    if (int rc = check(P)) return rc;
+    const size_t bytes = sizeof(double) * size_t(P->n);
+    NB_HIP(hipMemcpyAsync(P->d_t, a, bytes, hipMemcpyHostToDevice, stream()));
+    NB_HIP(hipMemcpyAsync(P->d_w, b, bytes, hipMemcpyHostToDevice, stream()));
+    NB_HIP(hipMemcpyAsync(P->d_f, m, bytes, hipMemcpyHostToDevice, stream()));
+    if (int rc = ensure_hist(P, 0)) return rc;
+    if (int rc = launch_glsc3(P, P->d_t, P->d_w, P->d_f, P->d_hist)) return rc;
+    NB_HIP(hipMemcpyAsync(result, P->d_hist, sizeof(double), hipMemcpyDeviceToHost, stream()));
+    NB_HIP(hipStreamSynchronize(stream()));
+    return NEKBONE_OK;
+}
+
+// ======================================================================
+// CG
+// ======================================================================
+static int copy_hist(nekbone_problem *P, double *hist, int niter)
+{
+    if (!hist) return NEKBONE_OK;
+    NB_HIP(hipMemcpyAsync(hist, P->d_hist, sizeof(double) * size_t(1 + 3 * niter),
+                          hipMemcpyDeviceToHost, stream()));
+    return NEKBONE_OK;
+}
+
+static int check_status(nekbone_problem *P)
+{
+    CgScalars s;
+    NB_HIP(hipMemcpyAsync(&s, P->d_scal, sizeof s, hipMemcpyDeviceToHost, stream()));
+    NB_HIP(hipStreamSynchronize(stream()));
+    if (s.status) return fail(NEKBONE_ERR_NUMERIC, "cg: non-positive or non-finite pap");
+    return NEKBONE_OK;
+}
+
+extern "C" int nekbone_cg_d(nekbone_problem *P, double *x, const double *f, double *r,
+                            int niter, int precond, double *hist)
+{
+    if (int rc = check(P)) return rc;
+    if (int rc = run_cg(P, x, f, r, niter, precond, 0.0)) return rc;
+    if (int rc = copy_hist(P, hist, niter)) return rc;
+    NB_HIP(hipStreamSynchronize(stream()));
+    return NEKBONE_OK;
+}
+
+extern "C" int nekbone_cg(nekbone_problem *P, double *x, const double *f, double *r, int niter,
+                          int precond, double *hist)
+{
+    if (int rc = check(P)) return rc;
+    const size_t bytes = sizeof(double) * size_t(P->n);
+    NB_HIP(hipMemcpyAsync(P->d_f, f, bytes, hipMemcpyHostToDevice, stream()));
+    if (int rc = run_cg(P, P->d_x, P->d_f, P->d_r, niter, precond, 0.0)) return rc;
+    if (int rc = copy_hist(P, hist, niter)) return rc;
+    if (x) NB_HIP(hipMemcpyAsync(x, P->d_x, bytes, hipMemcpyDeviceToHost, stream()));
+    if (r) NB_HIP(hipMemcpyAsync(r, P->d_r, bytes, hipMemcpyDeviceToHost, stream()));
+    NB_HIP(hipStreamSynchronize(stream()));
+    return NEKBONE_OK;
+}
+
+extern "C" int nekbone_cg_tol_d(nekbone_problem *P, double *x, const double *f, double *r,
+                                double tol, int maxiter, int precond, int *iters,
+                                double *final_rtr)
+{
+    if (int rc = check(P)) return rc;
+    if (!(tol > 0.0)) return fail(NEKBONE_ERR_ARG, "cg_tol: tol must be > 0");
+    if (precond == NEKBONE_PRECOND_JACOBI && !P->d_dinv)
+        return fail(NEKBONE_ERR_STATE, "cg: Jacobi preconditioner not set up");
+    if (int rc = ensure_hist(P, maxiter)) return rc;
+    const int mbit0 = next_mask_bit(P);
+    if (int rc = launch_cg_init(P, x, r, f, mbit0, precond, tol * tol)) return rc;
+    const int mbit = next_mask_bit(P);
+    // Iterations are queued in chunks; kernels of an already converged
+    // solve return at once, so only one host check per chunk is needed.
+    const int chunk = 10;
+    CgScalars s{};
+    for (int it = 0; it < maxiter; it += chunk) {
+        for (int k = it; k < std::min(maxiter, it + chunk); ++k)
+            if (int rc = launch_cg_iteration(P, x, r, precond, mbit)) return rc;
+        NB_HIP(hipMemcpyAsync(&s, P->d_scal, sizeof s, hipMemcpyDeviceToHost, stream()));
+        NB_HIP(hipStreamSynchronize(stream()));
+        if (s.done) break;
+    }
+    if (iters) *iters = s.iter;
+    if (final_rtr) *final_rtr = s.rtr;
+    if (s.status) return fail(NEKBONE_ERR_NUMERIC, "cg: non-positive or non-finite pap");
+    return NEKBONE_OK;
+}
+
+// ======================================================================
+// Device buffers and synthetic inputs
+// ======================================================================
+extern "C" int nekbone_alloc_vec(nekbone_problem *P, double **dptr)
+{
+    if (int rc = check(P)) return rc;
+    if (int rc = dalloc(dptr, P->n)) return rc;
+    NB_HIP(hipMemset(*dptr, 0, sizeof(double) * size_t(P->n)));
+    return NEKBONE_OK;
+}
+
+extern "C" int nekbone_free_vec(nekbone_problem *P, double *dptr)
+{
+    (void)P;
+    NB_HIP(hipFree(dptr));
+    return NEKBONE_OK;
+}
+
+extern "C" int nekbone_h2d(nekbone_problem *P, double *dst, const double *src, int64_t count)
+{
+    (void)P;
+    if (int rc = ensure_init()) return rc;
+    NB_HIP(hipMemcpyAsync(dst, src, sizeof(double) * size_t(count), hipMemcpyHostToDevice,
+                          stream()));
+    NB_HIP(hipStreamSynchronize(stream()));
+    return NEKBONE_OK;
+}
+
+extern "C" int nekbone_d2h(nekbone_problem *P, double *dst, const double *src, int64_t count)
+{
+    (void)P;
+    if (int rc = ensure_init()) return rc;
+    NB_HIP(hipMemcpyAsync(dst, src, sizeof(double) * size_t(count), hipMemcpyDeviceToHost,
+                          stream()));
+    NB_HIP(hipStreamSynchronize(stream()));
+    return NEKBONE_OK;
+}
+
+extern "C" int nekbone_rand_uniform_d(nekbone_problem *P, double *dst, int64_t count,
+                                      uint64_t seed)
+{
+    (void)P;
+    if (int rc = ensure_init()) return rc;
+    return launch_rand(dst, count, seed);
+}
+
+extern "C" int nekbone_rhs_from_raw_d(nekbone_problem *P, double *f)
+{
+    if (int rc = check(P)) return rc;
+    if (int rc = launch_gs(P, f, nullptr)) return rc;
+    return launch_col2(P, f, P->d_c);
+}
+
+// ======================================================================
+// Timing
+// ======================================================================
+extern "C" int nekbone_timing_enable(nekbone_problem *P, int mask)
+{
+    if (int rc = check(P)) return rc;
+    if (int rc = timing_resolve(P)) return rc;
+    P->timing.mask = mask & 31;
+    return NEKBONE_OK;
+}
+
+extern "C" int nekbone_timing_get(nekbone_problem *P, int kclass, int64_t *launches, double *ms)
+{
+    if (int rc = check(P)) return rc;
+    if (kclass < 0 || kclass > 4) return fail(NEKBONE_ERR_ARG, "bad kernel class");
+    if (int rc = timing_resolve(P)) return rc;
+    if (launches) *launches = P->timing.launches[kclass];
+    if (ms) *ms = P->timing.ms[kclass];
+    return NEKBONE_OK;
+}
+
+extern "C" int nekbone_timing_reset(nekbone_problem *P)
+{
+    if (int rc = check(P)) return rc;
+    if (int rc = timing_resolve(P)) return rc;
+    for (int k = 0; k < 5; ++k) {
+        P->timing.launches[k] = 0;
+        P->timing.ms[k] = 0.0;
+    }
+    return NEKBONE_OK;
+}

@@ -1,0 +1,146 @@
+// Internal declarations of the nekbone_amd library (not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/nekbone_amd.h"
+
+namespace nb {
+
+constexpr int kMaxNx = 16;
+
+// ---------------------------------------------------------------------
+// Errors: the C ABI returns a status and keeps a per-thread message.
+// ---------------------------------------------------------------------
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+#define NB_HIP(call)                                                               \
+    do {                                                                           \
+        hipError_t _e = (call);                                                    \
+        if (_e != hipSuccess)                                                      \
+            return ::nb::fail(NEKBONE_ERR_HIP, std::string(#call ": ") +           \
+                                                   hipGetErrorString(_e));         \
+    } while (0)
+
+// ---------------------------------------------------------------------
+// Host-side set-up (nb_setup.cpp): restatements of the reference's
+// set-up algorithms used by the library's own problem constructor.
+// ---------------------------------------------------------------------
+void gll_points_weights(double *z, double *w, int np);                  // speclib.f ZWGLL
+void fd_weights_full(double xx, const double *x, int n, int m, double *c); // semhat.f:39-93
+void semhat(double *a, double *b, double *c, double *d, double *z, int n); // semhat.f:2-37
+void box_numbering(std::vector<int64_t> &glo, int nx, int nelx, int nely, int nelz,
+                   const std::vector<int32_t> &lglel);                   // prox_dssum.f:42-86
+// Gather-scatter groups in the jl local order (gs.c:88-113,330-357):
+// only ids with >= 2 copies, groups ordered by their smallest index,
+// members ascending.  off has ngroups+1 entries.
+void gs_groups(const int64_t *glo, int64_t n, std::vector<int32_t> &off,
+               std::vector<int32_t> &idx);
+// Dirichlet lists (cg.f:214-368): every domain-boundary point, and the
+// element-1 positions the reference records on its first maskit call.
+void mask_lists(int nx, int nelt, int nelx, int nely, int nelz,
+                const std::vector<int32_t> &lglel, std::vector<int32_t> &full,
+                std::vector<int32_t> &ref);
+
+// ---------------------------------------------------------------------
+// Device work of one problem.
+// ---------------------------------------------------------------------
+struct GsMap {
+    int64_t ngroups = 0, nshared = 0;
+    int32_t *d_off = nullptr;  // ngroups+1
+    int32_t *d_idx = nullptr;  // nshared
+};
+
+// Per-kernel-class HIP-event timing.  Event pairs are recorded on the
+// library stream without synchronising and resolved when read.
+struct Timing {
+    int mask = 0;  // bit k: time kernel class k
+    int64_t launches[5] = {0, 0, 0, 0, 0};
+    double ms[5] = {0, 0, 0, 0, 0};
+    struct Pending {
+        hipEvent_t a, b;
+        int kclass;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
+    hipEvent_t cur = nullptr;
+};
+
+// Scalars of a CG solve, resident on the device; the finalize kernels
+// update them so that no host round trip is needed per iteration.
+struct CgScalars {
+    double rtz1, rtz2, beta, pap, alpha, rtr, rtr0, rtz_next, tol2;
+    int iter;   // iterations completed
+    int done;   // set once rtr <= tol2*rtr0 (tolerance runs)
+    int status; // sticky: 1 = non-finite or non-positive pap / rtz
+};
+
+}  // namespace nb
+
+struct nekbone_problem {
+    int nelt = 0, nx1 = 0, nelx = 0, nely = 0, nelz = 0;
+    int64_t n = 0;
+    std::vector<int32_t> lglel;
+    double dxm1[nb::kMaxNx * nb::kMaxNx] = {};
+    double zgll[nb::kMaxNx] = {}, wgll[nb::kMaxNx] = {};
+    std::vector<int64_t> glo;
+    std::vector<int32_t> mfull, mref;
+
+    // device data
+    double *d_D = nullptr;     // dxm1, nx1*nx1 column-major
+    double *d_g = nullptr;     // geometric factors, element-blocked SoA g[e][c][p]
+    double *d_c = nullptr;     // inverse multiplicity
+    double *d_dinv = nullptr;  // Jacobi
+    uint8_t *d_flag = nullptr; // bit0: full Dirichlet list, bit1: reference list
+    nb::GsMap gs;
+
+    // work vectors (n doubles)
+    double *d_x = nullptr, *d_r = nullptr, *d_p = nullptr, *d_w = nullptr, *d_f = nullptr,
+           *d_t = nullptr;
+    double *d_part = nullptr;  // reduction partials
+    nb::CgScalars *d_scal = nullptr;
+    double *d_hist = nullptr;
+    int hist_cap = 0;
+
+    int mask_mode = NEKBONE_MASK_REF;
+    int mask_built = 0;
+    int arith = NEKBONE_ARITH_EXACT;
+    bool jacobi = false;
+    nb::Timing timing;
+};
+
+namespace nb {
+hipStream_t stream();
+int ensure_init();
+// Mask bit to apply for the next maskit call, advancing the state.
+int next_mask_bit(nekbone_problem *P);
+
+// Kernel launchers (nb_kernels.hip / nb_ax.hip).  All on stream().
+int launch_ax_local(nekbone_problem *P, double *w, const double *u, int pmode, double *p,
+                    const double *r);
+int launch_gs(nekbone_problem *P, double *u, const CgScalars *guard);
+int launch_post(nekbone_problem *P, double *w, const double *p, int mbit, bool pap);
+int launch_mask(nekbone_problem *P, double *w, int mbit);
+int launch_glsc3(nekbone_problem *P, const double *a, const double *b, const double *m,
+                 double *d_out);
+int launch_brick_g(nekbone_problem *P);
+int launch_deformed_g(nekbone_problem *P, const double *d_vert);
+int launch_jacobi_local(nekbone_problem *P, double *dl);
+int launch_jacobi_finish(nekbone_problem *P, double *d);
+int launch_col2(nekbone_problem *P, double *a, const double *b);
+int launch_rand(double *dst, int64_t count, uint64_t seed);
+int launch_aos_to_soa(nekbone_problem *P, const double *aos, double *soa, int dir);
+// CG building blocks
+int launch_cg_init(nekbone_problem *P, double *x, double *r, const double *f, int mbit,
+                   int precond, double tol2);
+int launch_cg_iteration(nekbone_problem *P, double *x, double *r, int precond, int mbit);
+int timed_begin(nekbone_problem *P, int kclass);
+int timed_end(nekbone_problem *P, int kclass);
+int timing_resolve(nekbone_problem *P);
+int reduce_blocks(int64_t n);
+}  // namespace nb

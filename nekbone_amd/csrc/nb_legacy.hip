@@ -1,0 +1,440 @@
+// Reference-named, Fortran-callable entry points (trailing underscore, all
+// arguments by reference).  They act on the problem bound by
+// nekbone_setup_, standing in for the COMMON state the reference reads
+// (/dimn/, /dxyz/, /mymask/, /comm_handles/).  Host arrays in and out;
+// all arithmetic runs on the device through the same kernels as the
+// explicit API.
+#include <cstring>
+#include <map>
+#include <memory>
+
+#include "nb_internal.h"
+
+namespace nb {
+namespace {
+
+struct GsHandle {
+    int64_t n = 0;
+    std::vector<int64_t> glo;
+    std::unique_ptr<nekbone_problem, int (*)(nekbone_problem *)> host{nullptr,
+                                                                       nekbone_problem_destroy};
+    GsMap map;
+    double *d_u = nullptr;
+};
+
+struct LegacyState {
+    nekbone_problem *P = nullptr;
+    const double *g_ptr = nullptr;  // host g last uploaded
+    const double *c_ptr = nullptr;  // host c last uploaded
+    int precond = NEKBONE_PRECOND_IDENTITY;
+    std::map<int, GsHandle> gs;
+    int next_handle = 1;
+    nekbone_problem *elem = nullptr;  // one-element problem for ax_e_
+    double *d_scratch = nullptr;
+    int64_t scratch_len = 0;
+};
+
+LegacyState &S()
+{
+    static LegacyState s;
+    return s;
+}
+
+void report(int rc, const char *where)
+{
+    if (rc != NEKBONE_OK)
+        std::fprintf(stderr, "nekbone_amd: %s failed (%d): %s\n", where, rc, nekbone_last_error());
+}
+
+int scratch(int64_t count, double **out)
+{
+    LegacyState &L = S();
+    if (L.scratch_len < count) {
+        if (L.d_scratch) (void)hipFree(L.d_scratch);
+        L.d_scratch = nullptr;
+        L.scratch_len = 0;
+        NB_HIP(hipMalloc(&L.d_scratch, sizeof(double) * size_t(count)));
+        L.scratch_len = count;
+    }
+    *out = L.d_scratch;
+    return NEKBONE_OK;
+}
+
+__global__ void k_gs_plain(double *__restrict__ u, const int32_t *__restrict__ off,
+                           const int32_t *__restrict__ idx, int64_t ngroups)
+{
+    const int64_t gid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gid >= ngroups) return;
+    const int a = off[gid], b = off[gid + 1];
+    double t = u[idx[a]];
+    for (int q = a + 1; q < b; ++q) t += u[idx[q]];
+    for (int q = a; q < b; ++q) u[idx[q]] = t;
+}
+
+// local_grad3 (src/cg.f:170-187) for one element, one thread per point,
+// sums in mxf order.
+__global__ void k_grad3(double *ur, double *us, double *ut, const double *u, const double *D,
+                        const double *Dt, int nx)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nxyz = nx * nx * nx;
+    if (p >= nxyz) return;
+    const int i = p % nx, j = (p / nx) % nx, k = p / (nx * nx);
+    double a = D[i] * u[0 + nx * j + nx * nx * k];
+    double b = u[i + nx * 0 + nx * nx * k] * Dt[0 + nx * j];
+    double c = u[i + nx * j] * Dt[0 + nx * k];
+    for (int m = 1; m < nx; ++m) {
+        a = a + D[i + nx * m] * u[m + nx * j + nx * nx * k];
+        b = b + u[i + nx * m + nx * nx * k] * Dt[m + nx * j];
+        c = c + u[i + nx * j + nx * nx * m] * Dt[m + nx * k];
+    }
+    ur[p] = a;
+    us[p] = b;
+    ut[p] = c;
+}
+
+// local_grad3_t (src/cg.f:189-212): u = (Dt ur + us D) + ut D.
+__global__ void k_grad3_t(double *out, const double *ur, const double *us, const double *ut,
+                          const double *D, const double *Dt, int nx)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nxyz = nx * nx * nx;
+    if (p >= nxyz) return;
+    const int i = p % nx, j = (p / nx) % nx, k = p / (nx * nx);
+    double a = Dt[i] * ur[0 + nx * j + nx * nx * k];
+    double b = us[i + nx * 0 + nx * nx * k] * D[0 + nx * j];
+    double c = ut[i + nx * j] * D[0 + nx * k];
+    for (int m = 1; m < nx; ++m) {
+        a = a + Dt[i + nx * m] * ur[m + nx * j + nx * nx * k];
+        b = b + us[i + nx * m + nx * nx * k] * D[m + nx * j];
+        c = c + ut[i + nx * j + nx * nx * m] * D[m + nx * k];
+    }
+    out[p] = (a + b) + c;
+}
+
+__global__ void k_copy(double *dst, const double *src, int64_t n)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+__global__ void k_mul(double *dst, const double *a, const double *b, int64_t n)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = a[i] * b[i];
+}
+
+// Upload g and c when the caller hands a different array than last time.
+int sync_geometry(const double *g, const double *c)
+{
+    LegacyState &L = S();
+    if (g && g != L.g_ptr) {
+        if (int rc = nekbone_set_geometry(L.P, g)) return rc;
+        L.g_ptr = g;
+        if (L.precond == NEKBONE_PRECOND_JACOBI)
+            if (int rc = nekbone_setup_jacobi(L.P)) return rc;
+    }
+    if (c && c != L.c_ptr) {
+        NB_HIP(hipMemcpy(L.P->d_c, c, sizeof(double) * size_t(L.P->n), hipMemcpyHostToDevice));
+        L.c_ptr = c;
+    }
+    return NEKBONE_OK;
+}
+
+int bound(const char *where)
+{
+    if (!S().P) return fail(NEKBONE_ERR_STATE, std::string(where) + ": nekbone_setup_ not called");
+    return NEKBONE_OK;
+}
+
+}  // namespace
+}  // namespace nb
+
+using namespace nb;
+
+// COMMON /cflops/ flop_a, flop_cg, mflops (src/INPUT): updated when the
+// Fortran host defines it, as src/cg.f:81,122 do.
+extern "C" {
+struct nb_cflops {
+    double flop_a, flop_cg, mflops;
+};
+extern nb_cflops cflops_ __attribute__((weak));
+}
+
+extern "C" void nekbone_setup_(const int *nelt, const int *nx1, const int *nelx, const int *nely,
+                               const int *nelz, const int32_t *lglel, const double *dxm1,
+                               const int *gsh)
+{
+    LegacyState &L = S();
+    auto it = L.gs.find(*gsh);
+    if (it == L.gs.end()) {
+        report(fail(NEKBONE_ERR_ARG, "nekbone_setup_: unknown gs handle"), "nekbone_setup_");
+        return;
+    }
+    if (L.P) nekbone_problem_destroy(L.P);
+    L.P = nullptr;
+    L.g_ptr = L.c_ptr = nullptr;
+    int rc = nekbone_problem_create(&L.P, *nelt, *nx1, *nelx, *nely, *nelz, lglel, dxm1,
+                                    it->second.glo.data(), nullptr, nullptr);
+    report(rc, "nekbone_setup_");
+}
+
+extern "C" void nekbone_use_jacobi_(const int *on)
+{
+    LegacyState &L = S();
+    L.precond = (*on) ? NEKBONE_PRECOND_JACOBI : NEKBONE_PRECOND_IDENTITY;
+    if (L.P && L.precond == NEKBONE_PRECOND_JACOBI)
+        report(nekbone_setup_jacobi(L.P), "nekbone_use_jacobi_");
+}
+
+extern "C" void cg_(double *x, const double *f, const double *g, const double *c, double *r,
+                    double *w, double *p, double *z, const int *n, const int *niter,
+                    double *flop_cg)
+{
+    LegacyState &L = S();
+    int rc = bound("cg_");
+    if (!rc && *n != L.P->n) rc = fail(NEKBONE_ERR_ARG, "cg_: n differs from the bound mesh");
+    if (!rc) rc = sync_geometry(g, c);
+    if (!rc) rc = nekbone_cg(L.P, x, f, r, *niter, L.precond, nullptr);
+    if (!rc) {
+        const size_t bytes = sizeof(double) * size_t(L.P->n);
+        if (w) rc = hipMemcpy(w, L.P->d_w, bytes, hipMemcpyDeviceToHost) ? NEKBONE_ERR_HIP : 0;
+        if (!rc && p)
+            rc = hipMemcpy(p, L.P->d_p, bytes, hipMemcpyDeviceToHost) ? NEKBONE_ERR_HIP : 0;
+        (void)z;  // z is scratch in the reference; the fused path never stores it
+        // src/cg.f:79-81: after the loop iter = niter+1
+        const double iter = double(*niter + 1);
+        if (flop_cg) *flop_cg += iter * 15.0 * double(*n);
+        const double nxyz = double(L.P->nx1) * L.P->nx1 * L.P->nx1;
+        if (&cflops_)
+            cflops_.flop_a += double(*niter) * (19.0 * nxyz + 12.0 * L.P->nx1 * nxyz) * L.P->nelt;
+    }
+    report(rc, "cg_");
+}
+
+extern "C" void solvem_(double *z, double *r, const int *n)
+{
+    LegacyState &L = S();
+    int rc = bound("solvem_");
+    if (!rc && *n != L.P->n) rc = fail(NEKBONE_ERR_ARG, "solvem_: n differs from the bound mesh");
+    if (rc) return report(rc, "solvem_");
+    const int64_t nn = L.P->n;
+    const size_t bytes = sizeof(double) * size_t(nn);
+    if (hipMemcpy(L.P->d_t, r, bytes, hipMemcpyHostToDevice) != hipSuccess)
+        return report(fail(NEKBONE_ERR_HIP, "solvem_ upload"), "solvem_");
+    if (L.precond == NEKBONE_PRECOND_JACOBI && L.P->d_dinv)
+        hipLaunchKernelGGL(k_mul, dim3(unsigned((nn + 255) / 256)), dim3(256), 0, stream(),
+                           L.P->d_w, L.P->d_t, L.P->d_dinv, nn);
+    else
+        hipLaunchKernelGGL(k_copy, dim3(unsigned((nn + 255) / 256)), dim3(256), 0, stream(),
+                           L.P->d_w, L.P->d_t, nn);
+    (void)hipStreamSynchronize(stream());
+    if (hipMemcpy(z, L.P->d_w, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+        report(fail(NEKBONE_ERR_HIP, "solvem_ download"), "solvem_");
+}
+
+extern "C" void ax_(double *w, const double *u, const double *gxyz, double *ur, double *us,
+                    double *ut, double *wk, const int *n)
+{
+    (void)ur, (void)us, (void)ut, (void)wk;
+    LegacyState &L = S();
+    int rc = bound("ax_");
+    if (!rc && *n != L.P->n) rc = fail(NEKBONE_ERR_ARG, "ax_: n differs from the bound mesh");
+    if (!rc) rc = sync_geometry(gxyz, nullptr);
+    if (!rc) rc = nekbone_ax(L.P, w, u);
+    if (!rc && &cflops_) {
+        const double nxyz = double(L.P->nx1) * L.P->nx1 * L.P->nx1;
+        cflops_.flop_a += (19.0 * nxyz + 12.0 * L.P->nx1 * nxyz) * L.P->nelt;
+    }
+    report(rc, "ax_");
+}
+
+extern "C" void ax_e_(double *w, const double *u, const double *g, double *ur, double *us,
+                      double *ut, double *wk)
+{
+    (void)ur, (void)us, (void)ut, (void)wk;
+    LegacyState &L = S();
+    int rc = bound("ax_e_");
+    if (!rc && (!L.elem || L.elem->nx1 != L.P->nx1)) {
+        if (L.elem) nekbone_problem_destroy(L.elem);
+        L.elem = nullptr;
+        const int nx = L.P->nx1;
+        std::vector<int64_t> glo(size_t(nx) * nx * nx);
+        for (size_t i = 0; i < glo.size(); ++i) glo[i] = int64_t(i) + 1;
+        const int32_t one = 1;
+        rc = nekbone_problem_create(&L.elem, 1, nx, 1, 1, 1, &one, L.P->dxm1, glo.data(),
+                                    nullptr, nullptr);
+    }
+    if (!rc) rc = nekbone_set_geometry(L.elem, g);
+    if (!rc) rc = nekbone_ax_local(L.elem, w, u);
+    report(rc, "ax_e_");
+}
+
+static void grad_common(bool transpose, double *o0, double *o1, double *o2, double *out,
+                        const double *in0, const double *in1, const double *in2, const int *n,
+                        const double *D, const double *Dt)
+{
+    int rc = ensure_init();
+    const int nx = *n + 1;
+    const int64_t nxyz = int64_t(nx) * nx * nx;
+    double *d = nullptr;
+    if (!rc) rc = scratch(7 * nxyz + 2 * nx * nx, &d);
+    if (rc) return report(rc, "local_grad3");
+    double *dD = d + 7 * nxyz, *dDt = dD + nx * nx;
+    const size_t eb = sizeof(double) * size_t(nxyz), db = sizeof(double) * size_t(nx * nx);
+    bool ok = hipMemcpy(dD, D, db, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(dDt, Dt, db, hipMemcpyHostToDevice) == hipSuccess;
+    if (!transpose) {
+        ok = ok && hipMemcpy(d, in0, eb, hipMemcpyHostToDevice) == hipSuccess;
+        hipLaunchKernelGGL(k_grad3, dim3(unsigned((nxyz + 255) / 256)), dim3(256), 0, stream(),
+                           d + nxyz, d + 2 * nxyz, d + 3 * nxyz, d, dD, dDt, nx);
+        (void)hipStreamSynchronize(stream());
+        ok = ok && hipMemcpy(o0, d + nxyz, eb, hipMemcpyDeviceToHost) == hipSuccess &&
+             hipMemcpy(o1, d + 2 * nxyz, eb, hipMemcpyDeviceToHost) == hipSuccess &&
+             hipMemcpy(o2, d + 3 * nxyz, eb, hipMemcpyDeviceToHost) == hipSuccess;
+    } else {
+        ok = ok && hipMemcpy(d + nxyz, in0, eb, hipMemcpyHostToDevice) == hipSuccess &&
+             hipMemcpy(d + 2 * nxyz, in1, eb, hipMemcpyHostToDevice) == hipSuccess &&
+             hipMemcpy(d + 3 * nxyz, in2, eb, hipMemcpyHostToDevice) == hipSuccess;
+        hipLaunchKernelGGL(k_grad3_t, dim3(unsigned((nxyz + 255) / 256)), dim3(256), 0,
+                           stream(), d, d + nxyz, d + 2 * nxyz, d + 3 * nxyz, dD, dDt, nx);
+        (void)hipStreamSynchronize(stream());
+        ok = ok && hipMemcpy(out, d, eb, hipMemcpyDeviceToHost) == hipSuccess;
+    }
+    if (!ok) report(fail(NEKBONE_ERR_HIP, "local_grad3 copy"), "local_grad3");
+}
+
+extern "C" void local_grad3_(double *ur, double *us, double *ut, const double *u, const int *n,
+                             const double *D, const double *Dt)
+{
+    grad_common(false, ur, us, ut, nullptr, u, nullptr, nullptr, n, D, Dt);
+}
+
+extern "C" void local_grad3_t_(double *u, const double *ur, const double *us, const double *ut,
+                               const int *N, const double *D, const double *Dt, double *w)
+{
+    (void)w;
+    grad_common(true, nullptr, nullptr, nullptr, u, ur, us, ut, N, D, Dt);
+}
+
+extern "C" void dssum_(double *f)
+{
+    LegacyState &L = S();
+    int rc = bound("dssum_");
+    if (!rc) rc = nekbone_dssum(L.P, f);
+    report(rc, "dssum_");
+}
+
+extern "C" void maskit_(double *w, double *pmask, const int *nx, const int *ny, const int *nz)
+{
+    (void)pmask, (void)nx, (void)ny, (void)nz;
+    LegacyState &L = S();
+    int rc = bound("maskit_");
+    if (!rc) rc = nekbone_maskit(L.P, w);
+    report(rc, "maskit_");
+}
+
+extern "C" double glsc3_(const double *a, const double *b, const double *mult, const int *n)
+{
+    const int64_t nn = *n;
+    double *d = nullptr;
+    int rc = ensure_init();
+    if (!rc) rc = scratch(3 * nn + 2 * 1024 + 1, &d);
+    if (rc) {
+        report(rc, "glsc3_");
+        return 0.0;
+    }
+    const size_t bytes = sizeof(double) * size_t(nn);
+    double out = 0.0;
+    nekbone_problem tmp;  // carries only n and the partials for the reduction
+    tmp.n = nn;
+    tmp.d_part = d + 3 * nn;
+    bool ok = hipMemcpy(d, a, bytes, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(d + nn, b, bytes, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(d + 2 * nn, mult, bytes, hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && launch_glsc3(&tmp, d, d + nn, d + 2 * nn, d + 3 * nn + 2 * 1024) == NEKBONE_OK;
+    (void)hipStreamSynchronize(stream());
+    ok = ok && hipMemcpy(&out, d + 3 * nn + 2 * 1024, sizeof(double), hipMemcpyDeviceToHost) ==
+                   hipSuccess;
+    tmp.d_part = nullptr;
+    if (!ok) report(fail(NEKBONE_ERR_HIP, "glsc3_"), "glsc3_");
+    return out;
+}
+
+extern "C" void gs_setup_(int *handle, const int64_t *glo_num, const int *n, const int *comm,
+                          const int *np)
+{
+    (void)comm;
+    LegacyState &L = S();
+    if (np && *np != 1) {
+        report(fail(NEKBONE_ERR_UNSUPPORTED, "gs_setup_: one rank per GPU (np must be 1)"),
+               "gs_setup_");
+        *handle = -1;
+        return;
+    }
+    int rc = ensure_init();
+    GsHandle h;
+    h.n = *n;
+    h.glo.assign(glo_num, glo_num + *n);
+    std::vector<int32_t> off, idx;
+    gs_groups(glo_num, *n, off, idx);
+    h.map.ngroups = int64_t(off.size()) - 1;
+    h.map.nshared = int64_t(idx.size());
+    if (!rc && hipMalloc(&h.map.d_off, sizeof(int32_t) * off.size()) != hipSuccess)
+        rc = NEKBONE_ERR_HIP;
+    if (!rc && hipMalloc(&h.map.d_idx, sizeof(int32_t) * std::max<size_t>(idx.size(), 1)) !=
+                   hipSuccess)
+        rc = NEKBONE_ERR_HIP;
+    if (!rc && hipMalloc(&h.d_u, sizeof(double) * size_t(std::max<int64_t>(h.n, 1))) !=
+                   hipSuccess)
+        rc = NEKBONE_ERR_HIP;
+    if (!rc) {
+        (void)hipMemcpy(h.map.d_off, off.data(), sizeof(int32_t) * off.size(),
+                        hipMemcpyHostToDevice);
+        if (!idx.empty())
+            (void)hipMemcpy(h.map.d_idx, idx.data(), sizeof(int32_t) * idx.size(),
+                            hipMemcpyHostToDevice);
+    }
+    if (rc) {
+        report(fail(rc, "gs_setup_: device allocation"), "gs_setup_");
+        *handle = -1;
+        return;
+    }
+    *handle = L.next_handle++;
+    L.gs[*handle] = std::move(h);
+}
+
+extern "C" void gs_op_(const int *handle, double *u, const int *dom, const int *op,
+                       const int *transpose)
+{
+    (void)transpose;  // QQ^T is symmetric on one rank
+    LegacyState &L = S();
+    auto it = L.gs.find(*handle);
+    if (it == L.gs.end()) return report(fail(NEKBONE_ERR_ARG, "gs_op_: invalid handle"), "gs_op_");
+    // src/jl/gs.c:1178-1186 validates dom/op; this path implements
+    // double (1) and add (1), the only combination the CG path uses.
+    if (*dom != 1 || *op != 1)
+        return report(fail(NEKBONE_ERR_UNSUPPORTED, "gs_op_: only dom=1 (double), op=1 (add)"),
+                      "gs_op_");
+    GsHandle &h = it->second;
+    const size_t bytes = sizeof(double) * size_t(h.n);
+    bool ok = hipMemcpy(h.d_u, u, bytes, hipMemcpyHostToDevice) == hipSuccess;
+    if (h.map.ngroups > 0)
+        hipLaunchKernelGGL(k_gs_plain, dim3(unsigned((h.map.ngroups + 255) / 256)), dim3(256), 0,
+                           stream(), h.d_u, h.map.d_off, h.map.d_idx, h.map.ngroups);
+    (void)hipStreamSynchronize(stream());
+    ok = ok && hipMemcpy(u, h.d_u, bytes, hipMemcpyDeviceToHost) == hipSuccess;
+    if (!ok) report(fail(NEKBONE_ERR_HIP, "gs_op_ copy"), "gs_op_");
+}
+
+extern "C" void gs_free_(const int *handle)
+{
+    LegacyState &L = S();
+    auto it = L.gs.find(*handle);
+    if (it == L.gs.end()) return;
+    GsHandle &h = it->second;
+    if (h.map.d_off) (void)hipFree(h.map.d_off);
+    if (h.map.d_idx) (void)hipFree(h.map.d_idx);
+    if (h.d_u) (void)hipFree(h.d_u);
+    L.gs.erase(it);
+}

@@ -1,0 +1,217 @@
+"""Parity of the HIP path (through the C ABI) with the oracle and with the
+compiled reference's fixtures.
+
+Bars: integer/index work and the EXACT-arithmetic FP64 operators (ax,
+dssum, set-up arrays, deformed factors, Jacobi diagonal) bit for bit;
+glsc3 and CG scalars, whose only difference is the summation order of the
+reductions (tree on the GPU, sequential in the reference), to 1e-12
+relative - the tolerance the plain FP64 path is specified to."""
+import numpy as np
+import pytest
+
+from conftest import SMALL_CASES, splitmix_uniform, tag_of
+
+pytestmark = pytest.mark.gpu
+
+HIST_RTOL = 1e-12
+
+
+def _case(msg):
+    print("CASE", msg, flush=True)
+
+
+@pytest.mark.parametrize("nel,nx1", SMALL_CASES)
+def test_setup_arrays(nb, oracle, golden, nel, nx1):
+    _case("setup %s %d" % (nel, nx1))
+    t = tag_of(nel, nx1)
+    P = nb.Problem(*nel, nx1)
+    O = oracle.Problem(*nel, nx1)
+    assert np.array_equal(P.dxm1(), golden[t + "_dxm1"])
+    assert np.array_equal(P.c(), golden[t + "_c"])
+    assert np.array_equal(P.glo(), golden[t + "_glo"])
+    off, idx = O.gs_map()
+    assert P.gs_counts() == (off.size - 1, idx.size)
+    assert np.array_equal(P.geometry(), golden[t + "_g"])
+
+
+@pytest.mark.parametrize("nel,nx1", SMALL_CASES)
+def test_ax_local_bit_exact(nb, oracle, nel, nx1):
+    _case("ax_local %s %d" % (nel, nx1))
+    P = nb.Problem(*nel, nx1)
+    O = oracle.Problem(*nel, nx1)
+    u = splitmix_uniform(P.n, 21)
+    m = nx1 ** 3
+    ref = np.concatenate([O.ax_e(e, u[e * m:(e + 1) * m]) for e in range(O.nelt)])
+    assert np.array_equal(P.ax_local(u), ref)
+
+
+@pytest.mark.parametrize("nx1", list(range(2, 17)))
+def test_ax_local_every_order(nb, oracle, nx1):
+    _case("ax_local order %d" % nx1)
+    nel = (3, 2, 1)
+    P = nb.Problem(*nel, nx1)
+    O = oracle.Problem(*nel, nx1)
+    P.set_geometry_deformed(0.1, 5)
+    O.deformed_g(0.1, 5)
+    assert np.array_equal(P.geometry(), O.g())
+    u = splitmix_uniform(P.n, nx1)
+    m = nx1 ** 3
+    ref = np.concatenate([O.ax_e(e, u[e * m:(e + 1) * m]) for e in range(O.nelt)])
+    assert np.array_equal(P.ax_local(u), ref)
+
+
+@pytest.mark.parametrize("nel,nx1", SMALL_CASES)
+def test_ax_two_calls_match_reference(nb, golden, nel, nx1):
+    """ax twice on the same input: the first call masks every boundary
+    point and records the list, the second masks the recorded list
+    (src/cg.f:214-258) - both bit-identical to the compiled reference."""
+    _case("ax %s %d" % (nel, nx1))
+    t = tag_of(nel, nx1)
+    P = nb.Problem(*nel, nx1)
+    u = splitmix_uniform(P.n, 7)
+    assert np.array_equal(P.ax(u), golden[t + "_ax_w1"])
+    assert np.array_equal(P.ax(u), golden[t + "_ax_w2"])
+
+
+@pytest.mark.parametrize("nel,nx1", SMALL_CASES)
+def test_dssum_bit_exact(nb, golden, nel, nx1):
+    _case("dssum %s %d" % (nel, nx1))
+    t = tag_of(nel, nx1)
+    P = nb.Problem(*nel, nx1)
+    assert np.array_equal(P.dssum(splitmix_uniform(P.n, 7)), golden[t + "_dssum"])
+
+
+def test_dssum_idempotent_on_multiplicity(nb):
+    _case("dssum idempotence")
+    P = nb.Problem(4, 3, 2, 7)
+    c = P.c()
+    # QQ^T (c * QQ^T u) = QQ^T u
+    u = splitmix_uniform(P.n, 2)
+    a = P.dssum(u)
+    np.testing.assert_allclose(P.dssum(c * a), a, rtol=1e-15, atol=1e-15)
+
+
+def test_glsc3(nb, oracle):
+    _case("glsc3")
+    P = nb.Problem(5, 4, 3, 8)
+    a, b, m = (splitmix_uniform(P.n, s) for s in (1, 2, 3))
+    ref = oracle.glsc3(a, b, m)
+    assert abs(P.glsc3(a, b, m) - ref) <= 1e-13 * np.sum(np.abs(a * b * m))
+
+
+def test_deformed_geometry_and_jacobi_bit_exact(nb, oracle):
+    _case("deformed + jacobi")
+    for nel, nx1 in [((3, 2, 2), 8), ((2, 2, 2), 10), ((1, 1, 1), 2)]:
+        P = nb.Problem(*nel, nx1)
+        O = oracle.Problem(*nel, nx1)
+        P.set_geometry_deformed(0.1, 42)
+        O.deformed_g(0.1, 42)
+        assert np.array_equal(P.geometry(), O.g())
+        P.setup_jacobi()
+        O.setup_jacobi()
+        assert np.array_equal(P.dinv(), O.dinv())
+
+
+def test_deformed_amp0_is_brick(nb):
+    _case("deformed amp 0")
+    P = nb.Problem(3, 3, 2, 9)
+    g0 = P.geometry()
+    P.set_geometry_deformed(0.0, 7)
+    assert np.array_equal(P.geometry(), g0)
+
+
+def test_geometry_roundtrip(nb):
+    _case("geometry roundtrip")
+    P = nb.Problem(2, 2, 3, 6)
+    g = splitmix_uniform(6 * P.n, 9)
+    P.set_geometry(g)
+    assert np.array_equal(P.geometry(), g)
+
+
+@pytest.mark.parametrize("tag,nel,nx1,niter,ncalls,seed", [
+    ("cg_b888_n8", (8, 8, 8), 8, 100, 2, 0),
+    ("cg_b345_n6", (3, 4, 5), 6, 40, 1, 3),
+])
+def test_cg_history_vs_reference(nb, oracle, golden, tag, nel, nx1, niter, ncalls, seed):
+    """Identity preconditioner, the reference's mask: the full glsc3 history
+    of the driver's consecutive cg calls against the compiled reference."""
+    _case("cg ref %s" % tag)
+    P = nb.Problem(*nel, nx1)
+    O = oracle.Problem(*nel, nx1)
+    f = O.rhs_from_raw(splitmix_uniform(O.n, seed))
+    ref = golden[tag + "_hist"]
+    for call in range(ncalls):
+        x, r, h = P.cg(f, niter)
+        np.testing.assert_allclose(h, ref[call], rtol=HIST_RTOL, atol=0)
+    if tag == "cg_b888_n8":
+        np.testing.assert_allclose(x[:4096], golden[tag + "_x2_head"], rtol=0,
+                                   atol=1e-12 * np.abs(x).max())
+
+
+@pytest.mark.parametrize("deformed", [False, True])
+def test_cg_jacobi_dirichlet_vs_oracle(nb, oracle, deformed):
+    _case("cg jacobi deformed=%s" % deformed)
+    nel, nx1 = (6, 5, 4), 10
+    P = nb.Problem(*nel, nx1)
+    O = oracle.Problem(*nel, nx1)
+    for Q in (P,):
+        Q.set_mask_mode(nb.MASK_DIRICHLET)
+    O.set_mask_mode(oracle.MASK_DIRICHLET)
+    if deformed:
+        P.set_geometry_deformed(0.1, 3)
+        O.deformed_g(0.1, 3)
+    P.setup_jacobi()
+    O.setup_jacobi()
+    f = O.rhs_from_raw(splitmix_uniform(O.n, 0))
+    xo, ro, ho = O.cg(f, 100, oracle.PRECOND_JACOBI)
+    x, r, h = P.cg(f, 100, nb.PRECOND_JACOBI)
+    np.testing.assert_allclose(h, ho, rtol=HIST_RTOL, atol=0)
+    np.testing.assert_allclose(x, xo, rtol=0, atol=1e-12 * np.abs(xo).max())
+
+
+def test_fma_arithmetic_close_to_exact(nb):
+    _case("fma arithmetic")
+    P = nb.Problem(4, 4, 4, 10)
+    P.set_geometry_deformed(0.1, 1)
+    u = splitmix_uniform(P.n, 3)
+    w0 = P.ax_local(u)
+    P.set_arith(nb.ARITH_FMA)
+    w1 = P.ax_local(u)
+    assert not np.array_equal(w0, w1)
+    np.testing.assert_allclose(w1, w0, rtol=0, atol=1e-13 * np.abs(w0).max())
+
+
+def test_drop_in_create_matches_box(nb, oracle):
+    """Set-up from the reference's own arrays gives the same operator."""
+    _case("drop-in create")
+    nel, nx1 = (3, 2, 4), 7
+    O = oracle.Problem(*nel, nx1)
+    O.deformed_g(0.08, 6)
+    nelt = O.nelt
+    P = nb.Problem.from_arrays(nelt, nx1, *nel, np.arange(1, nelt + 1), O.dxm1(), O.glo(),
+                               O.g(), O.c())
+    u = splitmix_uniform(O.n, 4)
+    assert np.array_equal(P.ax(u), O.ax(u))
+    assert np.array_equal(P.ax(u), O.ax(u))
+
+
+def test_cg_tolerance_stop(nb, oracle):
+    """Tolerance runs stop at the first iteration with rtr <= tol^2 rtr0,
+    the same iteration the oracle's history gives."""
+    _case("cg tolerance")
+    nel, nx1 = (4, 4, 4), 8
+    P = nb.Problem(*nel, nx1)
+    O = oracle.Problem(*nel, nx1)
+    for Q, mode in ((P, nb.MASK_DIRICHLET), (O, oracle.MASK_DIRICHLET)):
+        Q.set_mask_mode(mode)
+        Q.setup_jacobi()
+    f = O.rhs_from_raw(splitmix_uniform(O.n, 8))
+    _, _, ho = O.cg(f, 300, oracle.PRECOND_JACOBI)
+    tol = 1e-8
+    rtr = ho[3::3]
+    expect = int(np.argmax(rtr <= tol * tol * ho[0])) + 1
+    fd, xd, rd = P.vec(), P.vec(), P.vec()
+    fd.upload(f)
+    iters, final = P.cg_tol_d(xd, fd, rd, tol, 300, nb.PRECOND_JACOBI)
+    assert iters == expect
+    assert final <= tol * tol * ho[0]
