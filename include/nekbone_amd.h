@@ -104,6 +104,22 @@ int64_t nekbone_gs_nshared(const nekbone_problem *P);
 int nekbone_set_mask_mode(nekbone_problem *P, int mode);
 int nekbone_reset_mask(nekbone_problem *P);
 int nekbone_set_arith(nekbone_problem *P, int arith);
+/* CG kernel sequence: 1 (default when c = 1/multiplicity) fuses the
+ * direction and x updates, the Helmholtz shift, the mask and pap into the
+ * element and gather-scatter kernels; 0 runs the reference's sequence of
+ * separate passes.  Values agree bit for bit between the two; only the
+ * dot products' summation trees differ. */
+int nekbone_set_fused(nekbone_problem *P, int on);
+
+/* Element-kernel shape: elements per workgroup, geometric factors
+ * prefetched at kernel entry (0/1), x update carried by the element kernel
+ * (0/1).  nekbone_autotune times every shape compiled for this order on
+ * this problem (a few fused CG iterations each), keeps the fastest and
+ * remembers it for later problems of the same order, preconditioner and
+ * arithmetic in this process; *best_ms = its time per iteration. */
+int nekbone_autotune(nekbone_problem *P, int precond, double *best_ms);
+int nekbone_get_variant(const nekbone_problem *P, int *epb, int *pf, int *xu);
+int nekbone_set_variant(nekbone_problem *P, int epb, int pf, int xu);
 
 /* Jacobi preconditioner (the reference has none): dinv = 1/(QQ^T
  * diag(A_L) + 0.1), computed on the device. */
@@ -169,6 +185,10 @@ int nekbone_rhs_from_raw_d(nekbone_problem *P, double *f);
 int nekbone_timing_enable(nekbone_problem *P, int mask);
 int nekbone_timing_get(nekbone_problem *P, int kclass, int64_t *launches, double *ms);
 int nekbone_timing_reset(nekbone_problem *P);
+/* Streaming device copy of `bytes` bytes (half read, half written), reps
+ * times: the achievable-bandwidth calibration for roofline figures and
+ * the reference point for HBM byte counters.  *ms = mean per copy. */
+int nekbone_bw_probe(int64_t bytes, int reps, double *ms);
 int nekbone_sync(void);
 
 /* ---------------------------------------------------------------------- */

@@ -21,7 +21,10 @@ LIBDIR = os.path.join(HERE, "lib")
 LIBNAME = "libnekbone_amd.so"
 ARCH = os.environ.get("NEKBONE_ARCH", "gfx950")
 
-SOURCES = ["nb_setup.cpp", "nb_ax.hip", "nb_kernels.hip", "nb_api.hip", "nb_legacy.hip",
+SOURCES = ["nb_ax_i10.hip", "nb_ax_i12.hip", "nb_ax_i8.hip", "nb_ax_i6.hip", "nb_ax_ilow.hip",
+           "nb_ax_ihigh.hip", "nb_setup.cpp", "nb_ax.hip", "nb_kernels.hip", "nb_cg.hip",
+           "nb_api.hip",
+           "nb_legacy.hip",
            "nb_mixed.hip", "nb_batch.hip", "nb_hsmg.hip", "nb_hsmg_setup.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -35,9 +38,18 @@ def lib_path():
     return os.path.join(LIBDIR, LIBNAME)
 
 
-def _compile(src, verbose):
+def _newest_header():
+    hdrs = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".h")]
+    hdrs += [os.path.join(ROOT, "include", f) for f in os.listdir(os.path.join(ROOT, "include"))]
+    return max(os.path.getmtime(h) for h in hdrs)
+
+
+def _compile(src, verbose, incremental=False):
     path = os.path.join(CSRC, src)
     obj = os.path.join(BUILD, src.rsplit(".", 1)[0] + ".o")
+    if incremental and os.path.exists(obj):
+        if os.path.getmtime(obj) > max(os.path.getmtime(path), _newest_header()):
+            return obj
     if src.endswith(".hip"):
         cmd = [HIPCC, "-x", "hip"] + COMMON + DEVICE + ["-c", path, "-o", obj]
     else:
@@ -53,13 +65,15 @@ def _compile(src, verbose):
     return obj
 
 
-def build(verbose=False, jobs=None):
+def build(verbose=False, jobs=None, incremental=False):
+    """Compile every source (incremental=True skips objects newer than
+    their source and every header) and link the library."""
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     jobs = jobs or min(len(srcs), max(1, min(8, os.cpu_count() or 1)))
     with cf.ThreadPoolExecutor(jobs) as ex:
-        objs = list(ex.map(lambda s: _compile(s, verbose), srcs))
+        objs = list(ex.map(lambda s: _compile(s, verbose, incremental), srcs))
     tmp = lib_path() + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs
     if verbose:
@@ -76,4 +90,4 @@ def clean():
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    print(build(verbose="-v" in sys.argv, incremental="--incremental" in sys.argv))

@@ -51,6 +51,7 @@ _SIGS = {
     "nekbone_set_mask_mode": (ctypes.c_int, [_vp, ctypes.c_int]),
     "nekbone_reset_mask": (ctypes.c_int, [_vp]),
     "nekbone_set_arith": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "nekbone_set_fused": (ctypes.c_int, [_vp, ctypes.c_int]),
     "nekbone_setup_jacobi": (ctypes.c_int, [_vp]),
     "nekbone_get_dinv": (ctypes.c_int, [_vp, _dp]),
     "nekbone_ax": (ctypes.c_int, [_vp, _dp, _dp]),
@@ -73,6 +74,10 @@ _SIGS = {
     "nekbone_timing_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
     "nekbone_timing_get": (ctypes.c_int, [_vp, ctypes.c_int, _i64p, _dp]),
     "nekbone_timing_reset": (ctypes.c_int, [_vp]),
+    "nekbone_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "nekbone_get_variant": (ctypes.c_int, [_vp] + [ctypes.POINTER(ctypes.c_int)] * 3),
+    "nekbone_set_variant": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "nekbone_bw_probe": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, _dp]),
 }
 
 _LIB = None
@@ -122,6 +127,13 @@ def init(device=0):
 
 def sync():
     _chk(lib().nekbone_sync())
+
+
+def bw_probe(nbytes, reps=10):
+    """Device copy of nbytes (half read, half written); returns ms/copy."""
+    ms = ctypes.c_double()
+    _chk(lib().nekbone_bw_probe(int(nbytes), int(reps), ctypes.byref(ms)))
+    return ms.value
 
 
 class DeviceVec:
@@ -237,6 +249,9 @@ class Problem:
     def set_arith(self, arith):
         _chk(lib().nekbone_set_arith(self.handle, arith))
 
+    def set_fused(self, on):
+        _chk(lib().nekbone_set_fused(self.handle, int(bool(on))))
+
     def setup_jacobi(self):
         _chk(lib().nekbone_setup_jacobi(self.handle))
 
@@ -309,6 +324,19 @@ class Problem:
 
     def ax_d(self, w, u):
         _chk(lib().nekbone_ax_d(self.handle, w.ptr, u.ptr))
+
+    def autotune(self, precond=PRECOND_JACOBI):
+        ms = ctypes.c_double()
+        _chk(lib().nekbone_autotune(self.handle, precond, ctypes.byref(ms)))
+        return ms.value
+
+    def variant(self):
+        v = [ctypes.c_int() for _ in range(3)]
+        _chk(lib().nekbone_get_variant(self.handle, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def set_variant(self, epb, pf, xu):
+        _chk(lib().nekbone_set_variant(self.handle, epb, pf, xu))
 
     # timing --------------------------------------------------------------
     def timing_enable(self, mask):

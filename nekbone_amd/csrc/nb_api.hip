@@ -6,6 +6,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <tuple>
 
 #include "nb_internal.h"
 
@@ -63,6 +64,20 @@ int check(const nekbone_problem *P)
 {
     if (!P) return fail(NEKBONE_ERR_ARG, "null problem");
     return NEKBONE_OK;
+}
+
+std::map<std::tuple<int, int, int>, AxVariant> &tuned_cache()
+{
+    static std::map<std::tuple<int, int, int>, AxVariant> m;
+    return m;
+}
+
+// A shape found by nekbone_autotune for this order (any preconditioner),
+// else the built-in default.
+AxVariant tuned_or_default(int nx1)
+{
+    auto it = tuned_cache().find(std::make_tuple(nx1, -1, -1));
+    return it != tuned_cache().end() ? it->second : ax_default_variant(nx1);
 }
 
 }  // namespace
@@ -135,49 +150,153 @@ int timing_resolve(nekbone_problem *P)
     return NEKBONE_OK;
 }
 
+// Point flags: list membership, sharing, multiplicity (see kFlag*).
+static void build_flags(nekbone_problem *P, const std::vector<int32_t> &off,
+                        const std::vector<int32_t> &idx, std::vector<uint8_t> &flag,
+                        std::vector<int32_t> &mult)
+{
+    const int64_t n = P->n;
+    flag.assign(size_t(n), 0);
+    mult.assign(size_t(n), 1);
+    for (size_t g = 0; g + 1 < off.size(); ++g)
+        for (int32_t q = off[g]; q < off[g + 1]; ++q) mult[size_t(idx[size_t(q)])] = off[g + 1] - off[g];
+    for (int32_t l : P->mfull) flag[size_t(l)] |= kFlagFull;
+    for (int32_t l : P->mref) flag[size_t(l)] |= kFlagRef;
+    for (int64_t l = 0; l < n; ++l) {
+        const int m = mult[size_t(l)];
+        if (m >= 2) flag[size_t(l)] |= kFlagShared;
+        if (m < 32) flag[size_t(l)] |= uint8_t(m << kMultShift);
+    }
+}
+
+// Whether c equals 1/multiplicity at every point (what the reference's
+// set_multiplicity produces); then the fused CG path may derive c from the
+// flag byte instead of streaming the c array.
+static bool c_matches_mult(const double *c, const std::vector<int32_t> &mult)
+{
+    for (size_t l = 0; l < mult.size(); ++l)
+        if (mult[l] >= 32 || c[l] != 1. / double(mult[l])) return false;
+    return true;
+}
+
+// Device position of reference local index l.
+static int64_t dev_index(const nekbone_problem *P, int64_t l)
+{
+    const int64_t n3 = int64_t(P->nx1) * P->nx1 * P->nx1;
+    const int64_t e = l / n3;
+    return e * n3 + P->perm[size_t(l - e * n3)];
+}
+
+// Upload a host vector in reference order into a device-order array.
+int upload_dev(nekbone_problem *P, double *dst, const double *src)
+{
+    NB_HIP(hipMemcpyAsync(P->d_t, src, sizeof(double) * size_t(P->n), hipMemcpyHostToDevice,
+                          stream()));
+    return launch_to_dev(P, dst, P->d_t);
+}
+
+// Download a device-order array into a host vector in reference order.
+int download_ref(nekbone_problem *P, double *dst, const double *src)
+{
+    if (int rc = launch_to_ref(P, P->d_t, src)) return rc;
+    NB_HIP(hipMemcpyAsync(dst, P->d_t, sizeof(double) * size_t(P->n), hipMemcpyDeviceToHost,
+                          stream()));
+    NB_HIP(hipStreamSynchronize(stream()));
+    return NEKBONE_OK;
+}
+
+int set_c_host(nekbone_problem *P, const double *c)
+{
+    std::vector<int32_t> off, idx;
+    gs_groups(P->glo.data(), P->n, off, idx);
+    std::vector<uint8_t> flag;
+    std::vector<int32_t> mult;
+    build_flags(P, off, idx, flag, mult);
+    if (int rc = upload_dev(P, P->d_c, c)) return rc;
+    NB_HIP(hipStreamSynchronize(stream()));
+    P->c_is_mult = c_matches_mult(c, mult);
+    return NEKBONE_OK;
+}
+
 // Upload the set-up data common to both constructors and allocate the
 // work space.  Expects n, nelt, nx1, dxm1, lglel, glo, masks filled in.
 static int finish_problem(nekbone_problem *P, const double *c_host)
 {
     const int64_t n = P->n;
+    const int nx = P->nx1;
+    boundary_first_perm(nx, P->perm);
+    if (int rc = dalloc(&P->d_perm, int64_t(P->perm.size()))) return rc;
+    NB_HIP(hipMemcpy(P->d_perm, P->perm.data(), sizeof(int32_t) * P->perm.size(),
+                     hipMemcpyHostToDevice));
+
+    // Groups keep the reference's member order (ascending reference local
+    // index, which fixes the summation order); members are stored as
+    // device positions.
     std::vector<int32_t> off, idx;
     gs_groups(P->glo.data(), n, off, idx);
     P->gs.ngroups = int64_t(off.size()) - 1;
     P->gs.nshared = int64_t(idx.size());
-    if (int rc = dalloc(&P->gs.d_off, int64_t(off.size()))) return rc;
-    if (int rc = dalloc(&P->gs.d_idx, int64_t(idx.size()))) return rc;
-    NB_HIP(hipMemcpy(P->gs.d_off, off.data(), sizeof(int32_t) * off.size(),
-                     hipMemcpyHostToDevice));
-    if (!idx.empty())
-        NB_HIP(hipMemcpy(P->gs.d_idx, idx.data(), sizeof(int32_t) * idx.size(),
-                         hipMemcpyHostToDevice));
+    std::vector<int32_t> m2, m4, m8, roff{0}, ridx;
+    for (size_t g = 0; g + 1 < off.size(); ++g) {
+        const int sz = off[g + 1] - off[g];
+        std::vector<int32_t> &dst = sz == 2 ? m2 : sz == 4 ? m4 : sz == 8 ? m8 : ridx;
+        for (int32_t q = off[g]; q < off[g + 1]; ++q)
+            dst.push_back(int32_t(dev_index(P, idx[size_t(q)])));
+        if (sz != 2 && sz != 4 && sz != 8) roff.push_back(int32_t(ridx.size()));
+    }
+    GsMap &G = P->gs;
+    G.n2 = int64_t(m2.size()) / 2;
+    G.n4 = int64_t(m4.size()) / 4;
+    G.n8 = int64_t(m8.size()) / 8;
+    G.nr = int64_t(roff.size()) - 1;
+    auto up = [](int32_t **d, const std::vector<int32_t> &h) -> int {
+        if (int rc = dalloc(d, int64_t(h.size()))) return rc;
+        if (!h.empty())
+            NB_HIP(hipMemcpy(*d, h.data(), sizeof(int32_t) * h.size(), hipMemcpyHostToDevice));
+        return NEKBONE_OK;
+    };
+    for (auto pr : {std::make_pair(&G.d_m2, &m2), std::make_pair(&G.d_m4, &m4),
+                    std::make_pair(&G.d_m8, &m8), std::make_pair(&G.d_roff, &roff),
+                    std::make_pair(&G.d_ridx, &ridx)})
+        if (int rc = up(pr.first, *pr.second)) return rc;
 
-    if (int rc = dalloc(&P->d_D, int64_t(P->nx1) * P->nx1)) return rc;
-    NB_HIP(hipMemcpy(P->d_D, P->dxm1, sizeof(double) * P->nx1 * P->nx1, hipMemcpyHostToDevice));
+    std::vector<double> drow(size_t(nx) * nx);
+    for (int i = 0; i < nx; ++i)
+        for (int m = 0; m < nx; ++m) drow[size_t(i * nx + m)] = P->dxm1[i + nx * m];
+    if (int rc = dalloc(&P->d_D, int64_t(nx) * nx)) return rc;
+    if (int rc = dalloc(&P->d_Drow, int64_t(nx) * nx)) return rc;
+    NB_HIP(hipMemcpy(P->d_D, P->dxm1, sizeof(double) * nx * nx, hipMemcpyHostToDevice));
+    NB_HIP(hipMemcpy(P->d_Drow, drow.data(), sizeof(double) * nx * nx, hipMemcpyHostToDevice));
     if (int rc = dalloc(&P->d_g, 6 * n)) return rc;
     if (int rc = dalloc(&P->d_c, n)) return rc;
-    for (double **v : {&P->d_x, &P->d_r, &P->d_p, &P->d_w, &P->d_f, &P->d_t})
+    for (double **v : {&P->d_x, &P->d_r, &P->d_p, &P->d_w, &P->d_f, &P->d_t, &P->d_t2})
         if (int rc = dalloc(v, n)) return rc;
-    if (int rc = dalloc(&P->d_part, 2 * 1024)) return rc;
+    P->part_cap = cg_partials_needed(P);
+    if (int rc = dalloc(&P->d_part, P->part_cap)) return rc;
     if (int rc = dalloc(&P->d_scal, 1)) return rc;
     NB_HIP(hipMemset(P->d_scal, 0, sizeof(CgScalars)));
-    for (double *v : {P->d_x, P->d_r, P->d_p, P->d_w, P->d_f, P->d_t})
+    for (double *v : {P->d_x, P->d_r, P->d_p, P->d_w, P->d_f, P->d_t, P->d_t2})
         NB_HIP(hipMemset(v, 0, sizeof(double) * size_t(n)));
 
-    std::vector<uint8_t> flag(size_t(n), 0);
-    for (int32_t l : P->mfull) flag[size_t(l)] |= 1;
-    for (int32_t l : P->mref) flag[size_t(l)] |= 2;
+    std::vector<uint8_t> flag, flag_dev(static_cast<size_t>(n));
+    std::vector<int32_t> mult;
+    build_flags(P, off, idx, flag, mult);
+    for (int64_t l = 0; l < n; ++l) flag_dev[size_t(dev_index(P, l))] = flag[size_t(l)];
     if (int rc = dalloc(&P->d_flag, n)) return rc;
-    NB_HIP(hipMemcpy(P->d_flag, flag.data(), size_t(n), hipMemcpyHostToDevice));
+    NB_HIP(hipMemcpy(P->d_flag, flag_dev.data(), size_t(n), hipMemcpyHostToDevice));
 
     if (c_host) {
-        NB_HIP(hipMemcpy(P->d_c, c_host, sizeof(double) * size_t(n), hipMemcpyHostToDevice));
+        if (int rc = upload_dev(P, P->d_c, c_host)) return rc;
+        P->c_is_mult = c_matches_mult(c_host, mult);
     } else {
         // set_multiplicity, src/driver.f:274-291: c = 1 / QQ^T(1)
         if (int rc = fill(P->d_c, 1.0, n)) return rc;
         if (int rc = launch_gs(P, P->d_c, nullptr)) return rc;
         hipLaunchKernelGGL(k_recip, dim3(grid(n)), dim3(256), 0, stream(), P->d_c, n);
         NB_HIP(hipGetLastError());
+        bool ok = true;
+        for (int32_t m : mult) ok = ok && m < 32;
+        P->c_is_mult = ok;
     }
     NB_HIP(hipStreamSynchronize(stream()));
     return NEKBONE_OK;
@@ -208,7 +327,7 @@ static int run_cg(nekbone_problem *P, double *x, const double *f, double *r, int
     const int mbit = next_mask_bit(P);  // maskit in ax, src/cg.f:119 (state is now fixed)
     for (int it = 0; it < niter; ++it)
         if (int rc = launch_cg_iteration(P, x, r, precond, mbit)) return rc;
-    return NEKBONE_OK;
+    return launch_cg_finish(P, x);
 }
 
 }  // namespace nb
@@ -271,6 +390,7 @@ extern "C" int nekbone_problem_create_box(nekbone_problem **out, int nelx, int n
     P->nelz = nelz;
     P->nelt = int(nelt64);
     P->nx1 = nx1;
+    P->axv = tuned_or_default(nx1);
     P->n = n64;
     P->lglel.resize(size_t(P->nelt));
     for (int e = 0; e < P->nelt; ++e) P->lglel[size_t(e)] = e + 1;
@@ -308,6 +428,7 @@ extern "C" int nekbone_problem_create(nekbone_problem **out, int nelt, int nx1, 
     P->nelz = nelz;
     P->nelt = nelt;
     P->nx1 = nx1;
+    P->axv = tuned_or_default(nx1);
     P->n = int64_t(nelt) * nx1 * nx1 * nx1;
     P->lglel.assign(lglel, lglel + nelt);
     std::memcpy(P->dxm1, dxm1, sizeof(double) * nx1 * nx1);
@@ -338,13 +459,18 @@ extern "C" int nekbone_problem_destroy(nekbone_problem *P)
     if (!P) return NEKBONE_OK;
     if (g_stream) (void)hipStreamSynchronize(g_stream);
     dfree(P->d_D);
+    dfree(P->d_Drow);
     dfree(P->d_g);
     dfree(P->d_c);
     dfree(P->d_dinv);
     dfree(P->d_flag);
-    dfree(P->gs.d_off);
-    dfree(P->gs.d_idx);
-    for (double **v : {&P->d_x, &P->d_r, &P->d_p, &P->d_w, &P->d_f, &P->d_t, &P->d_part,
+    dfree(P->d_perm);
+    dfree(P->gs.d_m2);
+    dfree(P->gs.d_m4);
+    dfree(P->gs.d_m8);
+    dfree(P->gs.d_roff);
+    dfree(P->gs.d_ridx);
+    for (double **v : {&P->d_x, &P->d_r, &P->d_p, &P->d_w, &P->d_f, &P->d_t, &P->d_t2, &P->d_part,
                        &P->d_hist})
         dfree(*v);
     dfree(P->d_scal);
@@ -429,9 +555,7 @@ extern "C" int nekbone_get_dxm1(const nekbone_problem *P, double *dxm1)
 extern "C" int nekbone_get_c(nekbone_problem *P, double *c)
 {
     if (int rc = check(P)) return rc;
-    NB_HIP(hipStreamSynchronize(stream()));
-    NB_HIP(hipMemcpy(c, P->d_c, sizeof(double) * size_t(P->n), hipMemcpyDeviceToHost));
-    return NEKBONE_OK;
+    return download_ref(P, c, P->d_c);
 }
 
 extern "C" int nekbone_get_glo(const nekbone_problem *P, int64_t *glo)
@@ -469,6 +593,82 @@ extern "C" int nekbone_set_arith(nekbone_problem *P, int arith)
     return NEKBONE_OK;
 }
 
+extern "C" int nekbone_get_variant(const nekbone_problem *P, int *epb, int *pf, int *xu)
+{
+    if (int rc = check(P)) return rc;
+    if (epb) *epb = P->axv.epb;
+    if (pf) *pf = P->axv.pf;
+    if (xu) *xu = P->axv.xu;
+    return NEKBONE_OK;
+}
+
+extern "C" int nekbone_set_variant(nekbone_problem *P, int epb, int pf, int xu)
+{
+    if (int rc = check(P)) return rc;
+    for (const AxVariant &v : ax_variants(P->nx1))
+        if (v.epb == epb && v.pf == pf && v.xu == xu) {
+            P->axv = v;
+            return NEKBONE_OK;
+        }
+    return fail(NEKBONE_ERR_ARG, "set_variant: no such kernel shape for this order");
+}
+
+extern "C" int nekbone_autotune(nekbone_problem *P, int precond, double *best_ms)
+{
+    if (int rc = check(P)) return rc;
+    if (precond == NEKBONE_PRECOND_JACOBI && !P->d_dinv)
+        return fail(NEKBONE_ERR_STATE, "autotune: Jacobi not set up");
+    if (precond != NEKBONE_PRECOND_JACOBI && precond != NEKBONE_PRECOND_IDENTITY)
+        return fail(NEKBONE_ERR_ARG, "autotune: identity or Jacobi");
+    std::vector<AxVariant> cands = ax_variants(P->nx1);
+    if (cands.empty()) cands.push_back(ax_default_variant(P->nx1));
+    const AxVariant keep = P->axv;
+    const int mbit = P->mask_mode == NEKBONE_MASK_DIRICHLET ? 1 : 2;
+    const int warm = 2, reps = 6;
+    if (int rc = ensure_hist(P, warm + reps)) return rc;
+    if (int rc = launch_rand(P->d_f, P->n, 12345)) return rc;
+    hipEvent_t e0, e1;
+    NB_HIP(hipEventCreate(&e0));
+    NB_HIP(hipEventCreate(&e1));
+    double best = 1e300;
+    AxVariant win = keep;
+    int rc = NEKBONE_OK;
+    for (const AxVariant &v : cands) {
+        P->axv = v;
+        rc = launch_cg_init(P, P->d_x, P->d_r, P->d_f, 1, precond, 0.0);
+        for (int i = 0; i < warm && !rc; ++i) rc = launch_cg_iteration(P, P->d_x, P->d_r, precond, mbit);
+        if (!rc && hipEventRecord(e0, stream()) != hipSuccess) rc = NEKBONE_ERR_HIP;
+        for (int i = 0; i < reps && !rc; ++i) rc = launch_cg_iteration(P, P->d_x, P->d_r, precond, mbit);
+        if (!rc && hipEventRecord(e1, stream()) != hipSuccess) rc = NEKBONE_ERR_HIP;
+        if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = NEKBONE_ERR_HIP;
+        float ms = 0.f;
+        if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = NEKBONE_ERR_HIP;
+        if (rc) break;
+        if (ms / reps < best) {
+            best = ms / reps;
+            win = v;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) {
+        P->axv = keep;
+        return rc == NEKBONE_ERR_HIP ? fail(rc, "autotune: HIP error") : rc;
+    }
+    P->axv = win;
+    tuned_cache()[std::make_tuple(P->nx1, precond, P->arith)] = win;
+    tuned_cache()[std::make_tuple(P->nx1, -1, -1)] = win;
+    if (best_ms) *best_ms = best;
+    return NEKBONE_OK;
+}
+
+extern "C" int nekbone_set_fused(nekbone_problem *P, int on)
+{
+    if (int rc = check(P)) return rc;
+    P->fused = on != 0;
+    return NEKBONE_OK;
+}
+
 extern "C" int nekbone_setup_jacobi(nekbone_problem *P)
 {
     if (int rc = check(P)) return rc;
@@ -486,73 +686,66 @@ extern "C" int nekbone_get_dinv(nekbone_problem *P, double *dinv)
 {
     if (int rc = check(P)) return rc;
     if (!P->d_dinv) return fail(NEKBONE_ERR_STATE, "Jacobi not set up");
-    NB_HIP(hipMemcpy(dinv, P->d_dinv, sizeof(double) * size_t(P->n), hipMemcpyDeviceToHost));
-    return NEKBONE_OK;
+    return download_ref(P, dinv, P->d_dinv);
 }
 
 // ======================================================================
-// Operators
+// Operators.  Caller vectors are in reference order; the work runs on
+// device-order copies (d_t2 in, d_w out).
 // ======================================================================
+static int ax_dev(nekbone_problem *P)
+{
+    if (int rc = launch_ax_local(P, P->d_w, P->d_t2)) return rc;
+    if (int rc = launch_gs(P, P->d_w, nullptr)) return rc;
+    return launch_post(P, P->d_w, P->d_t2, next_mask_bit(P), false);
+}
+
 extern "C" int nekbone_ax_d(nekbone_problem *P, double *w, const double *u)
 {
     if (int rc = check(P)) return rc;
-    if (int rc = launch_ax_local(P, w, u, 0, nullptr, nullptr)) return rc;
-    if (int rc = launch_gs(P, w, nullptr)) return rc;
-    return launch_post(P, w, u, next_mask_bit(P), false);
+    if (int rc = launch_to_dev(P, P->d_t2, u)) return rc;
+    if (int rc = ax_dev(P)) return rc;
+    return launch_to_ref(P, w, P->d_w);
 }
 
 extern "C" int nekbone_ax(nekbone_problem *P, double *w, const double *u)
 {
     if (int rc = check(P)) return rc;
-    NB_HIP(hipMemcpyAsync(P->d_t, u, sizeof(double) * size_t(P->n), hipMemcpyHostToDevice,
-                          stream()));
-    if (int rc = nekbone_ax_d(P, P->d_w, P->d_t)) return rc;
-    NB_HIP(hipMemcpyAsync(w, P->d_w, sizeof(double) * size_t(P->n), hipMemcpyDeviceToHost,
-                          stream()));
-    NB_HIP(hipStreamSynchronize(stream()));
-    return NEKBONE_OK;
+    if (int rc = upload_dev(P, P->d_t2, u)) return rc;
+    if (int rc = ax_dev(P)) return rc;
+    return download_ref(P, w, P->d_w);
 }
 
 extern "C" int nekbone_ax_local(nekbone_problem *P, double *w, const double *u)
 {
     if (int rc = check(P)) return rc;
-    NB_HIP(hipMemcpyAsync(P->d_t, u, sizeof(double) * size_t(P->n), hipMemcpyHostToDevice,
-                          stream()));
-    if (int rc = launch_ax_local(P, P->d_w, P->d_t, 0, nullptr, nullptr)) return rc;
-    NB_HIP(hipMemcpyAsync(w, P->d_w, sizeof(double) * size_t(P->n), hipMemcpyDeviceToHost,
-                          stream()));
-    NB_HIP(hipStreamSynchronize(stream()));
-    return NEKBONE_OK;
+    if (int rc = upload_dev(P, P->d_t2, u)) return rc;
+    if (int rc = launch_ax_local(P, P->d_w, P->d_t2)) return rc;
+    return download_ref(P, w, P->d_w);
 }
 
 extern "C" int nekbone_dssum_d(nekbone_problem *P, double *u)
 {
     if (int rc = check(P)) return rc;
-    return launch_gs(P, u, nullptr);
+    if (int rc = launch_to_dev(P, P->d_t2, u)) return rc;
+    if (int rc = launch_gs(P, P->d_t2, nullptr)) return rc;
+    return launch_to_ref(P, u, P->d_t2);
 }
 
 extern "C" int nekbone_dssum(nekbone_problem *P, double *u)
 {
     if (int rc = check(P)) return rc;
-    NB_HIP(hipMemcpyAsync(P->d_t, u, sizeof(double) * size_t(P->n), hipMemcpyHostToDevice,
-                          stream()));
-    if (int rc = launch_gs(P, P->d_t, nullptr)) return rc;
-    NB_HIP(hipMemcpyAsync(u, P->d_t, sizeof(double) * size_t(P->n), hipMemcpyDeviceToHost,
-                          stream()));
-    NB_HIP(hipStreamSynchronize(stream()));
-    return NEKBONE_OK;
+    if (int rc = upload_dev(P, P->d_t2, u)) return rc;
+    if (int rc = launch_gs(P, P->d_t2, nullptr)) return rc;
+    return download_ref(P, u, P->d_t2);
 }
 
 extern "C" int nekbone_maskit(nekbone_problem *P, double *w)
 {
     if (int rc = check(P)) return rc;
-    NB_HIP(hipMemcpyAsync(P->d_t, w, sizeof(double) * size_t(P->n), hipMemcpyHostToDevice,
-                          stream()));
-    if (int rc = launch_mask(P, P->d_t, next_mask_bit(P))) return rc;
-    NB_HIP(hipMemcpyAsync(w, P->d_t, sizeof(double) * size_t(P->n), hipMemcpyDeviceToHost,
-                          stream()));
-    NB_HIP(hipStreamSynchronize(stream()));
-    return NEKBONE_OK;
+    if (int rc = upload_dev(P, P->d_t2, w)) return rc;
+    if (int rc = launch_mask(P, P->d_t2, next_mask_bit(P))) return rc;
+    return download_ref(P, w, P->d_t2);
 }
 
 extern "C" int nekbone_glsc3(nekbone_problem *P, const double *a, const double *b,
@@ -594,8 +787,13 @@ extern "C" int nekbone_cg_d(nekbone_problem *P, double *x, const double *f, doub
                             int niter, int precond, double *hist)
 {
     if (int rc = check(P)) return rc;
-    if (int rc = run_cg(P, x, f, r, niter, precond, 0.0)) return rc;
+    if (int rc = launch_to_dev(P, P->d_f, f)) return rc;
+    if (int rc = run_cg(P, P->d_x, P->d_f, P->d_r, niter, precond, 0.0)) return rc;
     if (int rc = copy_hist(P, hist, niter)) return rc;
+    if (x)
+        if (int rc = launch_to_ref(P, x, P->d_x)) return rc;
+    if (r)
+        if (int rc = launch_to_ref(P, r, P->d_r)) return rc;
     NB_HIP(hipStreamSynchronize(stream()));
     return NEKBONE_OK;
 }
@@ -604,12 +802,13 @@ extern "C" int nekbone_cg(nekbone_problem *P, double *x, const double *f, double
                           int precond, double *hist)
 {
     if (int rc = check(P)) return rc;
-    const size_t bytes = sizeof(double) * size_t(P->n);
-    NB_HIP(hipMemcpyAsync(P->d_f, f, bytes, hipMemcpyHostToDevice, stream()));
+    if (int rc = upload_dev(P, P->d_f, f)) return rc;
     if (int rc = run_cg(P, P->d_x, P->d_f, P->d_r, niter, precond, 0.0)) return rc;
     if (int rc = copy_hist(P, hist, niter)) return rc;
-    if (x) NB_HIP(hipMemcpyAsync(x, P->d_x, bytes, hipMemcpyDeviceToHost, stream()));
-    if (r) NB_HIP(hipMemcpyAsync(r, P->d_r, bytes, hipMemcpyDeviceToHost, stream()));
+    if (x)
+        if (int rc = download_ref(P, x, P->d_x)) return rc;
+    if (r)
+        if (int rc = download_ref(P, r, P->d_r)) return rc;
     NB_HIP(hipStreamSynchronize(stream()));
     return NEKBONE_OK;
 }
@@ -623,8 +822,10 @@ extern "C" int nekbone_cg_tol_d(nekbone_problem *P, double *x, const double *f, 
     if (precond == NEKBONE_PRECOND_JACOBI && !P->d_dinv)
         return fail(NEKBONE_ERR_STATE, "cg: Jacobi preconditioner not set up");
     if (int rc = ensure_hist(P, maxiter)) return rc;
+    if (int rc = launch_to_dev(P, P->d_f, f)) return rc;
     const int mbit0 = next_mask_bit(P);
-    if (int rc = launch_cg_init(P, x, r, f, mbit0, precond, tol * tol)) return rc;
+    if (int rc = launch_cg_init(P, P->d_x, P->d_r, P->d_f, mbit0, precond, tol * tol))
+        return rc;
     const int mbit = next_mask_bit(P);
     // Iterations are queued in chunks; kernels of an already converged
     // solve return at once, so only one host check per chunk is needed.
@@ -632,11 +833,17 @@ extern "C" int nekbone_cg_tol_d(nekbone_problem *P, double *x, const double *f, 
     CgScalars s{};
     for (int it = 0; it < maxiter; it += chunk) {
         for (int k = it; k < std::min(maxiter, it + chunk); ++k)
-            if (int rc = launch_cg_iteration(P, x, r, precond, mbit)) return rc;
+            if (int rc = launch_cg_iteration(P, P->d_x, P->d_r, precond, mbit)) return rc;
         NB_HIP(hipMemcpyAsync(&s, P->d_scal, sizeof s, hipMemcpyDeviceToHost, stream()));
         NB_HIP(hipStreamSynchronize(stream()));
         if (s.done) break;
     }
+    if (int rc = launch_cg_finish(P, P->d_x)) return rc;
+    if (x)
+        if (int rc = launch_to_ref(P, x, P->d_x)) return rc;
+    if (r)
+        if (int rc = launch_to_ref(P, r, P->d_r)) return rc;
+    NB_HIP(hipStreamSynchronize(stream()));
     if (iters) *iters = s.iter;
     if (final_rtr) *final_rtr = s.rtr;
     if (s.status) return fail(NEKBONE_ERR_NUMERIC, "cg: non-positive or non-finite pap");
@@ -692,8 +899,10 @@ extern "C" int nekbone_rand_uniform_d(nekbone_problem *P, double *dst, int64_t c
 extern "C" int nekbone_rhs_from_raw_d(nekbone_problem *P, double *f)
 {
     if (int rc = check(P)) return rc;
-    if (int rc = launch_gs(P, f, nullptr)) return rc;
-    return launch_col2(P, f, P->d_c);
+    if (int rc = launch_to_dev(P, P->d_t2, f)) return rc;
+    if (int rc = launch_gs(P, P->d_t2, nullptr)) return rc;
+    if (int rc = launch_col2(P, P->d_t2, P->d_c)) return rc;
+    return launch_to_ref(P, f, P->d_t2);
 }
 
 // ======================================================================
@@ -714,6 +923,50 @@ extern "C" int nekbone_timing_get(nekbone_problem *P, int kclass, int64_t *launc
     if (int rc = timing_resolve(P)) return rc;
     if (launches) *launches = P->timing.launches[kclass];
     if (ms) *ms = P->timing.ms[kclass];
+    return NEKBONE_OK;
+}
+
+namespace nb {
+namespace {
+__global__ void k_copy2(double2 *__restrict__ dst, const double2 *__restrict__ src, int64_t n)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+}  // namespace
+}  // namespace nb
+
+extern "C" int nekbone_bw_probe(int64_t bytes, int reps, double *ms)
+{
+    if (int rc = ensure_init()) return rc;
+    if (bytes < 1024 || reps < 1) return fail(NEKBONE_ERR_ARG, "bw_probe: bytes >= 1024, reps >= 1");
+    const int64_t n2 = bytes / 2 / 16;  // double2 elements per buffer
+    double2 *a = nullptr, *b = nullptr;
+    NB_HIP(hipMalloc(&a, sizeof(double2) * size_t(n2)));
+    if (hipMalloc(&b, sizeof(double2) * size_t(n2)) != hipSuccess) {
+        (void)hipFree(a);
+        return fail(NEKBONE_ERR_HIP, "bw_probe: allocation");
+    }
+    (void)hipMemsetAsync(a, 0, sizeof(double2) * size_t(n2), stream());
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    const unsigned blocks = unsigned((n2 + 255) / 256);
+    hipLaunchKernelGGL(k_copy2, dim3(blocks), dim3(256), 0, stream(), b, a, n2);
+    (void)hipEventRecord(e0, stream());
+    for (int i = 0; i < reps; ++i)
+        hipLaunchKernelGGL(k_copy2, dim3(blocks), dim3(256), 0, stream(), (i & 1) ? a : b,
+                           (i & 1) ? b : a, n2);
+    (void)hipEventRecord(e1, stream());
+    hipError_t err = hipEventSynchronize(e1);
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (err != hipSuccess) return fail(NEKBONE_ERR_HIP, "bw_probe: kernel");
+    if (ms) *ms = double(t) / reps;
     return NEKBONE_OK;
 }
 

@@ -1,0 +1,352 @@
+// Element operator kernel template (included by the nb_ax*.hip files).
+// w_e = A_L u_e = D^T G D u_e for every element
+// (ax_e, src/cg.f:142-212), FP64, for gfx950, plus the fused CG variant.
+//
+// Design (one workgroup = EPB elements, NX*NX threads per element):
+//  * Every 1-D contraction is done by one thread along a whole grid line.
+//    All lanes of a wave use the same coefficient at the same time, so D
+//    comes through the scalar cache as SGPR operands and the only LDS
+//    traffic is the element data: about 20 LDS accesses per point instead
+//    of ~120 for a thread-per-point scheme.
+//  * D is stored twice, row-major (Drow[i*NX+m] = D(i,m)) and
+//    column-major (Dcol[i*NX+m] = D(m,i)), so every contraction reads one
+//    contiguous row of NX doubles through the constant address space
+//    (scalar loads).  An empty asm statement on the uniform row pointer
+//    keeps the compiler from hoisting all NX^2 coefficients into SGPRs at
+//    once, which would spill them into VGPR lanes and double the VALU work.
+//  * A thread owns line (a,b) in three roles: t-line (i,j)=(a,b), r-line
+//    (j,k)=(a,b), s-line (i,k)=(a,b).  The t-direction never leaves
+//    registers; r and s lines are exchanged through two LDS buffers of
+//    NX^3 doubles per element (16 KB at NX=10).
+//  * Every sum runs over m = 0..NX-1 in order starting from the first
+//    product, i.e. the order of mxfN (src/mxm_std.f:68-631):
+//      ur(i,j,k) = sum_m D(i,m) u(m,j,k)      mxm(D,m1,u,m1,ur,m2)
+//      us(i,j,k) = sum_m u(i,m,k) D(j,m)      mxm(u(,,k),m1,Dt,m1,us,m1)
+//      ut(i,j,k) = sum_m u(i,j,m) D(k,m)      mxm(u,m2,Dt,m1,ut,m1)
+//      w = (sum_m D(m,i) wr(m,j,k) + sum_m ws(i,m,k) D(m,j))
+//          + sum_m wt(i,j,m) D(m,k)           local_grad3_t
+//    and wr = (g1 ur + g2 us) + g3 ut etc. (src/cg.f:156-163).  With
+//    FMA=false no contraction happens (the library is compiled with
+//    -ffp-contract=off), so results are bit-identical to the reference.
+//  * Geometric factors are element-blocked SoA, g[e][c][p]: the six loads
+//    of a t-line step are each one coalesced row.  PF=1 issues all of them
+//    at kernel entry (deeper memory pipelining, more VGPRs); PF=0 issues
+//    them in the geometric phase (higher occupancy).
+//
+// CG mode (MODE 1 = identity, 2 = Jacobi) fuses, per point of the
+// thread's t-line:
+//   prologue  [XU] x += alpha_prev p_prev     (add2s2, src/cg.f:63, one
+//                                              iteration late)
+//             p = beta p + z,  z = r or r*dinv (add2s1, src/cg.f:56)
+//   epilogue  points not shared with another element are final after
+//             the element operator: w = w + 0.1 p (src/cg.f:118), mask
+//             (src/cg.f:119), and their part of pap = sum (w c) p
+//             (src/cg.f:59) with c = 1 there.  Shared points are left
+//             for the gather-scatter kernel, which finishes them.
+// The variant (EPB, PF, XU) is chosen per order by the autotuner.
+#pragma once
+
+#include <map>
+#include <tuple>
+
+#include "nb_internal.h"
+
+namespace nb {
+namespace axk {
+
+template <bool FMA>
+__device__ __forceinline__ double madd(double acc, double a, double b)
+{
+    if constexpr (FMA)
+        return __builtin_fma(a, b, acc);
+    else
+        return acc + a * b;
+}
+
+constexpr int round64(int x) { return (x + 63) / 64 * 64; }
+
+typedef const __attribute__((address_space(4))) double *const_dptr;
+
+// sum_m row[m] * x[m], m ascending, first term a plain product.
+template <int NX, bool FMA>
+__device__ __forceinline__ double dot_row(const double *__restrict__ base, int row,
+                                          const double (&x)[NX])
+{
+    const_dptr p = (const_dptr)(base + row * NX);
+    asm volatile("" : "+s"(p));
+    double s = p[0] * x[0];
+#pragma unroll
+    for (int m = 1; m < NX; ++m) s = madd<FMA>(s, p[m], x[m]);
+    return s;
+}
+
+struct AxArgs {
+    double *w;
+    const double *u;     // MODE 0 input
+    double *p;           // CG: direction, updated in place
+    const double *r;     // CG: residual
+    const double *dinv;  // CG Jacobi
+    double *x;           // CG: solution, updated one iteration late (XU)
+    const double *g;
+    const double *Drow, *Dcol;
+    const uint8_t *flag;
+    const int32_t *perm;  // device point order within an element
+    int mbit;
+    const CgScalars *scal;
+    double *part;        // CG: per-block pap partials
+    int nelt;
+};
+
+template <int NX, int EPB, bool FMA, int MODE, int PF, bool XU>
+__global__ void __launch_bounds__(round64(NX *NX *EPB)) k_ax(AxArgs A_)
+{
+    constexpr int N2 = NX * NX, N3 = NX * NX * NX;
+    constexpr int NT = round64(N2 * EPB);
+    __shared__ double sA[EPB * N3];
+    __shared__ double sB[EPB * N3];
+
+    const int tid = threadIdx.x;
+    const int el = tid / N2;
+    const int ab = tid - el * N2;
+    const int a = ab % NX, b = ab / NX;
+    const int64_t e = int64_t(blockIdx.x) * EPB + el;
+    const bool act = (el < EPB) && (e < A_.nelt);
+    double *A = sA + (act ? el : 0) * N3;
+    double *B = sB + (act ? el : 0) * N3;
+    const int64_t base = e * N3;
+    const double *__restrict__ Drow = A_.Drow;
+    const double *__restrict__ Dcol = A_.Dcol;
+    const double *ge = A_.g + e * 6 * N3;
+
+    if constexpr (MODE != 0) {
+        if (A_.scal->done) return;  // converged tolerance run: nothing to do
+    }
+
+    double gv[PF ? 6 : 1][PF ? NX : 1];
+    if constexpr (PF) {
+        if (act) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+#pragma unroll
+                for (int k = 0; k < NX; ++k) gv[c][k] = ge[c * N3 + a + NX * b + N2 * k];
+        }
+    }
+
+    // Device positions of the thread's t-line points (reference point
+    // a + NX*b + N2*k sits at base + off[k]): the boundary-first order of
+    // boundary_first_perm in closed form.  Surface points in reference
+    // order: slab k=0 (N2 points), then the ring of each middle slab
+    // (R = N2 - (NX-2)^2 points: row 0, two points per middle row, last
+    // row), then slab k=NX-1; interior points follow in reference order.
+    int off[NX];
+    {
+        constexpr int I = NX - 2, R = N2 - I * I, NB = N3 - I * I * I;
+        const bool lat = a == 0 || b == 0 || a == NX - 1 || b == NX - 1;
+        const int ring = b == 0 ? a : b == NX - 1 ? NX + 2 * I + a : NX + 2 * (b - 1) + (a != 0);
+        const int inner = NB + (a - 1) + I * (b - 1);
+#pragma unroll
+        for (int k = 0; k < NX; ++k) {
+            if (k == 0)
+                off[k] = a + NX * b;
+            else if (k == NX - 1)
+                off[k] = N2 + I * R + a + NX * b;
+            else
+                off[k] = lat ? N2 + (k - 1) * R + ring : inner + I * I * (k - 1);
+        }
+    }
+
+    double uc[NX];  // the thread's t-line of the input
+    if (act) {
+        if constexpr (MODE == 0) {
+#pragma unroll
+            for (int k = 0; k < NX; ++k) uc[k] = A_.u[base + off[k]];
+        } else {
+            const double beta = A_.scal->beta;
+            double po[NX], z[NX];
+#pragma unroll
+            for (int k = 0; k < NX; ++k) {
+                const int64_t q = base + off[k];
+                po[k] = A_.p[q];
+                z[k] = A_.r[q];
+            }
+            if constexpr (MODE == 2) {
+#pragma unroll
+                for (int k = 0; k < NX; ++k) z[k] = z[k] * A_.dinv[base + off[k]];
+            }
+            if constexpr (XU) {
+                if (A_.scal->iter > 0) {
+                    const double alpha = A_.scal->alpha;
+#pragma unroll
+                    for (int k = 0; k < NX; ++k) {
+                        const int64_t q = base + off[k];
+                        if constexpr (FMA)
+                            A_.x[q] = __builtin_fma(alpha, po[k], A_.x[q]);
+                        else
+                            A_.x[q] = A_.x[q] + alpha * po[k];
+                    }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < NX; ++k) {
+                double pn;
+                if constexpr (FMA)
+                    pn = __builtin_fma(beta, po[k], z[k]);
+                else
+                    pn = beta * po[k] + z[k];
+                A_.p[base + off[k]] = pn;
+                uc[k] = pn;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NX; ++k) A[a + NX * b + N2 * k] = uc[k];
+    }
+    __syncthreads();
+
+    double lr[NX], ls[NX];
+    if (act) {
+#pragma unroll
+        for (int m = 0; m < NX; ++m) lr[m] = A[m + NX * a + N2 * b];  // r-line (j,k)=(a,b)
+#pragma unroll
+        for (int m = 0; m < NX; ++m) ls[m] = A[a + NX * m + N2 * b];  // s-line (i,k)=(a,b)
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+        for (int i = 0; i < NX; ++i) B[i + NX * a + N2 * b] = dot_row<NX, FMA>(Drow, i, lr);
+#pragma unroll
+        for (int j = 0; j < NX; ++j) A[a + NX * j + N2 * b] = dot_row<NX, FMA>(Drow, j, ls);
+    }
+    __syncthreads();
+
+    double wt[NX];
+    if (act) {
+#pragma unroll
+        for (int k = 0; k < NX; ++k) {
+            const int q = a + NX * b + N2 * k;
+            const double ur = B[q], us = A[q];
+            const double ut = dot_row<NX, FMA>(Drow, k, uc);
+            double g1, g2, g3, g4, g5, g6;
+            if constexpr (PF) {
+                g1 = gv[0][k], g2 = gv[1][k], g3 = gv[2][k];
+                g4 = gv[3][k], g5 = gv[4][k], g6 = gv[5][k];
+            } else {
+                g1 = ge[q], g2 = ge[N3 + q], g3 = ge[2 * N3 + q];
+                g4 = ge[3 * N3 + q], g5 = ge[4 * N3 + q], g6 = ge[5 * N3 + q];
+            }
+            double wr, ws;
+            if constexpr (FMA) {
+                wr = __builtin_fma(g3, ut, __builtin_fma(g2, us, g1 * ur));
+                ws = __builtin_fma(g5, ut, __builtin_fma(g4, us, g2 * ur));
+                wt[k] = __builtin_fma(g6, ut, __builtin_fma(g5, us, g3 * ur));
+            } else {
+                wr = g1 * ur + g2 * us + g3 * ut;
+                ws = g2 * ur + g4 * us + g5 * ut;
+                wt[k] = g3 * ur + g5 * us + g6 * ut;
+            }
+            B[q] = wr;
+            A[q] = ws;
+        }
+    }
+    __syncthreads();
+
+    if (act) {
+        // r-line and s-line are each read and rewritten by their owner
+        // only, so no barrier is needed between the reads and the writes.
+#pragma unroll
+        for (int m = 0; m < NX; ++m) lr[m] = B[m + NX * a + N2 * b];
+#pragma unroll
+        for (int i = 0; i < NX; ++i) B[i + NX * a + N2 * b] = dot_row<NX, FMA>(Dcol, i, lr);
+#pragma unroll
+        for (int m = 0; m < NX; ++m) ls[m] = A[a + NX * m + N2 * b];
+#pragma unroll
+        for (int j = 0; j < NX; ++j) A[a + NX * j + N2 * b] = dot_row<NX, FMA>(Dcol, j, ls);
+    }
+    __syncthreads();
+
+    double acc = 0.0;
+    if (act) {
+#pragma unroll
+        for (int k = 0; k < NX; ++k) {
+            const int q = a + NX * b + N2 * k;
+            const double wl = (B[q] + A[q]) + dot_row<NX, FMA>(Dcol, k, wt);
+            if constexpr (MODE == 0) {
+                A_.w[base + off[k]] = wl;
+            } else {
+                const uint8_t fl = A_.flag[base + off[k]];
+                if (fl & kFlagShared) {
+                    A_.w[base + off[k]] = wl;  // finished by the gather-scatter kernel
+                } else {
+                    double v = wl + 0.1 * uc[k];
+                    if (fl & A_.mbit) v = 0.0;
+                    A_.w[base + off[k]] = v;
+                    acc = acc + v * uc[k];
+                }
+            }
+        }
+    }
+    if constexpr (MODE != 0) {
+        // deterministic block sum of the pap partials
+        __shared__ double red[NT];
+        red[tid] = acc;
+        __syncthreads();
+#pragma unroll
+        for (int h = NT / 2; h > 0; h >>= 1) {
+            if (tid < h) red[tid] = red[tid] + red[tid + h];
+            __syncthreads();
+        }
+        if (tid == 0) A_.part[blockIdx.x] = red[0];
+    }
+}
+
+// Default elements per workgroup for each NX: >= 2 waves per group and
+// LDS well inside 160 KB.
+constexpr int epb_for(int nx)
+{
+    return nx <= 3 ? 16 : nx <= 4 ? 8 : nx <= 5 ? 4 : nx <= 8 ? 2 : 1;
+}
+
+using AxFn = void (*)(AxArgs);
+
+// Kernel table: (nx, epb, fma, mode, pf, xu) -> function.
+using Key = std::tuple<int, int, int, int, int, int>;
+using Table = std::map<Key, AxFn>;
+
+template <int NX, int EPB, int PF>
+void reg_modes(Table &t)
+{
+    t[Key(NX, EPB, 0, 0, PF, 0)] = k_ax<NX, EPB, false, 0, PF, false>;
+    t[Key(NX, EPB, 1, 0, PF, 0)] = k_ax<NX, EPB, true, 0, PF, false>;
+    t[Key(NX, EPB, 0, 1, PF, 0)] = k_ax<NX, EPB, false, 1, PF, false>;
+    t[Key(NX, EPB, 1, 1, PF, 0)] = k_ax<NX, EPB, true, 1, PF, false>;
+    t[Key(NX, EPB, 0, 2, PF, 0)] = k_ax<NX, EPB, false, 2, PF, false>;
+    t[Key(NX, EPB, 1, 2, PF, 0)] = k_ax<NX, EPB, true, 2, PF, false>;
+    t[Key(NX, EPB, 0, 1, PF, 1)] = k_ax<NX, EPB, false, 1, PF, true>;
+    t[Key(NX, EPB, 1, 1, PF, 1)] = k_ax<NX, EPB, true, 1, PF, true>;
+    t[Key(NX, EPB, 0, 2, PF, 1)] = k_ax<NX, EPB, false, 2, PF, true>;
+    t[Key(NX, EPB, 1, 2, PF, 1)] = k_ax<NX, EPB, true, 2, PF, true>;
+}
+
+// Orders the benchmark configurations use get the full variant space;
+// the others get the default shape.
+template <int NX>
+void reg_order(Table &t, bool full)
+{
+    constexpr int E = epb_for(NX);
+    reg_modes<NX, E, 0>(t);
+    if (full) {
+        reg_modes<NX, E, 1>(t);
+        reg_modes<NX, 2 * E, 0>(t);
+        reg_modes<NX, 2 * E, 1>(t);
+    }
+}
+
+// Defined in the instantiation files nb_ax_i*.hip.
+void reg_low(Table &t);
+void reg_high(Table &t);
+void reg_6(Table &t);
+void reg_8(Table &t);
+void reg_10(Table &t);
+void reg_12(Table &t);
+
+}  // namespace axk
+}  // namespace nb

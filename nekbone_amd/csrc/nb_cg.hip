@@ -1,0 +1,451 @@
+// CG (src/cg.f:2-84) as device kernels with all scalars on the device.
+//
+// Fused path (c == 1/multiplicity, which is what set_multiplicity,
+// src/driver.f:274-291, produces), per iteration:
+//   1. k_ax<...,MODE>   x += alpha_prev p_prev; p = beta p + z; w_L = A_L p;
+//                       unshared points finished (+0.1p, mask, pap part)
+//   2. k_gs_cls<true>   shared groups: sum in ascending local index
+//                       (src/jl/gs_local.c:60-87), then +0.1p, mask and
+//                       pap part per copy
+//   3. k_pap_final      pap, alpha = rtz1/pap
+//   4. k_update         r += (-alpha) w; parts of rtr = (r,c,r) and of
+//                       rtz = (r,c,z) for the next iteration
+//   5. k_update_final   rtr, beta for the next iteration, convergence
+// Values are bit-identical to the reference's sequence; only the
+// summation order of the dot products differs (fixed tree, so results are
+// reproducible run to run).
+//
+// Generic path (any c): the reference's sequence with separate kernels
+// (p update, ax, dssum, shift/mask/pap, x/r update with rtr).
+#include "nb_device.h"
+
+namespace nb {
+namespace {
+
+constexpr int kFinalThreads = 1024;
+
+__device__ __forceinline__ double inv_mult(uint8_t fl)
+{
+    return 1. / double(fl >> kMultShift);
+}
+
+// Sum of a partials region in a fixed order by one block.
+template <int NT>
+__device__ __forceinline__ void sum_region2(const double *__restrict__ a, int64_t na,
+                                            const double *__restrict__ b, int64_t nb, double &s0,
+                                            double &s1)
+{
+    double a0 = 0.0, a1 = 0.0;
+    for (int64_t i = threadIdx.x; i < na; i += NT) a0 = a0 + a[i];
+    if (b)
+        for (int64_t i = threadIdx.x; i < nb; i += NT) a1 = a1 + b[i];
+    block_sum2<NT>(a0, a1);
+    s0 = a0;
+    s1 = a1;
+}
+
+__device__ void prepare_next(CgScalars *s, double *hist, int hcap)
+{
+    s->rtz2 = s->rtz1;
+    s->rtz1 = s->rtz_next;
+    s->beta = (s->iter == 0) ? 0.0 : s->rtz1 / s->rtz2;
+    const int h = 1 + 3 * s->iter;
+    if (h < hcap) hist[h] = s->rtz1;
+}
+
+// x = 0, r = mask(f) (src/cg.f:38-40); parts of glsc3(r,c,r) and of
+// glsc3(r,c,z) with z = M^-1 r.
+template <int PRE>
+__global__ void __launch_bounds__(kRedThreads)
+    k_cg_init(double *__restrict__ x, double *__restrict__ r, const double *__restrict__ f,
+              const double *__restrict__ c, const double *__restrict__ dinv,
+              const uint8_t *__restrict__ flag, int mbit, int64_t n, double *__restrict__ part,
+              int nblk)
+{
+    double a0 = 0.0, a1 = 0.0;
+    const int64_t stride = int64_t(gridDim.x) * kRedThreads;
+    for (int64_t i = int64_t(blockIdx.x) * kRedThreads + threadIdx.x; i < n; i += stride) {
+        x[i] = 0.0;
+        const double v = (flag[i] & mbit) ? 0.0 : f[i];
+        r[i] = v;
+        const double rc = v * c[i];
+        a0 = a0 + rc * v;
+        if constexpr (PRE == NEKBONE_PRECOND_JACOBI) a1 = a1 + rc * (v * dinv[i]);
+    }
+    block_sum2(a0, a1);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = a0;
+        part[nblk + blockIdx.x] = a1;
+    }
+}
+
+template <int PRE>
+__global__ void __launch_bounds__(kFinalThreads)
+    k_cg_init_final(const double *__restrict__ part, int nblk, CgScalars *__restrict__ s,
+                    double *__restrict__ hist, int hcap, double tol2)
+{
+    double rtr, rtz;
+    sum_region2<kFinalThreads>(part, nblk, part + nblk, nblk, rtr, rtz);
+    if (threadIdx.x == 0) {
+        s->rtr0 = rtr;
+        s->rtr = rtr;
+        s->rtz1 = 1.0;  // src/cg.f:36
+        s->rtz_next = (PRE == NEKBONE_PRECOND_JACOBI) ? rtz : rtr;
+        s->iter = 0;
+        s->alpha = 0.0;
+        s->done = (tol2 > 0.0 && rtr == 0.0) ? 1 : 0;
+        s->status = 0;
+        s->xpending = 0;
+        s->tol2 = tol2;
+        if (hcap > 0) hist[0] = rtr;
+        prepare_next(s, hist, hcap);
+    }
+}
+
+__global__ void __launch_bounds__(kFinalThreads)
+    k_pap_final(const double *__restrict__ part, int64_t npart, CgScalars *__restrict__ s,
+                double *__restrict__ hist, int hcap)
+{
+    if (s->done) return;
+    double pap, dummy;
+    sum_region2<kFinalThreads>(part, npart, nullptr, 0, pap, dummy);
+    if (threadIdx.x == 0) {
+        s->pap = pap;
+        s->alpha = s->rtz1 / pap;
+        s->xpending = 0;  // the element kernel of this iteration applied it
+        if (!(pap > 0.0) || !isfinite(pap)) {
+            s->status = 1;
+            if (s->tol2 > 0.0) s->done = 1;
+        }
+        const int h = 2 + 3 * s->iter;
+        if (h < hcap) hist[h] = pap;
+    }
+}
+
+// [XP] x += alpha p (add2s2, src/cg.f:63) when the element kernel does not
+// carry it; r += (-alpha) w (add2s2, src/cg.f:64); parts of rtr = (r*c)*r
+// (src/cg.f:66) and of the next rtz = (r*c)*z.  Four independent
+// elements per thread per sweep keep enough loads in flight.
+template <int PRE, bool FMA, bool XP>
+__global__ void __launch_bounds__(kRedThreads)
+    k_update(double *__restrict__ r, const double *__restrict__ w,
+             const double *__restrict__ dinv, const uint8_t *__restrict__ flag,
+             double *__restrict__ x, const double *__restrict__ p, int64_t n,
+             double *__restrict__ part, int nblk, const CgScalars *__restrict__ s)
+{
+    if (s->done) return;
+    const double alpha = s->alpha, alphm = -alpha;
+    double a0 = 0.0, a1 = 0.0;
+    constexpr int U = 4;
+    const int64_t stride = int64_t(gridDim.x) * kRedThreads * U;
+    for (int64_t i0 = int64_t(blockIdx.x) * kRedThreads * U + threadIdx.x; i0 < n; i0 += stride) {
+        double rv[U], wv[U], dv[U], xv[U], pv[U];
+        uint8_t fv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t i = i0 + int64_t(u) * kRedThreads;
+            if (i < n) {
+                rv[u] = r[i];
+                wv[u] = w[i];
+                fv[u] = flag[i];
+                if constexpr (PRE == NEKBONE_PRECOND_JACOBI) dv[u] = dinv[i];
+                if constexpr (XP) {
+                    xv[u] = x[i];
+                    pv[u] = p[i];
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t i = i0 + int64_t(u) * kRedThreads;
+            if (i < n) {
+                if constexpr (XP) {
+                    if constexpr (FMA)
+                        x[i] = __builtin_fma(alpha, pv[u], xv[u]);
+                    else
+                        x[i] = xv[u] + alpha * pv[u];
+                }
+                double ri;
+                if constexpr (FMA)
+                    ri = __builtin_fma(alphm, wv[u], rv[u]);
+                else
+                    ri = rv[u] + alphm * wv[u];
+                r[i] = ri;
+                const double rc = ri * inv_mult(fv[u]);
+                a0 = a0 + rc * ri;
+                if constexpr (PRE == NEKBONE_PRECOND_JACOBI) a1 = a1 + rc * (ri * dv[u]);
+            }
+        }
+    }
+    block_sum2(a0, a1);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = a0;
+        part[nblk + blockIdx.x] = a1;
+    }
+}
+
+template <int PRE>
+__global__ void __launch_bounds__(kFinalThreads)
+    k_update_final(const double *__restrict__ part, int nblk, CgScalars *__restrict__ s,
+                   double *__restrict__ hist, int hcap, int set_xpending)
+{
+    if (s->done) return;
+    double rtr, rtz;
+    sum_region2<kFinalThreads>(part, nblk, part + nblk, nblk, rtr, rtz);
+    if (threadIdx.x == 0) {
+        s->rtr = rtr;
+        const int h = 3 + 3 * s->iter;
+        if (h < hcap) hist[h] = rtr;
+        s->iter += 1;
+        if (set_xpending) s->xpending = 1;
+        s->rtz_next = (PRE == NEKBONE_PRECOND_JACOBI) ? rtz : rtr;
+        if (s->tol2 > 0.0 && rtr <= s->tol2 * s->rtr0) s->done = 1;
+        prepare_next(s, hist, hcap);
+    }
+}
+
+// The x update of the last completed iteration (the element kernel of
+// the next iteration would have applied it).
+template <bool FMA>
+__global__ void k_x_finish(double *__restrict__ x, const double *__restrict__ p, int64_t n,
+                           const CgScalars *__restrict__ s)
+{
+    if (!s->xpending) return;
+    const double alpha = s->alpha;
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if constexpr (FMA)
+        x[i] = __builtin_fma(alpha, p[i], x[i]);
+    else
+        x[i] = x[i] + alpha * p[i];
+}
+
+// ---- generic path (any c) ----------------------------------------------
+
+// p = beta p + z (add2s1, src/math.f:534-541), z = r or r*dinv.
+template <int PRE, bool FMA>
+__global__ void k_pupdate(double *__restrict__ p, const double *__restrict__ r,
+                          const double *__restrict__ dinv, int64_t n,
+                          const CgScalars *__restrict__ s)
+{
+    if (s->done) return;
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double z = r[i];
+    if constexpr (PRE == NEKBONE_PRECOND_JACOBI) z = z * dinv[i];
+    if constexpr (FMA)
+        p[i] = __builtin_fma(s->beta, p[i], z);
+    else
+        p[i] = s->beta * p[i] + z;
+}
+
+// x += alpha p; r += (-alpha) w; parts of rtr and rtz with the c array.
+template <int PRE, bool FMA>
+__global__ void __launch_bounds__(kRedThreads)
+    k_update_generic(double *__restrict__ x, double *__restrict__ r, const double *__restrict__ p,
+                     const double *__restrict__ w, const double *__restrict__ c,
+                     const double *__restrict__ dinv, int64_t n, double *__restrict__ part,
+                     int nblk, const CgScalars *__restrict__ s)
+{
+    if (s->done) return;
+    const double alpha = s->alpha, alphm = -alpha;
+    double a0 = 0.0, a1 = 0.0;
+    const int64_t stride = int64_t(gridDim.x) * kRedThreads;
+    for (int64_t i = int64_t(blockIdx.x) * kRedThreads + threadIdx.x; i < n; i += stride) {
+        double xi, ri;
+        if constexpr (FMA) {
+            xi = __builtin_fma(alpha, p[i], x[i]);
+            ri = __builtin_fma(alphm, w[i], r[i]);
+        } else {
+            xi = x[i] + alpha * p[i];
+            ri = r[i] + alphm * w[i];
+        }
+        x[i] = xi;
+        r[i] = ri;
+        const double rc = ri * c[i];
+        a0 = a0 + rc * ri;
+        if constexpr (PRE == NEKBONE_PRECOND_JACOBI) a1 = a1 + rc * (ri * dinv[i]);
+    }
+    block_sum2(a0, a1);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = a0;
+        part[nblk + blockIdx.x] = a1;
+    }
+}
+
+template <bool PAP>
+__global__ void __launch_bounds__(kRedThreads)
+    k_post_cg(double *__restrict__ w, const double *__restrict__ p, const double *__restrict__ c,
+              const uint8_t *__restrict__ flag, int mbit, int64_t n, double *__restrict__ part,
+              const CgScalars *__restrict__ s)
+{
+    if (s->done) return;
+    double acc = 0.0, dummy = 0.0;
+    const int64_t stride = int64_t(gridDim.x) * kRedThreads;
+    for (int64_t i = int64_t(blockIdx.x) * kRedThreads + threadIdx.x; i < n; i += stride) {
+        double v = w[i] + 0.1 * p[i];
+        if (flag[i] & mbit) v = 0.0;
+        w[i] = v;
+        acc = acc + v * c[i] * p[i];
+    }
+    block_sum2(acc, dummy);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+int grid1d(int64_t n, int t) { return int((n + t - 1) / t); }
+
+// Partials layout of the fused path:
+//   [0, NA)            element-kernel pap parts
+//   [NA, NA+NG)        gather-scatter pap parts
+//   [NA+NG, +2*NU)     update rtr parts, then rtz parts
+int64_t n_ax(const nekbone_problem *P) { return ax_blocks(P); }
+int64_t n_gs(const nekbone_problem *P) { return gs_blocks(P); }
+int n_upd(const nekbone_problem *P)
+{
+    int64_t b = (P->n + int64_t(kRedThreads) * 16 - 1) / (int64_t(kRedThreads) * 16);
+    return int(std::max<int64_t>(1, std::min<int64_t>(b, 8192)));
+}
+
+}  // namespace
+
+int64_t cg_partials_needed(const nekbone_problem *P)
+{
+    // element-kernel parts: at most one per element (smallest workgroup)
+    return std::max<int64_t>(P->nelt + n_gs(P) + 2 * n_upd(P), 2 * int64_t(kMaxRedBlocks));
+}
+
+int launch_cg_init(nekbone_problem *P, double *x, double *r, const double *f, int mbit,
+                   int precond, double tol2)
+{
+    const int nb = reduce_blocks(P->n);
+    if (precond == NEKBONE_PRECOND_JACOBI) {
+        hipLaunchKernelGGL(k_cg_init<NEKBONE_PRECOND_JACOBI>, dim3(nb), dim3(kRedThreads), 0,
+                           stream(), x, r, f, P->d_c, P->d_dinv, P->d_flag, mbit, P->n,
+                           P->d_part, nb);
+        hipLaunchKernelGGL(k_cg_init_final<NEKBONE_PRECOND_JACOBI>, dim3(1), dim3(kFinalThreads),
+                           0, stream(), P->d_part, nb, P->d_scal, P->d_hist, P->hist_cap, tol2);
+    } else {
+        hipLaunchKernelGGL(k_cg_init<NEKBONE_PRECOND_IDENTITY>, dim3(nb), dim3(kRedThreads), 0,
+                           stream(), x, r, f, P->d_c, P->d_dinv, P->d_flag, mbit, P->n,
+                           P->d_part, nb);
+        hipLaunchKernelGGL(k_cg_init_final<NEKBONE_PRECOND_IDENTITY>, dim3(1),
+                           dim3(kFinalThreads), 0, stream(), P->d_part, nb, P->d_scal, P->d_hist,
+                           P->hist_cap, tol2);
+    }
+    NB_HIP(hipGetLastError());
+    return NEKBONE_OK;
+}
+
+#define NB_PRE_FMA(FN, ...)                                          \
+    do {                                                             \
+        if (jac) {                                                   \
+            if (fma)                                                 \
+                FN(NEKBONE_PRECOND_JACOBI, true, __VA_ARGS__);       \
+            else                                                     \
+                FN(NEKBONE_PRECOND_JACOBI, false, __VA_ARGS__);      \
+        } else {                                                     \
+            if (fma)                                                 \
+                FN(NEKBONE_PRECOND_IDENTITY, true, __VA_ARGS__);     \
+            else                                                     \
+                FN(NEKBONE_PRECOND_IDENTITY, false, __VA_ARGS__);    \
+        }                                                            \
+    } while (0)
+
+static int iteration_fused(nekbone_problem *P, double *x, double *r, int precond, int mbit)
+{
+    const bool jac = precond == NEKBONE_PRECOND_JACOBI;
+    const bool fma = P->arith == NEKBONE_ARITH_FMA;
+    const int64_t na = n_ax(P), ng = n_gs(P);
+    const int nu = n_upd(P);
+    double *part_upd = P->d_part + na + ng;
+    if (int rc = launch_ax_cg(P, x, r, precond, mbit)) return rc;
+    if (int rc = launch_gs_cg(P, P->d_w, P->d_p, mbit, P->d_part + na)) return rc;
+    if (int rc = timed_begin(P, 4)) return rc;
+    hipLaunchKernelGGL(k_pap_final, dim3(1), dim3(kFinalThreads), 0, stream(), P->d_part, na + ng,
+                       P->d_scal, P->d_hist, P->hist_cap);
+    NB_HIP(hipGetLastError());
+    if (int rc = timed_end(P, 4)) return rc;
+    if (int rc = timed_begin(P, 3)) return rc;
+    const bool xp = !P->axv.xu;  // x update here unless the element kernel carries it
+#define NB_UPD2(PRE, F, XP)                                                                  \
+    hipLaunchKernelGGL((k_update<PRE, F, XP>), dim3(nu), dim3(kRedThreads), 0, stream(), r,    \
+                       P->d_w, P->d_dinv, P->d_flag, x, P->d_p, P->n, part_upd, nu, P->d_scal)
+#define NB_UPD(PRE, F, _)     \
+    do {                      \
+        if (xp)               \
+            NB_UPD2(PRE, F, true); \
+        else                  \
+            NB_UPD2(PRE, F, false); \
+    } while (0)
+    NB_PRE_FMA(NB_UPD, 0);
+#undef NB_UPD
+#undef NB_UPD2
+    NB_HIP(hipGetLastError());
+    if (int rc = timed_end(P, 3)) return rc;
+    if (int rc = timed_begin(P, 4)) return rc;
+    if (jac)
+        hipLaunchKernelGGL(k_update_final<NEKBONE_PRECOND_JACOBI>, dim3(1), dim3(kFinalThreads),
+                           0, stream(), part_upd, nu, P->d_scal, P->d_hist, P->hist_cap,
+                           P->axv.xu);
+    else
+        hipLaunchKernelGGL(k_update_final<NEKBONE_PRECOND_IDENTITY>, dim3(1),
+                           dim3(kFinalThreads), 0, stream(), part_upd, nu, P->d_scal, P->d_hist,
+                           P->hist_cap, P->axv.xu);
+    NB_HIP(hipGetLastError());
+    return timed_end(P, 4);
+}
+
+static int iteration_generic(nekbone_problem *P, double *x, double *r, int precond, int mbit)
+{
+    const bool jac = precond == NEKBONE_PRECOND_JACOBI;
+    const bool fma = P->arith == NEKBONE_ARITH_FMA;
+    const int nb = reduce_blocks(P->n);
+#define NB_PUPD(PRE, F, _)                                                                  \
+    hipLaunchKernelGGL((k_pupdate<PRE, F>), dim3(grid1d(P->n, 256)), dim3(256), 0, stream(), \
+                       P->d_p, r, P->d_dinv, P->n, P->d_scal)
+    NB_PRE_FMA(NB_PUPD, 0);
+#undef NB_PUPD
+    NB_HIP(hipGetLastError());
+    if (int rc = launch_ax_local(P, P->d_w, P->d_p)) return rc;
+    if (int rc = launch_gs(P, P->d_w, P->d_scal)) return rc;
+    hipLaunchKernelGGL(k_post_cg<true>, dim3(nb), dim3(kRedThreads), 0, stream(), P->d_w, P->d_p,
+                       P->d_c, P->d_flag, mbit, P->n, P->d_part, P->d_scal);
+    hipLaunchKernelGGL(k_pap_final, dim3(1), dim3(kFinalThreads), 0, stream(), P->d_part,
+                       int64_t(nb), P->d_scal, P->d_hist, P->hist_cap);
+#define NB_UPD(PRE, F, _)                                                                       \
+    hipLaunchKernelGGL((k_update_generic<PRE, F>), dim3(nb), dim3(kRedThreads), 0, stream(), x, \
+                       r, P->d_p, P->d_w, P->d_c, P->d_dinv, P->n, P->d_part, nb, P->d_scal)
+    NB_PRE_FMA(NB_UPD, 0);
+#undef NB_UPD
+    if (jac)
+        hipLaunchKernelGGL(k_update_final<NEKBONE_PRECOND_JACOBI>, dim3(1), dim3(kFinalThreads),
+                           0, stream(), P->d_part, nb, P->d_scal, P->d_hist, P->hist_cap, 0);
+    else
+        hipLaunchKernelGGL(k_update_final<NEKBONE_PRECOND_IDENTITY>, dim3(1),
+                           dim3(kFinalThreads), 0, stream(), P->d_part, nb, P->d_scal, P->d_hist,
+                           P->hist_cap, 0);
+    NB_HIP(hipGetLastError());
+    return NEKBONE_OK;
+}
+
+// One PCG iteration (src/cg.f:48-75), no host round trip.
+int launch_cg_iteration(nekbone_problem *P, double *x, double *r, int precond, int mbit)
+{
+    if (P->c_is_mult && P->fused) return iteration_fused(P, x, r, precond, mbit);
+    return iteration_generic(P, x, r, precond, mbit);
+}
+
+int launch_cg_finish(nekbone_problem *P, double *x)
+{
+    if (!(P->c_is_mult && P->fused) || !P->axv.xu) return NEKBONE_OK;
+    if (P->arith == NEKBONE_ARITH_FMA)
+        hipLaunchKernelGGL(k_x_finish<true>, dim3(grid1d(P->n, 256)), dim3(256), 0, stream(), x,
+                           P->d_p, P->n, P->d_scal);
+    else
+        hipLaunchKernelGGL(k_x_finish<false>, dim3(grid1d(P->n, 256)), dim3(256), 0, stream(), x,
+                           P->d_p, P->n, P->d_scal);
+    NB_HIP(hipGetLastError());
+    return NEKBONE_OK;
+}
+
+}  // namespace nb

@@ -13,6 +13,14 @@ namespace nb {
 
 constexpr int kMaxNx = 16;
 
+// Per-point flag byte: Dirichlet list membership, whether the point is
+// shared with another element, and its multiplicity (c = 1/multiplicity,
+// src/driver.f:274-291) when that fits in 5 bits.
+constexpr uint8_t kFlagFull = 1;    // in the full boundary list
+constexpr uint8_t kFlagRef = 2;     // in the reference's element-1 list
+constexpr uint8_t kFlagShared = 4;  // multiplicity >= 2
+constexpr int kMultShift = 3;       // bits 3..7: multiplicity (1..31)
+
 // ---------------------------------------------------------------------
 // Errors: the C ABI returns a status and keeps a per-thread message.
 // ---------------------------------------------------------------------
@@ -46,18 +54,38 @@ void gs_groups(const int64_t *glo, int64_t n, std::vector<int32_t> &off,
 void mask_lists(int nx, int nelt, int nelx, int nely, int nelz,
                 const std::vector<int32_t> &lglel, std::vector<int32_t> &full,
                 std::vector<int32_t> &ref);
+// Device point order within an element (surface points first).
+void boundary_first_perm(int nx, std::vector<int32_t> &perm);
 
 // ---------------------------------------------------------------------
 // Device work of one problem.
 // ---------------------------------------------------------------------
+// Shared-node groups, split by size so that a wave handles groups of one
+// size with a fixed trip count: pairs, quadruples and octuples (faces,
+// edges and corners of a hex mesh) as packed member lists, anything else
+// as CSR.  Members are device positions in the reference's member order
+// (ascending reference local index), which fixes the summation order.
 struct GsMap {
     int64_t ngroups = 0, nshared = 0;
-    int32_t *d_off = nullptr;  // ngroups+1
-    int32_t *d_idx = nullptr;  // nshared
+    int64_t n2 = 0, n4 = 0, n8 = 0, nr = 0;  // groups per class
+    int32_t *d_m2 = nullptr;                  // 2*n2
+    int32_t *d_m4 = nullptr;                  // 4*n4
+    int32_t *d_m8 = nullptr;                  // 8*n8
+    int32_t *d_roff = nullptr;                // nr+1
+    int32_t *d_ridx = nullptr;                // members of the other groups
 };
 
 // Per-kernel-class HIP-event timing.  Event pairs are recorded on the
 // library stream without synchronising and resolved when read.
+// Shape of the element-operator kernel (chosen per order by the
+// autotuner): elements per workgroup, geometric factors prefetched at
+// kernel entry, x update fused into the element kernel.
+struct AxVariant {
+    int epb = 1;
+    int pf = 0;
+    int xu = 0;
+};
+
 struct Timing {
     int mask = 0;  // bit k: time kernel class k
     int64_t launches[5] = {0, 0, 0, 0, 0};
@@ -75,9 +103,10 @@ struct Timing {
 // update them so that no host round trip is needed per iteration.
 struct CgScalars {
     double rtz1, rtz2, beta, pap, alpha, rtr, rtr0, rtz_next, tol2;
-    int iter;   // iterations completed
-    int done;   // set once rtr <= tol2*rtr0 (tolerance runs)
-    int status; // sticky: 1 = non-finite or non-positive pap / rtz
+    int iter;     // iterations completed
+    int done;     // set once rtr <= tol2*rtr0 (tolerance runs)
+    int status;   // sticky: 1 = non-finite or non-positive pap / rtz
+    int xpending; // x += alpha p of the last completed iteration not yet applied
 };
 
 }  // namespace nb
@@ -92,17 +121,28 @@ struct nekbone_problem {
     std::vector<int32_t> mfull, mref;
 
     // device data
-    double *d_D = nullptr;     // dxm1, nx1*nx1 column-major
+    double *d_D = nullptr;     // dxm1, nx1*nx1 column-major: Dcol[i*nx+m] = D(m,i)
+    double *d_Drow = nullptr;  // row-major copy: Drow[i*nx+m] = D(i,m)
     double *d_g = nullptr;     // geometric factors, element-blocked SoA g[e][c][p]
     double *d_c = nullptr;     // inverse multiplicity
     double *d_dinv = nullptr;  // Jacobi
-    uint8_t *d_flag = nullptr; // bit0: full Dirichlet list, bit1: reference list
+    uint8_t *d_flag = nullptr; // per-point flags (kFlag*), device order
+    // Device point order: every per-point vector on the device (x, r, p,
+    // w, f, c, dinv, flag) stores reference point p of element e at
+    // e*nx^3 + perm[p]; host-facing entry points convert.  g keeps the
+    // reference order.
+    std::vector<int32_t> perm;
+    int32_t *d_perm = nullptr;
     nb::GsMap gs;
 
     // work vectors (n doubles)
     double *d_x = nullptr, *d_r = nullptr, *d_p = nullptr, *d_w = nullptr, *d_f = nullptr,
-           *d_t = nullptr;
+           *d_t = nullptr, *d_t2 = nullptr;
     double *d_part = nullptr;  // reduction partials
+    int64_t part_cap = 0;
+    bool c_is_mult = false;    // c == 1/multiplicity everywhere (fused CG path allowed)
+    bool fused = true;         // use the fused CG path when c_is_mult
+    nb::AxVariant axv;         // element-kernel shape
     nb::CgScalars *d_scal = nullptr;
     double *d_hist = nullptr;
     int hist_cap = 0;
@@ -121,9 +161,16 @@ int ensure_init();
 int next_mask_bit(nekbone_problem *P);
 
 // Kernel launchers (nb_kernels.hip / nb_ax.hip).  All on stream().
-int launch_ax_local(nekbone_problem *P, double *w, const double *u, int pmode, double *p,
-                    const double *r);
+int ax_blocks(const nekbone_problem *P);
+AxVariant ax_default_variant(int nx1);
+std::vector<AxVariant> ax_variants(int nx1);
+int launch_ax_local(nekbone_problem *P, double *w, const double *u);
+int launch_ax_cg(nekbone_problem *P, double *x, const double *r, int precond, int mbit);
 int launch_gs(nekbone_problem *P, double *u, const CgScalars *guard);
+// Fused CG variant: finishes shared points (+0.1p, mask) and writes one
+// pap part per workgroup (gs_blocks of them) to part.
+int launch_gs_cg(nekbone_problem *P, double *w, const double *p, int mbit, double *part);
+int gs_blocks(const nekbone_problem *P);
 int launch_post(nekbone_problem *P, double *w, const double *p, int mbit, bool pap);
 int launch_mask(nekbone_problem *P, double *w, int mbit);
 int launch_glsc3(nekbone_problem *P, const double *a, const double *b, const double *m,
@@ -135,10 +182,19 @@ int launch_jacobi_finish(nekbone_problem *P, double *d);
 int launch_col2(nekbone_problem *P, double *a, const double *b);
 int launch_rand(double *dst, int64_t count, uint64_t seed);
 int launch_aos_to_soa(nekbone_problem *P, const double *aos, double *soa, int dir);
+// Reference order <-> device order of a per-point vector (dst != src).
+int launch_to_dev(nekbone_problem *P, double *dst, const double *src);
+int launch_to_ref(nekbone_problem *P, double *dst, const double *src);
+// Host vector (reference order) <-> device-order array, through d_t.
+int upload_dev(nekbone_problem *P, double *dst, const double *src);
+int download_ref(nekbone_problem *P, double *dst, const double *src);
 // CG building blocks
 int launch_cg_init(nekbone_problem *P, double *x, double *r, const double *f, int mbit,
                    int precond, double tol2);
 int launch_cg_iteration(nekbone_problem *P, double *x, double *r, int precond, int mbit);
+int launch_cg_finish(nekbone_problem *P, double *x);
+int64_t cg_partials_needed(const nekbone_problem *P);
+int set_c_host(nekbone_problem *P, const double *c);
 int timed_begin(nekbone_problem *P, int kclass);
 int timed_end(nekbone_problem *P, int kclass);
 int timing_resolve(nekbone_problem *P);

@@ -1,50 +1,112 @@
 // Gather-scatter, fused CG vector kernels, deterministic reductions and
 // set-up kernels.  All FP64; the library is compiled with
 // -ffp-contract=off so each expression rounds exactly as written.
-#include "nb_internal.h"
+#include "nb_device.h"
 
 namespace nb {
 namespace {
 
-constexpr int kRedThreads = 256;
-constexpr int kMaxRedBlocks = 1024;
-
-// Fixed-shape block tree reduction of two values (deterministic: the
-// combination order depends only on thread indices).
-__device__ __forceinline__ void block_sum2(double &v0, double &v1)
-{
-    __shared__ double s0[kRedThreads], s1[kRedThreads];
-    const int t = threadIdx.x;
-    s0[t] = v0;
-    s1[t] = v1;
-    __syncthreads();
-#pragma unroll
-    for (int h = kRedThreads / 2; h > 0; h >>= 1) {
-        if (t < h) {
-            s0[t] = s0[t] + s0[t + h];
-            s1[t] = s1[t] + s1[t + h];
-        }
-        __syncthreads();
-    }
-    v0 = s0[0];
-    v1 = s1[0];
-}
-
 // ---------------------------------------------------------------------
 // dssum: per shared group, t = u[i0] + u[i1] + ... in ascending local
 // index, then t written to every member (src/jl/gs_local.c:60-87).
+// With CG=true each member is finished as well: w = t + 0.1 p
+// (src/cg.f:118), mask (src/cg.f:119), and its part of
+// pap = sum (w*c)*p with c = 1/(group size) (src/cg.f:59).
+// One launch covers all size classes; every workgroup handles one class,
+// so its lanes run the same trip count.
 // ---------------------------------------------------------------------
-__global__ void k_gs(double *__restrict__ u, const int32_t *__restrict__ off,
-                     const int32_t *__restrict__ idx, int64_t ngroups,
-                     const CgScalars *__restrict__ scal)
+struct GsArgs {
+    const int32_t *m2, *m4, *m8, *roff, *ridx;
+    int64_t n2, n4, n8, nr;
+    int b2, b4, b8, br;  // workgroups per class
+};
+
+template <bool CG>
+__device__ __forceinline__ void gs_finish(double *__restrict__ w, const double *__restrict__ p,
+                                          const uint8_t *__restrict__ flag, int mbit, int m,
+                                          double t, double c, double &acc)
+{
+    if constexpr (CG) {
+        const double pm = p[m];
+        double v = t + 0.1 * pm;
+        if (flag[m] & mbit) v = 0.0;
+        w[m] = v;
+        acc = acc + v * c * pm;
+    } else {
+        w[m] = t;
+    }
+}
+
+template <int S, bool CG>
+__device__ __forceinline__ void gs_group(double *__restrict__ w, const double *__restrict__ p,
+                                         const uint8_t *__restrict__ flag, int mbit,
+                                         const int32_t *__restrict__ mem, double &acc)
+{
+    int m[S];
+    if constexpr (S == 2) {
+        const int2 v = *reinterpret_cast<const int2 *>(mem);
+        m[0] = v.x, m[1] = v.y;
+    } else {
+#pragma unroll
+        for (int h = 0; h < S / 4; ++h) {
+            const int4 v = reinterpret_cast<const int4 *>(mem)[h];
+            m[4 * h] = v.x, m[4 * h + 1] = v.y, m[4 * h + 2] = v.z, m[4 * h + 3] = v.w;
+        }
+    }
+    double t = w[m[0]];
+#pragma unroll
+    for (int q = 1; q < S; ++q) t += w[m[q]];
+    const double c = 1. / double(S);
+#pragma unroll
+    for (int q = 0; q < S; ++q) gs_finish<CG>(w, p, flag, mbit, m[q], t, c, acc);
+}
+
+template <bool CG>
+__global__ void __launch_bounds__(kRedThreads)
+    k_gs_cls(double *__restrict__ w, const double *__restrict__ p,
+             const uint8_t *__restrict__ flag, int mbit, GsArgs G, double *__restrict__ part,
+             const CgScalars *__restrict__ scal)
 {
     if (scal && scal->done) return;
-    const int64_t gid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (gid >= ngroups) return;
-    const int a = off[gid], b = off[gid + 1];
-    double t = u[idx[a]];
-    for (int q = a + 1; q < b; ++q) t += u[idx[q]];
-    for (int q = a; q < b; ++q) u[idx[q]] = t;
+    int blk = blockIdx.x;
+    double acc = 0.0;
+    if (blk < G.b2) {
+        const int64_t g = int64_t(blk) * kRedThreads + threadIdx.x;
+        if (g < G.n2) gs_group<2, CG>(w, p, flag, mbit, G.m2 + 2 * g, acc);
+    } else if ((blk -= G.b2) < G.b4) {
+        const int64_t g = int64_t(blk) * kRedThreads + threadIdx.x;
+        if (g < G.n4) gs_group<4, CG>(w, p, flag, mbit, G.m4 + 4 * g, acc);
+    } else if ((blk -= G.b4) < G.b8) {
+        const int64_t g = int64_t(blk) * kRedThreads + threadIdx.x;
+        if (g < G.n8) gs_group<8, CG>(w, p, flag, mbit, G.m8 + 8 * g, acc);
+    } else {
+        blk -= G.b8;
+        const int64_t g = int64_t(blk) * kRedThreads + threadIdx.x;
+        if (g < G.nr) {
+            const int a = G.roff[g], b = G.roff[g + 1];
+            double t = w[G.ridx[a]];
+            for (int q = a + 1; q < b; ++q) t += w[G.ridx[q]];
+            const double c = 1. / double(b - a);
+            for (int q = a; q < b; ++q) gs_finish<CG>(w, p, flag, mbit, G.ridx[q], t, c, acc);
+        }
+    }
+    if constexpr (CG) {
+        double dummy = 0.0;
+        block_sum2(acc, dummy);
+        if (threadIdx.x == 0) part[blockIdx.x] = acc;
+    }
+}
+
+GsArgs gs_args(const nekbone_problem *P)
+{
+    const GsMap &M = P->gs;
+    GsArgs G{M.d_m2, M.d_m4, M.d_m8, M.d_roff, M.d_ridx, M.n2, M.n4, M.n8, M.nr, 0, 0, 0, 0};
+    auto nb = [](int64_t n) { return int((n + kRedThreads - 1) / kRedThreads); };
+    G.b2 = nb(M.n2);
+    G.b4 = nb(M.n4);
+    G.b8 = nb(M.n8);
+    G.br = nb(M.nr);
+    return G;
 }
 
 // ---------------------------------------------------------------------
@@ -115,136 +177,6 @@ __global__ void __launch_bounds__(kRedThreads)
     double s0, s1;
     sum_partials(part, nblk, s0, s1);
     if (threadIdx.x == 0) *out = s0;
-}
-
-// ---------------------------------------------------------------------
-// CG scalar bookkeeping (src/cg.f:44-66), all on the device.
-// ---------------------------------------------------------------------
-__device__ void prepare_next(CgScalars *s, double *hist, int hcap)
-{
-    s->rtz2 = s->rtz1;
-    s->rtz1 = s->rtz_next;
-    s->beta = (s->iter == 0) ? 0.0 : s->rtz1 / s->rtz2;
-    const int h = 1 + 3 * s->iter;
-    if (h < hcap) hist[h] = s->rtz1;
-}
-
-// x = 0, r = mask(f); partials of glsc3(r,c,r) and glsc3(r,c,z(r)).
-template <int PRE>
-__global__ void __launch_bounds__(kRedThreads)
-    k_cg_init(double *__restrict__ x, double *__restrict__ r, const double *__restrict__ f,
-              const double *__restrict__ c, const double *__restrict__ dinv,
-              const uint8_t *__restrict__ flag, int mbit, int64_t n, double *__restrict__ part)
-{
-    double a0 = 0.0, a1 = 0.0;
-    const int64_t stride = int64_t(gridDim.x) * kRedThreads;
-    for (int64_t i = int64_t(blockIdx.x) * kRedThreads + threadIdx.x; i < n; i += stride) {
-        x[i] = 0.0;
-        const double v = (flag[i] & mbit) ? 0.0 : f[i];
-        r[i] = v;
-        const double rc = v * c[i];
-        a0 = a0 + rc * v;
-        if constexpr (PRE == NEKBONE_PRECOND_JACOBI) a1 = a1 + rc * (v * dinv[i]);
-    }
-    block_sum2(a0, a1);
-    if (threadIdx.x == 0) {
-        part[blockIdx.x] = a0;
-        part[kMaxRedBlocks + blockIdx.x] = a1;
-    }
-}
-
-template <int PRE>
-__global__ void __launch_bounds__(kRedThreads)
-    k_cg_init_final(const double *__restrict__ part, int nblk, CgScalars *__restrict__ s,
-                    double *__restrict__ hist, int hcap, double tol2)
-{
-    double rtr, rtz;
-    sum_partials(part, nblk, rtr, rtz);
-    if (threadIdx.x == 0) {
-        s->rtr0 = rtr;
-        s->rtr = rtr;
-        s->rtz1 = 1.0;
-        s->rtz_next = (PRE == NEKBONE_PRECOND_JACOBI) ? rtz : rtr;
-        s->iter = 0;
-        s->done = (tol2 > 0.0 && rtr == 0.0) ? 1 : 0;
-        s->status = 0;
-        s->tol2 = tol2;
-        if (hcap > 0) hist[0] = rtr;
-        prepare_next(s, hist, hcap);
-    }
-}
-
-__global__ void __launch_bounds__(kRedThreads)
-    k_pap_final(const double *__restrict__ part, int nblk, CgScalars *__restrict__ s,
-                double *__restrict__ hist, int hcap)
-{
-    if (s->done) return;
-    double pap, dummy;
-    sum_partials(part, nblk, pap, dummy);
-    if (threadIdx.x == 0) {
-        s->pap = pap;
-        s->alpha = s->rtz1 / pap;
-        if (!(pap > 0.0) || !isfinite(pap)) {
-            s->status = 1;
-            if (s->tol2 > 0.0) s->done = 1;
-        }
-        const int h = 2 + 3 * s->iter;
-        if (h < hcap) hist[h] = pap;
-    }
-}
-
-// x += alpha p; r += (-alpha) w (add2s2 x2, src/cg.f:61-64); partials of
-// rtr = glsc3(r,c,r) (src/cg.f:66) and of the next glsc3(r,c,z).
-template <int PRE, bool FMA>
-__global__ void __launch_bounds__(kRedThreads)
-    k_cg_update(double *__restrict__ x, double *__restrict__ r, const double *__restrict__ p,
-                const double *__restrict__ w, const double *__restrict__ c,
-                const double *__restrict__ dinv, int64_t n, double *__restrict__ part,
-                const CgScalars *__restrict__ s)
-{
-    if (s->done) return;
-    const double alpha = s->alpha, alphm = -alpha;
-    double a0 = 0.0, a1 = 0.0;
-    const int64_t stride = int64_t(gridDim.x) * kRedThreads;
-    for (int64_t i = int64_t(blockIdx.x) * kRedThreads + threadIdx.x; i < n; i += stride) {
-        double xi, ri;
-        if constexpr (FMA) {
-            xi = __builtin_fma(alpha, p[i], x[i]);
-            ri = __builtin_fma(alphm, w[i], r[i]);
-        } else {
-            xi = x[i] + alpha * p[i];
-            ri = r[i] + alphm * w[i];
-        }
-        x[i] = xi;
-        r[i] = ri;
-        const double rc = ri * c[i];
-        a0 = a0 + rc * ri;
-        if constexpr (PRE == NEKBONE_PRECOND_JACOBI) a1 = a1 + rc * (ri * dinv[i]);
-    }
-    block_sum2(a0, a1);
-    if (threadIdx.x == 0) {
-        part[blockIdx.x] = a0;
-        part[kMaxRedBlocks + blockIdx.x] = a1;
-    }
-}
-
-template <int PRE>
-__global__ void __launch_bounds__(kRedThreads)
-    k_update_final(const double *__restrict__ part, int nblk, CgScalars *__restrict__ s,
-                   double *__restrict__ hist, int hcap)
-{
-    if (s->done) return;
-    double rtr, rtz;
-    sum_partials(part, nblk, rtr, rtz);
-    if (threadIdx.x == 0) {
-        s->rtr = rtr;
-        const int h = 3 + 3 * s->iter;
-        if (h < hcap) hist[h] = rtr;
-        s->iter += 1;
-        s->rtz_next = (PRE == NEKBONE_PRECOND_JACOBI) ? rtz : rtr;
-        if (s->tol2 > 0.0 && rtr <= s->tol2 * s->rtr0) s->done = 1;
-        prepare_next(s, hist, hcap);
-    }
 }
 
 // ---------------------------------------------------------------------
@@ -337,8 +269,10 @@ __global__ void k_deformed_g(double *__restrict__ g, const double *__restrict__ 
 //   t3 = sum_m (D(m,k)^2) g6(i,j,m), m ascending;
 //   d = ((((t1+t2)+t3) + 2 D(i,i)D(j,j) g2) + 2 D(i,i)D(k,k) g3)
 //       + 2 D(j,j)D(k,k) g5
+// (output in device point order)
 __global__ void k_jacobi_local(double *__restrict__ dl, const double *__restrict__ g,
-                               const double *__restrict__ D, int nx, int64_t nelt)
+                               const double *__restrict__ D, const int32_t *__restrict__ perm,
+                               int nx, int64_t nelt)
 {
     const int64_t nxyz = int64_t(nx) * nx * nx;
     const int64_t gi = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -360,7 +294,21 @@ __global__ void k_jacobi_local(double *__restrict__ dl, const double *__restrict
     const double x2 = (2.0 * (dii * djj)) * ge[nxyz + p];
     const double x3 = (2.0 * (dii * dkk)) * ge[2 * nxyz + p];
     const double x5 = (2.0 * (djj * dkk)) * ge[4 * nxyz + p];
-    dl[gi] = ((((t1 + t2) + t3) + x2) + x3) + x5;
+    dl[e * nxyz + perm[p]] = ((((t1 + t2) + t3) + x2) + x3) + x5;
+}
+
+// Reference order <-> device order of a per-point vector.
+__global__ void k_perm(double *__restrict__ dst, const double *__restrict__ src,
+                       const int32_t *__restrict__ perm, int64_t nxyz, int64_t n, int to_dev)
+{
+    const int64_t gi = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gi >= n) return;
+    const int64_t e = gi / nxyz;
+    const int64_t d = e * nxyz + perm[gi - e * nxyz];
+    if (to_dev)
+        dst[d] = src[gi];
+    else
+        dst[gi] = src[d];
 }
 
 __global__ void k_jacobi_finish(double *__restrict__ d, int64_t n)
@@ -416,12 +364,31 @@ int reduce_blocks(int64_t n)
     return int(b);
 }
 
+int gs_blocks(const nekbone_problem *P)
+{
+    const GsArgs G = gs_args(P);
+    return G.b2 + G.b4 + G.b8 + G.br;
+}
+
 int launch_gs(nekbone_problem *P, double *u, const CgScalars *guard)
 {
-    if (P->gs.ngroups == 0) return NEKBONE_OK;
+    const int nb = gs_blocks(P);
+    if (nb == 0) return NEKBONE_OK;
     if (int rc = timed_begin(P, 1)) return rc;
-    hipLaunchKernelGGL(k_gs, dim3(grid1d(P->gs.ngroups, 256)), dim3(256), 0, stream(), u,
-                       P->gs.d_off, P->gs.d_idx, P->gs.ngroups, guard);
+    hipLaunchKernelGGL(k_gs_cls<false>, dim3(nb), dim3(kRedThreads), 0, stream(), u,
+                       static_cast<const double *>(nullptr), static_cast<const uint8_t *>(nullptr),
+                       0, gs_args(P), static_cast<double *>(nullptr), guard);
+    NB_HIP(hipGetLastError());
+    return timed_end(P, 1);
+}
+
+int launch_gs_cg(nekbone_problem *P, double *w, const double *p, int mbit, double *part)
+{
+    const int nb = gs_blocks(P);
+    if (nb == 0) return NEKBONE_OK;
+    if (int rc = timed_begin(P, 1)) return rc;
+    hipLaunchKernelGGL(k_gs_cls<true>, dim3(nb), dim3(kRedThreads), 0, stream(), w, p, P->d_flag,
+                       mbit, gs_args(P), part, static_cast<const CgScalars *>(P->d_scal));
     NB_HIP(hipGetLastError());
     return timed_end(P, 1);
 }
@@ -498,7 +465,25 @@ int launch_deformed_g(nekbone_problem *P, const double *d_vert)
 int launch_jacobi_local(nekbone_problem *P, double *dl)
 {
     hipLaunchKernelGGL(k_jacobi_local, dim3(grid1d(P->n, 256)), dim3(256), 0, stream(), dl,
-                       P->d_g, P->d_D, P->nx1, int64_t(P->nelt));
+                       P->d_g, P->d_D, P->d_perm, P->nx1, int64_t(P->nelt));
+    NB_HIP(hipGetLastError());
+    return NEKBONE_OK;
+}
+
+int launch_to_dev(nekbone_problem *P, double *dst, const double *src)
+{
+    const int64_t nxyz = int64_t(P->nx1) * P->nx1 * P->nx1;
+    hipLaunchKernelGGL(k_perm, dim3(grid1d(P->n, 256)), dim3(256), 0, stream(), dst, src,
+                       P->d_perm, nxyz, P->n, 1);
+    NB_HIP(hipGetLastError());
+    return NEKBONE_OK;
+}
+
+int launch_to_ref(nekbone_problem *P, double *dst, const double *src)
+{
+    const int64_t nxyz = int64_t(P->nx1) * P->nx1 * P->nx1;
+    hipLaunchKernelGGL(k_perm, dim3(grid1d(P->n, 256)), dim3(256), 0, stream(), dst, src,
+                       P->d_perm, nxyz, P->n, 0);
     NB_HIP(hipGetLastError());
     return NEKBONE_OK;
 }
@@ -535,73 +520,6 @@ int launch_aos_to_soa(nekbone_problem *P, const double *aos, double *soa, int di
                        P->n, dir);
     NB_HIP(hipGetLastError());
     return NEKBONE_OK;
-}
-
-int launch_cg_init(nekbone_problem *P, double *x, double *r, const double *f, int mbit,
-                   int precond, double tol2)
-{
-    const int nb = reduce_blocks(P->n);
-    if (precond == NEKBONE_PRECOND_JACOBI) {
-        hipLaunchKernelGGL(k_cg_init<NEKBONE_PRECOND_JACOBI>, dim3(nb), dim3(kRedThreads), 0,
-                           stream(), x, r, f, P->d_c, P->d_dinv, P->d_flag, mbit, P->n,
-                           P->d_part);
-        hipLaunchKernelGGL(k_cg_init_final<NEKBONE_PRECOND_JACOBI>, dim3(1), dim3(kRedThreads),
-                           0, stream(), P->d_part, nb, P->d_scal, P->d_hist, P->hist_cap, tol2);
-    } else {
-        hipLaunchKernelGGL(k_cg_init<NEKBONE_PRECOND_IDENTITY>, dim3(nb), dim3(kRedThreads), 0,
-                           stream(), x, r, f, P->d_c, P->d_dinv, P->d_flag, mbit, P->n,
-                           P->d_part);
-        hipLaunchKernelGGL(k_cg_init_final<NEKBONE_PRECOND_IDENTITY>, dim3(1),
-                           dim3(kRedThreads), 0, stream(), P->d_part, nb, P->d_scal, P->d_hist,
-                           P->hist_cap, tol2);
-    }
-    NB_HIP(hipGetLastError());
-    return NEKBONE_OK;
-}
-
-// One PCG iteration (src/cg.f:48-75) as 6 launches, no host round trip:
-// ax (with the fused p update) -> dssum -> shift+mask+pap -> alpha ->
-// x/r update with rtr (and rtz) -> beta.
-int launch_cg_iteration(nekbone_problem *P, double *x, double *r, int precond, int mbit)
-{
-    const int nb = reduce_blocks(P->n);
-    const bool jac = precond == NEKBONE_PRECOND_JACOBI;
-    const bool fma = P->arith == NEKBONE_ARITH_FMA;
-    if (int rc = launch_ax_local(P, P->d_w, nullptr, jac ? 2 : 1, P->d_p, r)) return rc;
-    if (int rc = launch_gs(P, P->d_w, P->d_scal)) return rc;
-    if (int rc = launch_post(P, P->d_w, P->d_p, mbit, true)) return rc;
-    if (int rc = timed_begin(P, 4)) return rc;
-    hipLaunchKernelGGL(k_pap_final, dim3(1), dim3(kRedThreads), 0, stream(), P->d_part, nb,
-                       P->d_scal, P->d_hist, P->hist_cap);
-    NB_HIP(hipGetLastError());
-    if (int rc = timed_end(P, 4)) return rc;
-    if (int rc = timed_begin(P, 3)) return rc;
-#define NB_UPD(PRE, F)                                                                      \
-    hipLaunchKernelGGL((k_cg_update<PRE, F>), dim3(nb), dim3(kRedThreads), 0, stream(), x, r, \
-                       P->d_p, P->d_w, P->d_c, P->d_dinv, P->n, P->d_part, P->d_scal)
-    if (jac) {
-        if (fma)
-            NB_UPD(NEKBONE_PRECOND_JACOBI, true);
-        else
-            NB_UPD(NEKBONE_PRECOND_JACOBI, false);
-    } else {
-        if (fma)
-            NB_UPD(NEKBONE_PRECOND_IDENTITY, true);
-        else
-            NB_UPD(NEKBONE_PRECOND_IDENTITY, false);
-    }
-#undef NB_UPD
-    NB_HIP(hipGetLastError());
-    if (int rc = timed_end(P, 3)) return rc;
-    if (int rc = timed_begin(P, 4)) return rc;
-    if (jac)
-        hipLaunchKernelGGL(k_update_final<NEKBONE_PRECOND_JACOBI>, dim3(1), dim3(kRedThreads), 0,
-                           stream(), P->d_part, nb, P->d_scal, P->d_hist, P->hist_cap);
-    else
-        hipLaunchKernelGGL(k_update_final<NEKBONE_PRECOND_IDENTITY>, dim3(1), dim3(kRedThreads),
-                           0, stream(), P->d_part, nb, P->d_scal, P->d_hist, P->hist_cap);
-    NB_HIP(hipGetLastError());
-    return timed_end(P, 4);
 }
 
 }  // namespace nb

@@ -18,7 +18,11 @@ struct GsHandle {
     std::vector<int64_t> glo;
     std::unique_ptr<nekbone_problem, int (*)(nekbone_problem *)> host{nullptr,
                                                                        nekbone_problem_destroy};
-    GsMap map;
+    // groups in the jl order, CSR over reference local indices
+    struct {
+        int64_t ngroups = 0, nshared = 0;
+        int32_t *d_off = nullptr, *d_idx = nullptr;
+    } map;
     double *d_u = nullptr;
 };
 
@@ -135,7 +139,7 @@ int sync_geometry(const double *g, const double *c)
             if (int rc = nekbone_setup_jacobi(L.P)) return rc;
     }
     if (c && c != L.c_ptr) {
-        NB_HIP(hipMemcpy(L.P->d_c, c, sizeof(double) * size_t(L.P->n), hipMemcpyHostToDevice));
+        if (int rc = set_c_host(L.P, c)) return rc;
         L.c_ptr = c;
     }
     return NEKBONE_OK;
@@ -197,10 +201,8 @@ extern "C" void cg_(double *x, const double *f, const double *g, const double *c
     if (!rc) rc = sync_geometry(g, c);
     if (!rc) rc = nekbone_cg(L.P, x, f, r, *niter, L.precond, nullptr);
     if (!rc) {
-        const size_t bytes = sizeof(double) * size_t(L.P->n);
-        if (w) rc = hipMemcpy(w, L.P->d_w, bytes, hipMemcpyDeviceToHost) ? NEKBONE_ERR_HIP : 0;
-        if (!rc && p)
-            rc = hipMemcpy(p, L.P->d_p, bytes, hipMemcpyDeviceToHost) ? NEKBONE_ERR_HIP : 0;
+        if (w) rc = download_ref(L.P, w, L.P->d_w);
+        if (!rc && p) rc = download_ref(L.P, p, L.P->d_p);
         (void)z;  // z is scratch in the reference; the fused path never stores it
         // src/cg.f:79-81: after the loop iter = niter+1
         const double iter = double(*niter + 1);
@@ -219,18 +221,14 @@ extern "C" void solvem_(double *z, double *r, const int *n)
     if (!rc && *n != L.P->n) rc = fail(NEKBONE_ERR_ARG, "solvem_: n differs from the bound mesh");
     if (rc) return report(rc, "solvem_");
     const int64_t nn = L.P->n;
-    const size_t bytes = sizeof(double) * size_t(nn);
-    if (hipMemcpy(L.P->d_t, r, bytes, hipMemcpyHostToDevice) != hipSuccess)
-        return report(fail(NEKBONE_ERR_HIP, "solvem_ upload"), "solvem_");
+    if (int rc2 = upload_dev(L.P, L.P->d_t2, r)) return report(rc2, "solvem_");
     if (L.precond == NEKBONE_PRECOND_JACOBI && L.P->d_dinv)
         hipLaunchKernelGGL(k_mul, dim3(unsigned((nn + 255) / 256)), dim3(256), 0, stream(),
-                           L.P->d_w, L.P->d_t, L.P->d_dinv, nn);
+                           L.P->d_w, L.P->d_t2, L.P->d_dinv, nn);
     else
         hipLaunchKernelGGL(k_copy, dim3(unsigned((nn + 255) / 256)), dim3(256), 0, stream(),
-                           L.P->d_w, L.P->d_t, nn);
-    (void)hipStreamSynchronize(stream());
-    if (hipMemcpy(z, L.P->d_w, bytes, hipMemcpyDeviceToHost) != hipSuccess)
-        report(fail(NEKBONE_ERR_HIP, "solvem_ download"), "solvem_");
+                           L.P->d_w, L.P->d_t2, nn);
+    report(download_ref(L.P, z, L.P->d_w), "solvem_");
 }
 
 extern "C" void ax_(double *w, const double *u, const double *gxyz, double *ur, double *us,

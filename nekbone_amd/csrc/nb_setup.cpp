@@ -311,6 +311,25 @@ void gs_groups(const int64_t *glo, int64_t n, std::vector<int32_t> &off,
     }
 }
 
+// Device point order inside an element: the points on the element's
+// surface first (in reference order), then the interior points (in
+// reference order).  perm[p] = device position of reference point
+// p = i + nx*j + nx*nx*k.  All shared points lie on element surfaces, so
+// the gather-scatter touches only the dense leading block of each element.
+void boundary_first_perm(int nx, std::vector<int32_t> &perm)
+{
+    const int n3 = nx * nx * nx;
+    perm.assign(size_t(n3), -1);
+    int pos = 0;
+    for (int pass = 0; pass < 2; ++pass)
+        for (int p = 0; p < n3; ++p) {
+            const int i = p % nx, j = (p / nx) % nx, k = p / (nx * nx);
+            const bool surf = i == 0 || j == 0 || k == 0 || i == nx - 1 || j == nx - 1 ||
+                              k == nx - 1;
+            if (surf == (pass == 0)) perm[size_t(p)] = pos++;
+        }
+}
+
 namespace {
 // Faces of element e (0-based) on the domain boundary, as get_face
 // decides, src/cg.f:302-368: {x0 (Face4), x1 (Face2), y0 (Face1),

@@ -167,6 +167,45 @@ def test_cg_jacobi_dirichlet_vs_oracle(nb, oracle, deformed):
     x, r, h = P.cg(f, 100, nb.PRECOND_JACOBI)
     np.testing.assert_allclose(h, ho, rtol=HIST_RTOL, atol=0)
     np.testing.assert_allclose(x, xo, rtol=0, atol=1e-12 * np.abs(xo).max())
+    # r has been reduced by ~1e-11 after 100 iterations; it is compared on
+    # the scale of the initial residual, the quantity CG reduces.
+    np.testing.assert_allclose(r, ro, rtol=0, atol=1e-12 * np.abs(f).max())
+
+
+@pytest.mark.parametrize("precond", [0, 1])
+def test_cg_generic_sequence_vs_oracle(nb, oracle, precond):
+    """The unfused kernel sequence (used when c is not 1/multiplicity)."""
+    _case("cg generic precond=%d" % precond)
+    nel, nx1 = (5, 3, 4), 8
+    P = nb.Problem(*nel, nx1)
+    O = oracle.Problem(*nel, nx1)
+    P.set_fused(False)
+    P.set_mask_mode(nb.MASK_DIRICHLET)
+    O.set_mask_mode(oracle.MASK_DIRICHLET)
+    P.set_geometry_deformed(0.1, 8)
+    O.deformed_g(0.1, 8)
+    if precond:
+        P.setup_jacobi()
+        O.setup_jacobi()
+    f = O.rhs_from_raw(splitmix_uniform(O.n, 4))
+    xo, ro, ho = O.cg(f, 60, precond)
+    x, r, h = P.cg(f, 60, precond)
+    np.testing.assert_allclose(h, ho, rtol=HIST_RTOL, atol=0)
+    np.testing.assert_allclose(x, xo, rtol=0, atol=1e-12 * np.abs(xo).max())
+    np.testing.assert_allclose(r, ro, rtol=0, atol=1e-12 * np.abs(f).max())
+
+
+def test_fused_and_generic_sequences_agree(nb):
+    _case("fused vs generic")
+    P = nb.Problem(4, 4, 3, 10)
+    P.set_mask_mode(nb.MASK_DIRICHLET)
+    P.setup_jacobi()
+    f = P.dssum(splitmix_uniform(P.n, 3)) * P.c()
+    x1, r1, h1 = P.cg(f, 50, nb.PRECOND_JACOBI)
+    P.set_fused(False)
+    x2, r2, h2 = P.cg(f, 50, nb.PRECOND_JACOBI)
+    np.testing.assert_allclose(h1, h2, rtol=1e-13, atol=0)
+    np.testing.assert_allclose(x1, x2, rtol=0, atol=1e-13 * np.abs(x2).max())
 
 
 def test_fma_arithmetic_close_to_exact(nb):
