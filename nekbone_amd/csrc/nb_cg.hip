@@ -24,9 +24,27 @@ namespace {
 
 constexpr int kFinalThreads = 1024;
 
+// c = 1/multiplicity (exactly the reference's 1./c(i)); the common hex
+// multiplicities are powers of two, exact without a division.
 __device__ __forceinline__ double inv_mult(uint8_t fl)
 {
-    return 1. / double(fl >> kMultShift);
+    const int m = fl >> kMultShift;
+    return m == 1 ? 1.0 : m == 2 ? 0.5 : m == 4 ? 0.25 : m == 8 ? 0.125 : 1. / double(m);
+}
+
+// Fixed-order sum of a strided slice with four interleaved accumulators
+// (four loads in flight per thread), combined as (s0+s1)+(s2+s3).
+template <int NT>
+__device__ __forceinline__ double slice_sum(const double *__restrict__ a, int64_t na)
+{
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    int64_t i = threadIdx.x;
+    for (; i + 3 * NT < na; i += 4 * NT) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) s[u] = s[u] + a[i + u * NT];
+    }
+    for (int u = 0; i < na; i += NT, ++u) s[u] = s[u] + a[i];
+    return (s[0] + s[1]) + (s[2] + s[3]);
 }
 
 // Sum of a partials region in a fixed order by one block.
@@ -35,10 +53,8 @@ __device__ __forceinline__ void sum_region2(const double *__restrict__ a, int64_
                                             const double *__restrict__ b, int64_t nb, double &s0,
                                             double &s1)
 {
-    double a0 = 0.0, a1 = 0.0;
-    for (int64_t i = threadIdx.x; i < na; i += NT) a0 = a0 + a[i];
-    if (b)
-        for (int64_t i = threadIdx.x; i < nb; i += NT) a1 = a1 + b[i];
+    double a0 = slice_sum<NT>(a, na);
+    double a1 = b ? slice_sum<NT>(b, nb) : 0.0;
     block_sum2<NT>(a0, a1);
     s0 = a0;
     s1 = a1;

@@ -254,3 +254,28 @@ def test_cg_tolerance_stop(nb, oracle):
     iters, final = P.cg_tol_d(xd, fd, rd, tol, 300, nb.PRECOND_JACOBI)
     assert iters == expect
     assert final <= tol * tol * ho[0]
+
+
+def test_cg_history_full_size_vs_reference(nb, golden_c1):
+    """32x32x32 elements, N=9 (32,768,000 points), identity preconditioner,
+    the reference's mask: the full 100-iteration glsc3 history of the
+    flang-compiled reference (tests/golden/ref_c1.npz)."""
+    _case("cg 32^3 N=9 vs reference")
+    P = nb.Problem(32, 32, 32, 10)
+    f, x, r = P.vec(), P.vec(), P.vec()
+    P.rand_uniform_d(f, 0)
+    P.rhs_from_raw_d(f)
+    h = P.cg_d(x, f, r, 100, nb.PRECOND_IDENTITY, want_hist=True)
+    ref = golden_c1["cg_b323232_n10_hist"][0]
+    # At 32.7M points the reference's strictly sequential glsc3 carries its
+    # own rounding error: 1.5e-13 relative on the first dot product against
+    # an exactly rounded sum (numpy pairwise: 1.8e-15), and CG propagates
+    # such last-bit differences.  The histories are compared at 1e-11,
+    # above that floor; the small-mesh fixtures hold 1e-12.
+    np.testing.assert_allclose(h, ref, rtol=1e-11, atol=0)
+    assert np.max(np.abs(h / ref - 1.0)) < 1e-11
+    # the Jacobi-preconditioned, fully masked variant converges too
+    P.set_mask_mode(nb.MASK_DIRICHLET)
+    P.setup_jacobi()
+    hj = P.cg_d(x, f, r, 100, nb.PRECOND_JACOBI, want_hist=True)
+    assert hj[-1] < 1e-12 * hj[0]
