@@ -17,12 +17,15 @@ bounded sample of the same workload.
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 import argparse
+import atexit
 import json
 import os
 import re
 import resource
+import shutil
 import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -39,8 +42,10 @@ def cpu_baseline(nel, nx1, niter_sample):
     ref = os.path.join(ROOT, "oracle", "_ref", "c1", "refharness")
     if os.path.exists(ref) and nx1 <= 10 and nel[0] * nel[1] * nel[2] <= 32768:
         import numpy as np
-        tmp = os.path.join(ROOT, "gpurun_out", "cpu_baseline")
-        os.makedirs(tmp, exist_ok=True)
+        # the RHS file and the harness outputs are ~260 MB each at this size:
+        # keep them out of the results folder and remove them afterwards
+        tmp = tempfile.mkdtemp(prefix="nb_cpu_")
+        atexit.register(shutil.rmtree, tmp, True)
         from oracle import oracle as O
         O.rand_uniform(n, 0).tofile(os.path.join(tmp, "f.bin"))
 
@@ -51,11 +56,19 @@ def cpu_baseline(nel, nx1, niter_sample):
         inp = "bench\n%d %d %d %d %d 1\n%s\n%s\n" % (nel[0], nel[1], nel[2], nx1, niter_sample,
                                                     os.path.join(tmp, "f.bin"),
                                                     os.path.join(tmp, "o"))
-        r = subprocess.run([ref], input=inp.encode(), capture_output=True, preexec_fn=pre,
-                           timeout=900)
-        out = r.stdout.decode()
-        m = re.search(r"HK cg call\s+1 niter\s+(\d+) n\s+(\d+) seconds\s+(\S+)", out)
-        if r.returncode == 0 and m:
+        # the harness prints a line per finished setup phase; pass those on
+        proc = subprocess.Popen([ref], stdin=subprocess.PIPE, stdout=subprocess.PIPE,
+                                preexec_fn=pre, text=True)
+        proc.stdin.write(inp)
+        proc.stdin.close()
+        lines = []
+        for line in proc.stdout:
+            lines.append(line)
+            if "HK" in line:
+                print("cpu_baseline:" + line.rstrip(), file=sys.stderr, flush=True)
+        rc = proc.wait(timeout=900)
+        m = re.search(r"HK cg call\s+1 niter\s+(\d+) n\s+(\d+) seconds\s+(\S+)", "".join(lines))
+        if rc == 0 and m:
             secs = float(m.group(3))
             del np
             return {"value": n * niter_sample / secs, "unit": "DOF*iter/s", "cores": 1,
@@ -78,6 +91,20 @@ def cpu_baseline(nel, nx1, niter_sample):
                       (niter_sample, secs)}
 
 
+def measured_traffic(kname, mesh):
+    """HBM bytes per launch of kernel `kname` on `mesh`, as recorded in the
+    counter summary under profiles/ (a line
+    "traffic <kernel> <mesh> <bytes>"), or None if not recorded."""
+    path = os.path.join(ROOT, "profiles", "ax_cg_hbm_traffic_32x32x32_n9.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.strip().strip("`").split()
+        if len(f) == 4 and f[0] == "traffic" and f[1] == kname and f[2] == mesh:
+            return float(f[3])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,7 +113,12 @@ def main():
     ap.add_argument("--niter", type=int, default=100)
     ap.add_argument("--nel", type=int, default=32)
     ap.add_argument("--nx1", type=int, default=10)
-    ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--cpu-iters", type=int, default=20)
+    ap.add_argument("--traffic", type=float, default=None,
+                    help="HBM bytes per launch of the dominant kernel (default: the "
+                         "figure recorded under profiles/ for this kernel and mesh)")
+    ap.add_argument("--copy-probe", type=float, default=None, metavar="BYTES",
+                    help="only run the copy kernel over BYTES (read+write) and print GB/s")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tol", action="store_true")
     args = ap.parse_args()
@@ -97,6 +129,11 @@ def main():
 
     import nekbone_amd as nb
     nb.init(0)
+    if args.copy_probe:
+        ms = nb.bw_probe(int(args.copy_probe), 10)
+        print(json.dumps({"copy_bytes": int(args.copy_probe), "ms": ms,
+                          "GBs": args.copy_probe / ms / 1e6}))
+        return
     nel = (args.nel, args.nel, args.nel)
     nx1, niter = args.nx1, args.niter
     P = nb.Problem(*nel, nx1)
@@ -128,15 +165,24 @@ def main():
     print("bench: %.3f ms/solve" % ms_step, file=sys.stderr, flush=True)
 
     # Dominant kernel: the element operator as launched inside CG, with the
-    # direction update fused in (reads p, r, dinv, g; writes p, w).
+    # direction update fused in.  Compulsory streams per local point:
+    # reads r, dinv, p (8 B each), g (48 B), the point flag byte (1 B);
+    # writes p, w (8 B each); with the deferred x update also reads and
+    # writes x (16 B).
+    epb, pf, xu = P.variant()
+    bpp = 89 + (16 if xu else 0)
+    kname = "k_ax<%d,%d,false,2,%d,%d>" % (nx1, epb, pf, xu)
     ax_avg_s = ax_ms / max(launches, 1) / 1e3
-    bytes_fused = 88.0 * n
+    bytes_fused = float(bpp) * n
     flops_ax = (19.0 + 12.0 * nx1) * nx1 ** 3 * P.nelt
     roofline = {"bound": "hbm", "achieved": bytes_fused / ax_avg_s / 1e9, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": bytes_fused / ax_avg_s / 1e9 / HBM_PEAK_GBS,
-                "traffic": None, "kernel": "k_ax_local<10,1,false,2> (ax_e + fused p update)",
-                "bytes_per_point": 88, "avg_us": ax_avg_s * 1e6, "launches": launches,
-                "gflops": flops_ax / ax_avg_s / 1e9}
+                "traffic": None, "kernel": kname,
+                "bytes_per_point": bpp, "avg_us": ax_avg_s * 1e6, "launches": launches,
+                "gflops": flops_ax / ax_avg_s / 1e9,
+                "fused": "ax_e + Jacobi p update" + (" + deferred x update" if xu else "")}
+    roofline["traffic"] = args.traffic if args.traffic is not None else \
+        measured_traffic(kname, "%dx%dx%d" % nel)
 
     # The plain operator w = A_L u alone (64 B/point: u, 6 factors, w).
     P.timing_enable(1 << nb.KCLASS_AX)

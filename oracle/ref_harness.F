@@ -62,7 +62,9 @@ c     offs=0 and the element loop runs x fastest, so lglel(e)=e.
          lglel(e) = e
       enddo
 
+      t0 = hk_clock()
       call proxy_setupds(gsh,nx1)
+      call hkphase('gs_setup', t0)
 
 c     set_multiplicity (src/driver.f:274-291)
       call rone(c,n)
@@ -71,8 +73,11 @@ c     set_multiplicity (src/driver.f:274-291)
          c(i) = 1./c(i)
       enddo
 
+      call hkphase('multiplicity', t0)
       call proxy_setup(ah,bh,ch,dh,zh,wh,g)
+      call hkphase('proxy_setup', t0)
       call h1mg_setup
+      call hkphase('h1mg_setup', t0)
 
       flop_a  = 0
       flop_cg = 0
@@ -107,10 +112,13 @@ c        cmask(-1)>=0 and builds the mask list (src/cg.f:230-253).
       elseif (task.eq.'cg' .or. task.eq.'bench') then
 c        RHS as set_f does (src/driver.f:96-97): dssum, then col2 by c,
 c        applied to the raw vector read from the input file.
+         t0 = hk_clock()
          call hkrd(fin, f, n)
+         call hkphase('read rhs', t0)
          call dssum(f)
          call col2 (f,c,n)
-         call hkwr(trim(fout)//'.f', f, n)
+         if (task.eq.'cg') call hkwr(trim(fout)//'.f', f, n)
+         call hkphase('rhs', t0)
          do icall=1,ncalls
             t0 = hk_clock()
             call cg(x,f,g,c,r,w,p,z,n,niter,flop_cg)
@@ -143,12 +151,25 @@ c        place, so both the output and the modified rhs are written.
       write(6,*) 'HK done ', trim(task)
       end
 c-----------------------------------------------------------------------
+      subroutine hkphase(name,t0)
+c     Progress line for a finished setup phase, flushed at once so that
+c     a caller streaming the output sees it; t0 restarts the clock.
+      character*(*) name
+      real*8 t0, t1, hk_clock
+      external hk_clock
+      t1 = hk_clock()
+      write(6,'(a,a,a,f10.3)') ' HK phase ', name, ' seconds ', t1-t0
+      flush(6)
+      t0 = t1
+      return
+      end
+c-----------------------------------------------------------------------
       subroutine hkwr(name,a,n)
       character*(*) name
       real a(n)
       open(unit=21,file=name,access='stream',form='unformatted',
      $     status='replace')
-      write(21) (a(i),i=1,n)
+      write(21) a
       close(21)
       return
       end
@@ -158,7 +179,7 @@ c-----------------------------------------------------------------------
       integer*8 a(n)
       open(unit=21,file=name,access='stream',form='unformatted',
      $     status='replace')
-      write(21) (a(i),i=1,n)
+      write(21) a
       close(21)
       return
       end
@@ -168,7 +189,7 @@ c-----------------------------------------------------------------------
       real a(n)
       open(unit=22,file=trim(name),access='stream',form='unformatted',
      $     status='old')
-      read(22) (a(i),i=1,n)
+      read(22) a
       close(22)
       return
       end
