@@ -223,3 +223,20 @@ def test_cg_converges_to_dense_solution(oracle, precond):
     np.testing.assert_allclose(x, xg[glo], rtol=0, atol=1e-10 * np.abs(xg).max())
     # glsc3 history: rtr decreases overall and ends tiny
     assert h[-1] < 1e-20 * h[0]
+
+
+def test_rand_uniform_splitmix64_known_answer(oracle):
+    """The synthetic-input generator is SplitMix64 (Steele, Lea, Flood,
+    OOPSLA 2014; Vigna's reference splitmix64.c): value k of seed s is the
+    (k+1)-th output of a generator whose state starts at s.  Published
+    first outputs for state 0 pin the constants and the shift order; the
+    mapping to U[-1,1) uses the top 53 bits."""
+    known = [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+    expect = np.array([2.0 * ((v >> 11) * 2.0 ** -53) - 1.0 for v in known])
+    assert np.array_equal(oracle.rand_uniform(3, 0), expect)
+    # a seed shifts the state: value k of seed s equals value k+1 of seed
+    # s - 0x9E3779B97F4A7C15 (mod 2^64)
+    s = 12345
+    prev = (s - 0x9E3779B97F4A7C15) % (1 << 64)
+    assert np.array_equal(oracle.rand_uniform(4, s)[:3], oracle.rand_uniform(4, prev)[1:4])
+    assert np.array_equal(splitmix_uniform(1000, 7), oracle.rand_uniform(1000, 7))
