@@ -91,6 +91,17 @@ def cpu_baseline(nel, nx1, niter_sample):
                       (niter_sample, secs)}
 
 
+def Q_rtr0(Q, fq):
+    """glsc3(r0, c, r0) of r0 = mask(f): the denominator of the relative
+    residual (a zero-iteration tolerance solve reports it)."""
+    import nekbone_amd as nb
+    x, r = Q.vec(), Q.vec()
+    _, rtr0 = Q.cg_tol_d(x, fq, r, 1e-300, 0, nb.PRECOND_JACOBI)
+    x.free()
+    r.free()
+    return rtr0
+
+
 def measured_traffic(kname, mesh):
     """HBM bytes per launch of kernel `kname` on `mesh`, as recorded in the
     counter summary under profiles/ (a line
@@ -119,6 +130,9 @@ def main():
                          "figure recorded under profiles/ for this kernel and mesh)")
     ap.add_argument("--copy-probe", type=float, default=None, metavar="BYTES",
                     help="only run the copy kernel over BYTES (read+write) and print GB/s")
+    ap.add_argument("--delta", type=float, default=0.0,
+                    help="mixed precision: FP32 residual reduction between FP64 "
+                         "residual updates (0: library default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tol", action="store_true")
     args = ap.parse_args()
@@ -126,6 +140,8 @@ def main():
         print("bench.py: this build runs on one GPU only (one rank's share of the mesh); "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
+
+    import numpy as np
 
     import nekbone_amd as nb
     nb.init(0)
@@ -212,9 +228,29 @@ def main():
         iters, _ = Q.cg_tol_d(xq, fq, rq, 1e-10, 5000, nb.PRECOND_JACOBI)
         nb.sync()
         tb = time.perf_counter()
-        extra["time_to_1e-10"] = {"fp64_ms": (tb - ta) * 1e3, "fp64_iters": iters,
-                                  "geometry": "deformed +-10% vertices, seed 1"}
-        del Q, fq, xq, rq
+        x64 = xq.download()
+        # mixed precision: FP32 inner iterations, FP64 reliable updates,
+        # the same FP64 stopping rule; first call builds the FP32 copies
+        xm = Q.vec()
+        Q.cg_mixed_d(xm, fq, rq, 1e-10, 5000, nb.PRECOND_JACOBI, args.delta)
+        nb.sync()
+        tc = time.perf_counter()
+        m_it, m_upd, m_rtr = Q.cg_mixed_d(xm, fq, rq, 1e-10, 5000, nb.PRECOND_JACOBI,
+                                          args.delta)
+        nb.sync()
+        td = time.perf_counter()
+        xmh = xm.download()
+        d = xmh - x64
+        fp64_ms, mixed_ms = (tb - ta) * 1e3, (td - tc) * 1e3
+        extra["time_to_1e-10"] = {
+            "fp64_ms": fp64_ms, "fp64_iters": iters, "mixed_ms": mixed_ms,
+            "mixed_iters": m_it, "mixed_fp64_updates": m_upd, "speedup": fp64_ms / mixed_ms,
+            "mixed_final_rel_residual": (m_rtr / max(Q_rtr0(Q, fq), 1e-300)) ** 0.5,
+            "mixed_vs_fp64_solution": {
+                "l2_rel": float(np.linalg.norm(d) / np.linalg.norm(x64)),
+                "max_rel": float(np.abs(d).max() / np.abs(x64).max())},
+            "geometry": "deformed +-10% vertices, seed 1", "delta": args.delta or 1e-3}
+        del Q, fq, xq, rq, xm
 
     cpu = None
     if not args.no_cpu:

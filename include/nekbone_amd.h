@@ -162,6 +162,27 @@ int nekbone_cg_d(nekbone_problem *P, double *x, const double *f, double *r, int 
 int nekbone_cg_tol_d(nekbone_problem *P, double *x, const double *f, double *r, double tol,
                      int maxiter, int precond, int *iters, double *final_rtr);
 
+/* Mixed precision.  No counterpart in the reference (its CG,
+ * src/cg.f:2-84, is FP64 throughout); the stopping rule is that of
+ * nekbone_cg_tol_d, evaluated on the FP64 residual of the FP64 operator:
+ * glsc3(r,c,r) <= tol^2 glsc3(r0,c,r0) with r = mask(f - A x).
+ * Inner iterations run in FP32 (geometric factors, D, vectors and
+ * gather-scatter in FP32; dot products accumulated in FP64); x is
+ * accumulated in FP64 and the residual is recomputed in FP64 ("reliable
+ * update", the search direction is kept) whenever the FP32 residual norm
+ * has dropped by the factor delta since the last update (0 selects 1e-3)
+ * or the FP32 iteration reports convergence.  x, f, r as for
+ * nekbone_cg_tol_d.  *iters: inner iterations; *updates: FP64 residual
+ * recomputations; *final_rtr: glsc3(r,c,r) of the returned x.  Requires
+ * c = 1/multiplicity.  The FP32 copies of g, D and dinv are made on first
+ * use and remade after the geometry or the Jacobi diagonal changes. */
+int nekbone_cg_mixed_d(nekbone_problem *P, double *x, const double *f, double *r, double tol,
+                       int maxiter, int precond, double delta, int *iters, int *updates,
+                       double *final_rtr);
+/* The FP32 element operator of the mixed solver, w = A_L u element by
+ * element (host float arrays, reference point order, n each). */
+int nekbone_ax_local_f32(nekbone_problem *P, float *w, const float *u);
+
 /* Device buffers owned by the problem (n doubles each), for *_d calls. */
 int nekbone_alloc_vec(nekbone_problem *P, double **dptr);
 int nekbone_free_vec(nekbone_problem *P, double *dptr);

@@ -24,6 +24,7 @@ KCLASS_AX, KCLASS_GS, KCLASS_POST, KCLASS_UPDATE, KCLASS_REDUCE = range(5)
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _i64p = ctypes.POINTER(ctypes.c_int64)
+_fp = ctypes.POINTER(ctypes.c_float)
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _vp = ctypes.c_void_p
 
@@ -65,6 +66,11 @@ _SIGS = {
     "nekbone_cg_d": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _dp]),
     "nekbone_cg_tol_d": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int,
                                         ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp]),
+    "nekbone_cg_mixed_d": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_double,
+                                          ctypes.POINTER(ctypes.c_int),
+                                          ctypes.POINTER(ctypes.c_int), _dp]),
+    "nekbone_ax_local_f32": (ctypes.c_int, [_vp, _fp, _fp]),
     "nekbone_alloc_vec": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
     "nekbone_free_vec": (ctypes.c_int, [_vp, _vp]),
     "nekbone_h2d": (ctypes.c_int, [_vp, _vp, _dp, ctypes.c_int64]),
@@ -321,6 +327,36 @@ class Problem:
         _chk(lib().nekbone_cg_tol_d(self.handle, x.ptr, f.ptr, r.ptr, tol, maxiter, precond,
                                     ctypes.byref(it), ctypes.byref(rtr)))
         return it.value, rtr.value
+
+    def cg_mixed_d(self, x, f, r, tol, maxiter, precond=PRECOND_JACOBI, delta=0.0):
+        """Mixed-precision CG to a relative residual on device vectors;
+        returns (inner iterations, FP64 residual updates, final rtr)."""
+        it, nu = ctypes.c_int(), ctypes.c_int()
+        rtr = ctypes.c_double()
+        _chk(lib().nekbone_cg_mixed_d(self.handle, x.ptr, f.ptr, r.ptr, tol, maxiter, precond,
+                                      delta, ctypes.byref(it), ctypes.byref(nu),
+                                      ctypes.byref(rtr)))
+        return it.value, nu.value, rtr.value
+
+    def cg_mixed(self, f, tol, maxiter, precond=PRECOND_JACOBI, delta=0.0):
+        """Host-array form: returns x, r (reference order), iterations,
+        updates, final rtr."""
+        fv, xv, rv = self.vec(), self.vec(), self.vec()
+        try:
+            fv.upload(f)
+            it, nu, rtr = self.cg_mixed_d(xv, fv, rv, tol, maxiter, precond, delta)
+            return xv.download(), rv.download(), it, nu, rtr
+        finally:
+            for v in (fv, xv, rv):
+                v.free()
+
+    def ax_local_f32(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float32)
+        assert u.size == self.n
+        w = np.empty(self.n, dtype=np.float32)
+        _chk(lib().nekbone_ax_local_f32(self.handle, w.ctypes.data_as(_fp),
+                                        u.ctypes.data_as(_fp)))
+        return w
 
     def ax_d(self, w, u):
         _chk(lib().nekbone_ax_d(self.handle, w.ptr, u.ptr))

@@ -44,28 +44,6 @@ int fill(double *a, double v, int64_t n)
     return NEKBONE_OK;
 }
 
-template <class T>
-int dalloc(T **p, int64_t count)
-{
-    *p = nullptr;
-    if (count <= 0) count = 1;
-    NB_HIP(hipMalloc(reinterpret_cast<void **>(p), sizeof(T) * size_t(count)));
-    return NEKBONE_OK;
-}
-
-template <class T>
-void dfree(T *&p)
-{
-    if (p) (void)hipFree(p);
-    p = nullptr;
-}
-
-int check(const nekbone_problem *P)
-{
-    if (!P) return fail(NEKBONE_ERR_ARG, "null problem");
-    return NEKBONE_OK;
-}
-
 std::map<std::tuple<int, int, int>, AxVariant> &tuned_cache()
 {
     static std::map<std::tuple<int, int, int>, AxVariant> m;
@@ -316,7 +294,7 @@ static int ensure_hist(nekbone_problem *P, int niter)
 static int run_cg(nekbone_problem *P, double *x, const double *f, double *r, int niter,
                   int precond, double tol2)
 {
-    if (precond == NEKBONE_PRECOND_JACOBI && !P->d_dinv)
+    if (precond == NEKBONE_PRECOND_JACOBI && !P->jacobi)
         return fail(NEKBONE_ERR_STATE, "cg: Jacobi preconditioner not set up");
     if (precond != NEKBONE_PRECOND_JACOBI && precond != NEKBONE_PRECOND_IDENTITY)
         return fail(NEKBONE_ERR_UNSUPPORTED, "cg: preconditioner not available on this path");
@@ -474,6 +452,7 @@ extern "C" int nekbone_problem_destroy(nekbone_problem *P)
                        &P->d_hist})
         dfree(*v);
     dfree(P->d_scal);
+    free_mixed(P);
     (void)timing_resolve(P);
     for (hipEvent_t e : P->timing.pool) (void)hipEventDestroy(e);
     if (P->timing.cur) (void)hipEventDestroy(P->timing.cur);
@@ -496,6 +475,7 @@ extern "C" int nekbone_set_geometry(nekbone_problem *P, const double *gxyz)
     (void)hipStreamSynchronize(stream());
     dfree(tmp);
     P->jacobi = false;
+    P->geom_version++;
     return rc;
 }
 
@@ -542,6 +522,7 @@ extern "C" int nekbone_set_geometry_deformed(nekbone_problem *P, double amp, uin
     (void)hipStreamSynchronize(stream());
     dfree(d_v);
     P->jacobi = false;
+    P->geom_version++;
     return rc;
 }
 
@@ -616,7 +597,7 @@ extern "C" int nekbone_set_variant(nekbone_problem *P, int epb, int pf, int xu)
 extern "C" int nekbone_autotune(nekbone_problem *P, int precond, double *best_ms)
 {
     if (int rc = check(P)) return rc;
-    if (precond == NEKBONE_PRECOND_JACOBI && !P->d_dinv)
+    if (precond == NEKBONE_PRECOND_JACOBI && !P->jacobi)
         return fail(NEKBONE_ERR_STATE, "autotune: Jacobi not set up");
     if (precond != NEKBONE_PRECOND_JACOBI && precond != NEKBONE_PRECOND_IDENTITY)
         return fail(NEKBONE_ERR_ARG, "autotune: identity or Jacobi");
@@ -679,6 +660,7 @@ extern "C" int nekbone_setup_jacobi(nekbone_problem *P)
     if (int rc = launch_jacobi_finish(P, P->d_dinv)) return rc;
     NB_HIP(hipStreamSynchronize(stream()));
     P->jacobi = true;
+    P->dinv_version++;
     return NEKBONE_OK;
 }
 
@@ -819,7 +801,7 @@ extern "C" int nekbone_cg_tol_d(nekbone_problem *P, double *x, const double *f, 
 {
     if (int rc = check(P)) return rc;
     if (!(tol > 0.0)) return fail(NEKBONE_ERR_ARG, "cg_tol: tol must be > 0");
-    if (precond == NEKBONE_PRECOND_JACOBI && !P->d_dinv)
+    if (precond == NEKBONE_PRECOND_JACOBI && !P->jacobi)
         return fail(NEKBONE_ERR_STATE, "cg: Jacobi preconditioner not set up");
     if (int rc = ensure_hist(P, maxiter)) return rc;
     if (int rc = launch_to_dev(P, P->d_f, f)) return rc;
@@ -843,6 +825,7 @@ extern "C" int nekbone_cg_tol_d(nekbone_problem *P, double *x, const double *f, 
         if (int rc = launch_to_ref(P, x, P->d_x)) return rc;
     if (r)
         if (int rc = launch_to_ref(P, r, P->d_r)) return rc;
+    NB_HIP(hipMemcpyAsync(&s, P->d_scal, sizeof s, hipMemcpyDeviceToHost, stream()));
     NB_HIP(hipStreamSynchronize(stream()));
     if (iters) *iters = s.iter;
     if (final_rtr) *final_rtr = s.rtr;

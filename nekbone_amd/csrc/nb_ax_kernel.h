@@ -54,8 +54,8 @@
 namespace nb {
 namespace axk {
 
-template <bool FMA>
-__device__ __forceinline__ double madd(double acc, double a, double b)
+template <bool FMA, typename T>
+__device__ __forceinline__ T madd(T acc, T a, T b)
 {
     if constexpr (FMA)
         return __builtin_fma(a, b, acc);
@@ -65,30 +65,32 @@ __device__ __forceinline__ double madd(double acc, double a, double b)
 
 constexpr int round64(int x) { return (x + 63) / 64 * 64; }
 
-typedef const __attribute__((address_space(4))) double *const_dptr;
-
 // sum_m row[m] * x[m], m ascending, first term a plain product.
-template <int NX, bool FMA>
-__device__ __forceinline__ double dot_row(const double *__restrict__ base, int row,
-                                          const double (&x)[NX])
+template <int NX, bool FMA, typename T>
+__device__ __forceinline__ T dot_row(const T *__restrict__ base, int row, const T (&x)[NX])
 {
-    const_dptr p = (const_dptr)(base + row * NX);
+    typedef const __attribute__((address_space(4))) T *const_ptr;
+    const_ptr p = (const_ptr)(base + row * NX);
     asm volatile("" : "+s"(p));
-    double s = p[0] * x[0];
+    T s = p[0] * x[0];
 #pragma unroll
     for (int m = 1; m < NX; ++m) s = madd<FMA>(s, p[m], x[m]);
     return s;
 }
 
-struct AxArgs {
-    double *w;
-    const double *u;     // MODE 0 input
-    double *p;           // CG: direction, updated in place
-    const double *r;     // CG: residual
-    const double *dinv;  // CG Jacobi
-    double *x;           // CG: solution, updated one iteration late (XU)
-    const double *g;
-    const double *Drow, *Dcol;
+// T is the storage and arithmetic type of the operator (double; float
+// for the inner solve of the mixed-precision CG).  Dot-product partials
+// and the CG scalars are FP64 for both.
+template <typename T>
+struct AxArgsT {
+    T *w;
+    const T *u;     // MODE 0 input
+    T *p;           // CG: direction, updated in place
+    const T *r;     // CG: residual
+    const T *dinv;  // CG Jacobi
+    T *x;           // CG: solution, updated one iteration late (XU)
+    const T *g;
+    const T *Drow, *Dcol;
     const uint8_t *flag;
     const int32_t *perm;  // device point order within an element
     int mbit;
@@ -96,14 +98,15 @@ struct AxArgs {
     double *part;        // CG: per-block pap partials
     int nelt;
 };
+using AxArgs = AxArgsT<double>;
 
-template <int NX, int EPB, bool FMA, int MODE, int PF, bool XU>
-__global__ void __launch_bounds__(round64(NX *NX *EPB)) k_ax(AxArgs A_)
+template <typename T, int NX, int EPB, bool FMA, int MODE, int PF, bool XU>
+__global__ void __launch_bounds__(round64(NX *NX *EPB)) k_ax(AxArgsT<T> A_)
 {
     constexpr int N2 = NX * NX, N3 = NX * NX * NX;
     constexpr int NT = round64(N2 * EPB);
-    __shared__ double sA[EPB * N3];
-    __shared__ double sB[EPB * N3];
+    __shared__ T sA[EPB * N3];
+    __shared__ T sB[EPB * N3];
 
     const int tid = threadIdx.x;
     const int el = tid / N2;
@@ -111,18 +114,18 @@ __global__ void __launch_bounds__(round64(NX *NX *EPB)) k_ax(AxArgs A_)
     const int a = ab % NX, b = ab / NX;
     const int64_t e = int64_t(blockIdx.x) * EPB + el;
     const bool act = (el < EPB) && (e < A_.nelt);
-    double *A = sA + (act ? el : 0) * N3;
-    double *B = sB + (act ? el : 0) * N3;
+    T *A = sA + (act ? el : 0) * N3;
+    T *B = sB + (act ? el : 0) * N3;
     const int64_t base = e * N3;
-    const double *__restrict__ Drow = A_.Drow;
-    const double *__restrict__ Dcol = A_.Dcol;
-    const double *ge = A_.g + e * 6 * N3;
+    const T *__restrict__ Drow = A_.Drow;
+    const T *__restrict__ Dcol = A_.Dcol;
+    const T *ge = A_.g + e * 6 * N3;
 
     if constexpr (MODE != 0) {
         if (A_.scal->done) return;  // converged tolerance run: nothing to do
     }
 
-    double gv[PF ? 6 : 1][PF ? NX : 1];
+    T gv[PF ? 6 : 1][PF ? NX : 1];
     if constexpr (PF) {
         if (act) {
 #pragma unroll
@@ -155,14 +158,14 @@ __global__ void __launch_bounds__(round64(NX *NX *EPB)) k_ax(AxArgs A_)
         }
     }
 
-    double uc[NX];  // the thread's t-line of the input
+    T uc[NX];  // the thread's t-line of the input
     if (act) {
         if constexpr (MODE == 0) {
 #pragma unroll
             for (int k = 0; k < NX; ++k) uc[k] = A_.u[base + off[k]];
         } else {
-            const double beta = A_.scal->beta;
-            double po[NX], z[NX];
+            const T beta = T(A_.scal->beta);
+            T po[NX], z[NX];
 #pragma unroll
             for (int k = 0; k < NX; ++k) {
                 const int64_t q = base + off[k];
@@ -174,8 +177,8 @@ __global__ void __launch_bounds__(round64(NX *NX *EPB)) k_ax(AxArgs A_)
                 for (int k = 0; k < NX; ++k) z[k] = z[k] * A_.dinv[base + off[k]];
             }
             if constexpr (XU) {
-                if (A_.scal->iter > 0) {
-                    const double alpha = A_.scal->alpha;
+                if (A_.scal->xpending) {  // the previous iteration's alpha p
+                    const T alpha = T(A_.scal->alpha);
 #pragma unroll
                     for (int k = 0; k < NX; ++k) {
                         const int64_t q = base + off[k];
@@ -188,7 +191,7 @@ __global__ void __launch_bounds__(round64(NX *NX *EPB)) k_ax(AxArgs A_)
             }
 #pragma unroll
             for (int k = 0; k < NX; ++k) {
-                double pn;
+                T pn;
                 if constexpr (FMA)
                     pn = __builtin_fma(beta, po[k], z[k]);
                 else
@@ -202,7 +205,7 @@ __global__ void __launch_bounds__(round64(NX *NX *EPB)) k_ax(AxArgs A_)
     }
     __syncthreads();
 
-    double lr[NX], ls[NX];
+    T lr[NX], ls[NX];
     if (act) {
 #pragma unroll
         for (int m = 0; m < NX; ++m) lr[m] = A[m + NX * a + N2 * b];  // r-line (j,k)=(a,b)
@@ -218,14 +221,14 @@ __global__ void __launch_bounds__(round64(NX *NX *EPB)) k_ax(AxArgs A_)
     }
     __syncthreads();
 
-    double wt[NX];
+    T wt[NX];
     if (act) {
 #pragma unroll
         for (int k = 0; k < NX; ++k) {
             const int q = a + NX * b + N2 * k;
-            const double ur = B[q], us = A[q];
-            const double ut = dot_row<NX, FMA>(Drow, k, uc);
-            double g1, g2, g3, g4, g5, g6;
+            const T ur = B[q], us = A[q];
+            const T ut = dot_row<NX, FMA>(Drow, k, uc);
+            T g1, g2, g3, g4, g5, g6;
             if constexpr (PF) {
                 g1 = gv[0][k], g2 = gv[1][k], g3 = gv[2][k];
                 g4 = gv[3][k], g5 = gv[4][k], g6 = gv[5][k];
@@ -233,7 +236,7 @@ __global__ void __launch_bounds__(round64(NX *NX *EPB)) k_ax(AxArgs A_)
                 g1 = ge[q], g2 = ge[N3 + q], g3 = ge[2 * N3 + q];
                 g4 = ge[3 * N3 + q], g5 = ge[4 * N3 + q], g6 = ge[5 * N3 + q];
             }
-            double wr, ws;
+            T wr, ws;
             if constexpr (FMA) {
                 wr = __builtin_fma(g3, ut, __builtin_fma(g2, us, g1 * ur));
                 ws = __builtin_fma(g5, ut, __builtin_fma(g4, us, g2 * ur));
@@ -268,7 +271,7 @@ __global__ void __launch_bounds__(round64(NX *NX *EPB)) k_ax(AxArgs A_)
 #pragma unroll
         for (int k = 0; k < NX; ++k) {
             const int q = a + NX * b + N2 * k;
-            const double wl = (B[q] + A[q]) + dot_row<NX, FMA>(Dcol, k, wt);
+            const T wl = (B[q] + A[q]) + dot_row<NX, FMA>(Dcol, k, wt);
             if constexpr (MODE == 0) {
                 A_.w[base + off[k]] = wl;
             } else {
@@ -276,10 +279,10 @@ __global__ void __launch_bounds__(round64(NX *NX *EPB)) k_ax(AxArgs A_)
                 if (fl & kFlagShared) {
                     A_.w[base + off[k]] = wl;  // finished by the gather-scatter kernel
                 } else {
-                    double v = wl + 0.1 * uc[k];
-                    if (fl & A_.mbit) v = 0.0;
+                    T v = wl + T(0.1) * uc[k];
+                    if (fl & A_.mbit) v = T(0);
                     A_.w[base + off[k]] = v;
-                    acc = acc + v * uc[k];
+                    acc = acc + double(v) * double(uc[k]);
                 }
             }
         }
@@ -314,16 +317,16 @@ using Table = std::map<Key, AxFn>;
 template <int NX, int EPB, int PF>
 void reg_modes(Table &t)
 {
-    t[Key(NX, EPB, 0, 0, PF, 0)] = k_ax<NX, EPB, false, 0, PF, false>;
-    t[Key(NX, EPB, 1, 0, PF, 0)] = k_ax<NX, EPB, true, 0, PF, false>;
-    t[Key(NX, EPB, 0, 1, PF, 0)] = k_ax<NX, EPB, false, 1, PF, false>;
-    t[Key(NX, EPB, 1, 1, PF, 0)] = k_ax<NX, EPB, true, 1, PF, false>;
-    t[Key(NX, EPB, 0, 2, PF, 0)] = k_ax<NX, EPB, false, 2, PF, false>;
-    t[Key(NX, EPB, 1, 2, PF, 0)] = k_ax<NX, EPB, true, 2, PF, false>;
-    t[Key(NX, EPB, 0, 1, PF, 1)] = k_ax<NX, EPB, false, 1, PF, true>;
-    t[Key(NX, EPB, 1, 1, PF, 1)] = k_ax<NX, EPB, true, 1, PF, true>;
-    t[Key(NX, EPB, 0, 2, PF, 1)] = k_ax<NX, EPB, false, 2, PF, true>;
-    t[Key(NX, EPB, 1, 2, PF, 1)] = k_ax<NX, EPB, true, 2, PF, true>;
+    t[Key(NX, EPB, 0, 0, PF, 0)] = k_ax<double, NX, EPB, false, 0, PF, false>;
+    t[Key(NX, EPB, 1, 0, PF, 0)] = k_ax<double, NX, EPB, true, 0, PF, false>;
+    t[Key(NX, EPB, 0, 1, PF, 0)] = k_ax<double, NX, EPB, false, 1, PF, false>;
+    t[Key(NX, EPB, 1, 1, PF, 0)] = k_ax<double, NX, EPB, true, 1, PF, false>;
+    t[Key(NX, EPB, 0, 2, PF, 0)] = k_ax<double, NX, EPB, false, 2, PF, false>;
+    t[Key(NX, EPB, 1, 2, PF, 0)] = k_ax<double, NX, EPB, true, 2, PF, false>;
+    t[Key(NX, EPB, 0, 1, PF, 1)] = k_ax<double, NX, EPB, false, 1, PF, true>;
+    t[Key(NX, EPB, 1, 1, PF, 1)] = k_ax<double, NX, EPB, true, 1, PF, true>;
+    t[Key(NX, EPB, 0, 2, PF, 1)] = k_ax<double, NX, EPB, false, 2, PF, true>;
+    t[Key(NX, EPB, 1, 2, PF, 1)] = k_ax<double, NX, EPB, true, 2, PF, true>;
 }
 
 // Orders the benchmark configurations use get the full variant space;
@@ -340,7 +343,37 @@ void reg_order(Table &t, bool full)
     }
 }
 
-// Defined in the instantiation files nb_ax_i*.hip.
+// FP32 kernels (inner iterations of the mixed-precision CG), always with
+// fused multiply-adds.  Key: (nx, epb, mode, pf, xu).
+using AxFn32 = void (*)(AxArgsT<float>);
+using Key32 = std::tuple<int, int, int, int, int>;
+using Table32 = std::map<Key32, AxFn32>;
+
+template <int NX, int EPB, int PF>
+void reg32_modes(Table32 &t)
+{
+    t[Key32(NX, EPB, 0, PF, 0)] = k_ax<float, NX, EPB, true, 0, PF, false>;
+    t[Key32(NX, EPB, 1, PF, 0)] = k_ax<float, NX, EPB, true, 1, PF, false>;
+    t[Key32(NX, EPB, 2, PF, 0)] = k_ax<float, NX, EPB, true, 2, PF, false>;
+    t[Key32(NX, EPB, 1, PF, 1)] = k_ax<float, NX, EPB, true, 1, PF, true>;
+    t[Key32(NX, EPB, 2, PF, 1)] = k_ax<float, NX, EPB, true, 2, PF, true>;
+}
+
+template <int NX>
+void reg32_order(Table32 &t, bool full)
+{
+    constexpr int E = epb_for(NX);
+    reg32_modes<NX, E, 0>(t);
+    if (full) {
+        reg32_modes<NX, E, 1>(t);
+        reg32_modes<NX, 2 * E, 0>(t);
+        reg32_modes<NX, 2 * E, 1>(t);
+    }
+}
+
+// Defined in the instantiation files nb_ax_i*.hip and nb_ax32_*.hip.
+void reg32_low(Table32 &t);
+void reg32_high(Table32 &t);
 void reg_low(Table &t);
 void reg_high(Table &t);
 void reg_6(Table &t);

@@ -324,6 +324,32 @@ int n_upd(const nekbone_problem *P)
 
 }  // namespace
 
+int cg_update_blocks(const nekbone_problem *P) { return n_upd(P); }
+
+int launch_pap_final(nekbone_problem *P, const double *part, int64_t npart)
+{
+    if (int rc = timed_begin(P, 4)) return rc;
+    hipLaunchKernelGGL(k_pap_final, dim3(1), dim3(kFinalThreads), 0, stream(), part, npart,
+                       P->d_scal, P->d_hist, 0);
+    NB_HIP(hipGetLastError());
+    return timed_end(P, 4);
+}
+
+int launch_update_final(nekbone_problem *P, const double *part, int nu, int precond,
+                        int set_xpending)
+{
+    if (int rc = timed_begin(P, 4)) return rc;
+    if (precond == NEKBONE_PRECOND_JACOBI)
+        hipLaunchKernelGGL(k_update_final<NEKBONE_PRECOND_JACOBI>, dim3(1), dim3(kFinalThreads),
+                           0, stream(), part, nu, P->d_scal, P->d_hist, 0, set_xpending);
+    else
+        hipLaunchKernelGGL(k_update_final<NEKBONE_PRECOND_IDENTITY>, dim3(1),
+                           dim3(kFinalThreads), 0, stream(), part, nu, P->d_scal, P->d_hist, 0,
+                           set_xpending);
+    NB_HIP(hipGetLastError());
+    return timed_end(P, 4);
+}
+
 int64_t cg_partials_needed(const nekbone_problem *P)
 {
     // element-kernel parts: at most one per element (smallest workgroup)

@@ -35,6 +35,23 @@ int fail(int code, const std::string &msg);
                                                    hipGetErrorString(_e));         \
     } while (0)
 
+// Device allocation helpers (at least one element is allocated).
+template <class T>
+int dalloc(T **p, int64_t count)
+{
+    *p = nullptr;
+    if (count <= 0) count = 1;
+    NB_HIP(hipMalloc(reinterpret_cast<void **>(p), sizeof(T) * size_t(count)));
+    return NEKBONE_OK;
+}
+
+template <class T>
+void dfree(T *&p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
 // ---------------------------------------------------------------------
 // Host-side set-up (nb_setup.cpp): restatements of the reference's
 // set-up algorithms used by the library's own problem constructor.
@@ -107,6 +124,18 @@ struct CgScalars {
     int done;     // set once rtr <= tol2*rtr0 (tolerance runs)
     int status;   // sticky: 1 = non-finite or non-positive pap / rtz
     int xpending; // x += alpha p of the last completed iteration not yet applied
+    double rtr_upd;  // mixed precision: FP64 rtr at the last reliable update
+};
+
+// FP32 copies for the inner iterations of the mixed-precision CG:
+// operator data converted from the FP64 originals (rebuilt when the
+// geometry or the Jacobi diagonal changes) and the FP32 work vectors,
+// all in the device point order.
+struct MixedState {
+    float *g = nullptr, *Drow = nullptr, *Dcol = nullptr, *dinv = nullptr;
+    float *x = nullptr, *r = nullptr, *p = nullptr, *w = nullptr;
+    int geom_version = -1, dinv_version = -1;
+    AxVariant axv;
 };
 
 }  // namespace nb
@@ -151,10 +180,18 @@ struct nekbone_problem {
     int mask_built = 0;
     int arith = NEKBONE_ARITH_EXACT;
     bool jacobi = false;
+    int geom_version = 0;  // bumped whenever d_g changes
+    int dinv_version = 0;  // bumped whenever d_dinv is recomputed
+    nb::MixedState *mx = nullptr;
     nb::Timing timing;
 };
 
 namespace nb {
+inline int check(const nekbone_problem *P)
+{
+    if (!P) return fail(NEKBONE_ERR_ARG, "null problem");
+    return NEKBONE_OK;
+}
 hipStream_t stream();
 int ensure_init();
 // Mask bit to apply for the next maskit call, advancing the state.
@@ -170,6 +207,10 @@ int launch_gs(nekbone_problem *P, double *u, const CgScalars *guard);
 // Fused CG variant: finishes shared points (+0.1p, mask) and writes one
 // pap part per workgroup (gs_blocks of them) to part.
 int launch_gs_cg(nekbone_problem *P, double *w, const double *p, int mbit, double *part);
+// FP32 gather-scatter (mixed-precision inner solve): plain, and the fused
+// CG variant (pap parts in FP64).
+int launch_gs32(nekbone_problem *P, float *u);
+int launch_gs_cg32(nekbone_problem *P, float *w, const float *p, int mbit, double *part);
 int gs_blocks(const nekbone_problem *P);
 int launch_post(nekbone_problem *P, double *w, const double *p, int mbit, bool pap);
 int launch_mask(nekbone_problem *P, double *w, int mbit);
@@ -194,6 +235,14 @@ int launch_cg_init(nekbone_problem *P, double *x, double *r, const double *f, in
 int launch_cg_iteration(nekbone_problem *P, double *x, double *r, int precond, int mbit);
 int launch_cg_finish(nekbone_problem *P, double *x);
 int64_t cg_partials_needed(const nekbone_problem *P);
+// Single-block scalar kernels of the fused iteration, shared with the
+// mixed-precision solver: alpha from npart pap parts; rtr / next rtz from
+// 2*nu parts (rtr parts, then rtz parts).
+int launch_pap_final(nekbone_problem *P, const double *part, int64_t npart);
+int launch_update_final(nekbone_problem *P, const double *part, int nu, int precond,
+                        int set_xpending);
+int cg_update_blocks(const nekbone_problem *P);
+void free_mixed(nekbone_problem *P);
 int set_c_host(nekbone_problem *P, const double *c);
 int timed_begin(nekbone_problem *P, int kclass);
 int timed_end(nekbone_problem *P, int kclass);

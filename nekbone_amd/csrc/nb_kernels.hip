@@ -21,24 +21,26 @@ struct GsArgs {
     int b2, b4, b8, br;  // workgroups per class
 };
 
-template <bool CG>
-__device__ __forceinline__ void gs_finish(double *__restrict__ w, const double *__restrict__ p,
-                                          const uint8_t *__restrict__ flag, int mbit, int m,
-                                          double t, double c, double &acc)
+// T = double, or float for the inner solve of the mixed-precision CG (its
+// pap partials are accumulated in FP64 all the same).
+template <typename T, bool CG>
+__device__ __forceinline__ void gs_finish(T *__restrict__ w, const T *__restrict__ p,
+                                          const uint8_t *__restrict__ flag, int mbit, int m, T t,
+                                          double c, double &acc)
 {
     if constexpr (CG) {
-        const double pm = p[m];
-        double v = t + 0.1 * pm;
-        if (flag[m] & mbit) v = 0.0;
+        const T pm = p[m];
+        T v = t + T(0.1) * pm;
+        if (flag[m] & mbit) v = T(0);
         w[m] = v;
-        acc = acc + v * c * pm;
+        acc = acc + double(v) * c * double(pm);
     } else {
         w[m] = t;
     }
 }
 
-template <int S, bool CG>
-__device__ __forceinline__ void gs_group(double *__restrict__ w, const double *__restrict__ p,
+template <typename T, int S, bool CG>
+__device__ __forceinline__ void gs_group(T *__restrict__ w, const T *__restrict__ p,
                                          const uint8_t *__restrict__ flag, int mbit,
                                          const int32_t *__restrict__ mem, double &acc)
 {
@@ -53,17 +55,17 @@ __device__ __forceinline__ void gs_group(double *__restrict__ w, const double *_
             m[4 * h] = v.x, m[4 * h + 1] = v.y, m[4 * h + 2] = v.z, m[4 * h + 3] = v.w;
         }
     }
-    double t = w[m[0]];
+    T t = w[m[0]];
 #pragma unroll
     for (int q = 1; q < S; ++q) t += w[m[q]];
     const double c = 1. / double(S);
 #pragma unroll
-    for (int q = 0; q < S; ++q) gs_finish<CG>(w, p, flag, mbit, m[q], t, c, acc);
+    for (int q = 0; q < S; ++q) gs_finish<T, CG>(w, p, flag, mbit, m[q], t, c, acc);
 }
 
-template <bool CG>
+template <typename T, bool CG>
 __global__ void __launch_bounds__(kRedThreads)
-    k_gs_cls(double *__restrict__ w, const double *__restrict__ p,
+    k_gs_cls(T *__restrict__ w, const T *__restrict__ p,
              const uint8_t *__restrict__ flag, int mbit, GsArgs G, double *__restrict__ part,
              const CgScalars *__restrict__ scal)
 {
@@ -72,22 +74,22 @@ __global__ void __launch_bounds__(kRedThreads)
     double acc = 0.0;
     if (blk < G.b2) {
         const int64_t g = int64_t(blk) * kRedThreads + threadIdx.x;
-        if (g < G.n2) gs_group<2, CG>(w, p, flag, mbit, G.m2 + 2 * g, acc);
+        if (g < G.n2) gs_group<T, 2, CG>(w, p, flag, mbit, G.m2 + 2 * g, acc);
     } else if ((blk -= G.b2) < G.b4) {
         const int64_t g = int64_t(blk) * kRedThreads + threadIdx.x;
-        if (g < G.n4) gs_group<4, CG>(w, p, flag, mbit, G.m4 + 4 * g, acc);
+        if (g < G.n4) gs_group<T, 4, CG>(w, p, flag, mbit, G.m4 + 4 * g, acc);
     } else if ((blk -= G.b4) < G.b8) {
         const int64_t g = int64_t(blk) * kRedThreads + threadIdx.x;
-        if (g < G.n8) gs_group<8, CG>(w, p, flag, mbit, G.m8 + 8 * g, acc);
+        if (g < G.n8) gs_group<T, 8, CG>(w, p, flag, mbit, G.m8 + 8 * g, acc);
     } else {
         blk -= G.b8;
         const int64_t g = int64_t(blk) * kRedThreads + threadIdx.x;
         if (g < G.nr) {
             const int a = G.roff[g], b = G.roff[g + 1];
-            double t = w[G.ridx[a]];
+            T t = w[G.ridx[a]];
             for (int q = a + 1; q < b; ++q) t += w[G.ridx[q]];
             const double c = 1. / double(b - a);
-            for (int q = a; q < b; ++q) gs_finish<CG>(w, p, flag, mbit, G.ridx[q], t, c, acc);
+            for (int q = a; q < b; ++q) gs_finish<T, CG>(w, p, flag, mbit, G.ridx[q], t, c, acc);
         }
     }
     if constexpr (CG) {
@@ -375,7 +377,7 @@ int launch_gs(nekbone_problem *P, double *u, const CgScalars *guard)
     const int nb = gs_blocks(P);
     if (nb == 0) return NEKBONE_OK;
     if (int rc = timed_begin(P, 1)) return rc;
-    hipLaunchKernelGGL(k_gs_cls<false>, dim3(nb), dim3(kRedThreads), 0, stream(), u,
+    hipLaunchKernelGGL((k_gs_cls<double, false>), dim3(nb), dim3(kRedThreads), 0, stream(), u,
                        static_cast<const double *>(nullptr), static_cast<const uint8_t *>(nullptr),
                        0, gs_args(P), static_cast<double *>(nullptr), guard);
     NB_HIP(hipGetLastError());
@@ -387,10 +389,35 @@ int launch_gs_cg(nekbone_problem *P, double *w, const double *p, int mbit, doubl
     const int nb = gs_blocks(P);
     if (nb == 0) return NEKBONE_OK;
     if (int rc = timed_begin(P, 1)) return rc;
-    hipLaunchKernelGGL(k_gs_cls<true>, dim3(nb), dim3(kRedThreads), 0, stream(), w, p, P->d_flag,
-                       mbit, gs_args(P), part, static_cast<const CgScalars *>(P->d_scal));
+    hipLaunchKernelGGL((k_gs_cls<double, true>), dim3(nb), dim3(kRedThreads), 0, stream(), w, p,
+                       P->d_flag, mbit, gs_args(P), part,
+                       static_cast<const CgScalars *>(P->d_scal));
     NB_HIP(hipGetLastError());
     return timed_end(P, 1);
+}
+
+int launch_gs_cg32(nekbone_problem *P, float *w, const float *p, int mbit, double *part)
+{
+    const int nb = gs_blocks(P);
+    if (nb == 0) return NEKBONE_OK;
+    if (int rc = timed_begin(P, 1)) return rc;
+    hipLaunchKernelGGL((k_gs_cls<float, true>), dim3(nb), dim3(kRedThreads), 0, stream(), w, p,
+                       P->d_flag, mbit, gs_args(P), part,
+                       static_cast<const CgScalars *>(P->d_scal));
+    NB_HIP(hipGetLastError());
+    return timed_end(P, 1);
+}
+
+int launch_gs32(nekbone_problem *P, float *u)
+{
+    const int nb = gs_blocks(P);
+    if (nb == 0) return NEKBONE_OK;
+    hipLaunchKernelGGL((k_gs_cls<float, false>), dim3(nb), dim3(kRedThreads), 0, stream(), u,
+                       static_cast<const float *>(nullptr), static_cast<const uint8_t *>(nullptr),
+                       0, gs_args(P), static_cast<double *>(nullptr),
+                       static_cast<const CgScalars *>(nullptr));
+    NB_HIP(hipGetLastError());
+    return NEKBONE_OK;
 }
 
 int launch_post(nekbone_problem *P, double *w, const double *p, int mbit, bool pap)

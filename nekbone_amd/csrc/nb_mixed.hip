@@ -1,0 +1,467 @@
+// Mixed-precision CG: FP32 inner iterations with FP64 reliable updates.
+// There is no counterpart in the reference, whose CG (src/cg.f:2-84) is
+// FP64 throughout; the solve stops on the same criterion as
+// nekbone_cg_tol_d, evaluated on the FP64 residual of the FP64 operator.
+//
+// Algorithm (reliable updates, Clark et al., "Solving lattice QCD systems
+// of equations using mixed precision solvers on GPUs", 2010, applied to
+// Jacobi-preconditioned CG):
+//   x64 = 0, r64 = mask(f), rtr0 = (r64, c r64)          FP64
+//   r32 = r64, x32 = 0, p32 = 0
+//   repeat  (FP32 vectors and operator; dot products accumulated in FP64)
+//     z = M^-1 r32, beta = rtz/rtz_prev, p32 = z + beta p32
+//     w32 = A p32, alpha = rtz / (w32, c p32)
+//     x32 += alpha p32, r32 -= alpha w32
+//     when (r32, c r32) <= delta^2 * rtr_upd, or it claims convergence:
+//       x64 += x32, x32 = 0
+//       r64 = mask(f - A x64) with the FP64 operator, r32 = r64,
+//       rtr_upd = (r64, c r64); rtz recomputed from r64 (p32 is kept)
+//       stop once rtr_upd <= tol^2 rtr0
+// The inner iteration runs the fused kernel sequence of the FP64 solver
+// instantiated for float: element kernel (p update, A_L, x update one
+// iteration late), gather-scatter with shift/mask/pap, and the update of
+// r with both dot products, all without a host round trip.  The host
+// checks the device scalars once per chunk of iterations.
+#include <algorithm>
+#include <cstring>
+
+#include "nb_ax_kernel.h"
+#include "nb_device.h"
+
+namespace nb {
+namespace {
+
+using axk::AxArgsT;
+using axk::AxFn32;
+using axk::Key32;
+using axk::Table32;
+
+Table32 &table32()
+{
+    static Table32 t;
+    static bool done = false;
+    if (!done) {
+        done = true;
+        axk::reg32_low(t);
+        axk::reg32_high(t);
+    }
+    return t;
+}
+
+AxFn32 lookup32(int nx, int epb, int mode, int pf, int xu)
+{
+    auto it = table32().find(Key32(nx, epb, mode, pf, mode == 0 ? 0 : xu));
+    return it == table32().end() ? nullptr : it->second;
+}
+
+constexpr int kThreads = 256;
+constexpr int kFinalThreads = 1024;
+int grid(int64_t n) { return int((n + kThreads - 1) / kThreads); }
+
+__device__ __forceinline__ double inv_mult(uint8_t fl)
+{
+    const int m = fl >> kMultShift;
+    return m == 1 ? 1.0 : m == 2 ? 0.5 : m == 4 ? 0.25 : m == 8 ? 0.125 : 1. / double(m);
+}
+
+__global__ void k_to_f32(float *__restrict__ dst, const double *__restrict__ src, int64_t n)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = float(src[i]);
+}
+
+// Reference point order <-> device order for float vectors (perm gives
+// the device position of each reference point within an element).
+template <bool TO_DEV>
+__global__ void k_perm32(float *__restrict__ dst, const float *__restrict__ src,
+                         const int32_t *__restrict__ perm, int n3, int64_t n)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t e = i / n3;
+    const int64_t d = e * n3 + perm[i - e * n3];
+    if constexpr (TO_DEV)
+        dst[d] = src[i];
+    else
+        dst[i] = src[d];
+}
+
+// Start of the inner iterations: r32 = r64, x32 = p32 = 0; the FP64
+// residual of the initial guess is the first reliable residual.
+__global__ void k_mx_start(float *__restrict__ r32, float *__restrict__ x32,
+                           float *__restrict__ p32, const double *__restrict__ r64, int64_t n,
+                           CgScalars *__restrict__ s)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i == 0) s->rtr_upd = s->rtr;
+    if (i >= n) return;
+    r32[i] = float(r64[i]);
+    x32[i] = 0.0f;
+    p32[i] = 0.0f;
+}
+
+// r32 -= alpha w32; [XP] x32 += alpha p32; parts of rtr = (r c r) and of
+// rtz = (r c z), z = M^-1 r, accumulated in FP64.
+template <int PRE, bool XP>
+__global__ void __launch_bounds__(kRedThreads)
+    k_mx_update(float *__restrict__ r, const float *__restrict__ w,
+                const float *__restrict__ dinv, const uint8_t *__restrict__ flag,
+                float *__restrict__ x, const float *__restrict__ p, int64_t n,
+                double *__restrict__ part, int nblk, const CgScalars *__restrict__ s)
+{
+    if (s->done) return;
+    const float alpha = float(s->alpha);
+    double a0 = 0.0, a1 = 0.0;
+    constexpr int U = 4;
+    const int64_t stride = int64_t(gridDim.x) * kRedThreads * U;
+    for (int64_t i0 = int64_t(blockIdx.x) * kRedThreads * U + threadIdx.x; i0 < n; i0 += stride) {
+        float rv[U], wv[U], dv[U], xv[U], pv[U];
+        uint8_t fv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t i = i0 + int64_t(u) * kRedThreads;
+            if (i < n) {
+                rv[u] = r[i];
+                wv[u] = w[i];
+                fv[u] = flag[i];
+                if constexpr (PRE == NEKBONE_PRECOND_JACOBI) dv[u] = dinv[i];
+                if constexpr (XP) {
+                    xv[u] = x[i];
+                    pv[u] = p[i];
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t i = i0 + int64_t(u) * kRedThreads;
+            if (i < n) {
+                if constexpr (XP) x[i] = __builtin_fmaf(alpha, pv[u], xv[u]);
+                const float ri = __builtin_fmaf(-alpha, wv[u], rv[u]);
+                r[i] = ri;
+                const double rd = double(ri);
+                const double rc = rd * inv_mult(fv[u]);
+                a0 = a0 + rc * rd;
+                if constexpr (PRE == NEKBONE_PRECOND_JACOBI) a1 = a1 + rc * double(ri * dv[u]);
+            }
+        }
+    }
+    block_sum2(a0, a1);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = a0;
+        part[nblk + blockIdx.x] = a1;
+    }
+}
+
+// Reliable update, first step: apply a pending x32 += alpha p32 (the
+// element kernel defers it by one iteration), then x64 += x32, x32 = 0.
+__global__ void k_mx_xflush(double *__restrict__ x64, float *__restrict__ x32,
+                            const float *__restrict__ p32, int64_t n,
+                            const CgScalars *__restrict__ s)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float xi = x32[i];
+    if (s->xpending) xi = __builtin_fmaf(float(s->alpha), p32[i], xi);
+    x64[i] = x64[i] + double(xi);
+    x32[i] = 0.0f;
+}
+
+// Reliable update, residual: r64 = mask(f - (A_L x64 assembled + 0.1 x64))
+// (the FP64 operator of src/cg.f:96-125), r32 = r64, parts of
+// rtr = (r c r) and rtz = (r c M^-1 r).
+template <int PRE>
+__global__ void __launch_bounds__(kRedThreads)
+    k_mx_resid(double *__restrict__ r64, float *__restrict__ r32, const double *__restrict__ f,
+               const double *__restrict__ w, const double *__restrict__ x,
+               const double *__restrict__ dinv, const uint8_t *__restrict__ flag, int mbit,
+               int64_t n, double *__restrict__ part, int nblk)
+{
+    double a0 = 0.0, a1 = 0.0;
+    const int64_t stride = int64_t(gridDim.x) * kRedThreads;
+    for (int64_t i = int64_t(blockIdx.x) * kRedThreads + threadIdx.x; i < n; i += stride) {
+        const uint8_t fl = flag[i];
+        double v = f[i] - (w[i] + 0.1 * x[i]);
+        if (fl & mbit) v = 0.0;
+        r64[i] = v;
+        r32[i] = float(v);
+        const double rc = v * inv_mult(fl);
+        a0 = a0 + rc * v;
+        if constexpr (PRE == NEKBONE_PRECOND_JACOBI) a1 = a1 + rc * (v * dinv[i]);
+    }
+    block_sum2(a0, a1);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = a0;
+        part[nblk + blockIdx.x] = a1;
+    }
+}
+
+template <int NT>
+__device__ __forceinline__ double fixed_sum(const double *__restrict__ a, int64_t na)
+{
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < na; i += NT) s = s + a[i];
+    return s;
+}
+
+// The replaced residual becomes the recurrence's: rtr, rtr_upd, and the
+// rtz of this iteration (beta against the previous rtz), convergence.
+template <int PRE>
+__global__ void __launch_bounds__(kFinalThreads)
+    k_mx_resid_final(const double *__restrict__ part, int nblk, CgScalars *__restrict__ s)
+{
+    double rtr = fixed_sum<kFinalThreads>(part, nblk);
+    double rtz = fixed_sum<kFinalThreads>(part + nblk, nblk);
+    block_sum2<kFinalThreads>(rtr, rtz);
+    if (threadIdx.x == 0) {
+        if (PRE != NEKBONE_PRECOND_JACOBI) rtz = rtr;
+        s->rtr = rtr;
+        s->rtr_upd = rtr;
+        s->rtz1 = rtz;
+        s->beta = (s->iter == 0) ? 0.0 : rtz / s->rtz2;
+        s->xpending = 0;
+        s->done = (rtr <= s->tol2 * s->rtr0) ? 1 : 0;
+        if (!isfinite(rtr)) s->status = 1;
+    }
+}
+
+int upd_blocks(int64_t n)
+{
+    const int64_t b = (n + int64_t(kRedThreads) * 16 - 1) / (int64_t(kRedThreads) * 16);
+    return int(std::max<int64_t>(1, std::min<int64_t>(b, 8192)));
+}
+
+int ax32_blocks(const nekbone_problem *P) { return (P->nelt + P->mx->axv.epb - 1) / P->mx->axv.epb; }
+
+// FP32 operator data, rebuilt when the FP64 originals changed.
+int ensure_mixed(nekbone_problem *P, int precond)
+{
+    if (!P->mx) {
+        auto *M = new MixedState();
+        P->mx = M;
+        const int nx = P->nx1;
+        if (int rc = dalloc(&M->g, 6 * P->n)) return rc;
+        if (int rc = dalloc(&M->Drow, int64_t(nx) * nx)) return rc;
+        if (int rc = dalloc(&M->Dcol, int64_t(nx) * nx)) return rc;
+        for (float **v : {&M->x, &M->r, &M->p, &M->w})
+            if (int rc = dalloc(v, P->n)) return rc;
+        std::vector<float> dr(size_t(nx) * nx), dc(size_t(nx) * nx);
+        for (int i = 0; i < nx; ++i)
+            for (int m = 0; m < nx; ++m) {
+                dr[size_t(i * nx + m)] = float(P->dxm1[i + nx * m]);  // D(i,m)
+                dc[size_t(i * nx + m)] = float(P->dxm1[m + nx * i]);  // D(m,i)
+            }
+        NB_HIP(hipMemcpy(M->Drow, dr.data(), sizeof(float) * dr.size(), hipMemcpyHostToDevice));
+        NB_HIP(hipMemcpy(M->Dcol, dc.data(), sizeof(float) * dc.size(), hipMemcpyHostToDevice));
+        // default FP32 shape: the FP64 default for this order
+        M->axv = ax_default_variant(nx);
+        M->axv.xu = 1;
+        if (!lookup32(nx, M->axv.epb, 2, M->axv.pf, 1)) {
+            M->axv.epb = axk::epb_for(nx);
+            M->axv.pf = 0;
+        }
+    }
+    MixedState *M = P->mx;
+    if (M->geom_version != P->geom_version) {
+        hipLaunchKernelGGL(k_to_f32, dim3(grid(6 * P->n)), dim3(kThreads), 0, stream(), M->g,
+                           P->d_g, 6 * P->n);
+        NB_HIP(hipGetLastError());
+        M->geom_version = P->geom_version;
+    }
+    if (precond == NEKBONE_PRECOND_JACOBI && M->dinv_version != P->dinv_version) {
+        if (!M->dinv)
+            if (int rc = dalloc(&M->dinv, P->n)) return rc;
+        hipLaunchKernelGGL(k_to_f32, dim3(grid(P->n)), dim3(kThreads), 0, stream(), M->dinv,
+                           P->d_dinv, P->n);
+        NB_HIP(hipGetLastError());
+        M->dinv_version = P->dinv_version;
+    }
+    return NEKBONE_OK;
+}
+
+int launch_ax32(nekbone_problem *P, int mode, int mbit, float *w, const float *u)
+{
+    MixedState *M = P->mx;
+    AxFn32 fn = lookup32(P->nx1, M->axv.epb, mode, M->axv.pf, M->axv.xu);
+    if (!fn) return fail(NEKBONE_ERR_UNSUPPORTED, "mixed: no FP32 element kernel for this order");
+    AxArgsT<float> A{};
+    A.w = w;
+    A.u = u;
+    A.p = M->p;
+    A.r = M->r;
+    A.dinv = M->dinv;
+    A.x = M->x;
+    A.g = M->g;
+    A.Drow = M->Drow;
+    A.Dcol = M->Dcol;
+    A.flag = P->d_flag;
+    A.perm = P->d_perm;
+    A.mbit = mbit;
+    A.scal = P->d_scal;
+    A.part = P->d_part;
+    A.nelt = P->nelt;
+    const int threads = axk::round64(P->nx1 * P->nx1 * M->axv.epb);
+    if (int rc = timed_begin(P, 0)) return rc;
+    hipLaunchKernelGGL(fn, dim3(ax32_blocks(P)), dim3(threads), 0, stream(), A);
+    NB_HIP(hipGetLastError());
+    return timed_end(P, 0);
+}
+
+int iteration32(nekbone_problem *P, int precond, int mbit)
+{
+    MixedState *M = P->mx;
+    const bool jac = precond == NEKBONE_PRECOND_JACOBI;
+    const int64_t na = ax32_blocks(P), ng = gs_blocks(P);
+    const int nu = upd_blocks(P->n);
+    double *part_upd = P->d_part + na + ng;
+    if (int rc = launch_ax32(P, jac ? 2 : 1, mbit, M->w, nullptr)) return rc;
+    if (int rc = launch_gs_cg32(P, M->w, M->p, mbit, P->d_part + na)) return rc;
+    if (int rc = launch_pap_final(P, P->d_part, na + ng)) return rc;
+    if (int rc = timed_begin(P, 3)) return rc;
+    const bool xp = !M->axv.xu;
+#define NB_MXU(PRE, XP)                                                                         \
+    hipLaunchKernelGGL((k_mx_update<PRE, XP>), dim3(nu), dim3(kRedThreads), 0, stream(), M->r, \
+                       M->w, M->dinv, P->d_flag, M->x, M->p, P->n, part_upd, nu, P->d_scal)
+    if (jac) {
+        if (xp)
+            NB_MXU(NEKBONE_PRECOND_JACOBI, true);
+        else
+            NB_MXU(NEKBONE_PRECOND_JACOBI, false);
+    } else {
+        if (xp)
+            NB_MXU(NEKBONE_PRECOND_IDENTITY, true);
+        else
+            NB_MXU(NEKBONE_PRECOND_IDENTITY, false);
+    }
+#undef NB_MXU
+    NB_HIP(hipGetLastError());
+    if (int rc = timed_end(P, 3)) return rc;
+    return launch_update_final(P, part_upd, nu, precond, M->axv.xu);
+}
+
+// x64 += x32; r64 = mask(f - A x64) in FP64; the recurrence restarts from
+// it (p32 kept).
+int reliable_update(nekbone_problem *P, int precond, int mbit)
+{
+    MixedState *M = P->mx;
+    hipLaunchKernelGGL(k_mx_xflush, dim3(grid(P->n)), dim3(kThreads), 0, stream(), P->d_x, M->x,
+                       M->p, P->n, static_cast<const CgScalars *>(P->d_scal));
+    NB_HIP(hipGetLastError());
+    if (int rc = launch_ax_local(P, P->d_w, P->d_x)) return rc;
+    if (int rc = launch_gs(P, P->d_w, nullptr)) return rc;
+    const int nb = reduce_blocks(P->n);
+    if (precond == NEKBONE_PRECOND_JACOBI) {
+        hipLaunchKernelGGL(k_mx_resid<NEKBONE_PRECOND_JACOBI>, dim3(nb), dim3(kRedThreads), 0,
+                           stream(), P->d_r, M->r, P->d_f, P->d_w, P->d_x, P->d_dinv, P->d_flag,
+                           mbit, P->n, P->d_part, nb);
+        hipLaunchKernelGGL(k_mx_resid_final<NEKBONE_PRECOND_JACOBI>, dim3(1),
+                           dim3(kFinalThreads), 0, stream(), P->d_part, nb, P->d_scal);
+    } else {
+        hipLaunchKernelGGL(k_mx_resid<NEKBONE_PRECOND_IDENTITY>, dim3(nb), dim3(kRedThreads), 0,
+                           stream(), P->d_r, M->r, P->d_f, P->d_w, P->d_x,
+                           static_cast<const double *>(nullptr), P->d_flag, mbit, P->n,
+                           P->d_part, nb);
+        hipLaunchKernelGGL(k_mx_resid_final<NEKBONE_PRECOND_IDENTITY>, dim3(1),
+                           dim3(kFinalThreads), 0, stream(), P->d_part, nb, P->d_scal);
+    }
+    NB_HIP(hipGetLastError());
+    return NEKBONE_OK;
+}
+
+int read_scalars(CgScalars &s, const nekbone_problem *P)
+{
+    NB_HIP(hipMemcpyAsync(&s, P->d_scal, sizeof s, hipMemcpyDeviceToHost, stream()));
+    NB_HIP(hipStreamSynchronize(stream()));
+    return NEKBONE_OK;
+}
+
+}  // namespace
+
+void free_mixed(nekbone_problem *P)
+{
+    if (!P->mx) return;
+    MixedState *M = P->mx;
+    for (float **v : {&M->g, &M->Drow, &M->Dcol, &M->dinv, &M->x, &M->r, &M->p, &M->w}) dfree(*v);
+    delete M;
+    P->mx = nullptr;
+}
+
+}  // namespace nb
+
+using namespace nb;
+
+extern "C" int nekbone_cg_mixed_d(nekbone_problem *P, double *x, const double *f, double *r,
+                                  double tol, int maxiter, int precond, double delta, int *iters,
+                                  int *updates, double *final_rtr)
+{
+    if (int rc = check(P)) return rc;
+    if (!(tol > 0.0)) return fail(NEKBONE_ERR_ARG, "cg_mixed: tol must be > 0");
+    if (maxiter < 0) return fail(NEKBONE_ERR_ARG, "cg_mixed: maxiter < 0");
+    if (delta == 0.0) delta = 1e-3;
+    if (!(delta > 0.0 && delta < 1.0)) return fail(NEKBONE_ERR_ARG, "cg_mixed: need 0 < delta < 1");
+    if (precond != NEKBONE_PRECOND_JACOBI && precond != NEKBONE_PRECOND_IDENTITY)
+        return fail(NEKBONE_ERR_UNSUPPORTED, "cg_mixed: preconditioner not available");
+    if (precond == NEKBONE_PRECOND_JACOBI && !P->jacobi)
+        return fail(NEKBONE_ERR_STATE, "cg_mixed: Jacobi preconditioner not set up");
+    if (!P->c_is_mult)
+        return fail(NEKBONE_ERR_UNSUPPORTED,
+                    "cg_mixed: needs c = 1/multiplicity (the inner kernels derive c from it)");
+    if (int rc = ensure_mixed(P, precond)) return rc;
+    if (int rc = launch_to_dev(P, P->d_f, f)) return rc;
+    const int mbit0 = next_mask_bit(P);
+    if (int rc = launch_cg_init(P, P->d_x, P->d_r, P->d_f, mbit0, precond, tol * tol)) return rc;
+    const int mbit = next_mask_bit(P);
+    MixedState *M = P->mx;
+    hipLaunchKernelGGL(k_mx_start, dim3(grid(P->n)), dim3(kThreads), 0, stream(), M->r, M->x,
+                       M->p, P->d_r, P->n, P->d_scal);
+    NB_HIP(hipGetLastError());
+
+    const double delta2 = delta * delta;
+    const int chunk = 8;
+    int nupd = 0;
+    CgScalars s{};
+    if (int rc = read_scalars(s, P)) return rc;
+    while (!s.done && s.iter < maxiter && !s.status) {
+        const int todo = std::min(chunk, maxiter - s.iter);
+        for (int k = 0; k < todo; ++k)
+            if (int rc = iteration32(P, precond, mbit)) return rc;
+        if (int rc = read_scalars(s, P)) return rc;
+        if (s.status) break;
+        if (s.done || s.iter >= maxiter || s.rtr <= delta2 * s.rtr_upd) {
+            if (int rc = reliable_update(P, precond, mbit)) return rc;
+            ++nupd;
+            if (int rc = read_scalars(s, P)) return rc;
+        }
+    }
+    if (x)
+        if (int rc = launch_to_ref(P, x, P->d_x)) return rc;
+    if (r)
+        if (int rc = launch_to_ref(P, r, P->d_r)) return rc;
+    NB_HIP(hipStreamSynchronize(stream()));
+    if (iters) *iters = s.iter;
+    if (updates) *updates = nupd;
+    if (final_rtr) *final_rtr = s.rtr;
+    if (s.status) return fail(NEKBONE_ERR_NUMERIC, "cg_mixed: non-positive or non-finite pap");
+    return NEKBONE_OK;
+}
+
+extern "C" int nekbone_ax_local_f32(nekbone_problem *P, float *w, const float *u)
+{
+    if (int rc = check(P)) return rc;
+    if (!w || !u) return fail(NEKBONE_ERR_ARG, "ax_local_f32: null array");
+    if (int rc = ensure_mixed(P, NEKBONE_PRECOND_IDENTITY)) return rc;
+    MixedState *M = P->mx;
+    const int64_t n = P->n;
+    const int n3 = P->nx1 * P->nx1 * P->nx1;
+    // stage through the FP32 work vectors (their CG contents are not kept)
+    NB_HIP(hipMemcpyAsync(M->w, u, sizeof(float) * size_t(n), hipMemcpyHostToDevice, stream()));
+    hipLaunchKernelGGL(k_perm32<true>, dim3(grid(n)), dim3(kThreads), 0, stream(), M->r, M->w,
+                       P->d_perm, n3, n);
+    NB_HIP(hipGetLastError());
+    if (int rc = launch_ax32(P, 0, 0, M->p, M->r)) return rc;
+    hipLaunchKernelGGL(k_perm32<false>, dim3(grid(n)), dim3(kThreads), 0, stream(), M->w, M->p,
+                       P->d_perm, n3, n);
+    NB_HIP(hipGetLastError());
+    NB_HIP(hipMemcpyAsync(w, M->w, sizeof(float) * size_t(n), hipMemcpyDeviceToHost, stream()));
+    NB_HIP(hipStreamSynchronize(stream()));
+    return NEKBONE_OK;
+}

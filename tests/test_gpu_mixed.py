@@ -1,0 +1,135 @@
+"""Mixed-precision CG (FP32 inner iterations with FP64 reliable updates)
+and its FP32 element operator, through the C ABI, against the oracle's
+FP64 operator.
+
+Bars (there is no reference counterpart to be bit-compatible with):
+  * FP32 element operator: within 2e-6 of the FP64 oracle, relative to
+    max|w|, on the same (FP32-rounded) input - FP32 unit roundoff 6e-8
+    times the ~30 rounding steps on a point's path.
+  * Solve: the returned x meets the FP64 stopping rule
+    glsc3(r,c,r) <= tol^2 glsc3(r0,c,r0) when the oracle recomputes
+    r = mask(f - A x) in FP64, the reported final rtr is that residual,
+    and x is as close to the converged FP64 solution as the FP64 solve
+    to the same tolerance (within 10x, or 1e-13 of max|x|)."""
+import numpy as np
+import pytest
+
+from conftest import splitmix_uniform
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(msg):
+    print("CASE", msg, flush=True)
+
+
+def _pair(nb, oracle, nel, nx1, deformed, gseed=5, jacobi=True):
+    P = nb.Problem(*nel, nx1)
+    O = oracle.Problem(*nel, nx1)
+    P.set_mask_mode(nb.MASK_DIRICHLET)
+    O.set_mask_mode(oracle.MASK_DIRICHLET)
+    if deformed:
+        P.set_geometry_deformed(0.1, gseed)
+        O.deformed_g(0.1, gseed)
+    if jacobi:
+        P.setup_jacobi()
+        O.setup_jacobi()
+    return P, O
+
+
+@pytest.mark.parametrize("deformed", [False, True])
+@pytest.mark.parametrize("nel,nx1", [((2, 2, 2), 8), ((3, 2, 2), 10), ((2, 3, 1), 12),
+                                     ((1, 1, 1), 5), ((2, 1, 2), 16)])
+def test_ax_local_f32_vs_fp64_oracle(nb, oracle, nel, nx1, deformed):
+    _case("ax_local_f32 %s %d deformed=%s" % (nel, nx1, deformed))
+    P, O = _pair(nb, oracle, nel, nx1, deformed, jacobi=False)
+    u32 = splitmix_uniform(P.n, 11).astype(np.float32)
+    w32 = P.ax_local_f32(u32)
+    u = u32.astype(np.float64)
+    n3 = nx1 ** 3
+    w64 = np.concatenate([O.ax_e(e, u[e * n3:(e + 1) * n3]) for e in range(O.nelt)])
+    scale = np.abs(w64).max()
+    assert np.abs(w32.astype(np.float64) - w64).max() <= 2e-6 * scale
+
+
+def _converged_oracle(O, f, precond):
+    """The oracle's FP64 CG run to its attainable accuracy: iterations
+    stop before rtr reaches zero (where beta = 0/0 would follow)."""
+    x, r, h = O.cg(f, 1500, precond)
+    rtr = h[3::3]
+    bad = np.nonzero(~np.isfinite(rtr) | (rtr <= 1e-30 * h[0]))[0]
+    if bad.size:
+        x, r, h = O.cg(f, max(int(bad[0]) - 1, 1), precond)
+    assert np.all(np.isfinite(x))
+    return x
+
+
+def _fp64_tol_solve(nb, P, f, tol, precond):
+    fv, xv, rv = P.vec(), P.vec(), P.vec()
+    fv.upload(f)
+    it, _ = P.cg_tol_d(xv, fv, rv, tol, 5000, precond)
+    x = xv.download()
+    for v in (fv, xv, rv):
+        v.free()
+    return x, it
+
+
+@pytest.mark.parametrize("precond", [1, 0])
+@pytest.mark.parametrize("nel,nx1,deformed", [((4, 3, 2), 8, True), ((3, 3, 3), 10, True),
+                                              ((4, 4, 2), 6, False)])
+def test_cg_mixed_meets_fp64_criterion(nb, oracle, nel, nx1, deformed, precond):
+    _case("cg_mixed %s %d deformed=%s precond=%d" % (nel, nx1, deformed, precond))
+    tol = 1e-10
+    P, O = _pair(nb, oracle, nel, nx1, deformed, jacobi=bool(precond))
+    f = O.rhs_from_raw(splitmix_uniform(O.n, 2))
+    x, r, it, nupd, rtr = P.cg_mixed(f, tol, 3000, precond)
+    assert nupd >= 1 and it > 0
+    c = O.c()
+    f0 = O.maskit(f)
+    rtrue = f0 - O.ax(x)
+    rr = float(np.sum(rtrue * c * rtrue))
+    rr0 = float(np.sum(f0 * c * f0))
+    assert rr <= tol ** 2 * rr0 * 1.0001
+    assert abs(rtr - rr) <= 1e-6 * rr
+    np.testing.assert_allclose(r, rtrue, rtol=0, atol=1e-9 * np.abs(f0).max())
+    # converged FP64 solution, and the FP64 solve to the same tolerance
+    xs = _converged_oracle(O, f, precond)
+    x64, it64 = _fp64_tol_solve(nb, P, f, tol, precond)
+    err = np.abs(x - xs).max()
+    err64 = np.abs(x64 - xs).max()
+    assert err <= max(10 * err64, 1e-13 * np.abs(xs).max())
+    print("iters mixed %d (+%d FP64 updates), fp64 %d; err %.2e vs %.2e" %
+          (it, nupd, it64, err, err64))
+
+
+def test_cg_mixed_maxiter_and_geometry_change(nb, oracle):
+    _case("cg_mixed maxiter / geometry change")
+    P, O = _pair(nb, oracle, (3, 2, 2), 8, True, gseed=1)
+    f = O.rhs_from_raw(splitmix_uniform(O.n, 6))
+    # a tolerance it cannot reach: stops at maxiter with x64 and r64 current
+    x, r, it, nupd, rtr = P.cg_mixed(f, 1e-30, 25, 1)
+    assert it == 25 and nupd >= 1
+    f0 = O.maskit(f)
+    rtrue = f0 - O.ax(x)
+    assert abs(rtr - float(np.sum(rtrue * O.c() * rtrue))) <= 1e-6 * rtr
+    # new geometry and diagonal: the FP32 copies must follow
+    P.set_geometry_deformed(0.1, 9)
+    O.deformed_g(0.1, 9)
+    P.setup_jacobi()
+    O.setup_jacobi()
+    x, r, it, nupd, rtr = P.cg_mixed(f, 1e-10, 3000, 1)
+    rtrue = f0 - O.ax(x)
+    rr = float(np.sum(rtrue * O.c() * rtrue))
+    assert rr <= 1e-20 * float(np.sum(f0 * O.c() * f0)) * 1.0001
+
+
+def test_cg_mixed_rejects_bad_arguments(nb):
+    _case("cg_mixed arguments")
+    P = nb.Problem(2, 2, 2, 6)
+    f = np.zeros(P.n)
+    with pytest.raises(nb.NekboneError):
+        P.cg_mixed(f, 1e-10, 10, nb.PRECOND_JACOBI)  # Jacobi not set up
+    with pytest.raises(nb.NekboneError):
+        P.cg_mixed(f, 1e-10, 10, nb.PRECOND_IDENTITY, delta=2.0)
+    with pytest.raises(nb.NekboneError):
+        P.cg_mixed(f, -1.0, 10, nb.PRECOND_IDENTITY)
