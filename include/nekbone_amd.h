@@ -179,6 +179,12 @@ int nekbone_cg_tol_d(nekbone_problem *P, double *x, const double *f, double *r, 
 int nekbone_cg_mixed_d(nekbone_problem *P, double *x, const double *f, double *r, double tol,
                        int maxiter, int precond, double delta, int *iters, int *updates,
                        double *final_rtr);
+/* FP32 element-kernel shape of the mixed solver (as nekbone_get_variant /
+ * nekbone_set_variant), and its autotuner: times every compiled shape over
+ * a few inner iterations and keeps the fastest for this problem. */
+int nekbone_get_variant_f32(nekbone_problem *P, int *epb, int *pf, int *xu);
+int nekbone_set_variant_f32(nekbone_problem *P, int epb, int pf, int xu);
+int nekbone_autotune_mixed(nekbone_problem *P, int precond, double *best_ms);
 /* The FP32 element operator of the mixed solver, w = A_L u element by
  * element (host float arrays, reference point order, n each). */
 int nekbone_ax_local_f32(nekbone_problem *P, float *w, const float *u);

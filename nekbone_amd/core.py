@@ -71,6 +71,9 @@ _SIGS = {
                                           ctypes.POINTER(ctypes.c_int),
                                           ctypes.POINTER(ctypes.c_int), _dp]),
     "nekbone_ax_local_f32": (ctypes.c_int, [_vp, _fp, _fp]),
+    "nekbone_get_variant_f32": (ctypes.c_int, [_vp] + [ctypes.POINTER(ctypes.c_int)] * 3),
+    "nekbone_set_variant_f32": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "nekbone_autotune_mixed": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     "nekbone_alloc_vec": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
     "nekbone_free_vec": (ctypes.c_int, [_vp, _vp]),
     "nekbone_h2d": (ctypes.c_int, [_vp, _vp, _dp, ctypes.c_int64]),
@@ -349,6 +352,19 @@ class Problem:
         finally:
             for v in (fv, xv, rv):
                 v.free()
+
+    def variant_f32(self):
+        v = [ctypes.c_int() for _ in range(3)]
+        _chk(lib().nekbone_get_variant_f32(self.handle, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def set_variant_f32(self, epb, pf, xu):
+        _chk(lib().nekbone_set_variant_f32(self.handle, epb, pf, xu))
+
+    def autotune_mixed(self, precond=PRECOND_JACOBI):
+        ms = ctypes.c_double()
+        _chk(lib().nekbone_autotune_mixed(self.handle, precond, ctypes.byref(ms)))
+        return ms.value
 
     def ax_local_f32(self, u):
         u = np.ascontiguousarray(u, dtype=np.float32)

@@ -202,6 +202,8 @@ static int finish_problem(nekbone_problem *P, const double *c_host)
 {
     const int64_t n = P->n;
     const int nx = P->nx1;
+    if (!surface_order_consistent(nx))
+        return fail(NEKBONE_ERR_STATE, "internal: device point order is not a permutation");
     boundary_first_perm(nx, P->perm);
     if (int rc = dalloc(&P->d_perm, int64_t(P->perm.size()))) return rc;
     NB_HIP(hipMemcpy(P->d_perm, P->perm.data(), sizeof(int32_t) * P->perm.size(),

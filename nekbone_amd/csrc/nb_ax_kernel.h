@@ -49,7 +49,7 @@
 #include <map>
 #include <tuple>
 
-#include "nb_internal.h"
+#include "nb_device.h"
 
 namespace nb {
 namespace axk {
@@ -135,75 +135,65 @@ __global__ void __launch_bounds__(round64(NX *NX *EPB)) k_ax(AxArgsT<T> A_)
         }
     }
 
-    // Device positions of the thread's t-line points (reference point
-    // a + NX*b + N2*k sits at base + off[k]): the boundary-first order of
-    // boundary_first_perm in closed form.  Surface points in reference
-    // order: slab k=0 (N2 points), then the ring of each middle slab
-    // (R = N2 - (NX-2)^2 points: row 0, two points per middle row, last
-    // row), then slab k=NX-1; interior points follow in reference order.
-    int off[NX];
-    {
-        constexpr int I = NX - 2, R = N2 - I * I, NB = N3 - I * I * I;
-        const bool lat = a == 0 || b == 0 || a == NX - 1 || b == NX - 1;
-        const int ring = b == 0 ? a : b == NX - 1 ? NX + 2 * I + a : NX + 2 * (b - 1) + (a != 0);
-        const int inner = NB + (a - 1) + I * (b - 1);
+    // Linear phase: the element's N3 points in device order are split as
+    // d = ab + N2*m, m = 0..NX-1 (N3 = NX*N2), so every global access of
+    // the per-point vectors is one contiguous run per wave whatever the
+    // device order is.  ref[m] is the reference position of point d.
+    int ref[NX];
 #pragma unroll
-        for (int k = 0; k < NX; ++k) {
-            if (k == 0)
-                off[k] = a + NX * b;
-            else if (k == NX - 1)
-                off[k] = N2 + I * R + a + NX * b;
-            else
-                off[k] = lat ? N2 + (k - 1) * R + ring : inner + I * I * (k - 1);
-        }
-    }
+    for (int m = 0; m < NX; ++m) ref[m] = surface_order_ref(NX, ab + N2 * m);
 
-    T uc[NX];  // the thread's t-line of the input
+    T pd[MODE != 0 ? NX : 1];  // CG: the new direction at the linear points
     if (act) {
         if constexpr (MODE == 0) {
 #pragma unroll
-            for (int k = 0; k < NX; ++k) uc[k] = A_.u[base + off[k]];
+            for (int m = 0; m < NX; ++m) A[ref[m]] = A_.u[base + ab + N2 * m];
         } else {
             const T beta = T(A_.scal->beta);
             T po[NX], z[NX];
 #pragma unroll
-            for (int k = 0; k < NX; ++k) {
-                const int64_t q = base + off[k];
-                po[k] = A_.p[q];
-                z[k] = A_.r[q];
+            for (int m = 0; m < NX; ++m) {
+                const int64_t q = base + ab + N2 * m;
+                po[m] = A_.p[q];
+                z[m] = A_.r[q];
             }
             if constexpr (MODE == 2) {
 #pragma unroll
-                for (int k = 0; k < NX; ++k) z[k] = z[k] * A_.dinv[base + off[k]];
+                for (int m = 0; m < NX; ++m) z[m] = z[m] * A_.dinv[base + ab + N2 * m];
             }
             if constexpr (XU) {
                 if (A_.scal->xpending) {  // the previous iteration's alpha p
                     const T alpha = T(A_.scal->alpha);
 #pragma unroll
-                    for (int k = 0; k < NX; ++k) {
-                        const int64_t q = base + off[k];
+                    for (int m = 0; m < NX; ++m) {
+                        const int64_t q = base + ab + N2 * m;
                         if constexpr (FMA)
-                            A_.x[q] = __builtin_fma(alpha, po[k], A_.x[q]);
+                            A_.x[q] = __builtin_fma(alpha, po[m], A_.x[q]);
                         else
-                            A_.x[q] = A_.x[q] + alpha * po[k];
+                            A_.x[q] = A_.x[q] + alpha * po[m];
                     }
                 }
             }
 #pragma unroll
-            for (int k = 0; k < NX; ++k) {
+            for (int m = 0; m < NX; ++m) {
                 T pn;
                 if constexpr (FMA)
-                    pn = __builtin_fma(beta, po[k], z[k]);
+                    pn = __builtin_fma(beta, po[m], z[m]);
                 else
-                    pn = beta * po[k] + z[k];
-                A_.p[base + off[k]] = pn;
-                uc[k] = pn;
+                    pn = beta * po[m] + z[m];
+                A_.p[base + ab + N2 * m] = pn;
+                pd[m] = pn;
+                A[ref[m]] = pn;
             }
         }
-#pragma unroll
-        for (int k = 0; k < NX; ++k) A[a + NX * b + N2 * k] = uc[k];
     }
     __syncthreads();
+
+    T uc[NX];  // the thread's t-line of the input
+    if (act) {
+#pragma unroll
+        for (int k = 0; k < NX; ++k) uc[k] = A[a + NX * b + N2 * k];
+    }
 
     T lr[NX], ls[NX];
     if (act) {
@@ -266,38 +256,52 @@ __global__ void __launch_bounds__(round64(NX *NX *EPB)) k_ax(AxArgsT<T> A_)
     }
     __syncthreads();
 
-    double acc = 0.0;
+    // w at the thread's t-line points, kept in place of the consumed
+    // partials (each entry is read and rewritten by its owner only)
     if (act) {
 #pragma unroll
         for (int k = 0; k < NX; ++k) {
             const int q = a + NX * b + N2 * k;
-            const T wl = (B[q] + A[q]) + dot_row<NX, FMA>(Dcol, k, wt);
+            A[q] = (B[q] + A[q]) + dot_row<NX, FMA>(Dcol, k, wt);
+        }
+    }
+    __syncthreads();
+
+    double acc = 0.0;
+    if (act) {
+#pragma unroll
+        for (int m = 0; m < NX; ++m) {
+            const int64_t q = base + ab + N2 * m;
+            const T wl = A[ref[m]];
             if constexpr (MODE == 0) {
-                A_.w[base + off[k]] = wl;
+                A_.w[q] = wl;
             } else {
-                const uint8_t fl = A_.flag[base + off[k]];
+                const uint8_t fl = A_.flag[q];
                 if (fl & kFlagShared) {
-                    A_.w[base + off[k]] = wl;  // finished by the gather-scatter kernel
+                    A_.w[q] = wl;  // finished by the gather-scatter kernel
                 } else {
-                    T v = wl + T(0.1) * uc[k];
+                    T v = wl + T(0.1) * pd[m];
                     if (fl & A_.mbit) v = T(0);
-                    A_.w[base + off[k]] = v;
-                    acc = acc + double(v) * double(uc[k]);
+                    A_.w[q] = v;
+                    acc = acc + double(v) * double(pd[m]);
                 }
             }
         }
     }
     if constexpr (MODE != 0) {
-        // deterministic block sum of the pap partials
-        __shared__ double red[NT];
-        red[tid] = acc;
+        // deterministic block sum of the pap partials: wave butterflies,
+        // then the wave sums in wave order
+        constexpr int NW = NT / 64;
+        __shared__ double red[NW];
+        acc = wave_sum(acc);
+        if ((tid & 63) == 0) red[tid >> 6] = acc;
         __syncthreads();
+        if (tid == 0) {
+            double s = red[0];
 #pragma unroll
-        for (int h = NT / 2; h > 0; h >>= 1) {
-            if (tid < h) red[tid] = red[tid] + red[tid + h];
-            __syncthreads();
+            for (int w = 1; w < NW; ++w) s = s + red[w];
+            A_.part[blockIdx.x] = s;
         }
-        if (tid == 0) A_.part[blockIdx.x] = red[0];
     }
 }
 

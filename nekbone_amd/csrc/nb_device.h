@@ -8,26 +8,44 @@ namespace nb {
 constexpr int kRedThreads = 256;
 constexpr int kMaxRedBlocks = 1024;
 
-// Fixed-shape block tree reduction of two values over NT threads
-// (deterministic: the combination order depends only on thread indices).
+// Sum over the 64 lanes of a wave by a butterfly of cross-lane moves:
+// every lane ends with the same value, combined in an order fixed by the
+// lane indices.
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = v + __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Fixed-shape block reduction of two values over NT threads (a multiple
+// of 64): wave butterflies, then the wave sums in wave order.  The result
+// depends only on thread indices (deterministic); every thread returns
+// it.  One barrier for the exchange, one so that the helper can be used
+// again in the same kernel.
 template <int NT = kRedThreads>
 __device__ __forceinline__ void block_sum2(double &v0, double &v1)
 {
-    __shared__ double s0[NT], s1[NT];
+    static_assert(NT % 64 == 0, "block of whole waves");
+    constexpr int NW = NT / 64;
+    __shared__ double s0[NW], s1[NW];
     const int t = threadIdx.x;
-    s0[t] = v0;
-    s1[t] = v1;
-    __syncthreads();
-#pragma unroll
-    for (int h = NT / 2; h > 0; h >>= 1) {
-        if (t < h) {
-            s0[t] = s0[t] + s0[t + h];
-            s1[t] = s1[t] + s1[t + h];
-        }
-        __syncthreads();
+    v0 = wave_sum(v0);
+    v1 = wave_sum(v1);
+    if ((t & 63) == 0) {
+        s0[t >> 6] = v0;
+        s1[t >> 6] = v1;
     }
-    v0 = s0[0];
-    v1 = s1[0];
+    __syncthreads();
+    double a = s0[0], b = s1[0];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) {
+        a = a + s0[w];
+        b = b + s1[w];
+    }
+    __syncthreads();
+    v0 = a;
+    v1 = b;
 }
 
 }  // namespace nb

@@ -71,8 +71,114 @@ void gs_groups(const int64_t *glo, int64_t n, std::vector<int32_t> &off,
 void mask_lists(int nx, int nelt, int nelx, int nely, int nelz,
                 const std::vector<int32_t> &lglel, std::vector<int32_t> &full,
                 std::vector<int32_t> &ref);
-// Device point order within an element (surface points first).
+// Device point order within an element: surface points first, grouped
+// so that every shared-point group of a hex mesh touches contiguous runs
+// in each element.  Position of reference point (i,j,k), I = nx-2, with
+// bc = 0 for an interior coordinate, 1 on the low side, 2 on the high
+// side:
+//   8 corners            4(bk-1) + 2(bj-1) + (bi-1)
+//   12 edges x I         8 + eps*I + (t-1), t the edge's free coordinate;
+//                        eps = 2(bj-1)+(bi-1) for the 4 k-edges, then per
+//                        k-side 4+4(bk-1) + {bi-1 (j free), 2+bj-1 (i free)}
+//   6 faces x I^2        8 + 12I + phi*I^2 + in-face index, phi = x-, x+,
+//                        y-, y+, z-, z+; in-face index (k-1)I+(j-1) on x
+//                        faces, (k-1)I+(i-1) on y faces, (j-1)I+(i-1) on z
+//   interior             nx^3 - I^3 + (k-1)I^2 + (j-1)I + (i-1)
+// The element kernel evaluates the same function.
+__host__ __device__ inline int surface_order_pos(int nx, int i, int j, int k)
+{
+    const int I = nx - 2;
+    const int bi = i == 0 ? 1 : i == nx - 1 ? 2 : 0;
+    const int bj = j == 0 ? 1 : j == nx - 1 ? 2 : 0;
+    const int bk = k == 0 ? 1 : k == nx - 1 ? 2 : 0;
+    const int nb = (bi != 0) + (bj != 0) + (bk != 0);
+    if (nb == 3) return 4 * (bk - 1) + 2 * (bj - 1) + (bi - 1);
+    if (nb == 2) {
+        int eps, t;
+        if (bk == 0) {
+            eps = 2 * (bj - 1) + (bi - 1);
+            t = k;
+        } else if (bj == 0) {
+            eps = 4 + 4 * (bk - 1) + (bi - 1);
+            t = j;
+        } else {
+            eps = 4 + 4 * (bk - 1) + 2 + (bj - 1);
+            t = i;
+        }
+        return 8 + eps * I + (t - 1);
+    }
+    if (nb == 1) {
+        int phi, loc;
+        if (bi) {
+            phi = bi - 1;
+            loc = (k - 1) * I + (j - 1);
+        } else if (bj) {
+            phi = 2 + bj - 1;
+            loc = (k - 1) * I + (i - 1);
+        } else {
+            phi = 4 + bk - 1;
+            loc = (j - 1) * I + (i - 1);
+        }
+        return 8 + 12 * I + phi * I * I + loc;
+    }
+    return nx * nx * nx - I * I * I + (k - 1) * I * I + (j - 1) * I + (i - 1);
+}
+// Inverse of surface_order_pos: the reference position i + nx*j + nx^2*k
+// of device position d.
+__host__ __device__ inline int surface_order_ref(int nx, int d)
+{
+    const int I = nx - 2, hi = nx - 1;
+    int i, j, k;
+    if (d < 8) {
+        i = (d & 1) ? hi : 0;
+        j = (d & 2) ? hi : 0;
+        k = (d & 4) ? hi : 0;
+    } else if (I > 0 && d < 8 + 12 * I) {
+        const int e = (d - 8) / I, t = (d - 8) - e * I + 1;
+        if (e < 4) {
+            i = (e & 1) ? hi : 0;
+            j = (e & 2) ? hi : 0;
+            k = t;
+        } else {
+            const int r = (e - 4) & 3;
+            k = ((e - 4) >> 2) ? hi : 0;
+            if (r < 2) {
+                i = r ? hi : 0;
+                j = t;
+            } else {
+                j = (r - 2) ? hi : 0;
+                i = t;
+            }
+        }
+    } else if (I > 0 && d < 8 + 12 * I + 6 * I * I) {
+        const int f = d - 8 - 12 * I, phi = f / (I * I), loc = f - phi * (I * I);
+        const int u = loc / I + 1, v = loc - (u - 1) * I + 1;
+        const int side = (phi & 1) ? hi : 0;
+        if (phi < 2) {
+            i = side;
+            k = u;
+            j = v;
+        } else if (phi < 4) {
+            j = side;
+            k = u;
+            i = v;
+        } else {
+            k = side;
+            j = u;
+            i = v;
+        }
+    } else {
+        const int f = I > 0 ? d - (nx * nx * nx - I * I * I) : 0;
+        const int II = I > 0 ? I * I : 1, I1 = I > 0 ? I : 1;
+        k = f / II + 1;
+        j = (f - (k - 1) * II) / I1 + 1;
+        i = f - (k - 1) * II - (j - 1) * I1 + 1;
+    }
+    return i + nx * (j + nx * k);
+}
 void boundary_first_perm(int nx, std::vector<int32_t> &perm);
+// surface_order_pos is a permutation and surface_order_ref its inverse.
+bool surface_order_consistent(int nx);
 
 // ---------------------------------------------------------------------
 // Device work of one problem.

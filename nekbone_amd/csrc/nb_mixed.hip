@@ -24,6 +24,7 @@
 // checks the device scalars once per chunk of iterations.
 #include <algorithm>
 #include <cstring>
+#include <tuple>
 
 #include "nb_ax_kernel.h"
 #include "nb_device.h"
@@ -441,6 +442,84 @@ extern "C" int nekbone_cg_mixed_d(nekbone_problem *P, double *x, const double *f
     if (updates) *updates = nupd;
     if (final_rtr) *final_rtr = s.rtr;
     if (s.status) return fail(NEKBONE_ERR_NUMERIC, "cg_mixed: non-positive or non-finite pap");
+    return NEKBONE_OK;
+}
+
+extern "C" int nekbone_get_variant_f32(nekbone_problem *P, int *epb, int *pf, int *xu)
+{
+    if (int rc = check(P)) return rc;
+    if (int rc = ensure_mixed(P, NEKBONE_PRECOND_IDENTITY)) return rc;
+    if (epb) *epb = P->mx->axv.epb;
+    if (pf) *pf = P->mx->axv.pf;
+    if (xu) *xu = P->mx->axv.xu;
+    return NEKBONE_OK;
+}
+
+extern "C" int nekbone_set_variant_f32(nekbone_problem *P, int epb, int pf, int xu)
+{
+    if (int rc = check(P)) return rc;
+    if (int rc = ensure_mixed(P, NEKBONE_PRECOND_IDENTITY)) return rc;
+    if (!lookup32(P->nx1, epb, 2, pf, xu ? 1 : 0) || !lookup32(P->nx1, epb, 0, pf, 0))
+        return fail(NEKBONE_ERR_UNSUPPORTED, "set_variant_f32: shape not compiled for this order");
+    P->mx->axv = AxVariant{epb, pf, xu ? 1 : 0};
+    return NEKBONE_OK;
+}
+
+extern "C" int nekbone_autotune_mixed(nekbone_problem *P, int precond, double *best_ms)
+{
+    if (int rc = check(P)) return rc;
+    if (precond == NEKBONE_PRECOND_JACOBI && !P->jacobi)
+        return fail(NEKBONE_ERR_STATE, "autotune_mixed: Jacobi not set up");
+    if (precond != NEKBONE_PRECOND_JACOBI && precond != NEKBONE_PRECOND_IDENTITY)
+        return fail(NEKBONE_ERR_ARG, "autotune_mixed: identity or Jacobi");
+    if (!P->c_is_mult) return fail(NEKBONE_ERR_UNSUPPORTED, "autotune_mixed: needs c = 1/mult");
+    if (int rc = ensure_mixed(P, precond)) return rc;
+    MixedState *M = P->mx;
+    std::vector<AxVariant> cands;
+    for (const auto &kv : table32()) {
+        int nx, epb, mode, pf, xu;
+        std::tie(nx, epb, mode, pf, xu) = kv.first;
+        if (nx == P->nx1 && mode == 2) cands.push_back(AxVariant{epb, pf, xu});
+    }
+    const AxVariant keep = M->axv;
+    const int mbit = P->mask_mode == NEKBONE_MASK_DIRICHLET ? 1 : 2;
+    const int warm = 2, reps = 6;
+    if (int rc = launch_rand(P->d_f, P->n, 12345)) return rc;
+    hipEvent_t e0, e1;
+    NB_HIP(hipEventCreate(&e0));
+    NB_HIP(hipEventCreate(&e1));
+    double best = 1e300;
+    AxVariant win = keep;
+    int rc = NEKBONE_OK;
+    for (const AxVariant &v : cands) {
+        M->axv = v;
+        rc = launch_cg_init(P, P->d_x, P->d_r, P->d_f, 1, precond, 0.0);
+        if (!rc) {
+            hipLaunchKernelGGL(k_mx_start, dim3(grid(P->n)), dim3(kThreads), 0, stream(), M->r,
+                               M->x, M->p, P->d_r, P->n, P->d_scal);
+            if (hipGetLastError() != hipSuccess) rc = NEKBONE_ERR_HIP;
+        }
+        for (int i = 0; i < warm && !rc; ++i) rc = iteration32(P, precond, mbit);
+        if (!rc && hipEventRecord(e0, stream()) != hipSuccess) rc = NEKBONE_ERR_HIP;
+        for (int i = 0; i < reps && !rc; ++i) rc = iteration32(P, precond, mbit);
+        if (!rc && hipEventRecord(e1, stream()) != hipSuccess) rc = NEKBONE_ERR_HIP;
+        if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = NEKBONE_ERR_HIP;
+        float ms = 0.f;
+        if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = NEKBONE_ERR_HIP;
+        if (rc) break;
+        if (ms / reps < best) {
+            best = ms / reps;
+            win = v;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) {
+        M->axv = keep;
+        return rc == NEKBONE_ERR_HIP ? fail(rc, "autotune_mixed: HIP error") : rc;
+    }
+    M->axv = win;
+    if (best_ms) *best_ms = best;
     return NEKBONE_OK;
 }
 

@@ -320,14 +320,21 @@ void boundary_first_perm(int nx, std::vector<int32_t> &perm)
 {
     const int n3 = nx * nx * nx;
     perm.assign(size_t(n3), -1);
-    int pos = 0;
-    for (int pass = 0; pass < 2; ++pass)
-        for (int p = 0; p < n3; ++p) {
-            const int i = p % nx, j = (p / nx) % nx, k = p / (nx * nx);
-            const bool surf = i == 0 || j == 0 || k == 0 || i == nx - 1 || j == nx - 1 ||
-                              k == nx - 1;
-            if (surf == (pass == 0)) perm[size_t(p)] = pos++;
-        }
+    for (int p = 0; p < n3; ++p) {
+        const int i = p % nx, j = (p / nx) % nx, k = p / (nx * nx);
+        perm[size_t(p)] = surface_order_pos(nx, i, j, k);
+    }
+}
+
+bool surface_order_consistent(int nx)
+{
+    const int n3 = nx * nx * nx;
+    std::vector<int> seen(size_t(n3), 0);
+    for (int p = 0; p < n3; ++p) {
+        const int d = surface_order_pos(nx, p % nx, (p / nx) % nx, p / (nx * nx));
+        if (d < 0 || d >= n3 || seen[size_t(d)]++ || surface_order_ref(nx, d) != p) return false;
+    }
+    return true;
 }
 
 namespace {
