@@ -208,6 +208,13 @@ static int finish_problem(nekbone_problem *P, const double *c_host)
     if (int rc = dalloc(&P->d_perm, int64_t(P->perm.size()))) return rc;
     NB_HIP(hipMemcpy(P->d_perm, P->perm.data(), sizeof(int32_t) * P->perm.size(),
                      hipMemcpyHostToDevice));
+    {
+        std::vector<int16_t> iref(P->perm.size());
+        for (size_t p = 0; p < P->perm.size(); ++p) iref[size_t(P->perm[p])] = int16_t(p);
+        if (int rc = dalloc(&P->d_iref, int64_t(iref.size()))) return rc;
+        NB_HIP(hipMemcpy(P->d_iref, iref.data(), sizeof(int16_t) * iref.size(),
+                         hipMemcpyHostToDevice));
+    }
 
     // Groups keep the reference's member order (ascending reference local
     // index, which fixes the summation order); members are stored as
@@ -445,6 +452,7 @@ extern "C" int nekbone_problem_destroy(nekbone_problem *P)
     dfree(P->d_dinv);
     dfree(P->d_flag);
     dfree(P->d_perm);
+    dfree(P->d_iref);
     dfree(P->gs.d_m2);
     dfree(P->gs.d_m4);
     dfree(P->gs.d_m8);

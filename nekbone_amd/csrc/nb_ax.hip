@@ -81,7 +81,7 @@ int launch_ax_local(nekbone_problem *P, double *w, const double *u)
     A.g = P->d_g;
     A.Drow = P->d_Drow;
     A.Dcol = P->d_D;
-    A.perm = P->d_perm;
+    A.iref = P->d_iref;
     A.nelt = P->nelt;
     const int threads = round64(P->nx1 * P->nx1 * epb);
     const int blocks = (P->nelt + epb - 1) / epb;
@@ -108,7 +108,7 @@ int launch_ax_cg(nekbone_problem *P, double *x, const double *r, int precond, in
     A.Drow = P->d_Drow;
     A.Dcol = P->d_D;
     A.flag = P->d_flag;
-    A.perm = P->d_perm;
+    A.iref = P->d_iref;
     A.mbit = mbit;
     A.scal = P->d_scal;
     A.part = P->d_part;

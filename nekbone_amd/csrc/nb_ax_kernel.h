@@ -54,11 +54,19 @@
 namespace nb {
 namespace axk {
 
+// Fused multiply-add in the operand type (__builtin_fma alone would
+// promote float operands to double).
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_t(double a, double b, double c)
+{
+    return __builtin_fma(a, b, c);
+}
+
 template <bool FMA, typename T>
 __device__ __forceinline__ T madd(T acc, T a, T b)
 {
     if constexpr (FMA)
-        return __builtin_fma(a, b, acc);
+        return fma_t(a, b, acc);
     else
         return acc + a * b;
 }
@@ -92,7 +100,7 @@ struct AxArgsT {
     const T *g;
     const T *Drow, *Dcol;
     const uint8_t *flag;
-    const int32_t *perm;  // device point order within an element
+    const int16_t *iref;  // reference position of each device position (N3)
     int mbit;
     const CgScalars *scal;
     double *part;        // CG: per-block pap partials
@@ -138,10 +146,13 @@ __global__ void __launch_bounds__(round64(NX *NX *EPB)) k_ax(AxArgsT<T> A_)
     // Linear phase: the element's N3 points in device order are split as
     // d = ab + N2*m, m = 0..NX-1 (N3 = NX*N2), so every global access of
     // the per-point vectors is one contiguous run per wave whatever the
-    // device order is.  ref[m] is the reference position of point d.
+    // device order is.  ref[m] is the reference position of point d (a
+    // 2-byte table shared by all elements, so it stays in cache).
     int ref[NX];
+    if (act) {
 #pragma unroll
-    for (int m = 0; m < NX; ++m) ref[m] = surface_order_ref(NX, ab + N2 * m);
+        for (int m = 0; m < NX; ++m) ref[m] = A_.iref[ab + N2 * m];
+    }
 
     T pd[MODE != 0 ? NX : 1];  // CG: the new direction at the linear points
     if (act) {
@@ -168,7 +179,7 @@ __global__ void __launch_bounds__(round64(NX *NX *EPB)) k_ax(AxArgsT<T> A_)
                     for (int m = 0; m < NX; ++m) {
                         const int64_t q = base + ab + N2 * m;
                         if constexpr (FMA)
-                            A_.x[q] = __builtin_fma(alpha, po[m], A_.x[q]);
+                            A_.x[q] = fma_t(alpha, po[m], A_.x[q]);
                         else
                             A_.x[q] = A_.x[q] + alpha * po[m];
                     }
@@ -178,7 +189,7 @@ __global__ void __launch_bounds__(round64(NX *NX *EPB)) k_ax(AxArgsT<T> A_)
             for (int m = 0; m < NX; ++m) {
                 T pn;
                 if constexpr (FMA)
-                    pn = __builtin_fma(beta, po[m], z[m]);
+                    pn = fma_t(beta, po[m], z[m]);
                 else
                     pn = beta * po[m] + z[m];
                 A_.p[base + ab + N2 * m] = pn;
@@ -228,9 +239,9 @@ __global__ void __launch_bounds__(round64(NX *NX *EPB)) k_ax(AxArgsT<T> A_)
             }
             T wr, ws;
             if constexpr (FMA) {
-                wr = __builtin_fma(g3, ut, __builtin_fma(g2, us, g1 * ur));
-                ws = __builtin_fma(g5, ut, __builtin_fma(g4, us, g2 * ur));
-                wt[k] = __builtin_fma(g6, ut, __builtin_fma(g5, us, g3 * ur));
+                wr = fma_t(g3, ut, fma_t(g2, us, g1 * ur));
+                ws = fma_t(g5, ut, fma_t(g4, us, g2 * ur));
+                wt[k] = fma_t(g6, ut, fma_t(g5, us, g3 * ur));
             } else {
                 wr = g1 * ur + g2 * us + g3 * ut;
                 ws = g2 * ur + g4 * us + g5 * ut;
@@ -269,10 +280,14 @@ __global__ void __launch_bounds__(round64(NX *NX *EPB)) k_ax(AxArgsT<T> A_)
 
     double acc = 0.0;
     if (act) {
+        // positions read again (cache hits) rather than kept live through
+        // the contractions
+        const int16_t *iref = A_.iref;
+        asm volatile("" : "+s"(iref));
 #pragma unroll
         for (int m = 0; m < NX; ++m) {
             const int64_t q = base + ab + N2 * m;
-            const T wl = A[ref[m]];
+            const T wl = A[iref[ab + N2 * m]];
             if constexpr (MODE == 0) {
                 A_.w[q] = wl;
             } else {

@@ -268,6 +268,7 @@ struct nekbone_problem {
     // reference order.
     std::vector<int32_t> perm;
     int32_t *d_perm = nullptr;
+    int16_t *d_iref = nullptr;  // inverse: reference position of device position d
     nb::GsMap gs;
 
     // work vectors (n doubles)

@@ -295,7 +295,7 @@ int launch_ax32(nekbone_problem *P, int mode, int mbit, float *w, const float *u
     A.Drow = M->Drow;
     A.Dcol = M->Dcol;
     A.flag = P->d_flag;
-    A.perm = P->d_perm;
+    A.iref = P->d_iref;
     A.mbit = mbit;
     A.scal = P->d_scal;
     A.part = P->d_part;
