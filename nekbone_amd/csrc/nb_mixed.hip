@@ -253,9 +253,14 @@ int ensure_mixed(nekbone_problem *P, int precond)
             }
         NB_HIP(hipMemcpy(M->Drow, dr.data(), sizeof(float) * dr.size(), hipMemcpyHostToDevice));
         NB_HIP(hipMemcpy(M->Dcol, dc.data(), sizeof(float) * dc.size(), hipMemcpyHostToDevice));
-        // default FP32 shape: the FP64 default for this order
+        // default FP32 shape: the FP64 default's elements per workgroup
+        // with the geometric factors prefetched at entry (measured fastest
+        // at N=9 on MI355X; FP32 leaves the registers for it), where that
+        // shape is compiled
         M->axv = ax_default_variant(nx);
         M->axv.xu = 1;
+        M->axv.pf = 1;
+        if (!lookup32(nx, M->axv.epb, 2, M->axv.pf, 1)) M->axv.pf = 0;
         if (!lookup32(nx, M->axv.epb, 2, M->axv.pf, 1)) {
             M->axv.epb = axk::epb_for(nx);
             M->axv.pf = 0;

@@ -34,7 +34,9 @@ struct orc_problem {
     int mask_mode;
     int mask_built; /* 0: cmask(-1)>=0 (next maskit builds the list) */
     double *wk;     /* scratch n */
+    struct orc_hsmg *mg; /* HSMG preconditioner, NULL until orc_hsmg_setup */
 };
+static void hsmg_free(struct orc_hsmg *M);
 
 /* ------------------------------------------------------------------ */
 /* GLL points and weights: src/speclib.f:107-518                        */
@@ -604,6 +606,7 @@ void orc_free(orc_problem *P)
     free(P->mfull);
     free(P->mref);
     free(P->dinv);
+    hsmg_free(P->mg);
     free(P);
 }
 
@@ -801,6 +804,8 @@ void orc_cg(orc_problem *P, double *x, const double *f, double *r, int niter, in
         if (precond == ORC_PRECOND_JACOBI) {
 #pragma omp parallel for schedule(static) if (n > 1000000)
             for (int64_t i = 0; i < n; ++i) z[i] = r[i] * P->dinv[i];
+        } else if (precond == ORC_PRECOND_HSMG) {
+            orc_hsmg_solve(P, z, r); /* h1mg_solve masks r in place */
         } else {
 #pragma omp parallel for schedule(static) if (n > 1000000)
             for (int64_t i = 0; i < n; ++i) z[i] = r[i];
@@ -947,4 +952,740 @@ void orc_deformed_g(orc_problem *P, double amp, uint64_t seed)
                 }
     }
     free(V);
+}
+
+/* ================================================================== */
+/* HSMG preconditioner: src/hsmg.f                                      */
+/* ================================================================== */
+
+typedef struct {
+    int64_t ng;
+    int64_t *off;
+    int32_t *idx;
+} gsmap;
+
+typedef struct {
+    int nx, nh, nl; /* polynomial order, points per direction, nh+2 */
+    int64_t n;      /* nh^3 * nelt */
+    double ah[MAXNX * MAXNX], bh[MAXNX], ch[MAXNX * MAXNX], dh[MAXNX * MAXNX], zh[MAXNX];
+    double jh[MAXNX * MAXNX], jht[MAXNX * MAXNX]; /* to level l+1: jh(nf,nc), jht(nc,nf) */
+    gsmap gs, gss;  /* nh and nh+2 numberings */
+    double *rwt;    /* restriction weights wt(nh,nh,2,3,nelt) */
+    double *fs;     /* FDM s(nl*nl,2,3,nelt) */
+    double *fd;     /* FDM d(nl^3,nelt) */
+    double *swt;    /* Schwarz weights wt(nh,nh,4,3,nelt) */
+    int32_t *mask;
+    int64_t nmask;
+} mglevel;
+
+struct orc_hsmg {
+    int lmax;
+    mglevel L[4]; /* 1..lmax */
+    double *e, *w, *r, *work;
+};
+
+static void gs_from_glo(const int64_t *glo, int64_t n, gsmap *m)
+{
+    int64_t maxid = 0;
+    for (int64_t l = 0; l < n; ++l)
+        if (glo[l] > maxid) maxid = glo[l];
+    int64_t *cnt = calloc((size_t)maxid + 2, sizeof *cnt);
+    for (int64_t l = 0; l < n; ++l) cnt[glo[l] + 1]++;
+    for (int64_t i = 1; i <= maxid + 1; ++i) cnt[i] += cnt[i - 1];
+    int64_t *pos = malloc(sizeof *pos * ((size_t)maxid + 1));
+    memcpy(pos, cnt, sizeof *pos * ((size_t)maxid + 1));
+    int32_t *byid = malloc(sizeof *byid * (size_t)(n ? n : 1));
+    for (int64_t l = 0; l < n; ++l) byid[pos[glo[l]]++] = (int32_t)l;
+    int64_t ng = 0, ns = 0;
+    for (int64_t l = 0; l < n; ++l) {
+        int64_t c0 = cnt[glo[l]], c1 = cnt[glo[l] + 1];
+        if (c1 - c0 >= 2 && byid[c0] == l) {
+            ng++;
+            ns += c1 - c0;
+        }
+    }
+    m->ng = ng;
+    m->off = malloc(sizeof *m->off * (size_t)(ng + 1));
+    m->idx = malloc(sizeof *m->idx * (size_t)(ns ? ns : 1));
+    int64_t g = 0, s = 0;
+    m->off[0] = 0;
+    for (int64_t l = 0; l < n; ++l) {
+        int64_t c0 = cnt[glo[l]], c1 = cnt[glo[l] + 1];
+        if (c1 - c0 >= 2 && byid[c0] == l) {
+            for (int64_t q = c0; q < c1; ++q) m->idx[s++] = byid[q];
+            m->off[++g] = s;
+        }
+    }
+    free(cnt);
+    free(pos);
+    free(byid);
+}
+
+static void gs_apply(const gsmap *m, double *u)
+{
+    for (int64_t g = 0; g < m->ng; ++g) {
+        int64_t a = m->off[g], b = m->off[g + 1];
+        double t = u[m->idx[a]];
+        for (int64_t q = a + 1; q < b; ++q) t += u[m->idx[q]];
+        for (int64_t q = a; q < b; ++q) u[m->idx[q]] = t;
+    }
+}
+
+/* set_vert_box (src/prox_dssum.f:42-73) for nx points per direction. */
+static int64_t *vert_box_nx(const orc_problem *P, int nx)
+{
+    int nn = nx - 1;
+    int64_t nxyz = (int64_t)nx * nx * nx;
+    int64_t *glo = malloc(sizeof *glo * (size_t)(nxyz * P->nelt));
+    for (int e = 0; e < P->nelt; ++e) {
+        int ex, ey, ez;
+        get_exyz(&ex, &ey, &ez, e + 1, P->nelx, P->nely);
+        for (int k = 0; k <= nn; ++k)
+            for (int j = 0; j <= nn; ++j)
+                for (int i = 0; i <= nn; ++i) {
+                    int64_t kg = (int64_t)nn * (ez - 1) + k;
+                    int64_t jg = (int64_t)nn * (ey - 1) + j;
+                    int64_t ig = (int64_t)nn * (ex - 1) + i;
+                    glo[i + nx * j + (int64_t)nx * nx * k + nxyz * e] =
+                        1 + ig + jg * ((int64_t)nn * P->nelx + 1) +
+                        kg * ((int64_t)nn * P->nelx + 1) * ((int64_t)nn * P->nely + 1);
+                }
+    }
+    return glo;
+}
+
+/* Generalized symmetric-definite eigenproblem A x = lam B x
+ * (generalev, src/hsmg.f:483-523, which calls LAPACK DSYGV with
+ * itype=1, jobz='V', uplo='U').  Same definition of the result: lam
+ * ascending, eigenvectors in the columns of a with X^T B X = I.  Method:
+ * Cholesky B = U^T U, C = U^-T A U^-1, cyclic Jacobi rotations on C
+ * (Golub & Van Loan, Matrix Computations, 4th ed., sec. 8.5) until the
+ * off-diagonal part is below 1e-30 of the Frobenius norm, X = U^-1 Y.
+ * Eigenvector signs may differ from LAPACK's; every use below is of the
+ * form S diag S^T, which does not depend on them. */
+static void generalev(double *a, const double *b, double *lam, int n)
+{
+#define A_(i, j) a[(i) + n * (j)]
+    double U[MAXNX * MAXNX] = {0}, C[MAXNX * MAXNX], T[MAXNX * MAXNX], Y[MAXNX * MAXNX];
+    /* Cholesky, upper: B = U^T U */
+    for (int j = 0; j < n; ++j) {
+        double s = b[j + n * j];
+        for (int k = 0; k < j; ++k) s -= U[k + n * j] * U[k + n * j];
+        U[j + n * j] = sqrt(s);
+        for (int i = j + 1; i < n; ++i) {
+            double t = b[j + n * i];
+            for (int k = 0; k < j; ++k) t -= U[k + n * j] * U[k + n * i];
+            U[j + n * i] = t / U[j + n * j];
+        }
+    }
+    /* T = A U^-1 (solve T U = A row by row), C = U^-T T */
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            double t = A_(i, j);
+            for (int k = 0; k < j; ++k) t -= T[i + n * k] * U[k + n * j];
+            T[i + n * j] = t / U[j + n * j];
+        }
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            double t = T[i + n * j];
+            for (int k = 0; k < i; ++k) t -= U[k + n * i] * C[k + n * j];
+            C[i + n * j] = t / U[i + n * i];
+        }
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < j; ++i) {
+            double m = 0.5 * (C[i + n * j] + C[j + n * i]);
+            C[i + n * j] = C[j + n * i] = m;
+        }
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) Y[i + n * j] = (i == j) ? 1.0 : 0.0;
+    double fro = 0.0;
+    for (int i = 0; i < n * n; ++i) fro += C[i] * C[i];
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0;
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < j; ++i) off += C[i + n * j] * C[i + n * j];
+        if (off <= 1e-30 * fro) break;
+        for (int p = 0; p < n - 1; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                double apq = C[p + n * q];
+                if (apq == 0.0) continue;
+                double app = C[p + n * p], aqq = C[q + n * q];
+                double theta = (aqq - app) / (2.0 * apq);
+                double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < n; ++k) {
+                    double ckp = C[k + n * p], ckq = C[k + n * q];
+                    C[k + n * p] = c * ckp - s * ckq;
+                    C[k + n * q] = s * ckp + c * ckq;
+                }
+                for (int k = 0; k < n; ++k) {
+                    double cpk = C[p + n * k], cqk = C[q + n * k];
+                    C[p + n * k] = c * cpk - s * cqk;
+                    C[q + n * k] = s * cpk + c * cqk;
+                }
+                for (int k = 0; k < n; ++k) {
+                    double ykp = Y[k + n * p], ykq = Y[k + n * q];
+                    Y[k + n * p] = c * ykp - s * ykq;
+                    Y[k + n * q] = s * ykp + c * ykq;
+                }
+            }
+    }
+    int ord[MAXNX];
+    for (int i = 0; i < n; ++i) ord[i] = i;
+    for (int i = 1; i < n; ++i) /* stable insertion sort by eigenvalue */
+        for (int k = i; k > 0 && C[ord[k] * (n + 1)] < C[ord[k - 1] * (n + 1)]; --k) {
+            int t = ord[k];
+            ord[k] = ord[k - 1];
+            ord[k - 1] = t;
+        }
+    for (int jj = 0; jj < n; ++jj) {
+        int j = ord[jj];
+        lam[jj] = C[j * (n + 1)];
+        /* x = U^-1 y (back substitution) */
+        for (int i = n - 1; i >= 0; --i) {
+            double t = Y[i + n * j];
+            for (int k = i + 1; k < n; ++k) t -= U[i + n * k] * A_(k, jj);
+            A_(i, jj) = t / U[i + n * i];
+        }
+    }
+#undef A_
+}
+
+/* h1mg_setup_fast1d_a / _b (src/hsmg.f:402-481): the 1-D stiffness and
+ * mass of the element extended by one point of each neighbour; a and b
+ * are (n+3)x(n+3), ah (n+1)x(n+1), bh (n+1). */
+static void fast1d_ab(double *a, double *b, int lbc, int rbc, double ll, double lm, double lr,
+                      const double *ah, const double *bh, int n)
+{
+    const int m = n + 3;
+#define A_(i, j) a[(i) + m * (j)]
+#define B_(i, j) b[(i) + m * (j)]
+#define AH(i, j) ah[(i) + (n + 1) * (j)]
+    int i0 = (lbc == 1) ? 1 : 0, i1 = (rbc == 1) ? n - 1 : n;
+    for (int i = 0; i < m * m; ++i) a[i] = b[i] = 0.0;
+    double fac = 2.0 / lm;
+    A_(1, 1) = 1.0;
+    A_(n + 1, n + 1) = 1.0;
+    for (int j = i0; j <= i1; ++j)
+        for (int i = i0; i <= i1; ++i) A_(i + 1, j + 1) = fac * AH(i, j);
+    if (lbc == 0) {
+        fac = 2.0 / ll;
+        A_(0, 0) = fac * AH(n - 1, n - 1);
+        A_(1, 0) = fac * AH(n, n - 1);
+        A_(0, 1) = fac * AH(n - 1, n);
+        A_(1, 1) = A_(1, 1) + fac * AH(n, n);
+    } else {
+        A_(0, 0) = 1.0;
+    }
+    if (rbc == 0) {
+        fac = 2.0 / lr;
+        A_(n + 1, n + 1) = A_(n + 1, n + 1) + fac * AH(0, 0);
+        A_(n + 2, n + 1) = fac * AH(1, 0);
+        A_(n + 1, n + 2) = fac * AH(0, 1);
+        A_(n + 2, n + 2) = fac * AH(1, 1);
+    } else {
+        A_(n + 2, n + 2) = 1.0;
+    }
+    fac = 0.5 * lm;
+    B_(1, 1) = 1.0;
+    B_(n + 1, n + 1) = 1.0;
+    for (int i = i0; i <= i1; ++i) B_(i + 1, i + 1) = fac * bh[i];
+    if (lbc == 0) {
+        fac = 0.5 * ll;
+        B_(0, 0) = fac * bh[n - 1];
+        B_(1, 1) = B_(1, 1) + fac * bh[n];
+    } else {
+        B_(0, 0) = 1.0;
+    }
+    if (rbc == 0) {
+        fac = 0.5 * lr;
+        B_(n + 1, n + 1) = B_(n + 1, n + 1) + fac * bh[0];
+        B_(n + 2, n + 2) = fac * bh[1];
+    } else {
+        B_(n + 2, n + 2) = 1.0;
+    }
+#undef A_
+#undef B_
+#undef AH
+}
+
+/* h1mg_setup_fast1d (src/hsmg.f:378-400): s(nl,nl,2) = eigenvectors with
+ * the Dirichlet rows zeroed, and their transpose. */
+static void fast1d(double *s, double *lam, int nl, int lbc, int rbc, double ll, double lm,
+                   double lr, const double *ah, const double *bh, int n)
+{
+    double b[MAXNX * MAXNX];
+    fast1d_ab(s, b, lbc, rbc, ll, lm, lr, ah, bh, n);
+    generalev(s, b, lam, nl);
+    if (lbc > 0)
+        for (int j = 0; j < nl; ++j) s[0 + nl * j] = 0.;
+    if (lbc == 1)
+        for (int j = 0; j < nl; ++j) s[1 + nl * j] = 0.;
+    if (rbc > 0)
+        for (int j = 0; j < nl; ++j) s[nl - 1 + nl * j] = 0.;
+    if (rbc == 1)
+        for (int j = 0; j < nl; ++j) s[nl - 2 + nl * j] = 0.;
+    for (int j = 0; j < nl; ++j)
+        for (int i = 0; i < nl; ++i) s[nl * nl + i + nl * j] = s[j + nl * i];
+}
+
+/* h1mg_extrude (src/hsmg.f:1246-1283) on extended arrays of nx^3 per
+ * element: face points (tangential indices 1..nx-2) of layers l1 and
+ * nx-1-l1 become f1*arr1 + f2*arr2(layer l2 / nx-1-l2), x faces, then y,
+ * then z. */
+static void extrude(double *a1, int l1, double f1, const double *a2, int l2, double f2, int nx,
+                    int nelt)
+{
+    int64_t n3 = (int64_t)nx * nx * nx;
+#define IX(i, j, k) ((i) + nx * (j) + (int64_t)nx * nx * (k))
+    for (int e = 0; e < nelt; ++e) {
+        double *u = a1 + n3 * e;
+        const double *v = a2 + n3 * e;
+        for (int k = 1; k <= nx - 2; ++k)
+            for (int j = 1; j <= nx - 2; ++j) {
+                u[IX(l1, j, k)] = f1 * u[IX(l1, j, k)] + f2 * v[IX(l2, j, k)];
+                u[IX(nx - 1 - l1, j, k)] = f1 * u[IX(nx - 1 - l1, j, k)] + f2 * v[IX(nx - 1 - l2, j, k)];
+            }
+        for (int k = 1; k <= nx - 2; ++k)
+            for (int i = 1; i <= nx - 2; ++i) {
+                u[IX(i, l1, k)] = f1 * u[IX(i, l1, k)] + f2 * v[IX(i, l2, k)];
+                u[IX(i, nx - 1 - l1, k)] = f1 * u[IX(i, nx - 1 - l1, k)] + f2 * v[IX(i, nx - 1 - l2, k)];
+            }
+        for (int j = 1; j <= nx - 2; ++j)
+            for (int i = 1; i <= nx - 2; ++i) {
+                u[IX(i, j, l1)] = f1 * u[IX(i, j, l1)] + f2 * v[IX(i, j, l2)];
+                u[IX(i, j, nx - 1 - l1)] = f1 * u[IX(i, j, nx - 1 - l1)] + f2 * v[IX(i, j, nx - 1 - l2)];
+            }
+    }
+#undef IX
+}
+
+/* h1mg_schwarz_toext3d / toreg3d (src/hsmg.f:1207-1244). */
+static void toext3d(double *a, const double *b, int n, int nelt)
+{
+    int nl = n + 2;
+    int64_t n3 = (int64_t)n * n * n, l3 = (int64_t)nl * nl * nl;
+    memset(a, 0, sizeof *a * (size_t)(l3 * nelt));
+    for (int e = 0; e < nelt; ++e)
+        for (int k = 0; k < n; ++k)
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i)
+                    a[l3 * e + (i + 1) + nl * (j + 1) + (int64_t)nl * nl * (k + 1)] =
+                        b[n3 * e + i + n * j + (int64_t)n * n * k];
+}
+
+static void toreg3d(double *b, const double *a, int n, int nelt)
+{
+    int nl = n + 2;
+    int64_t n3 = (int64_t)n * n * n, l3 = (int64_t)nl * nl * nl;
+    for (int e = 0; e < nelt; ++e)
+        for (int k = 0; k < n; ++k)
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i)
+                    b[n3 * e + i + n * j + (int64_t)n * n * k] =
+                        a[l3 * e + (i + 1) + nl * (j + 1) + (int64_t)nl * nl * (k + 1)];
+}
+
+/* h1mg_tnsr3d_el (src/hsmg.f:1322-1338): v = [C x B x A] u on one
+ * element, A (nv,nu), Bt and Ct (nu,nv), through mxm as the reference. */
+static void tnsr3d_el(double *v, int nv, const double *u, int nu, const double *A,
+                      const double *Bt, const double *Ct)
+{
+    double work[MAXNX * MAXNX * MAXNX], work2[MAXNX * MAXNX * MAXNX];
+    mxm(A, nv, u, nu, work, nu * nu);
+    for (int i = 0; i < nu; ++i) mxm(work + nv * nu * i, nv, Bt, nu, work2 + nv * nv * i, nv);
+    mxm(work2, nv * nv, Ct, nu, v, nv);
+}
+
+static void hsmg_free(struct orc_hsmg *M)
+{
+    if (!M) return;
+    for (int l = 1; l <= 3; ++l) {
+        mglevel *L = &M->L[l];
+        free(L->gs.off);
+        free(L->gs.idx);
+        free(L->gss.off);
+        free(L->gss.idx);
+        free(L->rwt);
+        free(L->fs);
+        free(L->fd);
+        free(L->swt);
+        free(L->mask);
+    }
+    free(M->e);
+    free(M->w);
+    free(M->r);
+    free(M->work);
+    free(M);
+}
+
+int orc_hsmg_levels(const orc_problem *P) { return P->mg ? P->mg->lmax : 0; }
+int orc_hsmg_nh(const orc_problem *P, int l)
+{
+    return (P->mg && l >= 1 && l <= P->mg->lmax) ? P->mg->L[l].nh : 0;
+}
+
+/* h1mg_setup (src/hsmg.f:3-29) and everything it calls. */
+int orc_hsmg_setup(orc_problem *P)
+{
+    const int nx1 = P->nx1, nelt = P->nelt;
+    if (nx1 < 4) return 1;
+    hsmg_free(P->mg);
+    struct orc_hsmg *M = calloc(1, sizeof *M);
+    P->mg = M;
+    /* h1mg_setup_mg_nx (:45-97).  For nx1 > 10 the reference computes
+     * 2*(lx2/4)+1 from lx2, which its SIZE files do not define; any
+     * lx2 >= 4 gives 3 after the min(3,.) below, taken here. */
+    M->lmax = (nx1 == 4) ? 2 : 3;
+    static const int mgn2[10] = {1, 2, 2, 2, 2, 3, 3, 5, 5, 5};
+    int mgnx2 = 3;
+    if (nx1 <= 10) mgnx2 = mgn2[nx1 - 1];
+    if (nx1 == 8) mgnx2 = 3;
+    if (mgnx2 > 3) mgnx2 = 3;
+    int nxl[4] = {0, 1, mgnx2, mgnx2 + 1};
+    nxl[M->lmax] = nx1 - 1;
+    for (int l = 1; l <= M->lmax; ++l) {
+        mglevel *L = &M->L[l];
+        L->nx = nxl[l];
+        L->nh = nxl[l] + 1;
+        L->nl = L->nh + 2;
+        L->n = (int64_t)L->nh * L->nh * L->nh * nelt;
+        /* h1mg_setup_semhat (:99-118) */
+        orc_semhat(L->ah, L->bh, L->ch, L->dh, L->zh, L->nx);
+    }
+    /* h1mg_setup_intp (:120-152) */
+    for (int l = 1; l < M->lmax; ++l) {
+        mglevel *L = &M->L[l], *F = &M->L[l + 1];
+        int nf = F->nh, nc = L->nh;
+        double w[2 * MAXNX + 2];
+        for (int i = 0; i < nf; ++i) {
+            orc_fd_weights_full(F->zh[i], L->zh, nc - 1, 1, w);
+            for (int j = 0; j < nc; ++j) L->jh[i + nf * j] = w[j];
+        }
+        for (int i = 0; i < nf; ++i)
+            for (int j = 0; j < nc; ++j) L->jht[j + nc * i] = L->jh[i + nf * j];
+    }
+    /* h1mg_setup_dssum (:154-169) */
+    for (int l = 1; l <= M->lmax; ++l) {
+        mglevel *L = &M->L[l];
+        int64_t *glo = vert_box_nx(P, L->nh);
+        gs_from_glo(glo, L->n, &L->gs);
+        free(glo);
+        glo = vert_box_nx(P, L->nl);
+        gs_from_glo(glo, (int64_t)L->nl * L->nl * L->nl * nelt, &L->gss);
+        free(glo);
+    }
+    int64_t nlmax = M->L[M->lmax].nl;
+    M->work = calloc((size_t)(2 * nlmax * nlmax * nlmax * nelt), sizeof *M->work);
+    M->e = calloc((size_t)(2 * P->n), sizeof *M->e);
+    M->w = calloc((size_t)P->n, sizeof *M->w);
+    M->r = calloc((size_t)P->n, sizeof *M->r);
+    /* h1mg_setup_wtmask / h1mg_setup_rstr_wt (:171-249) */
+    for (int l = 1; l <= M->lmax; ++l) {
+        mglevel *L = &M->L[l];
+        int n = L->nh;
+        int64_t n3 = (int64_t)n * n * n;
+        double *w = M->work;
+        memset(w, 0, sizeof *w * (size_t)L->n);
+        for (int e = 0; e < nelt; ++e)
+            for (int k = 0; k < n; ++k)
+                for (int j = 0; j < n; ++j)
+                    for (int i = 0; i < n; ++i)
+                        if (i == 0 || j == 0 || k == 0 || i == n - 1 || j == n - 1 || k == n - 1)
+                            w[n3 * e + i + n * j + (int64_t)n * n * k] = 1.0;
+        gs_apply(&L->gs, w);
+        L->rwt = malloc(sizeof *L->rwt * (size_t)(n * n * 6 * nelt));
+#define W_(i, j, k) w[n3 * e + (i) + n * (j) + (int64_t)n * n * (k)]
+#define RW(a, b, s, d) L->rwt[(a) + n * ((b) + n * ((s) + 2 * ((d) + 3 * (int64_t)e)))]
+        for (int e = 0; e < nelt; ++e) {
+            for (int k = 0; k < n; ++k)
+                for (int j = 0; j < n; ++j) {
+                    RW(j, k, 0, 0) = 1.0 / W_(0, j, k);
+                    RW(j, k, 1, 0) = 1.0 / W_(n - 1, j, k);
+                }
+            for (int k = 0; k < n; ++k)
+                for (int i = 0; i < n; ++i) {
+                    RW(i, k, 0, 1) = 1.0 / W_(i, 0, k);
+                    RW(i, k, 1, 1) = 1.0 / W_(i, n - 1, k);
+                }
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i) {
+                    RW(i, j, 0, 2) = 1.0 / W_(i, j, 0);
+                    RW(i, j, 1, 2) = 1.0 / W_(i, j, n - 1);
+                }
+        }
+#undef RW
+#undef W_
+    }
+    /* h1mg_setup_fdm / h1mg_setup_fast (:251-376).  Neighbour widths
+     * through the fine-level dssum as the reference does. */
+    double *lw = calloc((size_t)P->n, sizeof *lw);
+    int64_t f3 = (int64_t)nx1 * nx1 * nx1;
+#define LF(i, j, k, e) lw[(i) + nx1 * (j) + (int64_t)nx1 * nx1 * (k) + f3 * (e)]
+    const double lm = 2.0;
+    for (int e = 0; e < nelt; ++e) {
+        LF(0, 1, 1, e) = lm;
+        LF(nx1 - 1, 1, 1, e) = lm;
+        LF(1, 0, 1, e) = lm;
+        LF(1, nx1 - 1, 1, e) = lm;
+        LF(1, 1, 0, e) = lm;
+        LF(1, 1, nx1 - 1, e) = lm;
+    }
+    orc_dssum(P, lw);
+    for (int l = 2; l <= M->lmax; ++l) {
+        mglevel *L = &M->L[l];
+        int nl = L->nl;
+        int64_t l3 = (int64_t)nl * nl * nl;
+        L->fs = malloc(sizeof *L->fs * (size_t)(nl * nl * 6 * nelt));
+        L->fd = malloc(sizeof *L->fd * (size_t)(l3 * nelt));
+        for (int e = 0; e < nelt; ++e) {
+            double llr = LF(0, 1, 1, e) - lm, lrr = LF(nx1 - 1, 1, 1, e) - lm;
+            double lls = LF(1, 0, 1, e) - lm, lrs = LF(1, nx1 - 1, 1, e) - lm;
+            double llt = LF(1, 1, 0, e) - lm, lrt = LF(1, 1, nx1 - 1, e) - lm;
+            int f[6];
+            boundary_faces(P, e, f); /* get_bcs, src/hsmg.f:534-570 */
+            double lr[MAXNX], ls[MAXNX], lt[MAXNX];
+            double *s = L->fs + (int64_t)nl * nl * 6 * e;
+            fast1d(s, lr, nl, f[0], f[1], llr, lm, lrr, L->ah, L->bh, L->nx);
+            fast1d(s + 2 * nl * nl, ls, nl, f[2], f[3], lls, lm, lrs, L->ah, L->bh, L->nx);
+            fast1d(s + 4 * nl * nl, lt, nl, f[4], f[5], llt, lm, lrt, L->ah, L->bh, L->nx);
+            /* vlmax(lr(2),nr-2): largest of the eigenvalues 2..nl-1 */
+            double mr = lr[1], ms = ls[1], mt = lt[1];
+            for (int i = 2; i <= nl - 2; ++i) {
+                if (lr[i] > mr) mr = lr[i];
+                if (ls[i] > ms) ms = ls[i];
+                if (lt[i] > mt) mt = lt[i];
+            }
+            double eps = 1.e-5 * (mr + ms + mt);
+            double *d = L->fd + l3 * e;
+            int64_t il = 0;
+            for (int k = 0; k < nl; ++k)
+                for (int j = 0; j < nl; ++j)
+                    for (int i = 0; i < nl; ++i) {
+                        double diag = lr[i] + ls[j] + lt[k];
+                        d[il++] = (diag > eps) ? 1.0 / diag : 0.0;
+                    }
+        }
+    }
+#undef LF
+    free(lw);
+    /* h1mg_setup_schwarz_wt / _wt3d_1 / _wt3d_2 (:572-689) */
+    for (int l = 2; l <= M->lmax; ++l) {
+        mglevel *L = &M->L[l];
+        int n = L->nh, nl = L->nl;
+        int64_t ns = (int64_t)nl * nl * nl * nelt;
+        double *w0 = M->work, *wi = M->work + ns;
+        for (int64_t i = 0; i < ns; ++i) wi[i] = 1.0;
+        extrude(w0, 0, 0.0, wi, 0, 1.0, nl, nelt);
+        gs_apply(&L->gss, wi);
+        extrude(wi, 0, 1.0, w0, 0, -1.0, nl, nelt);
+        extrude(wi, 2, 1.0, wi, 0, 1.0, nl, nelt);
+        toreg3d(w0, wi, n, nelt);
+        gs_apply(&L->gs, w0);
+        int64_t n3 = (int64_t)n * n * n;
+        L->swt = malloc(sizeof *L->swt * (size_t)(n * n * 12 * nelt));
+#define W_(i, j, k) w0[n3 * e + (i) + n * (j) + (int64_t)n * n * (k)]
+#define SW(a, b, s, d) L->swt[(a) + n * ((b) + n * ((s) + 4 * ((d) + 3 * (int64_t)e)))]
+        for (int e = 0; e < nelt; ++e) {
+            for (int k = 0; k < n; ++k)
+                for (int j = 0; j < n; ++j) {
+                    SW(j, k, 0, 0) = 1.0 / W_(0, j, k);
+                    SW(j, k, 1, 0) = 1.0 / W_(1, j, k);
+                    SW(j, k, 2, 0) = 1.0 / W_(n - 2, j, k);
+                    SW(j, k, 3, 0) = 1.0 / W_(n - 1, j, k);
+                }
+            for (int k = 0; k < n; ++k)
+                for (int i = 0; i < n; ++i) {
+                    SW(i, k, 0, 1) = 1.0 / W_(i, 0, k);
+                    SW(i, k, 1, 1) = 1.0 / W_(i, 1, k);
+                    SW(i, k, 2, 1) = 1.0 / W_(i, n - 2, k);
+                    SW(i, k, 3, 1) = 1.0 / W_(i, n - 1, k);
+                }
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i) {
+                    SW(i, j, 0, 2) = 1.0 / W_(i, j, 0);
+                    SW(i, j, 1, 2) = 1.0 / W_(i, j, 1);
+                    SW(i, j, 2, 2) = 1.0 / W_(i, j, n - 2);
+                    SW(i, j, 3, 2) = 1.0 / W_(i, j, n - 1);
+                }
+        }
+#undef SW
+#undef W_
+    }
+    /* mg_set_msk / h1mg_setup_mask (:824-911): every domain-boundary
+     * point of each level, element by element. */
+    for (int l = 1; l <= M->lmax; ++l) {
+        mglevel *L = &M->L[l];
+        int n = L->nh;
+        int64_t n3 = (int64_t)n * n * n, cnt = 0;
+        L->mask = malloc(sizeof *L->mask * (size_t)L->n);
+        for (int e = 0; e < nelt; ++e) {
+            int f[6];
+            boundary_faces(P, e, f);
+            for (int k = 0; k < n; ++k)
+                for (int j = 0; j < n; ++j)
+                    for (int i = 0; i < n; ++i)
+                        if (on_boundary(f, i, j, k, n))
+                            L->mask[cnt++] = (int32_t)(n3 * e + i + n * j + (int64_t)n * n * k);
+        }
+        L->nmask = cnt;
+    }
+    return 0;
+}
+
+static void mg_mask(const mglevel *L, double *u)
+{
+    for (int64_t q = 0; q < L->nmask; ++q) u[L->mask[q]] = 0.;
+}
+
+/* h1mg_schwarz_wt3d2 (src/hsmg.f:1144-1178): e *= sqrt(wt) on the two
+ * outer layers of each direction, each point once. */
+static void schwarz_wt(const mglevel *L, double *e, int nelt)
+{
+    int n = L->nh;
+    int64_t n3 = (int64_t)n * n * n;
+#define E_(i, j, k) e[n3 * ie + (i) + n * (j) + (int64_t)n * n * (k)]
+#define SW(a, b, s, d) L->swt[(a) + n * ((b) + n * ((s) + 4 * ((d) + 3 * (int64_t)ie)))]
+    for (int ie = 0; ie < nelt; ++ie) {
+        for (int k = 0; k < n; ++k)
+            for (int j = 0; j < n; ++j) {
+                E_(0, j, k) = E_(0, j, k) * sqrt(SW(j, k, 0, 0));
+                E_(1, j, k) = E_(1, j, k) * sqrt(SW(j, k, 1, 0));
+                E_(n - 2, j, k) = E_(n - 2, j, k) * sqrt(SW(j, k, 2, 0));
+                E_(n - 1, j, k) = E_(n - 1, j, k) * sqrt(SW(j, k, 3, 0));
+            }
+        for (int k = 0; k < n; ++k)
+            for (int i = 2; i <= n - 3; ++i) {
+                E_(i, 0, k) = E_(i, 0, k) * sqrt(SW(i, k, 0, 1));
+                E_(i, 1, k) = E_(i, 1, k) * sqrt(SW(i, k, 1, 1));
+                E_(i, n - 2, k) = E_(i, n - 2, k) * sqrt(SW(i, k, 2, 1));
+                E_(i, n - 1, k) = E_(i, n - 1, k) * sqrt(SW(i, k, 3, 1));
+            }
+        for (int j = 2; j <= n - 3; ++j)
+            for (int i = 2; i <= n - 3; ++i) {
+                E_(i, j, 0) = E_(i, j, 0) * sqrt(SW(i, j, 0, 2));
+                E_(i, j, 1) = E_(i, j, 1) * sqrt(SW(i, j, 1, 2));
+                E_(i, j, n - 2) = E_(i, j, n - 2) * sqrt(SW(i, j, 2, 2));
+                E_(i, j, n - 1) = E_(i, j, n - 1) * sqrt(SW(i, j, 3, 2));
+            }
+    }
+#undef SW
+#undef E_
+}
+
+/* h1mg_schwarz (src/hsmg.f:1036-1050) with h1mg_schwarz_part1
+ * (:1052-1095) and h1mg_fdm / h1mg_do_fast (:1286-1320); sigma = 1.
+ * The reference also weights e before part1, which part1 then
+ * overwrites entirely; that step has no effect and is omitted. */
+static void mg_schwarz(struct orc_hsmg *M, const mglevel *L, double *e, double *r, int nelt)
+{
+    int n = L->nh, nl = L->nl;
+    int64_t l3 = (int64_t)nl * nl * nl, ns = l3 * nelt;
+    double *w0 = M->work, *w1 = M->work + ns;
+    mg_mask(L, r);
+    toext3d(w0, r, n, nelt);
+    extrude(w0, 0, 0.0, w0, 2, 1.0, nl, nelt);
+    gs_apply(&L->gss, w0);
+    extrude(w0, 0, 1.0, w0, 2, -1.0, nl, nelt);
+    for (int ie = 0; ie < nelt; ++ie) {
+        const double *s = L->fs + (int64_t)nl * nl * 6 * ie;
+        const double *Sx = s, *SxT = s + nl * nl, *Sy = s + 2 * nl * nl, *SyT = s + 3 * nl * nl,
+                     *Sz = s + 4 * nl * nl, *SzT = s + 5 * nl * nl;
+        double *ee = w1 + l3 * ie, *rr = w0 + l3 * ie;
+        const double *d = L->fd + l3 * ie;
+        tnsr3d_el(ee, nl, rr, nl, SxT, Sy, Sz);
+        for (int64_t i = 0; i < l3; ++i) rr[i] = d[i] * ee[i];
+        tnsr3d_el(ee, nl, rr, nl, Sx, SyT, SzT);
+    }
+    extrude(w0, 0, 0.0, w1, 0, 1.0, nl, nelt);
+    gs_apply(&L->gss, w1);
+    extrude(w1, 0, 1.0, w0, 0, -1.0, nl, nelt);
+    extrude(w1, 2, 1.0, w1, 0, 1.0, nl, nelt);
+    toreg3d(e, w1, n, nelt);
+    gs_apply(&L->gs, e);
+    mg_mask(L, e);
+    schwarz_wt(L, e, nelt);
+    /* cmult(e, sigma=1) leaves e unchanged */
+}
+
+/* h1mg_rstr (src/hsmg.f:1340-1357): r := J^T (wt .* r) in place, from
+ * level l+1 to level l, then optionally dssum on level l. */
+static void mg_rstr(const mglevel *C, const mglevel *F, double *r, int ifdssum, int nelt)
+{
+    int n = F->nh;
+    int64_t n3 = (int64_t)n * n * n;
+#define U_(i, j, k) r[n3 * ie + (i) + n * (j) + (int64_t)n * n * (k)]
+#define RW(a, b, s, d) F->rwt[(a) + n * ((b) + n * ((s) + 2 * ((d) + 3 * (int64_t)ie)))]
+    for (int ie = 0; ie < nelt; ++ie) { /* h1mg_do_wt (:1359-1388) */
+        for (int k = 0; k < n; ++k)
+            for (int j = 0; j < n; ++j) {
+                U_(0, j, k) = U_(0, j, k) * RW(j, k, 0, 0);
+                U_(n - 1, j, k) = U_(n - 1, j, k) * RW(j, k, 1, 0);
+            }
+        for (int k = 0; k < n; ++k)
+            for (int i = 1; i <= n - 2; ++i) {
+                U_(i, 0, k) = U_(i, 0, k) * RW(i, k, 0, 1);
+                U_(i, n - 1, k) = U_(i, n - 1, k) * RW(i, k, 1, 1);
+            }
+        for (int j = 1; j <= n - 2; ++j)
+            for (int i = 1; i <= n - 2; ++i) {
+                U_(i, j, 0) = U_(i, j, 0) * RW(i, j, 0, 2);
+                U_(i, j, n - 1) = U_(i, j, n - 1) * RW(i, j, 1, 2);
+            }
+    }
+#undef RW
+#undef U_
+    /* h1mg_tnsr1_3d (:1402-1434) in place, elements in increasing order
+     * since the coarse result is smaller */
+    int nv = C->nh, nu = F->nh;
+    int64_t nu3 = (int64_t)nu * nu * nu, nv3 = (int64_t)nv * nv * nv;
+    double tmp[MAXNX * MAXNX * MAXNX];
+    for (int ie = 0; ie < nelt; ++ie) {
+        tnsr3d_el(tmp, nv, r + nu3 * ie, nu, C->jht, C->jh, C->jh);
+        memcpy(r + nv3 * ie, tmp, sizeof *tmp * (size_t)nv3);
+    }
+    if (ifdssum) gs_apply(&C->gs, r);
+}
+
+/* h1mg_intp (src/hsmg.f:1436-1443): uf = [J x J x J] uc */
+static void mg_intp(const mglevel *C, const mglevel *F, double *uf, const double *uc, int nelt)
+{
+    int nv = F->nh, nu = C->nh;
+    int64_t nu3 = (int64_t)nu * nu * nu, nv3 = (int64_t)nv * nv * nv;
+    for (int ie = 0; ie < nelt; ++ie)
+        tnsr3d_el(uf + nv3 * ie, nv, uc + nu3 * ie, nu, C->jh, C->jht, C->jht);
+}
+
+/* h1mg_solve (src/hsmg.f:960-1034) */
+void orc_hsmg_solve(orc_problem *P, double *z, double *rhs)
+{
+    struct orc_hsmg *M = P->mg;
+    const int nelt = P->nelt, lmax = M->lmax;
+    double *e = M->e, *w = M->w, *r = M->r;
+    mglevel *L = M->L;
+    mg_schwarz(M, &L[lmax], z, rhs, nelt);
+    int64_t n = L[lmax].n, is = 0;
+    memcpy(r, rhs, sizeof *r * (size_t)n);
+    for (int l = lmax - 1; l >= 2; --l) {
+        is += n;
+        n = L[l].n;
+        mg_rstr(&L[l], &L[l + 1], r, 1, nelt);
+        mg_schwarz(M, &L[l], e + is, r, nelt);
+    }
+    is += n;
+    mg_rstr(&L[1], &L[2], r, 0, nelt);
+    mg_mask(&L[1], r);
+    memcpy(e + is, r, sizeof *e * (size_t)L[1].n); /* hsmg_coarse_solve (:945-957) */
+    gs_apply(&L[1].gs, e + is);
+    mg_mask(&L[1], e + is);
+    for (int l = 2; l <= lmax - 1; ++l) {
+        int64_t im = is;
+        is -= n;
+        n = L[l].n;
+        mg_intp(&L[l - 1], &L[l], w, e + im, nelt);
+        for (int64_t i = 0; i < n; ++i) e[is + i] = e[is + i] + w[i];
+    }
+    n = L[lmax].n;
+    mg_intp(&L[lmax - 1], &L[lmax], w, e + is, nelt);
+    for (int64_t i = 0; i < n; ++i) z[i] = z[i] + w[i];
 }

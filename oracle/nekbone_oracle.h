@@ -23,7 +23,7 @@ extern "C" {
 
 typedef struct orc_problem orc_problem;
 
-enum { ORC_PRECOND_IDENTITY = 0, ORC_PRECOND_JACOBI = 1 };
+enum { ORC_PRECOND_IDENTITY = 0, ORC_PRECOND_JACOBI = 1, ORC_PRECOND_HSMG = 2 };
 enum { ORC_MASK_REF = 0, ORC_MASK_DIRICHLET = 1 };
 
 /* GLL points/weights (src/speclib.f:107-518) and 1-D operators
@@ -78,6 +78,17 @@ void orc_get_dinv(const orc_problem *P, double *dinv);
  * x, r are n-vectors; f is the masked-on-entry right-hand side. */
 void orc_cg(orc_problem *P, double *x, const double *f, double *r,
             int niter, int precond, double *hist);
+
+/* HSMG preconditioner (src/hsmg.f): h1mg_setup (:3-29) on the problem's
+ * current mesh, and one application z = M^-1 rhs, h1mg_solve (:960-1034),
+ * which also zeroes the Dirichlet points of rhs in place as the reference
+ * does (:1067).  Needs nx1 >= 4.  Returns 0 on success. */
+int orc_hsmg_setup(orc_problem *P);
+void orc_hsmg_solve(orc_problem *P, double *z, double *rhs);
+/* Level data for checks: number of levels, and points per direction of
+ * level l (1 = coarsest). */
+int orc_hsmg_levels(const orc_problem *P);
+int orc_hsmg_nh(const orc_problem *P, int l);
 
 /* Synthetic inputs. */
 void orc_rand_uniform(double *f, int64_t n, uint64_t seed); /* U[-1,1) */

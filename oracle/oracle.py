@@ -14,6 +14,7 @@ _LIB = None
 
 PRECOND_IDENTITY = 0
 PRECOND_JACOBI = 1
+PRECOND_HSMG = 2
 MASK_REF = 0
 MASK_DIRICHLET = 1
 
@@ -62,6 +63,13 @@ def lib():
         L.orc_zwgll.argtypes = [_dp, _dp, ctypes.c_int]
         L.orc_semhat.argtypes = [_dp, _dp, _dp, _dp, _dp, ctypes.c_int]
         L.orc_cubic.argtypes = [ctypes.POINTER(ctypes.c_int)] * 3 + [ctypes.c_int]
+        L.orc_hsmg_setup.restype = ctypes.c_int
+        L.orc_hsmg_setup.argtypes = [ctypes.c_void_p]
+        L.orc_hsmg_solve.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.orc_hsmg_levels.restype = ctypes.c_int
+        L.orc_hsmg_levels.argtypes = [ctypes.c_void_p]
+        L.orc_hsmg_nh.restype = ctypes.c_int
+        L.orc_hsmg_nh.argtypes = [ctypes.c_void_p, ctypes.c_int]
         _LIB = L
     return _LIB
 
@@ -204,6 +212,22 @@ class Problem:
         f = np.array(fraw, dtype=np.float64, copy=True)
         lib().orc_rhs_from_raw(self._h, _p(f))
         return f
+
+    def hsmg_setup(self):
+        if lib().orc_hsmg_setup(self._h) != 0:
+            raise ValueError("HSMG needs nx1 >= 4")
+
+    def hsmg_levels(self):
+        L = lib().orc_hsmg_levels(self._h)
+        return [lib().orc_hsmg_nh(self._h, l) for l in range(1, L + 1)]
+
+    def hsmg_solve(self, rhs):
+        """z = M^-1 rhs (h1mg_solve); returns z and the rhs as the
+        reference leaves it (Dirichlet points zeroed)."""
+        r = np.array(rhs, dtype=np.float64, copy=True)
+        z = np.zeros(self.n)
+        lib().orc_hsmg_solve(self._h, _p(z), _p(r))
+        return z, r
 
     def cg(self, f, niter, precond=PRECOND_IDENTITY):
         f = np.ascontiguousarray(f, dtype=np.float64)
