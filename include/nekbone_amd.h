@@ -126,6 +126,15 @@ int nekbone_set_variant(nekbone_problem *P, int epb, int pf, int xu);
 int nekbone_setup_jacobi(nekbone_problem *P);
 int nekbone_get_dinv(nekbone_problem *P, double *dinv);
 
+/* HSMG preconditioner (src/hsmg.f): h1mg_setup (:3-29) on this problem's
+ * box, then NEKBONE_PRECOND_HSMG is available to the CG calls.  Levels of
+ * order 1, min(3, ...) and nx1-1 as h1mg_setup_mg_nx (:45-97) chooses;
+ * needs nx1 >= 4.  nekbone_hsmg_solve applies it once, h1mg_solve
+ * (:960-1034): z = M^-1 r on host arrays, and r comes back with its
+ * Dirichlet points zeroed, as the reference leaves it (:1067). */
+int nekbone_setup_hsmg(nekbone_problem *P);
+int nekbone_hsmg_solve(nekbone_problem *P, double *z, double *r);
+
 /* ---------------------------------------------------------------------- */
 /* Operators (host arrays; *_d: device pointers on the library stream)     */
 /* ---------------------------------------------------------------------- */
@@ -236,6 +245,11 @@ void cg_(double *x, const double *f, const double *g, const double *c, double *r
          double *p, double *z, const int *n, const int *niter, double *flop_cg);
 /* src/cg.f:86-94 (identity, or Jacobi after nekbone_setup_jacobi) */
 void solvem_(double *z, double *r, const int *n);
+/* src/hsmg.f:3-29; after it solvem_ and cg_ use the HSMG preconditioner,
+ * as a build with MGRID does (src/cg.f:86-94) */
+void h1mg_setup_(void);
+/* src/hsmg.f:960-1034 (rhs is masked in place, as in the reference) */
+void h1mg_solve_(double *z, double *rhs, const int *n);
 /* src/cg.f:96-125 */
 void ax_(double *w, const double *u, const double *gxyz, double *ur, double *us, double *ut,
          double *wk, const int *n);

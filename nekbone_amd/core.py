@@ -54,6 +54,8 @@ _SIGS = {
     "nekbone_set_arith": (ctypes.c_int, [_vp, ctypes.c_int]),
     "nekbone_set_fused": (ctypes.c_int, [_vp, ctypes.c_int]),
     "nekbone_setup_jacobi": (ctypes.c_int, [_vp]),
+    "nekbone_setup_hsmg": (ctypes.c_int, [_vp]),
+    "nekbone_hsmg_solve": (ctypes.c_int, [_vp, _dp, _dp]),
     "nekbone_get_dinv": (ctypes.c_int, [_vp, _dp]),
     "nekbone_ax": (ctypes.c_int, [_vp, _dp, _dp]),
     "nekbone_ax_d": (ctypes.c_int, [_vp, _vp, _vp]),
@@ -263,6 +265,17 @@ class Problem:
 
     def setup_jacobi(self):
         _chk(lib().nekbone_setup_jacobi(self.handle))
+
+    def setup_hsmg(self):
+        _chk(lib().nekbone_setup_hsmg(self.handle))
+
+    def hsmg_solve(self, r):
+        """z = M^-1 r (h1mg_solve); returns z and r as the preconditioner
+        leaves it (Dirichlet points zeroed)."""
+        r = _f64(np.array(r, dtype=np.float64, copy=True), self.n)
+        z = np.empty(self.n)
+        _chk(lib().nekbone_hsmg_solve(self.handle, _p(z), _p(r)))
+        return z, r
 
     def dinv(self):
         v = np.empty(self.n)

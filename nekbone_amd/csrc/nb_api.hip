@@ -299,21 +299,42 @@ static int ensure_hist(nekbone_problem *P, int niter)
     return NEKBONE_OK;
 }
 
+static int check_precond(nekbone_problem *P, int precond)
+{
+    if (precond == NEKBONE_PRECOND_JACOBI && !P->jacobi)
+        return fail(NEKBONE_ERR_STATE, "cg: Jacobi preconditioner not set up");
+    if (precond == NEKBONE_PRECOND_HSMG && !P->mg)
+        return fail(NEKBONE_ERR_STATE, "cg: HSMG preconditioner not set up");
+    if (precond != NEKBONE_PRECOND_JACOBI && precond != NEKBONE_PRECOND_IDENTITY &&
+        precond != NEKBONE_PRECOND_HSMG)
+        return fail(NEKBONE_ERR_UNSUPPORTED, "cg: unknown preconditioner");
+    return NEKBONE_OK;
+}
+
+// One iteration: solveM (src/cg.f:86-94) is fused into the iteration for
+// the identity and Jacobi; HSMG runs h1mg_solve first (which masks r, as
+// the reference's does) and the iteration takes its z.
+static int cg_step(nekbone_problem *P, double *x, double *r, int precond, int mbit)
+{
+    if (precond == NEKBONE_PRECOND_HSMG) {
+        if (int rc = launch_hsmg_apply(P, P->mg->d_z, r)) return rc;
+        return launch_cg_iteration_z(P, x, r, P->mg->d_z, mbit);
+    }
+    return launch_cg_iteration(P, x, r, precond, mbit);
+}
+
 // CG on device vectors (src/cg.f:2-84).
 static int run_cg(nekbone_problem *P, double *x, const double *f, double *r, int niter,
                   int precond, double tol2)
 {
-    if (precond == NEKBONE_PRECOND_JACOBI && !P->jacobi)
-        return fail(NEKBONE_ERR_STATE, "cg: Jacobi preconditioner not set up");
-    if (precond != NEKBONE_PRECOND_JACOBI && precond != NEKBONE_PRECOND_IDENTITY)
-        return fail(NEKBONE_ERR_UNSUPPORTED, "cg: preconditioner not available on this path");
+    if (int rc = check_precond(P, precond)) return rc;
     if (niter < 0) return fail(NEKBONE_ERR_ARG, "cg: niter < 0");
     if (int rc = ensure_hist(P, niter)) return rc;
     const int mbit0 = next_mask_bit(P);  // maskit(r), src/cg.f:40
     if (int rc = launch_cg_init(P, x, r, f, mbit0, precond, tol2)) return rc;
     const int mbit = next_mask_bit(P);  // maskit in ax, src/cg.f:119 (state is now fixed)
     for (int it = 0; it < niter; ++it)
-        if (int rc = launch_cg_iteration(P, x, r, precond, mbit)) return rc;
+        if (int rc = cg_step(P, x, r, precond, mbit)) return rc;
     return launch_cg_finish(P, x);
 }
 
@@ -463,6 +484,7 @@ extern "C" int nekbone_problem_destroy(nekbone_problem *P)
         dfree(*v);
     dfree(P->d_scal);
     free_mixed(P);
+    hsmg_free(P);
     (void)timing_resolve(P);
     for (hipEvent_t e : P->timing.pool) (void)hipEventDestroy(e);
     if (P->timing.cur) (void)hipEventDestroy(P->timing.cur);
@@ -811,8 +833,7 @@ extern "C" int nekbone_cg_tol_d(nekbone_problem *P, double *x, const double *f, 
 {
     if (int rc = check(P)) return rc;
     if (!(tol > 0.0)) return fail(NEKBONE_ERR_ARG, "cg_tol: tol must be > 0");
-    if (precond == NEKBONE_PRECOND_JACOBI && !P->jacobi)
-        return fail(NEKBONE_ERR_STATE, "cg: Jacobi preconditioner not set up");
+    if (int rc = check_precond(P, precond)) return rc;
     if (int rc = ensure_hist(P, maxiter)) return rc;
     if (int rc = launch_to_dev(P, P->d_f, f)) return rc;
     const int mbit0 = next_mask_bit(P);
@@ -825,7 +846,7 @@ extern "C" int nekbone_cg_tol_d(nekbone_problem *P, double *x, const double *f, 
     CgScalars s{};
     for (int it = 0; it < maxiter; it += chunk) {
         for (int k = it; k < std::min(maxiter, it + chunk); ++k)
-            if (int rc = launch_cg_iteration(P, P->d_x, P->d_r, precond, mbit)) return rc;
+            if (int rc = cg_step(P, P->d_x, P->d_r, precond, mbit)) return rc;
         NB_HIP(hipMemcpyAsync(&s, P->d_scal, sizeof s, hipMemcpyDeviceToHost, stream()));
         NB_HIP(hipStreamSynchronize(stream()));
         if (s.done) break;

@@ -470,6 +470,93 @@ static int iteration_generic(nekbone_problem *P, double *x, double *r, int preco
     return NEKBONE_OK;
 }
 
+namespace {
+
+// rtz = glsc3(r, c, z) (src/cg.f:52) with an explicit z
+__global__ void __launch_bounds__(kRedThreads)
+    k_rtz(const double *__restrict__ r, const double *__restrict__ c,
+          const double *__restrict__ z, int64_t n, double *__restrict__ part,
+          const CgScalars *__restrict__ s)
+{
+    if (s->done) return;
+    double a0 = 0.0, dummy = 0.0;
+    const int64_t stride = int64_t(gridDim.x) * kRedThreads;
+    for (int64_t i = int64_t(blockIdx.x) * kRedThreads + threadIdx.x; i < n; i += stride)
+        a0 = a0 + r[i] * c[i] * z[i];
+    block_sum2(a0, dummy);
+    if (threadIdx.x == 0) part[blockIdx.x] = a0;
+}
+
+// rtz1 and beta of this iteration (src/cg.f:52-55) from the rtz parts;
+// replaces what the previous update's bookkeeping assumed (z = r).
+__global__ void __launch_bounds__(kFinalThreads)
+    k_rtz_final(const double *__restrict__ part, int nblk, CgScalars *__restrict__ s,
+                double *__restrict__ hist, int hcap)
+{
+    if (s->done) return;
+    double rtz, dummy;
+    sum_region2<kFinalThreads>(part, nblk, nullptr, 0, rtz, dummy);
+    if (threadIdx.x == 0) {
+        s->rtz1 = rtz;
+        s->beta = (s->iter == 0) ? 0.0 : rtz / s->rtz2;
+        const int h = 1 + 3 * s->iter;
+        if (h < hcap) hist[h] = rtz;
+    }
+}
+
+// p = beta p + z (add2s1, src/cg.f:56)
+template <bool FMA>
+__global__ void k_pupdate_z(double *__restrict__ p, const double *__restrict__ z, int64_t n,
+                            const CgScalars *__restrict__ s)
+{
+    if (s->done) return;
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if constexpr (FMA)
+        p[i] = __builtin_fma(s->beta, p[i], z[i]);
+    else
+        p[i] = s->beta * p[i] + z[i];
+}
+
+}  // namespace
+
+int launch_cg_iteration_z(nekbone_problem *P, double *x, double *r, const double *z, int mbit)
+{
+    const bool fma = P->arith == NEKBONE_ARITH_FMA;
+    const int nb = reduce_blocks(P->n);
+    hipLaunchKernelGGL(k_rtz, dim3(nb), dim3(kRedThreads), 0, stream(),
+                       static_cast<const double *>(r), static_cast<const double *>(P->d_c), z,
+                       P->n, P->d_part, static_cast<const CgScalars *>(P->d_scal));
+    hipLaunchKernelGGL(k_rtz_final, dim3(1), dim3(kFinalThreads), 0, stream(),
+                       static_cast<const double *>(P->d_part), nb, P->d_scal, P->d_hist,
+                       P->hist_cap);
+    if (fma)
+        hipLaunchKernelGGL(k_pupdate_z<true>, dim3(grid1d(P->n, 256)), dim3(256), 0, stream(),
+                           P->d_p, z, P->n, static_cast<const CgScalars *>(P->d_scal));
+    else
+        hipLaunchKernelGGL(k_pupdate_z<false>, dim3(grid1d(P->n, 256)), dim3(256), 0, stream(),
+                           P->d_p, z, P->n, static_cast<const CgScalars *>(P->d_scal));
+    NB_HIP(hipGetLastError());
+    if (int rc = launch_ax_local(P, P->d_w, P->d_p)) return rc;
+    if (int rc = launch_gs(P, P->d_w, P->d_scal)) return rc;
+    hipLaunchKernelGGL(k_post_cg<true>, dim3(nb), dim3(kRedThreads), 0, stream(), P->d_w, P->d_p,
+                       P->d_c, P->d_flag, mbit, P->n, P->d_part, P->d_scal);
+    hipLaunchKernelGGL(k_pap_final, dim3(1), dim3(kFinalThreads), 0, stream(), P->d_part,
+                       int64_t(nb), P->d_scal, P->d_hist, P->hist_cap);
+    if (fma)
+        hipLaunchKernelGGL((k_update_generic<NEKBONE_PRECOND_IDENTITY, true>), dim3(nb),
+                           dim3(kRedThreads), 0, stream(), x, r, P->d_p, P->d_w, P->d_c,
+                           P->d_dinv, P->n, P->d_part, nb, P->d_scal);
+    else
+        hipLaunchKernelGGL((k_update_generic<NEKBONE_PRECOND_IDENTITY, false>), dim3(nb),
+                           dim3(kRedThreads), 0, stream(), x, r, P->d_p, P->d_w, P->d_c,
+                           P->d_dinv, P->n, P->d_part, nb, P->d_scal);
+    hipLaunchKernelGGL(k_update_final<NEKBONE_PRECOND_IDENTITY>, dim3(1), dim3(kFinalThreads), 0,
+                       stream(), P->d_part, nb, P->d_scal, P->d_hist, P->hist_cap, 0);
+    NB_HIP(hipGetLastError());
+    return NEKBONE_OK;
+}
+
 // One PCG iteration (src/cg.f:48-75), no host round trip.
 int launch_cg_iteration(nekbone_problem *P, double *x, double *r, int precond, int mbit)
 {

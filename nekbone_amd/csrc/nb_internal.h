@@ -71,6 +71,11 @@ void gs_groups(const int64_t *glo, int64_t n, std::vector<int32_t> &off,
 void mask_lists(int nx, int nelt, int nelx, int nely, int nelz,
                 const std::vector<int32_t> &lglel, std::vector<int32_t> &full,
                 std::vector<int32_t> &ref);
+// Faces of global element eg (1-based) on the domain boundary, get_face
+// (src/cg.f:302-368): x0, x1, y0, y1, z0, z1.
+void boundary_faces(int eg, int nelx, int nely, int nelz, bool f[6]);
+bool on_face(const bool f[6], int i, int j, int k, int nx);
+
 // Device point order within an element: surface points first, grouped
 // so that every shared-point group of a hex mesh touches contiguous runs
 // in each element.  Position of reference point (i,j,k), I = nx-2, with
@@ -233,6 +238,38 @@ struct CgScalars {
     double rtr_upd;  // mixed precision: FP64 rtr at the last reliable update
 };
 
+// One level of the HSMG preconditioner (src/hsmg.f).  Level vectors are
+// element-blocked nh^3 per element; the finest level is the problem's own
+// (device point order, the problem's gather-scatter map), the coarser
+// levels use the reference's point order.
+struct HsmgLevel {
+    int nx = 0, nh = 0, nl = 0;  // order, points per direction, nh+2
+    int64_t n = 0;               // nh^3 * nelt
+    bool fine = false;
+    GsMap gs;                    // coarser levels: own numbering
+    int32_t *d_pos = nullptr;    // memory offset of reference point p within an element (nh^3)
+    uint8_t *d_bnd = nullptr;    // 1: zeroed by this level's Dirichlet mask
+    double *d_c = nullptr;       // inverse multiplicity (restriction weight)
+    double *d_sw = nullptr;      // square root of the Schwarz weight (1 away from faces)
+    int32_t *d_gsrc = nullptr;   // per element 6*nh^2: memory index of the point the ghost
+                                 // receives from the neighbour (-1: none)
+    int32_t *d_gin = nullptr;    // per element 6*nh^2: own point one inside that face (memory index)
+    int32_t *d_gpart = nullptr;  // per element 6*nh^2: the neighbour's matching ghost slot (-1: none)
+    double *d_ghost = nullptr;   // per element 6*nh^2 ghost values of the local solves
+    double *d_ops = nullptr;     // distinct 1-D FDM operators: S, S^T (nl*nl each), lambda (nl)
+    int nops = 0;
+    int16_t *d_eop = nullptr;    // per element: operator index in x, y, z
+    double *d_eps = nullptr;     // per element eigenvalue cut-off
+    double *d_jh = nullptr, *d_jht = nullptr;  // interpolation to the next finer level
+    double *d_r = nullptr, *d_e = nullptr, *d_eown = nullptr;  // level work vectors
+};
+
+struct Hsmg {
+    int lmax = 0;
+    HsmgLevel L[4];  // 1..lmax
+    double *d_z = nullptr;  // preconditioned residual of the CG (fine level)
+};
+
 // FP32 copies for the inner iterations of the mixed-precision CG:
 // operator data converted from the FP64 originals (rebuilt when the
 // geometry or the Jacobi diagonal changes) and the FP32 work vectors,
@@ -290,6 +327,7 @@ struct nekbone_problem {
     int geom_version = 0;  // bumped whenever d_g changes
     int dinv_version = 0;  // bumped whenever d_dinv is recomputed
     nb::MixedState *mx = nullptr;
+    nb::Hsmg *mg = nullptr;
     nb::Timing timing;
 };
 
@@ -350,6 +388,18 @@ int launch_update_final(nekbone_problem *P, const double *part, int nu, int prec
                         int set_xpending);
 int cg_update_blocks(const nekbone_problem *P);
 void free_mixed(nekbone_problem *P);
+// gather-scatter with an explicit map (HSMG levels)
+int launch_gs_map(const GsMap &M, double *u);
+int gs_map_upload(GsMap &G, const std::vector<int32_t> &off, const std::vector<int32_t> &idx,
+                  const int32_t *pos, int n3);
+void gs_map_free(GsMap &G);
+// HSMG (nb_hsmg_setup.cpp, nb_hsmg.hip)
+int hsmg_build(nekbone_problem *P);
+void hsmg_free(nekbone_problem *P);
+int launch_hsmg_apply(nekbone_problem *P, double *z, double *r);
+// One CG iteration with an externally computed z = M^-1 r (r as the
+// preconditioner left it): rtz, beta, p, w = A p, pap, alpha, x, r, rtr.
+int launch_cg_iteration_z(nekbone_problem *P, double *x, double *r, const double *z, int mbit);
 int set_c_host(nekbone_problem *P, const double *c);
 int timed_begin(nekbone_problem *P, int kclass);
 int timed_end(nekbone_problem *P, int kclass);

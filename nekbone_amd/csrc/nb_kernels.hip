@@ -99,9 +99,8 @@ __global__ void __launch_bounds__(kRedThreads)
     }
 }
 
-GsArgs gs_args(const nekbone_problem *P)
+GsArgs gs_args_map(const GsMap &M)
 {
-    const GsMap &M = P->gs;
     GsArgs G{M.d_m2, M.d_m4, M.d_m8, M.d_roff, M.d_ridx, M.n2, M.n4, M.n8, M.nr, 0, 0, 0, 0};
     auto nb = [](int64_t n) { return int((n + kRedThreads - 1) / kRedThreads); };
     G.b2 = nb(M.n2);
@@ -110,6 +109,8 @@ GsArgs gs_args(const nekbone_problem *P)
     G.br = nb(M.nr);
     return G;
 }
+
+GsArgs gs_args(const nekbone_problem *P) { return gs_args_map(P->gs); }
 
 // ---------------------------------------------------------------------
 // After dssum: w = w + 0.1 p (add2s2, src/cg.f:118), then the Dirichlet
@@ -394,6 +395,65 @@ int launch_gs_cg(nekbone_problem *P, double *w, const double *p, int mbit, doubl
                        static_cast<const CgScalars *>(P->d_scal));
     NB_HIP(hipGetLastError());
     return timed_end(P, 1);
+}
+
+int launch_gs_map(const GsMap &M, double *u)
+{
+    const GsArgs G = gs_args_map(M);
+    const int nb = G.b2 + G.b4 + G.b8 + G.br;
+    if (nb == 0) return NEKBONE_OK;
+    hipLaunchKernelGGL((k_gs_cls<double, false>), dim3(nb), dim3(kRedThreads), 0, stream(), u,
+                       static_cast<const double *>(nullptr), static_cast<const uint8_t *>(nullptr),
+                       0, G, static_cast<double *>(nullptr),
+                       static_cast<const CgScalars *>(nullptr));
+    NB_HIP(hipGetLastError());
+    return NEKBONE_OK;
+}
+
+// Device form of a group list (offsets, members in the reference's
+// order): members stored as memory positions e*n3 + pos[p] (pos null:
+// identity), split by size class.
+int gs_map_upload(GsMap &G, const std::vector<int32_t> &off, const std::vector<int32_t> &idx,
+                  const int32_t *pos, int n3)
+{
+    G.ngroups = int64_t(off.size()) - 1;
+    G.nshared = int64_t(idx.size());
+    std::vector<int32_t> m2, m4, m8, roff{0}, ridx;
+    auto at = [&](int32_t l) -> int32_t {
+        if (!pos) return l;
+        const int32_t e = l / n3;
+        return e * n3 + pos[l - e * n3];
+    };
+    for (size_t g = 0; g + 1 < off.size(); ++g) {
+        const int sz = off[g + 1] - off[g];
+        std::vector<int32_t> &dst = sz == 2 ? m2 : sz == 4 ? m4 : sz == 8 ? m8 : ridx;
+        for (int32_t q = off[g]; q < off[g + 1]; ++q) dst.push_back(at(idx[size_t(q)]));
+        if (sz != 2 && sz != 4 && sz != 8) roff.push_back(int32_t(ridx.size()));
+    }
+    G.n2 = int64_t(m2.size()) / 2;
+    G.n4 = int64_t(m4.size()) / 4;
+    G.n8 = int64_t(m8.size()) / 8;
+    G.nr = int64_t(roff.size()) - 1;
+    auto up = [](int32_t **d, const std::vector<int32_t> &h) -> int {
+        if (int rc = dalloc(d, int64_t(h.size()))) return rc;
+        if (!h.empty())
+            NB_HIP(hipMemcpy(*d, h.data(), sizeof(int32_t) * h.size(), hipMemcpyHostToDevice));
+        return NEKBONE_OK;
+    };
+    for (auto pr : {std::make_pair(&G.d_m2, &m2), std::make_pair(&G.d_m4, &m4),
+                    std::make_pair(&G.d_m8, &m8), std::make_pair(&G.d_roff, &roff),
+                    std::make_pair(&G.d_ridx, &ridx)})
+        if (int rc = up(pr.first, *pr.second)) return rc;
+    return NEKBONE_OK;
+}
+
+void gs_map_free(GsMap &G)
+{
+    dfree(G.d_m2);
+    dfree(G.d_m4);
+    dfree(G.d_m8);
+    dfree(G.d_roff);
+    dfree(G.d_ridx);
 }
 
 int launch_gs_cg32(nekbone_problem *P, float *w, const float *p, int mbit, double *part)

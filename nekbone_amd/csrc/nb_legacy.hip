@@ -220,6 +220,8 @@ extern "C" void solvem_(double *z, double *r, const int *n)
     int rc = bound("solvem_");
     if (!rc && *n != L.P->n) rc = fail(NEKBONE_ERR_ARG, "solvem_: n differs from the bound mesh");
     if (rc) return report(rc, "solvem_");
+    if (L.precond == NEKBONE_PRECOND_HSMG)  // MGRID build: h1mg_solve (src/cg.f:91)
+        return report(nekbone_hsmg_solve(L.P, z, r), "solvem_");
     const int64_t nn = L.P->n;
     if (int rc2 = upload_dev(L.P, L.P->d_t2, r)) return report(rc2, "solvem_");
     if (L.precond == NEKBONE_PRECOND_JACOBI && L.P->d_dinv)
@@ -229,6 +231,24 @@ extern "C" void solvem_(double *z, double *r, const int *n)
         hipLaunchKernelGGL(k_copy, dim3(unsigned((nn + 255) / 256)), dim3(256), 0, stream(),
                            L.P->d_w, L.P->d_t2, nn);
     report(download_ref(L.P, z, L.P->d_w), "solvem_");
+}
+
+extern "C" void h1mg_setup_(void)
+{
+    LegacyState &L = S();
+    int rc = bound("h1mg_setup_");
+    if (!rc) rc = nekbone_setup_hsmg(L.P);
+    if (!rc) L.precond = NEKBONE_PRECOND_HSMG;
+    report(rc, "h1mg_setup_");
+}
+
+extern "C" void h1mg_solve_(double *z, double *rhs, const int *n)
+{
+    LegacyState &L = S();
+    int rc = bound("h1mg_solve_");
+    if (!rc && *n != L.P->n) rc = fail(NEKBONE_ERR_ARG, "h1mg_solve_: n differs from the bound mesh");
+    if (!rc) rc = nekbone_hsmg_solve(L.P, z, rhs);
+    report(rc, "h1mg_solve_");
 }
 
 extern "C" void ax_(double *w, const double *u, const double *gxyz, double *ur, double *us,

@@ -337,9 +337,8 @@ bool surface_order_consistent(int nx)
     return true;
 }
 
-namespace {
-// Faces of element e (0-based) on the domain boundary, as get_face
-// decides, src/cg.f:302-368: {x0 (Face4), x1 (Face2), y0 (Face1),
+// Faces of element eg (1-based global number) on the domain boundary, as
+// get_face decides, src/cg.f:302-368: {x0 (Face4), x1 (Face2), y0 (Face1),
 // y1 (Face3), z0 (Face5), z1 (Face6)}.
 void boundary_faces(int eg, int nelx, int nely, int nelz, bool f[6])
 {
@@ -360,7 +359,6 @@ bool on_face(const bool f[6], int i, int j, int k, int nx)
     return (f[0] && i == 0) || (f[1] && i == nx - 1) || (f[2] && j == 0) ||
            (f[3] && j == nx - 1) || (f[4] && k == 0) || (f[5] && k == nx - 1);
 }
-}  // namespace
 
 void mask_lists(int nx, int nelt, int nelx, int nely, int nelz,
                 const std::vector<int32_t> &lglel, std::vector<int32_t> &full,
