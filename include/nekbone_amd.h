@@ -198,6 +198,34 @@ int nekbone_autotune_mixed(nekbone_problem *P, int precond, double *best_ms);
  * element (host float arrays, reference point order, n each). */
 int nekbone_ax_local_f32(nekbone_problem *P, float *w, const float *u);
 
+/* Batched solves (no counterpart in the reference, whose driver runs one
+ * mesh at a time, src/driver.f:28-79).  A batch holds any number of
+ * problems of any sizes and orders; each keeps its own mask mode,
+ * geometry and Jacobi set-up, and needs c = 1/multiplicity.  Every
+ * iteration of the whole batch is one launch per kernel per order group
+ * (element operator, gather-scatter) and one per kernel for all problems
+ * (scalars, vector update), replayed from a captured HIP graph in chunks;
+ * each problem's values are those of its own nekbone_cg_tol_d solve with
+ * the batch's element-kernel shape.  The batch sets that shape on its
+ * problems. */
+typedef struct nekbone_batch nekbone_batch;
+int nekbone_batch_create(nekbone_batch **B, nekbone_problem *const *probs, int nprob);
+int nekbone_batch_destroy(nekbone_batch *B);
+int nekbone_batch_size(const nekbone_batch *B);
+/* 1: the deferred x update rides in the element kernel (default); 0: in
+ * the vector update.  nekbone_batch_autotune times both and keeps the
+ * faster; *best_ms is its time per batched iteration. */
+int nekbone_batch_set_xu(nekbone_batch *B, int xu);
+int nekbone_batch_autotune(nekbone_batch *B, int precond, double *best_ms);
+/* CG to a relative residual for every problem k: x[k], f[k], r[k] as for
+ * nekbone_cg_tol_d (x, r may be NULL); stops when every problem has met
+ * tol or after maxiter iterations.  iters[k], final_rtr[k] may be NULL. */
+int nekbone_batch_cg_tol_d(nekbone_batch *B, double *const *x, const double *const *f,
+                           double *const *r, double tol, int maxiter, int precond, int *iters,
+                           double *final_rtr);
+/* Box factorization of np elements, cubic (src/driver.f:196-247). */
+int nekbone_cubic(int np, int *mx, int *my, int *mz);
+
 /* Device buffers owned by the problem (n doubles each), for *_d calls. */
 int nekbone_alloc_vec(nekbone_problem *P, double **dptr);
 int nekbone_free_vec(nekbone_problem *P, double *dptr);

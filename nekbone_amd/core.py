@@ -89,6 +89,16 @@ _SIGS = {
     "nekbone_get_variant": (ctypes.c_int, [_vp] + [ctypes.POINTER(ctypes.c_int)] * 3),
     "nekbone_set_variant": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "nekbone_bw_probe": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, _dp]),
+    "nekbone_cubic": (ctypes.c_int, [ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 3),
+    "nekbone_batch_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                            ctypes.c_int]),
+    "nekbone_batch_destroy": (ctypes.c_int, [_vp]),
+    "nekbone_batch_size": (ctypes.c_int, [_vp]),
+    "nekbone_batch_set_xu": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "nekbone_batch_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "nekbone_batch_cg_tol_d": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                              ctypes.POINTER(_vp), ctypes.c_double, ctypes.c_int,
+                                              ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp]),
 }
 
 _LIB = None
@@ -138,6 +148,54 @@ def init(device=0):
 
 def sync():
     _chk(lib().nekbone_sync())
+
+
+def cubic(np_):
+    """Box factorization of np_ elements (cubic, src/driver.f:196-247)."""
+    m = [ctypes.c_int() for _ in range(3)]
+    _chk(lib().nekbone_cubic(int(np_), *[ctypes.byref(v) for v in m]))
+    return tuple(v.value for v in m)
+
+
+class Batch:
+    """Problems solved together, one launch per kernel per order group."""
+
+    def __init__(self, problems):
+        self.problems = list(problems)
+        arr = (_vp * len(self.problems))(*[p.handle for p in self.problems])
+        h = _vp()
+        _chk(lib().nekbone_batch_create(ctypes.byref(h), arr, len(self.problems)))
+        self.handle = h.value
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().nekbone_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_xu(self, xu):
+        _chk(lib().nekbone_batch_set_xu(self.handle, int(xu)))
+
+    def autotune(self, precond=None):
+        precond = PRECOND_JACOBI if precond is None else precond
+        ms = ctypes.c_double()
+        _chk(lib().nekbone_batch_autotune(self.handle, precond, ctypes.byref(ms)))
+        return ms.value
+
+    def cg_tol_d(self, xs, fs, rs, tol, maxiter, precond=None):
+        precond = PRECOND_JACOBI if precond is None else precond
+        n = len(self.problems)
+        ptrs = lambda vs: (_vp * n)(*[(v.ptr if v is not None else None) for v in vs])
+        it = (ctypes.c_int * n)()
+        rtr = (ctypes.c_double * n)()
+        _chk(lib().nekbone_batch_cg_tol_d(self.handle, ptrs(xs), ptrs(fs), ptrs(rs), tol, maxiter,
+                                          precond, it, rtr))
+        return list(it), list(rtr)
 
 
 def bw_probe(nbytes, reps=10):

@@ -108,8 +108,10 @@ struct AxArgsT {
 };
 using AxArgs = AxArgsT<double>;
 
+// Body of the element kernel for workgroup blk of one problem (shared by
+// the single-problem kernel and the batched one).
 template <typename T, int NX, int EPB, bool FMA, int MODE, int PF, bool XU>
-__global__ void __launch_bounds__(round64(NX *NX *EPB)) k_ax(AxArgsT<T> A_)
+__device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
 {
     constexpr int N2 = NX * NX, N3 = NX * NX * NX;
     constexpr int NT = round64(N2 * EPB);
@@ -120,7 +122,7 @@ __global__ void __launch_bounds__(round64(NX *NX *EPB)) k_ax(AxArgsT<T> A_)
     const int el = tid / N2;
     const int ab = tid - el * N2;
     const int a = ab % NX, b = ab / NX;
-    const int64_t e = int64_t(blockIdx.x) * EPB + el;
+    const int64_t e = int64_t(blk) * EPB + el;
     const bool act = (el < EPB) && (e < A_.nelt);
     T *A = sA + (act ? el : 0) * N3;
     T *B = sB + (act ? el : 0) * N3;
@@ -315,9 +317,35 @@ __global__ void __launch_bounds__(round64(NX *NX *EPB)) k_ax(AxArgsT<T> A_)
             double s = red[0];
 #pragma unroll
             for (int w = 1; w < NW; ++w) s = s + red[w];
-            A_.part[blockIdx.x] = s;
+            A_.part[blk] = s;
         }
     }
+}
+
+template <typename T, int NX, int EPB, bool FMA, int MODE, int PF, bool XU>
+__global__ void __launch_bounds__(round64(NX *NX *EPB)) k_ax(AxArgsT<T> A_)
+{
+    ax_body<T, NX, EPB, FMA, MODE, PF, XU>(A_, blockIdx.x);
+}
+
+// Batched form: problem q owns workgroups [start[q], start[q+1]); its
+// arguments are read once per workgroup.  Each problem sees exactly the
+// workgroups (and partial sums) of its own single-problem launch.
+template <typename T, int NX, int EPB, bool FMA, int MODE, int PF, bool XU>
+__global__ void __launch_bounds__(round64(NX *NX *EPB))
+    k_ax_batch(const AxArgsT<T> *__restrict__ args, const int *__restrict__ start, int nprob)
+{
+    const int b = blockIdx.x;
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (start[mid] <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const AxArgsT<T> A = args[lo];
+    ax_body<T, NX, EPB, FMA, MODE, PF, XU>(A, b - start[lo]);
 }
 
 // Default elements per workgroup for each NX: >= 2 waves per group and
@@ -362,6 +390,30 @@ void reg_order(Table &t, bool full)
     }
 }
 
+// Batched CG kernels (double, modes 1 and 2), same keys as the table.
+using AxFnB = void (*)(const AxArgs *, const int *, int);
+using TableB = std::map<Key, AxFnB>;
+
+template <int NX, int EPB, int PF>
+void regb_modes(TableB &t)
+{
+    t[Key(NX, EPB, 0, 1, PF, 0)] = k_ax_batch<double, NX, EPB, false, 1, PF, false>;
+    t[Key(NX, EPB, 0, 2, PF, 0)] = k_ax_batch<double, NX, EPB, false, 2, PF, false>;
+    t[Key(NX, EPB, 0, 1, PF, 1)] = k_ax_batch<double, NX, EPB, false, 1, PF, true>;
+    t[Key(NX, EPB, 0, 2, PF, 1)] = k_ax_batch<double, NX, EPB, false, 2, PF, true>;
+    t[Key(NX, EPB, 1, 1, PF, 0)] = k_ax_batch<double, NX, EPB, true, 1, PF, false>;
+    t[Key(NX, EPB, 1, 2, PF, 0)] = k_ax_batch<double, NX, EPB, true, 2, PF, false>;
+    t[Key(NX, EPB, 1, 1, PF, 1)] = k_ax_batch<double, NX, EPB, true, 1, PF, true>;
+    t[Key(NX, EPB, 1, 2, PF, 1)] = k_ax_batch<double, NX, EPB, true, 2, PF, true>;
+}
+
+template <int NX>
+void regb_order(TableB &t)
+{
+    constexpr int E = epb_for(NX);
+    regb_modes<NX, E, 0>(t);
+}
+
 // FP32 kernels (inner iterations of the mixed-precision CG), always with
 // fused multiply-adds.  Key: (nx, epb, mode, pf, xu).
 using AxFn32 = void (*)(AxArgsT<float>);
@@ -393,6 +445,7 @@ void reg32_order(Table32 &t, bool full)
 // Defined in the instantiation files nb_ax_i*.hip and nb_ax32_*.hip.
 void reg32_low(Table32 &t);
 void reg32_high(Table32 &t);
+void regb_all(TableB &t);
 void reg_low(Table &t);
 void reg_high(Table &t);
 void reg_6(Table &t);

@@ -118,9 +118,9 @@ __global__ void __launch_bounds__(kFinalThreads)
     }
 }
 
-__global__ void __launch_bounds__(kFinalThreads)
-    k_pap_final(const double *__restrict__ part, int64_t npart, CgScalars *__restrict__ s,
-                double *__restrict__ hist, int hcap)
+__device__ __forceinline__ void pap_final_body(const double *__restrict__ part, int64_t npart,
+                                               CgScalars *__restrict__ s,
+                                               double *__restrict__ hist, int hcap)
 {
     if (s->done) return;
     double pap, dummy;
@@ -138,23 +138,31 @@ __global__ void __launch_bounds__(kFinalThreads)
     }
 }
 
+__global__ void __launch_bounds__(kFinalThreads)
+    k_pap_final(const double *__restrict__ part, int64_t npart, CgScalars *__restrict__ s,
+                double *__restrict__ hist, int hcap)
+{
+    pap_final_body(part, npart, s, hist, hcap);
+}
+
 // [XP] x += alpha p (add2s2, src/cg.f:63) when the element kernel does not
 // carry it; r += (-alpha) w (add2s2, src/cg.f:64); parts of rtr = (r*c)*r
 // (src/cg.f:66) and of the next rtz = (r*c)*z.  Four independent
 // elements per thread per sweep keep enough loads in flight.
 template <int PRE, bool FMA, bool XP>
-__global__ void __launch_bounds__(kRedThreads)
-    k_update(double *__restrict__ r, const double *__restrict__ w,
-             const double *__restrict__ dinv, const uint8_t *__restrict__ flag,
-             double *__restrict__ x, const double *__restrict__ p, int64_t n,
-             double *__restrict__ part, int nblk, const CgScalars *__restrict__ s)
+__device__ __forceinline__ void update_body(double *__restrict__ r, const double *__restrict__ w,
+                                            const double *__restrict__ dinv,
+                                            const uint8_t *__restrict__ flag,
+                                            double *__restrict__ x, const double *__restrict__ p,
+                                            int64_t n, double *__restrict__ part, int nblk,
+                                            const CgScalars *__restrict__ s, const int blk)
 {
     if (s->done) return;
     const double alpha = s->alpha, alphm = -alpha;
     double a0 = 0.0, a1 = 0.0;
     constexpr int U = 4;
-    const int64_t stride = int64_t(gridDim.x) * kRedThreads * U;
-    for (int64_t i0 = int64_t(blockIdx.x) * kRedThreads * U + threadIdx.x; i0 < n; i0 += stride) {
+    const int64_t stride = int64_t(nblk) * kRedThreads * U;
+    for (int64_t i0 = int64_t(blk) * kRedThreads * U + threadIdx.x; i0 < n; i0 += stride) {
         double rv[U], wv[U], dv[U], xv[U], pv[U];
         uint8_t fv[U];
 #pragma unroll
@@ -195,15 +203,26 @@ __global__ void __launch_bounds__(kRedThreads)
     }
     block_sum2(a0, a1);
     if (threadIdx.x == 0) {
-        part[blockIdx.x] = a0;
-        part[nblk + blockIdx.x] = a1;
+        part[blk] = a0;
+        part[nblk + blk] = a1;
     }
 }
 
+template <int PRE, bool FMA, bool XP>
+__global__ void __launch_bounds__(kRedThreads)
+    k_update(double *__restrict__ r, const double *__restrict__ w,
+             const double *__restrict__ dinv, const uint8_t *__restrict__ flag,
+             double *__restrict__ x, const double *__restrict__ p, int64_t n,
+             double *__restrict__ part, int nblk, const CgScalars *__restrict__ s)
+{
+    update_body<PRE, FMA, XP>(r, w, dinv, flag, x, p, n, part, nblk, s, blockIdx.x);
+}
+
 template <int PRE>
-__global__ void __launch_bounds__(kFinalThreads)
-    k_update_final(const double *__restrict__ part, int nblk, CgScalars *__restrict__ s,
-                   double *__restrict__ hist, int hcap, int set_xpending)
+__device__ __forceinline__ void update_final_body(const double *__restrict__ part, int nblk,
+                                                  CgScalars *__restrict__ s,
+                                                  double *__restrict__ hist, int hcap,
+                                                  int set_xpending)
 {
     if (s->done) return;
     double rtr, rtz;
@@ -218,6 +237,53 @@ __global__ void __launch_bounds__(kFinalThreads)
         if (s->tol2 > 0.0 && rtr <= s->tol2 * s->rtr0) s->done = 1;
         prepare_next(s, hist, hcap);
     }
+}
+
+template <int PRE>
+__global__ void __launch_bounds__(kFinalThreads)
+    k_update_final(const double *__restrict__ part, int nblk, CgScalars *__restrict__ s,
+                   double *__restrict__ hist, int hcap, int set_xpending)
+{
+    update_final_body<PRE>(part, nblk, s, hist, hcap, set_xpending);
+}
+
+// ---- batched forms: problem q owns workgroups [start[q], start[q+1]) of
+// the vector kernels and workgroup q of the scalar kernels ----------------
+__device__ __forceinline__ int owner(const int *__restrict__ start, int nprob, int b)
+{
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (start[mid] <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int PRE, bool FMA, bool XP>
+__global__ void __launch_bounds__(kRedThreads)
+    k_update_batch(const CgBatchArgs *__restrict__ args, const int *__restrict__ start, int nprob)
+{
+    const int q = owner(start, nprob, blockIdx.x);
+    const CgBatchArgs A = args[q];
+    update_body<PRE, FMA, XP>(A.r, A.w, A.dinv, A.flag, A.x, A.p, A.n, A.part_upd, A.nupd,
+                              A.scal, blockIdx.x - start[q]);
+}
+
+__global__ void __launch_bounds__(kFinalThreads) k_pap_final_batch(const CgBatchArgs *__restrict__ args)
+{
+    const CgBatchArgs A = args[blockIdx.x];
+    pap_final_body(A.part, A.npap, A.scal, A.hist, A.hcap);
+}
+
+template <int PRE>
+__global__ void __launch_bounds__(kFinalThreads)
+    k_update_final_batch(const CgBatchArgs *__restrict__ args, int set_xpending)
+{
+    const CgBatchArgs A = args[blockIdx.x];
+    update_final_body<PRE>(A.part_upd, A.nupd, A.scal, A.hist, A.hcap, set_xpending);
 }
 
 // The x update of the last completed iteration (the element kernel of
@@ -563,6 +629,64 @@ int launch_cg_iteration(nekbone_problem *P, double *x, double *r, int precond, i
     if (P->c_is_mult && P->fused) return iteration_fused(P, x, r, precond, mbit);
     return iteration_generic(P, x, r, precond, mbit);
 }
+
+CgBatchArgs cg_batch_args(nekbone_problem *P, double *x, double *r)
+{
+    CgBatchArgs A{};
+    A.r = r;
+    A.w = P->d_w;
+    A.dinv = P->d_dinv;
+    A.flag = P->d_flag;
+    A.x = x;
+    A.p = P->d_p;
+    A.n = P->n;
+    A.part = P->d_part;
+    A.npap = n_ax(P) + n_gs(P);
+    A.part_upd = P->d_part + A.npap;
+    A.nupd = n_upd(P);
+    A.scal = P->d_scal;
+    A.hist = P->d_hist;
+    A.hcap = P->hist_cap;
+    return A;
+}
+
+// The scalar and vector-update kernels of one fused iteration for a batch
+// of problems (the element and gather-scatter kernels are launched per
+// order group before): pap/alpha, r update with rtr and rtz, rtr/beta.
+int launch_cg_batch_tail(const CgBatchArgs *d_args, const int *d_ustart, int nprob, int nupd_total,
+                         int precond, bool fma, bool xp, int set_xpending)
+{
+    hipLaunchKernelGGL(k_pap_final_batch, dim3(nprob), dim3(kFinalThreads), 0, stream(), d_args);
+#define NB_UB(PRE, F, X)                                                                     \
+    hipLaunchKernelGGL((k_update_batch<PRE, F, X>), dim3(nupd_total), dim3(kRedThreads), 0, \
+                       stream(), d_args, d_ustart, nprob)
+    const bool jac = precond == NEKBONE_PRECOND_JACOBI;
+    if (jac) {
+        if (fma) {
+            if (xp) NB_UB(NEKBONE_PRECOND_JACOBI, true, true); else NB_UB(NEKBONE_PRECOND_JACOBI, true, false);
+        } else {
+            if (xp) NB_UB(NEKBONE_PRECOND_JACOBI, false, true); else NB_UB(NEKBONE_PRECOND_JACOBI, false, false);
+        }
+    } else {
+        if (fma) {
+            if (xp) NB_UB(NEKBONE_PRECOND_IDENTITY, true, true); else NB_UB(NEKBONE_PRECOND_IDENTITY, true, false);
+        } else {
+            if (xp) NB_UB(NEKBONE_PRECOND_IDENTITY, false, true); else NB_UB(NEKBONE_PRECOND_IDENTITY, false, false);
+        }
+    }
+#undef NB_UB
+    if (jac)
+        hipLaunchKernelGGL(k_update_final_batch<NEKBONE_PRECOND_JACOBI>, dim3(nprob),
+                           dim3(kFinalThreads), 0, stream(), d_args, set_xpending);
+    else
+        hipLaunchKernelGGL(k_update_final_batch<NEKBONE_PRECOND_IDENTITY>, dim3(nprob),
+                           dim3(kFinalThreads), 0, stream(), d_args, set_xpending);
+    NB_HIP(hipGetLastError());
+    return NEKBONE_OK;
+}
+
+int cg_ax_blocks(const nekbone_problem *P) { return int(n_ax(P)); }
+int cg_gs_blocks(const nekbone_problem *P) { return int(n_gs(P)); }
 
 int launch_cg_finish(nekbone_problem *P, double *x)
 {

@@ -238,6 +238,42 @@ struct CgScalars {
     double rtr_upd;  // mixed precision: FP64 rtr at the last reliable update
 };
 
+// Per-problem arguments of the batched fused CG kernels.
+struct CgBatchArgs {
+    double *r;
+    const double *w, *dinv;
+    const uint8_t *flag;
+    double *x;
+    const double *p;
+    int64_t n;
+    double *part;       // element + gather-scatter pap parts
+    int64_t npap;
+    double *part_upd;   // update parts (rtr, then rtz)
+    int nupd;
+    CgScalars *scal;
+    double *hist;
+    int hcap;
+};
+
+// Kernel view of a gather-scatter map: the class member lists and the
+// number of workgroups each class takes.
+struct GsArgs {
+    const int32_t *m2, *m4, *m8, *roff, *ridx;
+    int64_t n2, n4, n8, nr;
+    int b2, b4, b8, br;  // workgroups per class
+};
+// Arguments of the fused CG gather-scatter of one problem.
+struct GsCgArgs {
+    double *w;
+    const double *p;
+    const uint8_t *flag;
+    int mbit;
+    GsArgs G;
+    double *part;
+    const CgScalars *scal;
+};
+
+
 // One level of the HSMG preconditioner (src/hsmg.f).  Level vectors are
 // element-blocked nh^3 per element; the finest level is the problem's own
 // (device point order, the problem's gather-scatter map), the coarser
@@ -387,6 +423,14 @@ int launch_pap_final(nekbone_problem *P, const double *part, int64_t npart);
 int launch_update_final(nekbone_problem *P, const double *part, int nu, int precond,
                         int set_xpending);
 int cg_update_blocks(const nekbone_problem *P);
+// batched fused CG (nb_batch.hip drives these)
+GsCgArgs gs_cg_args(nekbone_problem *P, double *w, const double *p, int mbit, double *part);
+int launch_gs_cg_batch(const GsCgArgs *d_args, const int *d_start, int nprob, int nblocks);
+CgBatchArgs cg_batch_args(nekbone_problem *P, double *x, double *r);
+int launch_cg_batch_tail(const CgBatchArgs *d_args, const int *d_ustart, int nprob, int nupd_total,
+                         int precond, bool fma, bool xp, int set_xpending);
+int cg_ax_blocks(const nekbone_problem *P);
+int cg_gs_blocks(const nekbone_problem *P);
 void free_mixed(nekbone_problem *P);
 // gather-scatter with an explicit map (HSMG levels)
 int launch_gs_map(const GsMap &M, double *u);

@@ -15,11 +15,7 @@ namespace {
 // One launch covers all size classes; every workgroup handles one class,
 // so its lanes run the same trip count.
 // ---------------------------------------------------------------------
-struct GsArgs {
-    const int32_t *m2, *m4, *m8, *roff, *ridx;
-    int64_t n2, n4, n8, nr;
-    int b2, b4, b8, br;  // workgroups per class
-};
+
 
 // T = double, or float for the inner solve of the mixed-precision CG (its
 // pap partials are accumulated in FP64 all the same).
@@ -64,13 +60,13 @@ __device__ __forceinline__ void gs_group(T *__restrict__ w, const T *__restrict_
 }
 
 template <typename T, bool CG>
-__global__ void __launch_bounds__(kRedThreads)
-    k_gs_cls(T *__restrict__ w, const T *__restrict__ p,
-             const uint8_t *__restrict__ flag, int mbit, GsArgs G, double *__restrict__ part,
-             const CgScalars *__restrict__ scal)
+__device__ __forceinline__ void gs_body(T *__restrict__ w, const T *__restrict__ p,
+                                        const uint8_t *__restrict__ flag, int mbit,
+                                        const GsArgs &G, double *__restrict__ part,
+                                        const CgScalars *__restrict__ scal, const int blk0)
 {
     if (scal && scal->done) return;
-    int blk = blockIdx.x;
+    int blk = blk0;
     double acc = 0.0;
     if (blk < G.b2) {
         const int64_t g = int64_t(blk) * kRedThreads + threadIdx.x;
@@ -95,8 +91,35 @@ __global__ void __launch_bounds__(kRedThreads)
     if constexpr (CG) {
         double dummy = 0.0;
         block_sum2(acc, dummy);
-        if (threadIdx.x == 0) part[blockIdx.x] = acc;
+        if (threadIdx.x == 0) part[blk0] = acc;
     }
+}
+
+template <typename T, bool CG>
+__global__ void __launch_bounds__(kRedThreads)
+    k_gs_cls(T *__restrict__ w, const T *__restrict__ p,
+             const uint8_t *__restrict__ flag, int mbit, GsArgs G, double *__restrict__ part,
+             const CgScalars *__restrict__ scal)
+{
+    gs_body<T, CG>(w, p, flag, mbit, G, part, scal, blockIdx.x);
+}
+
+// Batched fused CG gather-scatter: problem q owns workgroups
+// [start[q], start[q+1]).
+__global__ void __launch_bounds__(kRedThreads)
+    k_gs_cg_batch(const GsCgArgs *__restrict__ args, const int *__restrict__ start, int nprob)
+{
+    const int b = blockIdx.x;
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (start[mid] <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const GsCgArgs A = args[lo];
+    gs_body<double, true>(A.w, A.p, A.flag, A.mbit, A.G, A.part, A.scal, b - start[lo]);
 }
 
 GsArgs gs_args_map(const GsMap &M)
@@ -395,6 +418,20 @@ int launch_gs_cg(nekbone_problem *P, double *w, const double *p, int mbit, doubl
                        static_cast<const CgScalars *>(P->d_scal));
     NB_HIP(hipGetLastError());
     return timed_end(P, 1);
+}
+
+GsCgArgs gs_cg_args(nekbone_problem *P, double *w, const double *p, int mbit, double *part)
+{
+    return GsCgArgs{w, p, P->d_flag, mbit, gs_args(P), part, P->d_scal};
+}
+
+int launch_gs_cg_batch(const GsCgArgs *d_args, const int *d_start, int nprob, int nblocks)
+{
+    if (nblocks == 0) return NEKBONE_OK;
+    hipLaunchKernelGGL(k_gs_cg_batch, dim3(nblocks), dim3(kRedThreads), 0, stream(), d_args,
+                       d_start, nprob);
+    NB_HIP(hipGetLastError());
+    return NEKBONE_OK;
 }
 
 int launch_gs_map(const GsMap &M, double *u)
