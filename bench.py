@@ -102,6 +102,66 @@ def Q_rtr0(Q, fq):
     return rtr0
 
 
+def splitmix_u01(n, seed):
+    """n values U[0,1) of the SplitMix64 stream with the given seed
+    (value k = mix(seed + (k+1)*0x9E3779B97F4A7C15), top 53 bits)."""
+    import numpy as np
+    k = np.arange(1, n + 1, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + k * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+
+
+def batched_workload(nb, count, seed=2024):
+    """count problems: E_k = round(exp(U[ln 64, ln 8192])) elements in the
+    box cubic(E_k), N_k uniform in {5, 7, 9, 11}, brick geometry,
+    Dirichlet faces, RHS U[-1,1) seed k -> dssum -> *c, Jacobi to 1e-8.
+    Returns the batched and the one-at-a-time solve records."""
+    import math
+    u = splitmix_u01(2 * count, seed)
+    probs, fs, xs = [], [], []
+    for k in range(count):
+        E = int(round(math.exp(math.log(64) + u[2 * k] * (math.log(8192) - math.log(64)))))
+        nx1 = (6, 8, 10, 12)[min(3, int(4 * u[2 * k + 1]))]
+        P = nb.Problem(*nb.cubic(E), nx1)
+        P.set_mask_mode(nb.MASK_DIRICHLET)
+        P.setup_jacobi()
+        f, x = P.vec(), P.vec()
+        P.rand_uniform_d(f, k)
+        P.rhs_from_raw_d(f)
+        probs.append(P), fs.append(f), xs.append(x)
+    nb.sync()
+    B = nb.Batch(probs)
+    B.autotune(nb.PRECOND_JACOBI)
+    tol, maxit = 1e-8, 5000
+    B.cg_tol_d(xs, fs, [None] * count, tol, maxit)
+    nb.sync()
+    t0 = time.perf_counter()
+    its, _ = B.cg_tol_d(xs, fs, [None] * count, tol, maxit)
+    nb.sync()
+    tb = time.perf_counter() - t0
+    rs = [P.vec() for P in probs]
+    nb.sync()
+    t0 = time.perf_counter()
+    its1 = []
+    for P, f, x, r in zip(probs, fs, xs, rs):
+        it, _ = P.cg_tol_d(x, f, r, tol, maxit, nb.PRECOND_JACOBI)
+        its1.append(it)
+    nb.sync()
+    ts = time.perf_counter() - t0
+    npts = sum(P.n for P in probs)
+    work = sum(P.n * it for P, it in zip(probs, its))
+    B.close()
+    return {"problems": count, "local_points": npts, "tol": tol,
+            "orders": sorted(set(P.nx1 - 1 for P in probs)),
+            "iters_min": min(its), "iters_max": max(its), "same_iters_as_single": its == its1,
+            "batched_ms": tb * 1e3, "one_at_a_time_ms": ts * 1e3, "speedup": ts / tb,
+            "DOF_iter_per_s": work / tb}
+
+
 def measured_traffic(kname, mesh):
     """HBM bytes per launch of kernel `kname` on `mesh`, as recorded in the
     counter summary under profiles/ (a line
@@ -133,6 +193,8 @@ def main():
     ap.add_argument("--delta", type=float, default=0.0,
                     help="mixed precision: FP32 residual reduction between FP64 "
                          "residual updates (0: library default)")
+    ap.add_argument("--batch", type=int, default=64,
+                    help="problems in the batched-solve measurement (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tol", action="store_true")
     args = ap.parse_args()
@@ -251,6 +313,10 @@ def main():
                 "max_rel": float(np.abs(d).max() / np.abs(x64).max())},
             "geometry": "deformed +-10% vertices, seed 1", "delta": args.delta or 1e-3}
         del Q, fq, xq, rq, xm
+
+    if args.batch > 0:
+        print("bench: batched solves", file=sys.stderr, flush=True)
+        extra["batched"] = batched_workload(nb, args.batch)
 
     cpu = None
     if not args.no_cpu:
