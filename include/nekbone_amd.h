@@ -110,6 +110,11 @@ int nekbone_set_arith(nekbone_problem *P, int arith);
  * separate passes.  Values agree bit for bit between the two; only the
  * dot products' summation trees differ. */
 int nekbone_set_fused(nekbone_problem *P, int on);
+/* 1 (default): CG iterations are replayed in chunks of 10 from a HIP graph
+ * captured on first use (re-captured when buffers or settings change; not
+ * while per-kernel timing is enabled); 0: every kernel launched directly.
+ * Same kernels, same results. */
+int nekbone_set_graph(nekbone_problem *P, int on);
 
 /* Element-kernel shape: elements per workgroup, geometric factors
  * prefetched at kernel entry (0/1), x update carried by the element kernel
@@ -188,6 +193,11 @@ int nekbone_cg_tol_d(nekbone_problem *P, double *x, const double *f, double *r, 
 int nekbone_cg_mixed_d(nekbone_problem *P, double *x, const double *f, double *r, double tol,
                        int maxiter, int precond, double delta, int *iters, int *updates,
                        double *final_rtr);
+/* 1: the mixed solver's inner iterations read the geometric factors in
+ * FP16 (per element and factor scaled by its largest magnitude, 12 bytes
+ * per point instead of 24); the FP64 residual updates still use the FP64
+ * factors.  0 (default): FP32 factors. */
+int nekbone_set_mixed_g16(nekbone_problem *P, int on);
 /* FP32 element-kernel shape of the mixed solver (as nekbone_get_variant /
  * nekbone_set_variant), and its autotuner: times every compiled shape over
  * a few inner iterations and keeps the fastest for this problem. */

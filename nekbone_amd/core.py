@@ -76,6 +76,7 @@ _SIGS = {
     "nekbone_get_variant_f32": (ctypes.c_int, [_vp] + [ctypes.POINTER(ctypes.c_int)] * 3),
     "nekbone_set_variant_f32": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "nekbone_autotune_mixed": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "nekbone_set_mixed_g16": (ctypes.c_int, [_vp, ctypes.c_int]),
     "nekbone_alloc_vec": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
     "nekbone_free_vec": (ctypes.c_int, [_vp, _vp]),
     "nekbone_h2d": (ctypes.c_int, [_vp, _vp, _dp, ctypes.c_int64]),
@@ -86,6 +87,7 @@ _SIGS = {
     "nekbone_timing_get": (ctypes.c_int, [_vp, ctypes.c_int, _i64p, _dp]),
     "nekbone_timing_reset": (ctypes.c_int, [_vp]),
     "nekbone_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "nekbone_set_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
     "nekbone_get_variant": (ctypes.c_int, [_vp] + [ctypes.POINTER(ctypes.c_int)] * 3),
     "nekbone_set_variant": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "nekbone_bw_probe": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, _dp]),
@@ -318,6 +320,9 @@ class Problem:
     def set_arith(self, arith):
         _chk(lib().nekbone_set_arith(self.handle, arith))
 
+    def set_graph(self, on):
+        _chk(lib().nekbone_set_graph(self.handle, int(on)))
+
     def set_fused(self, on):
         _chk(lib().nekbone_set_fused(self.handle, int(bool(on))))
 
@@ -423,6 +428,9 @@ class Problem:
         finally:
             for v in (fv, xv, rv):
                 v.free()
+
+    def set_mixed_g16(self, on):
+        _chk(lib().nekbone_set_mixed_g16(self.handle, int(on)))
 
     def variant_f32(self):
         v = [ctypes.c_int() for _ in range(3)]

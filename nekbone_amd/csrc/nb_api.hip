@@ -323,6 +323,55 @@ static int cg_step(nekbone_problem *P, double *x, double *r, int precond, int mb
     return launch_cg_iteration(P, x, r, precond, mbit);
 }
 
+static void drop_graph(nekbone_problem *P)
+{
+    if (P->graph) (void)hipGraphExecDestroy(P->graph);
+    P->graph = nullptr;
+    P->graph_key.clear();
+}
+
+// Iterations it0..it0+count-1, from a captured graph of kGraphChunk
+// iterations when possible.  Launch arguments of an iteration depend only
+// on the problem's buffers and settings (all scalars live on the device),
+// so a chunk captured once replays any later chunk of any solve with the
+// same key; event timing is not captured (it needs host bookkeeping).
+constexpr int kGraphChunk = 10;
+static int run_iterations(nekbone_problem *P, double *x, double *r, int precond, int mbit,
+                          int count)
+{
+    const bool graphable = P->use_graph && P->timing.mask == 0;
+    int done = 0;
+    if (graphable && count >= kGraphChunk) {
+        const std::vector<uintptr_t> key = {
+            uintptr_t(x), uintptr_t(r), uintptr_t(precond), uintptr_t(mbit),
+            uintptr_t(P->d_hist), uintptr_t(P->hist_cap), uintptr_t(P->axv.epb),
+            uintptr_t(P->axv.pf), uintptr_t(P->axv.xu), uintptr_t(P->arith),
+            uintptr_t(P->fused), uintptr_t(P->mg), uintptr_t(P->d_dinv)};
+        if (!P->graph || P->graph_key != key) {
+            drop_graph(P);
+            hipGraph_t g = nullptr;
+            NB_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
+            int rc = NEKBONE_OK;
+            for (int i = 0; i < kGraphChunk && !rc; ++i) rc = cg_step(P, x, r, precond, mbit);
+            hipError_t e = hipStreamEndCapture(stream(), &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            NB_HIP(e);
+            e = hipGraphInstantiate(&P->graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            NB_HIP(e);
+            P->graph_key = key;
+        }
+        for (; done + kGraphChunk <= count; done += kGraphChunk)
+            NB_HIP(hipGraphLaunch(P->graph, stream()));
+    }
+    for (; done < count; ++done)
+        if (int rc = cg_step(P, x, r, precond, mbit)) return rc;
+    return NEKBONE_OK;
+}
+
 // CG on device vectors (src/cg.f:2-84).
 static int run_cg(nekbone_problem *P, double *x, const double *f, double *r, int niter,
                   int precond, double tol2)
@@ -333,8 +382,7 @@ static int run_cg(nekbone_problem *P, double *x, const double *f, double *r, int
     const int mbit0 = next_mask_bit(P);  // maskit(r), src/cg.f:40
     if (int rc = launch_cg_init(P, x, r, f, mbit0, precond, tol2)) return rc;
     const int mbit = next_mask_bit(P);  // maskit in ax, src/cg.f:119 (state is now fixed)
-    for (int it = 0; it < niter; ++it)
-        if (int rc = cg_step(P, x, r, precond, mbit)) return rc;
+    if (int rc = run_iterations(P, x, r, precond, mbit, niter)) return rc;
     return launch_cg_finish(P, x);
 }
 
@@ -485,6 +533,7 @@ extern "C" int nekbone_problem_destroy(nekbone_problem *P)
     dfree(P->d_scal);
     free_mixed(P);
     hsmg_free(P);
+    drop_graph(P);
     (void)timing_resolve(P);
     for (hipEvent_t e : P->timing.pool) (void)hipEventDestroy(e);
     if (P->timing.cur) (void)hipEventDestroy(P->timing.cur);
@@ -675,6 +724,14 @@ extern "C" int nekbone_autotune(nekbone_problem *P, int precond, double *best_ms
     return NEKBONE_OK;
 }
 
+extern "C" int nekbone_set_graph(nekbone_problem *P, int on)
+{
+    if (int rc = check(P)) return rc;
+    P->use_graph = on != 0;
+    if (!P->use_graph) drop_graph(P);
+    return NEKBONE_OK;
+}
+
 extern "C" int nekbone_set_fused(nekbone_problem *P, int on)
 {
     if (int rc = check(P)) return rc;
@@ -845,8 +902,9 @@ extern "C" int nekbone_cg_tol_d(nekbone_problem *P, double *x, const double *f, 
     const int chunk = 10;
     CgScalars s{};
     for (int it = 0; it < maxiter; it += chunk) {
-        for (int k = it; k < std::min(maxiter, it + chunk); ++k)
-            if (int rc = cg_step(P, P->d_x, P->d_r, precond, mbit)) return rc;
+        if (int rc = run_iterations(P, P->d_x, P->d_r, precond, mbit,
+                                    std::min(maxiter, it + chunk) - it))
+            return rc;
         NB_HIP(hipMemcpyAsync(&s, P->d_scal, sizeof s, hipMemcpyDeviceToHost, stream()));
         NB_HIP(hipStreamSynchronize(stream()));
         if (s.done) break;

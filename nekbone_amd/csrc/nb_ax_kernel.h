@@ -49,6 +49,8 @@
 #include <map>
 #include <tuple>
 
+#include <hip/hip_fp16.h>
+
 #include "nb_device.h"
 
 namespace nb {
@@ -101,6 +103,8 @@ struct AxArgsT {
     const T *Drow, *Dcol;
     const uint8_t *flag;
     const int16_t *iref;  // reference position of each device position (N3)
+    const __half *gh;     // G16: geometric factors in FP16, g = gh * gsc[e][c]
+    const float *gsc;     // G16: per element and factor scale (6 per element)
     int mbit;
     const CgScalars *scal;
     double *part;        // CG: per-block pap partials
@@ -110,7 +114,7 @@ using AxArgs = AxArgsT<double>;
 
 // Body of the element kernel for workgroup blk of one problem (shared by
 // the single-problem kernel and the batched one).
-template <typename T, int NX, int EPB, bool FMA, int MODE, int PF, bool XU>
+template <typename T, int NX, int EPB, bool FMA, int MODE, int PF, bool XU, bool G16 = false>
 __device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
 {
     constexpr int N2 = NX * NX, N3 = NX * NX * NX;
@@ -130,6 +134,19 @@ __device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
     const T *__restrict__ Drow = A_.Drow;
     const T *__restrict__ Dcol = A_.Dcol;
     const T *ge = A_.g + e * 6 * N3;
+    // G16 (FP32 inner solves only): factor c at point q is gh*scale
+    const __half *gh = G16 ? A_.gh + e * 6 * N3 : nullptr;
+    T gsc[6] = {T(1), T(1), T(1), T(1), T(1), T(1)};
+    if constexpr (G16) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) gsc[c] = T(A_.gsc[e * 6 + c]);
+    }
+    auto gload = [&](int c, int q) -> T {
+        if constexpr (G16)
+            return T(__half2float(gh[c * N3 + q])) * gsc[c];
+        else
+            return ge[c * N3 + q];
+    };
 
     if constexpr (MODE != 0) {
         if (A_.scal->done) return;  // converged tolerance run: nothing to do
@@ -141,7 +158,7 @@ __device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
 #pragma unroll
             for (int c = 0; c < 6; ++c)
 #pragma unroll
-                for (int k = 0; k < NX; ++k) gv[c][k] = ge[c * N3 + a + NX * b + N2 * k];
+                for (int k = 0; k < NX; ++k) gv[c][k] = gload(c, a + NX * b + N2 * k);
         }
     }
 
@@ -236,8 +253,8 @@ __device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
                 g1 = gv[0][k], g2 = gv[1][k], g3 = gv[2][k];
                 g4 = gv[3][k], g5 = gv[4][k], g6 = gv[5][k];
             } else {
-                g1 = ge[q], g2 = ge[N3 + q], g3 = ge[2 * N3 + q];
-                g4 = ge[3 * N3 + q], g5 = ge[4 * N3 + q], g6 = ge[5 * N3 + q];
+                g1 = gload(0, q), g2 = gload(1, q), g3 = gload(2, q);
+                g4 = gload(3, q), g5 = gload(4, q), g6 = gload(5, q);
             }
             T wr, ws;
             if constexpr (FMA) {
@@ -322,10 +339,10 @@ __device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
     }
 }
 
-template <typename T, int NX, int EPB, bool FMA, int MODE, int PF, bool XU>
+template <typename T, int NX, int EPB, bool FMA, int MODE, int PF, bool XU, bool G16 = false>
 __global__ void __launch_bounds__(round64(NX *NX *EPB)) k_ax(AxArgsT<T> A_)
 {
-    ax_body<T, NX, EPB, FMA, MODE, PF, XU>(A_, blockIdx.x);
+    ax_body<T, NX, EPB, FMA, MODE, PF, XU, G16>(A_, blockIdx.x);
 }
 
 // Batched form: problem q owns workgroups [start[q], start[q+1]); its
@@ -417,17 +434,21 @@ void regb_order(TableB &t)
 // FP32 kernels (inner iterations of the mixed-precision CG), always with
 // fused multiply-adds.  Key: (nx, epb, mode, pf, xu).
 using AxFn32 = void (*)(AxArgsT<float>);
-using Key32 = std::tuple<int, int, int, int, int>;
+// Key: (nx, epb, mode, pf, xu, g16)
+using Key32 = std::tuple<int, int, int, int, int, int>;
 using Table32 = std::map<Key32, AxFn32>;
 
 template <int NX, int EPB, int PF>
 void reg32_modes(Table32 &t)
 {
-    t[Key32(NX, EPB, 0, PF, 0)] = k_ax<float, NX, EPB, true, 0, PF, false>;
-    t[Key32(NX, EPB, 1, PF, 0)] = k_ax<float, NX, EPB, true, 1, PF, false>;
-    t[Key32(NX, EPB, 2, PF, 0)] = k_ax<float, NX, EPB, true, 2, PF, false>;
-    t[Key32(NX, EPB, 1, PF, 1)] = k_ax<float, NX, EPB, true, 1, PF, true>;
-    t[Key32(NX, EPB, 2, PF, 1)] = k_ax<float, NX, EPB, true, 2, PF, true>;
+    t[Key32(NX, EPB, 0, PF, 0, 0)] = k_ax<float, NX, EPB, true, 0, PF, false>;
+    t[Key32(NX, EPB, 1, PF, 0, 0)] = k_ax<float, NX, EPB, true, 1, PF, false>;
+    t[Key32(NX, EPB, 2, PF, 0, 0)] = k_ax<float, NX, EPB, true, 2, PF, false>;
+    t[Key32(NX, EPB, 1, PF, 1, 0)] = k_ax<float, NX, EPB, true, 1, PF, true>;
+    t[Key32(NX, EPB, 2, PF, 1, 0)] = k_ax<float, NX, EPB, true, 2, PF, true>;
+    t[Key32(NX, EPB, 0, PF, 0, 1)] = k_ax<float, NX, EPB, true, 0, PF, false, true>;
+    t[Key32(NX, EPB, 1, PF, 1, 1)] = k_ax<float, NX, EPB, true, 1, PF, true, true>;
+    t[Key32(NX, EPB, 2, PF, 1, 1)] = k_ax<float, NX, EPB, true, 2, PF, true, true>;
 }
 
 template <int NX>

@@ -17,57 +17,10 @@
 //
 // Generic path (any c): the reference's sequence with separate kernels
 // (p update, ax, dssum, shift/mask/pap, x/r update with rtr).
-#include "nb_device.h"
+#include "nb_cg_device.h"
 
 namespace nb {
 namespace {
-
-constexpr int kFinalThreads = 1024;
-
-// c = 1/multiplicity (exactly the reference's 1./c(i)); the common hex
-// multiplicities are powers of two, exact without a division.
-__device__ __forceinline__ double inv_mult(uint8_t fl)
-{
-    const int m = fl >> kMultShift;
-    return m == 1 ? 1.0 : m == 2 ? 0.5 : m == 4 ? 0.25 : m == 8 ? 0.125 : 1. / double(m);
-}
-
-// Fixed-order sum of a strided slice with four interleaved accumulators
-// (four loads in flight per thread), combined as (s0+s1)+(s2+s3).
-template <int NT>
-__device__ __forceinline__ double slice_sum(const double *__restrict__ a, int64_t na)
-{
-    double s[4] = {0.0, 0.0, 0.0, 0.0};
-    int64_t i = threadIdx.x;
-    for (; i + 3 * NT < na; i += 4 * NT) {
-#pragma unroll
-        for (int u = 0; u < 4; ++u) s[u] = s[u] + a[i + u * NT];
-    }
-    for (int u = 0; i < na; i += NT, ++u) s[u] = s[u] + a[i];
-    return (s[0] + s[1]) + (s[2] + s[3]);
-}
-
-// Sum of a partials region in a fixed order by one block.
-template <int NT>
-__device__ __forceinline__ void sum_region2(const double *__restrict__ a, int64_t na,
-                                            const double *__restrict__ b, int64_t nb, double &s0,
-                                            double &s1)
-{
-    double a0 = slice_sum<NT>(a, na);
-    double a1 = b ? slice_sum<NT>(b, nb) : 0.0;
-    block_sum2<NT>(a0, a1);
-    s0 = a0;
-    s1 = a1;
-}
-
-__device__ void prepare_next(CgScalars *s, double *hist, int hcap)
-{
-    s->rtz2 = s->rtz1;
-    s->rtz1 = s->rtz_next;
-    s->beta = (s->iter == 0) ? 0.0 : s->rtz1 / s->rtz2;
-    const int h = 1 + 3 * s->iter;
-    if (h < hcap) hist[h] = s->rtz1;
-}
 
 // x = 0, r = mask(f) (src/cg.f:38-40); parts of glsc3(r,c,r) and of
 // glsc3(r,c,z) with z = M^-1 r.
@@ -118,31 +71,11 @@ __global__ void __launch_bounds__(kFinalThreads)
     }
 }
 
-__device__ __forceinline__ void pap_final_body(const double *__restrict__ part, int64_t npart,
-                                               CgScalars *__restrict__ s,
-                                               double *__restrict__ hist, int hcap)
-{
-    if (s->done) return;
-    double pap, dummy;
-    sum_region2<kFinalThreads>(part, npart, nullptr, 0, pap, dummy);
-    if (threadIdx.x == 0) {
-        s->pap = pap;
-        s->alpha = s->rtz1 / pap;
-        s->xpending = 0;  // the element kernel of this iteration applied it
-        if (!(pap > 0.0) || !isfinite(pap)) {
-            s->status = 1;
-            if (s->tol2 > 0.0) s->done = 1;
-        }
-        const int h = 2 + 3 * s->iter;
-        if (h < hcap) hist[h] = pap;
-    }
-}
-
 __global__ void __launch_bounds__(kFinalThreads)
     k_pap_final(const double *__restrict__ part, int64_t npart, CgScalars *__restrict__ s,
                 double *__restrict__ hist, int hcap)
 {
-    pap_final_body(part, npart, s, hist, hcap);
+    pap_final_body<kFinalThreads>(part, npart, s, hist, hcap);
 }
 
 // [XP] x += alpha p (add2s2, src/cg.f:63) when the element kernel does not
@@ -219,32 +152,11 @@ __global__ void __launch_bounds__(kRedThreads)
 }
 
 template <int PRE>
-__device__ __forceinline__ void update_final_body(const double *__restrict__ part, int nblk,
-                                                  CgScalars *__restrict__ s,
-                                                  double *__restrict__ hist, int hcap,
-                                                  int set_xpending)
-{
-    if (s->done) return;
-    double rtr, rtz;
-    sum_region2<kFinalThreads>(part, nblk, part + nblk, nblk, rtr, rtz);
-    if (threadIdx.x == 0) {
-        s->rtr = rtr;
-        const int h = 3 + 3 * s->iter;
-        if (h < hcap) hist[h] = rtr;
-        s->iter += 1;
-        if (set_xpending) s->xpending = 1;
-        s->rtz_next = (PRE == NEKBONE_PRECOND_JACOBI) ? rtz : rtr;
-        if (s->tol2 > 0.0 && rtr <= s->tol2 * s->rtr0) s->done = 1;
-        prepare_next(s, hist, hcap);
-    }
-}
-
-template <int PRE>
 __global__ void __launch_bounds__(kFinalThreads)
     k_update_final(const double *__restrict__ part, int nblk, CgScalars *__restrict__ s,
                    double *__restrict__ hist, int hcap, int set_xpending)
 {
-    update_final_body<PRE>(part, nblk, s, hist, hcap, set_xpending);
+    update_final_body<PRE, kFinalThreads>(part, nblk, s, hist, hcap, set_xpending);
 }
 
 // ---- batched forms: problem q owns workgroups [start[q], start[q+1]) of
@@ -275,7 +187,7 @@ __global__ void __launch_bounds__(kRedThreads)
 __global__ void __launch_bounds__(kFinalThreads) k_pap_final_batch(const CgBatchArgs *__restrict__ args)
 {
     const CgBatchArgs A = args[blockIdx.x];
-    pap_final_body(A.part, A.npap, A.scal, A.hist, A.hcap);
+    pap_final_body<kFinalThreads>(A.part, A.npap, A.scal, A.hist, A.hcap);
 }
 
 template <int PRE>
@@ -283,7 +195,7 @@ __global__ void __launch_bounds__(kFinalThreads)
     k_update_final_batch(const CgBatchArgs *__restrict__ args, int set_xpending)
 {
     const CgBatchArgs A = args[blockIdx.x];
-    update_final_body<PRE>(A.part_upd, A.nupd, A.scal, A.hist, A.hcap, set_xpending);
+    update_final_body<PRE, kFinalThreads>(A.part_upd, A.nupd, A.scal, A.hist, A.hcap, set_xpending);
 }
 
 // The x update of the last completed iteration (the element kernel of

@@ -1,0 +1,100 @@
+// Device helpers of the CG kernels: fixed-order reductions of partial
+// sums and the scalar bookkeeping of an iteration (alpha, beta, rtr,
+// convergence), shared by the single-problem and the batched kernels and
+// by the mixed-precision solver.
+#pragma once
+
+#include "nb_device.h"
+
+namespace nb {
+
+constexpr int kFinalThreads = 1024;
+
+// c = 1/multiplicity (exactly the reference's 1./c(i)); the common hex
+// multiplicities are powers of two, exact without a division.
+__device__ __forceinline__ double inv_mult(uint8_t fl)
+{
+    const int m = fl >> kMultShift;
+    return m == 1 ? 1.0 : m == 2 ? 0.5 : m == 4 ? 0.25 : m == 8 ? 0.125 : 1. / double(m);
+}
+
+// Fixed-order sum of a strided slice with four interleaved accumulators
+// (four loads in flight per thread), combined as (s0+s1)+(s2+s3).
+template <int NT>
+__device__ __forceinline__ double slice_sum(const double *__restrict__ a, int64_t na)
+{
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    int64_t i = threadIdx.x;
+    for (; i + 3 * NT < na; i += 4 * NT) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) s[u] = s[u] + a[i + u * NT];
+    }
+    for (int u = 0; i < na; i += NT, ++u) s[u] = s[u] + a[i];
+    return (s[0] + s[1]) + (s[2] + s[3]);
+}
+
+// Sum of a partials region in a fixed order by one block.
+template <int NT>
+__device__ __forceinline__ void sum_region2(const double *__restrict__ a, int64_t na,
+                                            const double *__restrict__ b, int64_t nb, double &s0,
+                                            double &s1)
+{
+    double a0 = slice_sum<NT>(a, na);
+    double a1 = b ? slice_sum<NT>(b, nb) : 0.0;
+    block_sum2<NT>(a0, a1);
+    s0 = a0;
+    s1 = a1;
+}
+
+__device__ __forceinline__ void prepare_next(CgScalars *s, double *hist, int hcap)
+{
+    s->rtz2 = s->rtz1;
+    s->rtz1 = s->rtz_next;
+    s->beta = (s->iter == 0) ? 0.0 : s->rtz1 / s->rtz2;
+    const int h = 1 + 3 * s->iter;
+    if (h < hcap) hist[h] = s->rtz1;
+}
+
+template <int NT>
+__device__ __forceinline__ void pap_final_body(const double *__restrict__ part, int64_t npart,
+                                               CgScalars *__restrict__ s,
+                                               double *__restrict__ hist, int hcap)
+{
+    if (s->done) return;
+    double pap, dummy;
+    sum_region2<NT>(part, npart, nullptr, 0, pap, dummy);
+    if (threadIdx.x == 0) {
+        s->pap = pap;
+        s->alpha = s->rtz1 / pap;
+        s->xpending = 0;  // the element kernel of this iteration applied it
+        if (!(pap > 0.0) || !isfinite(pap)) {
+            s->status = 1;
+            if (s->tol2 > 0.0) s->done = 1;
+        }
+        const int h = 2 + 3 * s->iter;
+        if (h < hcap) hist[h] = pap;
+    }
+}
+
+template <int PRE, int NT>
+__device__ __forceinline__ void update_final_body(const double *__restrict__ part, int nblk,
+                                                  CgScalars *__restrict__ s,
+                                                  double *__restrict__ hist, int hcap,
+                                                  int set_xpending)
+{
+    if (s->done) return;
+    double rtr, rtz;
+    sum_region2<NT>(part, nblk, part + nblk, nblk, rtr, rtz);
+    if (threadIdx.x == 0) {
+        s->rtr = rtr;
+        const int h = 3 + 3 * s->iter;
+        if (h < hcap) hist[h] = rtr;
+        s->iter += 1;
+        if (set_xpending) s->xpending = 1;
+        s->rtz_next = (PRE == NEKBONE_PRECOND_JACOBI) ? rtz : rtr;
+        if (s->tol2 > 0.0 && rtr <= s->tol2 * s->rtr0) s->done = 1;
+        prepare_next(s, hist, hcap);
+    }
+}
+
+}  // namespace nb

@@ -315,6 +315,11 @@ struct MixedState {
     float *x = nullptr, *r = nullptr, *p = nullptr, *w = nullptr;
     int geom_version = -1, dinv_version = -1;
     AxVariant axv;
+    // optional FP16 geometric factors: g = gh * gsc[e][c] (IEEE half bits)
+    int g16 = 0;
+    uint16_t *gh = nullptr;
+    float *gsc = nullptr;
+    int g16_version = -1;
 };
 
 }  // namespace nb
@@ -364,6 +369,10 @@ struct nekbone_problem {
     int dinv_version = 0;  // bumped whenever d_dinv is recomputed
     nb::MixedState *mx = nullptr;
     nb::Hsmg *mg = nullptr;
+    // captured chunk of CG iterations, replayed while the key matches
+    bool use_graph = true;
+    hipGraphExec_t graph = nullptr;
+    std::vector<uintptr_t> graph_key;
     nb::Timing timing;
 };
 
@@ -392,6 +401,7 @@ int launch_gs_cg(nekbone_problem *P, double *w, const double *p, int mbit, doubl
 // CG variant (pap parts in FP64).
 int launch_gs32(nekbone_problem *P, float *u);
 int launch_gs_cg32(nekbone_problem *P, float *w, const float *p, int mbit, double *part);
+
 int gs_blocks(const nekbone_problem *P);
 int launch_post(nekbone_problem *P, double *w, const double *p, int mbit, bool pap);
 int launch_mask(nekbone_problem *P, double *w, int mbit);

@@ -27,7 +27,7 @@
 #include <tuple>
 
 #include "nb_ax_kernel.h"
-#include "nb_device.h"
+#include "nb_cg_device.h"
 
 namespace nb {
 namespace {
@@ -49,26 +49,44 @@ Table32 &table32()
     return t;
 }
 
-AxFn32 lookup32(int nx, int epb, int mode, int pf, int xu)
+AxFn32 lookup32(int nx, int epb, int mode, int pf, int xu, int g16 = 0)
 {
-    auto it = table32().find(Key32(nx, epb, mode, pf, mode == 0 ? 0 : xu));
+    auto it = table32().find(Key32(nx, epb, mode, pf, mode == 0 ? 0 : xu, g16));
     return it == table32().end() ? nullptr : it->second;
 }
 
 constexpr int kThreads = 256;
-constexpr int kFinalThreads = 1024;
 int grid(int64_t n) { return int((n + kThreads - 1) / kThreads); }
-
-__device__ __forceinline__ double inv_mult(uint8_t fl)
-{
-    const int m = fl >> kMultShift;
-    return m == 1 ? 1.0 : m == 2 ? 0.5 : m == 4 ? 0.25 : m == 8 ? 0.125 : 1. / double(m);
-}
 
 __global__ void k_to_f32(float *__restrict__ dst, const double *__restrict__ src, int64_t n)
 {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) dst[i] = float(src[i]);
+}
+
+// FP16 geometric factors: one workgroup per (element, factor) finds the
+// largest magnitude, stores it as the scale, and the factor divided by it
+// rounded to half precision (relative error <= 2^-11 for every value above
+// 2^-14 of the largest).
+__global__ void __launch_bounds__(256)
+    k_to_g16(__half *__restrict__ gh, float *__restrict__ gsc, const double *__restrict__ g,
+             int n3)
+{
+    const int64_t base = int64_t(blockIdx.x) * n3;
+    double m = 0.0;
+    for (int q = threadIdx.x; q < n3; q += blockDim.x) m = fmax(m, fabs(g[base + q]));
+    __shared__ double sm[256];
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + h]);
+        __syncthreads();
+    }
+    const double scale = sm[0] > 0.0 ? sm[0] : 1.0;
+    if (threadIdx.x == 0) gsc[blockIdx.x] = float(scale);
+    const double inv = 1.0 / double(float(scale));
+    for (int q = threadIdx.x; q < n3; q += blockDim.x)
+        gh[base + q] = __float2half(float(g[base + q] * inv));
 }
 
 // Reference point order <-> device order for float vectors (perm gives
@@ -267,6 +285,18 @@ int ensure_mixed(nekbone_problem *P, int precond)
         }
     }
     MixedState *M = P->mx;
+    if (M->g16 && M->g16_version != P->geom_version) {
+        if (!M->gh)
+            if (int rc = dalloc(&M->gh, 6 * P->n)) return rc;
+        if (!M->gsc)
+            if (int rc = dalloc(&M->gsc, 6 * int64_t(P->nelt))) return rc;
+        const int n3 = P->nx1 * P->nx1 * P->nx1;
+        hipLaunchKernelGGL(k_to_g16, dim3(6 * P->nelt), dim3(256), 0, stream(),
+                           reinterpret_cast<__half *>(M->gh), M->gsc,
+                           static_cast<const double *>(P->d_g), n3);
+        NB_HIP(hipGetLastError());
+        M->g16_version = P->geom_version;
+    }
     if (M->geom_version != P->geom_version) {
         hipLaunchKernelGGL(k_to_f32, dim3(grid(6 * P->n)), dim3(kThreads), 0, stream(), M->g,
                            P->d_g, 6 * P->n);
@@ -287,7 +317,7 @@ int ensure_mixed(nekbone_problem *P, int precond)
 int launch_ax32(nekbone_problem *P, int mode, int mbit, float *w, const float *u)
 {
     MixedState *M = P->mx;
-    AxFn32 fn = lookup32(P->nx1, M->axv.epb, mode, M->axv.pf, M->axv.xu);
+    AxFn32 fn = lookup32(P->nx1, M->axv.epb, mode, M->axv.pf, M->axv.xu, M->g16);
     if (!fn) return fail(NEKBONE_ERR_UNSUPPORTED, "mixed: no FP32 element kernel for this order");
     AxArgsT<float> A{};
     A.w = w;
@@ -305,6 +335,8 @@ int launch_ax32(nekbone_problem *P, int mode, int mbit, float *w, const float *u
     A.scal = P->d_scal;
     A.part = P->d_part;
     A.nelt = P->nelt;
+    A.gh = reinterpret_cast<const __half *>(M->gh);
+    A.gsc = M->gsc;
     const int threads = axk::round64(P->nx1 * P->nx1 * M->axv.epb);
     if (int rc = timed_begin(P, 0)) return rc;
     hipLaunchKernelGGL(fn, dim3(ax32_blocks(P)), dim3(threads), 0, stream(), A);
@@ -387,6 +419,8 @@ void free_mixed(nekbone_problem *P)
     if (!P->mx) return;
     MixedState *M = P->mx;
     for (float **v : {&M->g, &M->Drow, &M->Dcol, &M->dinv, &M->x, &M->r, &M->p, &M->w}) dfree(*v);
+    dfree(M->gh);
+    dfree(M->gsc);
     delete M;
     P->mx = nullptr;
 }
@@ -450,6 +484,23 @@ extern "C" int nekbone_cg_mixed_d(nekbone_problem *P, double *x, const double *f
     return NEKBONE_OK;
 }
 
+extern "C" int nekbone_set_mixed_g16(nekbone_problem *P, int on)
+{
+    if (int rc = check(P)) return rc;
+    if (int rc = ensure_mixed(P, NEKBONE_PRECOND_IDENTITY)) return rc;
+    MixedState *M = P->mx;
+    const int g16 = on ? 1 : 0;
+    if (g16 && !lookup32(P->nx1, M->axv.epb, 2, M->axv.pf, M->axv.xu, 1)) {
+        // fall back to a shape compiled with FP16 factors
+        AxVariant v{axk::epb_for(P->nx1), 0, 1};
+        if (!lookup32(P->nx1, v.epb, 2, v.pf, v.xu, 1))
+            return fail(NEKBONE_ERR_UNSUPPORTED, "mixed: no FP16-factor kernel for this order");
+        M->axv = v;
+    }
+    M->g16 = g16;
+    return ensure_mixed(P, NEKBONE_PRECOND_IDENTITY);
+}
+
 extern "C" int nekbone_get_variant_f32(nekbone_problem *P, int *epb, int *pf, int *xu)
 {
     if (int rc = check(P)) return rc;
@@ -464,7 +515,8 @@ extern "C" int nekbone_set_variant_f32(nekbone_problem *P, int epb, int pf, int 
 {
     if (int rc = check(P)) return rc;
     if (int rc = ensure_mixed(P, NEKBONE_PRECOND_IDENTITY)) return rc;
-    if (!lookup32(P->nx1, epb, 2, pf, xu ? 1 : 0) || !lookup32(P->nx1, epb, 0, pf, 0))
+    if (!lookup32(P->nx1, epb, 2, pf, xu ? 1 : 0, P->mx->g16) ||
+        !lookup32(P->nx1, epb, 0, pf, 0, P->mx->g16))
         return fail(NEKBONE_ERR_UNSUPPORTED, "set_variant_f32: shape not compiled for this order");
     P->mx->axv = AxVariant{epb, pf, xu ? 1 : 0};
     return NEKBONE_OK;
@@ -482,9 +534,9 @@ extern "C" int nekbone_autotune_mixed(nekbone_problem *P, int precond, double *b
     MixedState *M = P->mx;
     std::vector<AxVariant> cands;
     for (const auto &kv : table32()) {
-        int nx, epb, mode, pf, xu;
-        std::tie(nx, epb, mode, pf, xu) = kv.first;
-        if (nx == P->nx1 && mode == 2) cands.push_back(AxVariant{epb, pf, xu});
+        int nx, epb, mode, pf, xu, g16;
+        std::tie(nx, epb, mode, pf, xu, g16) = kv.first;
+        if (nx == P->nx1 && mode == 2 && g16 == M->g16) cands.push_back(AxVariant{epb, pf, xu});
     }
     const AxVariant keep = M->axv;
     const int mbit = P->mask_mode == NEKBONE_MASK_DIRICHLET ? 1 : 2;

@@ -279,3 +279,26 @@ def test_cg_history_full_size_vs_reference(nb, golden_c1):
     P.setup_jacobi()
     hj = P.cg_d(x, f, r, 100, nb.PRECOND_JACOBI, want_hist=True)
     assert hj[-1] < 1e-12 * hj[0]
+
+
+@pytest.mark.parametrize("precond", [0, 1])
+def test_graph_replay_matches_direct_launches(nb, oracle, precond):
+    """Iterations replayed from the captured graph (chunks of 10) and the
+    same iterations launched one by one give identical vectors and
+    histories; 25 iterations cover two replays and a remainder."""
+    _case("graph replay precond=%d" % precond)
+    nel, nx1 = (3, 4, 2), 9
+    O = oracle.Problem(*nel, nx1)
+    f = O.rhs_from_raw(splitmix_uniform(O.n, 8))
+    res = []
+    for graph in (True, False):
+        P = nb.Problem(*nel, nx1)
+        P.set_mask_mode(nb.MASK_DIRICHLET)
+        P.set_geometry_deformed(0.1, 4)
+        P.setup_jacobi()
+        P.set_graph(graph)
+        res.append(P.cg(f, 25, precond))
+        res.append(P.cg(f, 25, precond))  # second solve replays the cached graph
+    for a, b in ((res[0], res[2]), (res[1], res[3]), (res[0], res[1])):
+        for u, v in zip(a, b):
+            assert np.array_equal(u, v)
