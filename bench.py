@@ -176,6 +176,90 @@ def measured_traffic(kname, mesh):
     return None
 
 
+def tol_workload(nb, nel, nx1, delta):
+    """Time to a 1e-10 relative residual on the deformed mesh (vertices
+    moved by U[-0.1,0.1) of the element width, seed 1): plain FP64 Jacobi
+    CG against the mixed-precision CG (FP32 inner iterations, FP64
+    reliable updates, the same FP64 stopping rule)."""
+    import numpy as np
+    Q = nb.Problem(*nel, nx1)
+    Q.set_mask_mode(nb.MASK_DIRICHLET)
+    Q.set_geometry_deformed(0.1, 1)
+    Q.setup_jacobi()
+    fq, xq, rq = Q.vec(), Q.vec(), Q.vec()
+    Q.rand_uniform_d(fq, 0)
+    Q.rhs_from_raw_d(fq)
+    Q.cg_tol_d(xq, fq, rq, 1e-10, 5000, nb.PRECOND_JACOBI)
+    nb.sync()
+    ta = time.perf_counter()
+    iters, _ = Q.cg_tol_d(xq, fq, rq, 1e-10, 5000, nb.PRECOND_JACOBI)
+    nb.sync()
+    tb = time.perf_counter()
+    x64 = xq.download()
+    # the first call builds the FP32 copies
+    xm = Q.vec()
+    Q.cg_mixed_d(xm, fq, rq, 1e-10, 5000, nb.PRECOND_JACOBI, delta)
+    nb.sync()
+    tc = time.perf_counter()
+    m_it, m_upd, m_rtr = Q.cg_mixed_d(xm, fq, rq, 1e-10, 5000, nb.PRECOND_JACOBI, delta)
+    nb.sync()
+    td = time.perf_counter()
+    xmh = xm.download()
+    d = xmh - x64
+    fp64_ms, mixed_ms = (tb - ta) * 1e3, (td - tc) * 1e3
+    return {
+        "fp64_ms": fp64_ms, "fp64_iters": iters, "mixed_ms": mixed_ms,
+        "mixed_iters": m_it, "mixed_fp64_updates": m_upd, "speedup": fp64_ms / mixed_ms,
+        "mixed_final_rel_residual": (m_rtr / max(Q_rtr0(Q, fq), 1e-300)) ** 0.5,
+        "mixed_vs_fp64_solution": {
+            "l2_rel": float(np.linalg.norm(d) / np.linalg.norm(x64)),
+            "max_rel": float(np.abs(d).max() / np.abs(x64).max())},
+        "geometry": "deformed +-10% vertices, seed 1", "delta": delta or 1e-3}
+
+
+def hsmg_workload(nb, tol=1e-10):
+    """HSMG-preconditioned FP64 CG on a 16x16x16-element box at N=7,
+    deformed +-10% (seed 1), Dirichlet faces, RHS U[-1,1) seed 0, to a
+    1e-10 relative residual: with the FP64 smoother (the reference's
+    arithmetic), with the FP32 local solves, and Jacobi CG beside them."""
+    import numpy as np
+    P = nb.Problem(16, 16, 16, 8)
+    P.set_mask_mode(nb.MASK_DIRICHLET)
+    P.set_geometry_deformed(0.1, 1)
+    t0 = time.perf_counter()
+    P.setup_hsmg()
+    setup_ms = (time.perf_counter() - t0) * 1e3
+    P.setup_jacobi()
+    f, x, r = P.vec(), P.vec(), P.vec()
+    P.rand_uniform_d(f, 0)
+    P.rhs_from_raw_d(f)
+    out = {"elements": P.nelt, "nx1": 8, "local_points": P.n, "tol": tol,
+           "geometry": "deformed +-10% vertices, seed 1", "setup_ms": setup_ms}
+    xs = {}
+    for name, pre, fp32 in (("fp64_smoother", nb.PRECOND_HSMG, 0),
+                            ("fp32_smoother", nb.PRECOND_HSMG, 1),
+                            ("jacobi", nb.PRECOND_JACOBI, 0)):
+        P.set_hsmg_fp32(fp32)
+        P.cg_tol_d(x, f, r, tol, 5000, pre)
+        nb.sync()
+        t0 = time.perf_counter()
+        it, _ = P.cg_tol_d(x, f, r, tol, 5000, pre)
+        nb.sync()
+        ms = (time.perf_counter() - t0) * 1e3
+        out[name] = {"iters": it, "ms": ms, "ms_per_iter": ms / max(it, 1)}
+        xs[name] = x.download()
+    P.set_hsmg_fp32(0)
+    d = xs["fp32_smoother"] - xs["fp64_smoother"]
+    ref = xs["fp64_smoother"]
+    out["fp32_vs_fp64_smoother_solution"] = {
+        "l2_rel": float(np.linalg.norm(d) / np.linalg.norm(ref)),
+        "max_rel": float(np.abs(d).max() / np.abs(ref).max())}
+    out["fp32_smoother_speedup"] = out["fp64_smoother"]["ms"] / out["fp32_smoother"]["ms"]
+    for v in (f, x, r):
+        v.free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -195,6 +279,10 @@ def main():
                          "residual updates (0: library default)")
     ap.add_argument("--batch", type=int, default=64,
                     help="problems in the batched-solve measurement (0: skip)")
+    ap.add_argument("--no-hsmg", dest="hsmg", action="store_false",
+                    help="skip the HSMG-preconditioned solve on 16x16x16 elements, N=7")
+    ap.add_argument("--only", choices=("tol", "batch", "hsmg"), default=None,
+                    help="run only this part (no timed CG steps) and print its record")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tol", action="store_true")
     args = ap.parse_args()
@@ -214,6 +302,12 @@ def main():
         return
     nel = (args.nel, args.nel, args.nel)
     nx1, niter = args.nx1, args.niter
+    if args.only:
+        part = {"tol": lambda: tol_workload(nb, nel, nx1, args.delta),
+                "batch": lambda: batched_workload(nb, args.batch),
+                "hsmg": lambda: hsmg_workload(nb)}[args.only]()
+        print(json.dumps({args.only: part}))
+        return
     P = nb.Problem(*nel, nx1)
     n = P.n
     P.set_mask_mode(nb.MASK_DIRICHLET)
@@ -277,42 +371,11 @@ def main():
 
     extra = {}
     if not args.no_tol:
-        Q = nb.Problem(*nel, nx1)
-        Q.set_mask_mode(nb.MASK_DIRICHLET)
-        Q.set_geometry_deformed(0.1, 1)
-        Q.setup_jacobi()
-        fq, xq, rq = Q.vec(), Q.vec(), Q.vec()
-        Q.rand_uniform_d(fq, 0)
-        Q.rhs_from_raw_d(fq)
-        Q.cg_tol_d(xq, fq, rq, 1e-10, 5000, nb.PRECOND_JACOBI)
-        nb.sync()
-        ta = time.perf_counter()
-        iters, _ = Q.cg_tol_d(xq, fq, rq, 1e-10, 5000, nb.PRECOND_JACOBI)
-        nb.sync()
-        tb = time.perf_counter()
-        x64 = xq.download()
-        # mixed precision: FP32 inner iterations, FP64 reliable updates,
-        # the same FP64 stopping rule; first call builds the FP32 copies
-        xm = Q.vec()
-        Q.cg_mixed_d(xm, fq, rq, 1e-10, 5000, nb.PRECOND_JACOBI, args.delta)
-        nb.sync()
-        tc = time.perf_counter()
-        m_it, m_upd, m_rtr = Q.cg_mixed_d(xm, fq, rq, 1e-10, 5000, nb.PRECOND_JACOBI,
-                                          args.delta)
-        nb.sync()
-        td = time.perf_counter()
-        xmh = xm.download()
-        d = xmh - x64
-        fp64_ms, mixed_ms = (tb - ta) * 1e3, (td - tc) * 1e3
-        extra["time_to_1e-10"] = {
-            "fp64_ms": fp64_ms, "fp64_iters": iters, "mixed_ms": mixed_ms,
-            "mixed_iters": m_it, "mixed_fp64_updates": m_upd, "speedup": fp64_ms / mixed_ms,
-            "mixed_final_rel_residual": (m_rtr / max(Q_rtr0(Q, fq), 1e-300)) ** 0.5,
-            "mixed_vs_fp64_solution": {
-                "l2_rel": float(np.linalg.norm(d) / np.linalg.norm(x64)),
-                "max_rel": float(np.abs(d).max() / np.abs(x64).max())},
-            "geometry": "deformed +-10% vertices, seed 1", "delta": args.delta or 1e-3}
-        del Q, fq, xq, rq, xm
+        print("bench: time to 1e-10", file=sys.stderr, flush=True)
+        extra["time_to_1e-10"] = tol_workload(nb, nel, nx1, args.delta)
+    if args.hsmg:
+        print("bench: hsmg", file=sys.stderr, flush=True)
+        extra["hsmg"] = hsmg_workload(nb)
 
     if args.batch > 0:
         print("bench: batched solves", file=sys.stderr, flush=True)

@@ -139,6 +139,12 @@ int nekbone_get_dinv(nekbone_problem *P, double *dinv);
  * Dirichlet points zeroed, as the reference leaves it (:1067). */
 int nekbone_setup_hsmg(nekbone_problem *P);
 int nekbone_hsmg_solve(nekbone_problem *P, double *z, double *r);
+/* on = 1: the local FDM solves of the Schwarz smoother (levels nx1-1 and
+ * the middle level) run in FP32 with FP32 copies of the eigenbases built
+ * by nekbone_setup_hsmg; restriction, interpolation, the coarse level and
+ * the outer CG stay FP64.  on = 0 (default): everything FP64.  The
+ * reference has no FP32 path (it is this library's option). */
+int nekbone_set_hsmg_fp32(nekbone_problem *P, int on);
 
 /* ---------------------------------------------------------------------- */
 /* Operators (host arrays; *_d: device pointers on the library stream)     */

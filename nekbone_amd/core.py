@@ -56,6 +56,7 @@ _SIGS = {
     "nekbone_setup_jacobi": (ctypes.c_int, [_vp]),
     "nekbone_setup_hsmg": (ctypes.c_int, [_vp]),
     "nekbone_hsmg_solve": (ctypes.c_int, [_vp, _dp, _dp]),
+    "nekbone_set_hsmg_fp32": (ctypes.c_int, [_vp, ctypes.c_int]),
     "nekbone_get_dinv": (ctypes.c_int, [_vp, _dp]),
     "nekbone_ax": (ctypes.c_int, [_vp, _dp, _dp]),
     "nekbone_ax_d": (ctypes.c_int, [_vp, _vp, _vp]),
@@ -331,6 +332,10 @@ class Problem:
 
     def setup_hsmg(self):
         _chk(lib().nekbone_setup_hsmg(self.handle))
+
+    def set_hsmg_fp32(self, on):
+        """FP32 local FDM solves in the HSMG smoother (FP64 otherwise)."""
+        _chk(lib().nekbone_set_hsmg_fp32(self.handle, int(bool(on))))
 
     def hsmg_solve(self, r):
         """z = M^-1 r (h1mg_solve); returns z and r as the preconditioner

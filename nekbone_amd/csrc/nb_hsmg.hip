@@ -30,31 +30,10 @@ struct FdmArgs {
     double *r;
     const uint8_t *bnd;
     const int32_t *pos, *gsrc, *gin;
-    const double *ops;
     const int16_t *eop;
-    const double *eps;
     double *eown, *ghost;
-    int nh, nl;
+    int nelt;
 };
-
-__device__ __forceinline__ void ghost_ext(int f, int t, int nh, int &I, int &J, int &K)
-{
-    const int nl = nh + 2, a = t % nh + 1, b = t / nh + 1;
-    const int side = (f & 1) ? nl - 1 : 0;
-    if (f < 2) {
-        I = side;
-        J = a;
-        K = b;
-    } else if (f < 4) {
-        I = a;
-        J = side;
-        K = b;
-    } else {
-        I = a;
-        J = b;
-        K = side;
-    }
-}
 
 // out(i,jk) = sum_m M(i,m) in(m,jk)       (first index)
 __device__ void tx(double *out, const double *in, const double *M, int nv, int nu, int n23)
@@ -88,75 +67,268 @@ __device__ void tz(double *out, const double *in, const double *Ct, int n12, int
     }
 }
 
-__global__ void __launch_bounds__(kT) k_fdm(FdmArgs a)
+// Sizes of the local FDM solve on a level with NH points per direction:
+// the extended element has NL = NH+2 points per direction; one thread per
+// grid line (NL^2 per element), EPB elements per workgroup.
+template <int NH>
+struct FdmShape {
+    static constexpr int NL = NH + 2, L2 = NL * NL, L3 = L2 * NL;
+    static constexpr int EPB = L2 >= 256 ? 1 : 256 / L2;
+    static constexpr int NT = (EPB * L2 + 63) / 64 * 64;
+    static constexpr int OPS = 2 * L2 + NL;  // S, S^T (column-major), eigenvalues
+};
+
+// v <- C v for a column-major NL x NL matrix C in LDS: out_i is summed
+// left to right from the first product, C(i,0) v_0 + C(i,1) v_1 + ...
+// (the mxm order); FP64 rounds every product and sum, FP32 fuses them.
+template <typename T, int NL>
+__device__ __forceinline__ void line_apply(T (&v)[NL], const T *__restrict__ C)
 {
-    extern __shared__ double sm[];
-    const int nh = a.nh, nl = a.nl, l2 = nl * nl, l3 = l2 * nl, n3 = nh * nh * nh;
-    const int h2 = nh * nh, g2 = 6 * h2, opsz = 2 * l2 + nl;
-    double *A = sm, *B = sm + l3, *S = sm + 2 * l3;
-    const int e = blockIdx.x, tid = threadIdx.x;
-    for (int d = 0; d < 3; ++d) {
-        const double *src = a.ops + int64_t(a.eop[3 * e + d]) * opsz;
-        for (int q = tid; q < opsz; q += blockDim.x) S[d * opsz + q] = src[q];
-    }
-    for (int q = tid; q < l3; q += blockDim.x) A[q] = 0.0;
-    __syncthreads();
-    const int64_t base = int64_t(e) * n3;
-    for (int p = tid; p < n3; p += blockDim.x) {
-        const int i = p % nh, j = (p / nh) % nh, k = p / h2;
-        const int64_t m = base + a.pos[p];
-        double v = a.r[m];
-        if (a.bnd[m]) {
-            v = 0.0;
-            a.r[m] = 0.0;
+    T o[NL];
+#pragma unroll
+    for (int i = 0; i < NL; ++i) o[i] = C[i] * v[0];
+#pragma unroll
+    for (int m = 1; m < NL; ++m)
+#pragma unroll
+        for (int i = 0; i < NL; ++i) {
+            if constexpr (sizeof(T) == 4)
+                o[i] = __builtin_fmaf(C[i + NL * m], v[m], o[i]);
+            else
+                o[i] = o[i] + C[i + NL * m] * v[m];
         }
-        A[(i + 1) + nl * (j + 1) + l2 * (k + 1)] = v;
-    }
-    for (int q = tid; q < g2; q += blockDim.x) {
-        int I, J, K;
-        ghost_ext(q / h2, q % h2, nh, I, J, K);
-        const int32_t s = a.gsrc[int64_t(e) * g2 + q];
-        double v = 0.0;
-        if (s >= 0) {
-            const int32_t o = a.gin[int64_t(e) * g2 + q];
-            const double vo = a.bnd[o] ? 0.0 : a.r[o];
-            const double vs = a.bnd[s] ? 0.0 : a.r[s];
-            v = (vo + vs) - vo;
+#pragma unroll
+    for (int i = 0; i < NL; ++i) v[i] = o[i];
+}
+
+template <typename T, int NL>
+__device__ __forceinline__ void line_load(T (&v)[NL], const T *b, int base, int stride)
+{
+#pragma unroll
+    for (int m = 0; m < NL; ++m) v[m] = b[base + stride * m];
+}
+
+template <typename T, int NL>
+__device__ __forceinline__ void line_store(const T (&v)[NL], T *b, int base, int stride)
+{
+#pragma unroll
+    for (int m = 0; m < NL; ++m) b[base + stride * m] = v[m];
+}
+
+// Extended-block position of ghost slot t of face f (x-, x+, y-, y+, z-,
+// z+): the face's (a, b) = (t % NH + 1, t / NH + 1) in its two tangential
+// directions, the normal coordinate 0 or NL-1.
+template <int NH>
+__device__ __forceinline__ int ghost_ext(int f, int t)
+{
+    constexpr int NL = NH + 2;
+    const int a = t % NH + 1, b = t / NH + 1, side = (f & 1) ? NL - 1 : 0;
+    if (f < 2) return side + NL * a + NL * NL * b;
+    if (f < 4) return a + NL * side + NL * NL * b;
+    return a + NL * b + NL * NL * side;
+}
+
+// Extended-block position of own point p = i + NH j + NH^2 k.
+template <int NH>
+__device__ __forceinline__ int own_ext(int p)
+{
+    constexpr int NL = NH + 2;
+    const int i = p % NH, j = (p / NH) % NH, k = p / (NH * NH);
+    return (i + 1) + NL * (j + 1) + NL * NL * (k + 1);
+}
+
+// Local solves of one Schwarz level (h1mg_schwarz_part1, src/hsmg.f:1052-1079):
+//   mask r in place (the reference's side effect, :1067); build the
+//   extended element (own points, each ghost face = the neighbour's point
+//   one inside, delivered as (own + neighbour) - own like the (n+2)^3
+//   exchange of :1071-1076; edges and corners zero); e = S D^-1 S^T r with
+//   S = S_z x S_y x S_x (h1mg_do_fast, :1297-1320): contractions along x,
+//   y, z with the S^T factors, the diagonal (l_x + l_y + l_z, cut to zero
+//   at or below the element's eps), then x, y, z with the S factors; write
+//   the own part and the ghost faces of e.
+// The element's block sits in LDS; each contraction reads a thread's line
+// into registers and writes it back in place.  T is the arithmetic of the
+// local solve (FP64, or FP32 for the single-precision smoother); the level
+// vectors stay FP64.
+template <typename T, int NH>
+__global__ void __launch_bounds__(FdmShape<NH>::NT)
+    k_fdm(FdmArgs a, const T *__restrict__ ops, const T *__restrict__ epsv)
+{
+    using S = FdmShape<NH>;
+    constexpr int NL = S::NL, L2 = S::L2, L3 = S::L3, EPB = S::EPB, OPS = S::OPS;
+    constexpr int H2 = NH * NH, N3 = H2 * NH, G2 = 6 * H2;
+    __shared__ T blk[EPB][L3];
+    __shared__ T op[EPB][3][OPS];
+    const int e0 = blockIdx.x * EPB, tid = threadIdx.x, nelt = a.nelt;
+    {
+        constexpr int IOP = (EPB * 3 * OPS + S::NT - 1) / S::NT;
+        T vop[IOP];
+#pragma unroll
+        for (int it = 0; it < IOP; ++it) {
+            const int q = tid + it * S::NT, el = q / (3 * OPS), w = q - el * (3 * OPS),
+                      d = w / OPS;
+            vop[it] = T(0);
+            if (q < EPB * 3 * OPS && e0 + el < nelt)
+                vop[it] = ops[int64_t(a.eop[3 * (e0 + el) + d]) * OPS + (w - d * OPS)];
         }
-        A[I + nl * J + l2 * K] = v;
+#pragma unroll
+        for (int it = 0; it < IOP; ++it) {
+            const int q = tid + it * S::NT;
+            if (q < EPB * 3 * OPS) (&op[0][0][0])[q] = vop[it];
+        }
+    }
+    // edges and corners of the extended block stay zero
+    for (int q = tid; q < EPB * L3; q += S::NT) (&blk[0][0])[q] = T(0);
+    __syncthreads();
+    // Gathers first (unrolled, no stores in between, so every thread's
+    // loads are in flight together), then the LDS writes.  Dirichlet
+    // points read as zero here; r itself is masked at the end.
+    constexpr int IO = (EPB * N3 + S::NT - 1) / S::NT, IG = (EPB * G2 + S::NT - 1) / S::NT;
+    double vown[IO], vgh[IG];
+#pragma unroll
+    for (int it = 0; it < IO; ++it) {
+        const int q = tid + it * S::NT, el = q / N3, e = e0 + el;
+        vown[it] = 0.0;
+        if (q < EPB * N3 && e < nelt) {
+            const int64_t m = int64_t(e) * N3 + a.pos[q - el * N3];
+            vown[it] = a.bnd[m] ? 0.0 : a.r[m];
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < IG; ++it) {
+        const int q = tid + it * S::NT, el = q / G2, e = e0 + el;
+        vgh[it] = 0.0;
+        if (q < EPB * G2 && e < nelt) {
+            const int64_t slot = int64_t(e) * G2 + (q - el * G2);
+            const int32_t sn = a.gsrc[slot];
+            if (sn >= 0) {
+                const int32_t o = a.gin[slot];
+                const double vo = a.bnd[o] ? 0.0 : a.r[o];
+                const double vs = a.bnd[sn] ? 0.0 : a.r[sn];
+                vgh[it] = (vo + vs) - vo;
+            }
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < IO; ++it) {
+        const int q = tid + it * S::NT, el = q / N3;
+        if (q < EPB * N3 && e0 + el < nelt) blk[el][own_ext<NH>(q - el * N3)] = T(vown[it]);
+    }
+#pragma unroll
+    for (int it = 0; it < IG; ++it) {
+        const int q = tid + it * S::NT, el = q / G2;
+        if (q < EPB * G2 && e0 + el < nelt) {
+            const int w = q - el * G2;
+            blk[el][ghost_ext<NH>(w / H2, w - (w / H2) * H2)] = T(vgh[it]);
+        }
     }
     __syncthreads();
-    const double *Sx = S, *SxT = S + l2, *lx = S + 2 * l2;
-    const double *Sy = S + opsz, *SyT = Sy + l2, *ly = Sy + 2 * l2;
-    const double *Sz = S + 2 * opsz, *SzT = Sz + l2, *lz = Sz + 2 * l2;
-    tx(B, A, SxT, nl, nl, l2);
-    __syncthreads();
-    ty(A, B, Sy, nl, nl, nl, nl);
-    __syncthreads();
-    tz(B, A, Sz, l2, nl, nl);
-    __syncthreads();
-    const double eps = a.eps[e];
-    for (int q = tid; q < l3; q += blockDim.x) {
-        const int i = q % nl, j = (q / nl) % nl, k = q / l2;
-        const double diag = lx[i] + ly[j] + lz[k];
-        const double d = (diag > eps) ? 1.0 / diag : 0.0;
-        A[q] = d * B[q];
+    const int el = tid / L2, l = tid - el * L2, la = l % NL, lb = l / NL;
+    const bool act = el < EPB && e0 + el < nelt;
+    T *b = blk[act ? el : 0];
+    const T *ox = op[act ? el : 0][0], *oy = op[act ? el : 0][1], *oz = op[act ? el : 0][2];
+    T v[NL];
+    // S^T r: x lines (la, lb) = (j, k), y lines (i, k), z lines (i, j)
+    if (act) {
+        line_load(v, b, NL * l, 1);
+        line_apply<T, NL>(v, ox + L2);
+        line_store(v, b, NL * l, 1);
     }
     __syncthreads();
-    tx(B, A, Sx, nl, nl, l2);
-    __syncthreads();
-    ty(A, B, SyT, nl, nl, nl, nl);
-    __syncthreads();
-    tz(B, A, SzT, l2, nl, nl);
-    __syncthreads();
-    for (int p = tid; p < n3; p += blockDim.x) {
-        const int i = p % nh, j = (p / nh) % nh, k = p / h2;
-        a.eown[base + a.pos[p]] = B[(i + 1) + nl * (j + 1) + l2 * (k + 1)];
+    if (act) {
+        line_load(v, b, la + L2 * lb, NL);
+        line_apply<T, NL>(v, oy + L2);
+        line_store(v, b, la + L2 * lb, NL);
     }
-    for (int q = tid; q < g2; q += blockDim.x) {
-        int I, J, K;
-        ghost_ext(q / h2, q % h2, nh, I, J, K);
-        a.ghost[int64_t(e) * g2 + q] = B[I + nl * J + l2 * K];
+    __syncthreads();
+    if (act) {
+        line_load(v, b, l, L2);
+        line_apply<T, NL>(v, oz + L2);
+        const T eps = epsv[e0 + el], lxy = ox[2 * L2 + la] + oy[2 * L2 + lb];
+#pragma unroll
+        for (int k = 0; k < NL; ++k) {
+            const T diag = lxy + oz[2 * L2 + k];
+            const T d = (diag > eps) ? T(1) / diag : T(0);
+            v[k] = d * v[k];
+        }
+        line_store(v, b, l, L2);
+    }
+    __syncthreads();
+    if (act) {
+        line_load(v, b, NL * l, 1);
+        line_apply<T, NL>(v, ox);
+        line_store(v, b, NL * l, 1);
+    }
+    __syncthreads();
+    if (act) {
+        line_load(v, b, la + L2 * lb, NL);
+        line_apply<T, NL>(v, oy);
+        line_store(v, b, la + L2 * lb, NL);
+    }
+    __syncthreads();
+    if (act) {
+        line_load(v, b, l, L2);
+        line_apply<T, NL>(v, oz);
+        line_store(v, b, l, L2);
+    }
+    __syncthreads();
+    {
+        int64_t mo[IO];
+        uint8_t bo[IO];
+#pragma unroll
+        for (int it = 0; it < IO; ++it) {
+            const int q = tid + it * S::NT, el2 = q / N3, e = e0 + el2;
+            mo[it] = -1;
+            bo[it] = 0;
+            if (q < EPB * N3 && e < nelt) {
+                mo[it] = int64_t(e) * N3 + a.pos[q - el2 * N3];
+                bo[it] = a.bnd[mo[it]];
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < IO; ++it) {
+            const int q = tid + it * S::NT, el2 = q / N3;
+            if (mo[it] < 0) continue;
+            a.eown[mo[it]] = double(blk[el2][own_ext<NH>(q - el2 * N3)]);
+            if (bo[it]) a.r[mo[it]] = 0.0;  // the caller's r leaves masked (src/hsmg.f:1067)
+        }
+    }
+    for (int q = tid; q < EPB * G2; q += S::NT) {
+        const int el2 = q / G2, e = e0 + el2;
+        if (e >= nelt) continue;
+        const int w = q - el2 * G2;
+        a.ghost[int64_t(e) * G2 + w] = double(blk[el2][ghost_ext<NH>(w / H2, w - (w / H2) * H2)]);
+    }
+}
+
+template <typename T, int NH>
+int launch_fdm_nh(const FdmArgs &a, const T *ops, const T *eps)
+{
+    using S = FdmShape<NH>;
+    hipLaunchKernelGGL((k_fdm<T, NH>), dim3((a.nelt + S::EPB - 1) / S::EPB), dim3(S::NT), 0,
+                       stream(), a, ops, eps);
+    NB_HIP(hipGetLastError());
+    return NEKBONE_OK;
+}
+
+template <typename T>
+int launch_fdm(const FdmArgs &a, const T *ops, const T *eps, int nh)
+{
+    switch (nh) {
+    case 3: return launch_fdm_nh<T, 3>(a, ops, eps);
+    case 4: return launch_fdm_nh<T, 4>(a, ops, eps);
+    case 5: return launch_fdm_nh<T, 5>(a, ops, eps);
+    case 6: return launch_fdm_nh<T, 6>(a, ops, eps);
+    case 7: return launch_fdm_nh<T, 7>(a, ops, eps);
+    case 8: return launch_fdm_nh<T, 8>(a, ops, eps);
+    case 9: return launch_fdm_nh<T, 9>(a, ops, eps);
+    case 10: return launch_fdm_nh<T, 10>(a, ops, eps);
+    case 11: return launch_fdm_nh<T, 11>(a, ops, eps);
+    case 12: return launch_fdm_nh<T, 12>(a, ops, eps);
+    case 13: return launch_fdm_nh<T, 13>(a, ops, eps);
+    case 14: return launch_fdm_nh<T, 14>(a, ops, eps);
+    case 15: return launch_fdm_nh<T, 15>(a, ops, eps);
+    case 16: return launch_fdm_nh<T, 16>(a, ops, eps);
+    default: return fail(NEKBONE_ERR_UNSUPPORTED, "hsmg: level size outside 3..16");
     }
 }
 
@@ -262,13 +434,10 @@ int grid(int64_t n) { return int((n + kT - 1) / kT); }
 // One Schwarz level: out = W^1/2 mask(QQ^T(ovl(fdm(mask(r))))); r masked in place.
 int schwarz(nekbone_problem *P, HsmgLevel &L, double *out, double *r)
 {
-    const int nelt = P->nelt;
-    FdmArgs a{r,       L.d_bnd, L.d_pos, L.d_gsrc, L.d_gin, L.d_ops,
-              L.d_eop, L.d_eps, L.d_eown, L.d_ghost, L.nh, L.nl};
-    const int l3 = L.nl * L.nl * L.nl;
-    const size_t lds = sizeof(double) * size_t(2 * l3 + 3 * (2 * L.nl * L.nl + L.nl));
-    hipLaunchKernelGGL(k_fdm, dim3(nelt), dim3(kT), lds, stream(), a);
-    NB_HIP(hipGetLastError());
+    const FdmArgs a{r, L.d_bnd, L.d_pos, L.d_gsrc, L.d_gin, L.d_eop, L.d_eown, L.d_ghost, P->nelt};
+    if (int rc = P->hsmg_fp32 ? launch_fdm<float>(a, L.d_ops32, L.d_eps32, L.nh)
+                              : launch_fdm<double>(a, L.d_ops, L.d_eps, L.nh))
+        return rc;
     hipLaunchKernelGGL(k_ovl, dim3(grid(L.n)), dim3(kT), 0, stream(), L.d_eown,
                        static_cast<const double *>(L.d_ghost),
                        static_cast<const int32_t *>(L.d_gpart),
@@ -369,6 +538,13 @@ extern "C" int nekbone_setup_hsmg(nekbone_problem *P)
         return rc;
     }
     NB_HIP(hipStreamSynchronize(stream()));
+    return NEKBONE_OK;
+}
+
+extern "C" int nekbone_set_hsmg_fp32(nekbone_problem *P, int on)
+{
+    if (int rc = check(P)) return rc;
+    P->hsmg_fp32 = on ? 1 : 0;
     return NEKBONE_OK;
 }
 

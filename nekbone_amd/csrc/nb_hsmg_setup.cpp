@@ -278,6 +278,8 @@ void hsmg_free(nekbone_problem *P)
         dfree(L.d_ops);
         dfree(L.d_eop);
         dfree(L.d_eps);
+        dfree(L.d_ops32);
+        dfree(L.d_eps32);
         dfree(L.d_jh);
         dfree(L.d_jht);
         dfree(L.d_r);
@@ -458,6 +460,8 @@ int hsmg_build(nekbone_problem *P)
         if (int rc = upload(&L.d_ops, ops)) return rc;
         if (int rc = upload(&L.d_eop, eop)) return rc;
         if (int rc = upload(&L.d_eps, eps)) return rc;
+        if (int rc = upload(&L.d_ops32, std::vector<float>(ops.begin(), ops.end()))) return rc;
+        if (int rc = upload(&L.d_eps32, std::vector<float>(eps.begin(), eps.end()))) return rc;
 
         // Extended numbering (nh+2 points, :154-169): each face ghost
         // point is shared with exactly one ghost of the neighbour.

@@ -296,6 +296,8 @@ struct HsmgLevel {
     int nops = 0;
     int16_t *d_eop = nullptr;    // per element: operator index in x, y, z
     double *d_eps = nullptr;     // per element eigenvalue cut-off
+    float *d_ops32 = nullptr;    // FP32 copies of d_ops and d_eps (single-precision smoother)
+    float *d_eps32 = nullptr;
     double *d_jh = nullptr, *d_jht = nullptr;  // interpolation to the next finer level
     double *d_r = nullptr, *d_e = nullptr, *d_eown = nullptr;  // level work vectors
 };
@@ -369,6 +371,7 @@ struct nekbone_problem {
     int dinv_version = 0;  // bumped whenever d_dinv is recomputed
     nb::MixedState *mx = nullptr;
     nb::Hsmg *mg = nullptr;
+    int hsmg_fp32 = 0;  // 1: the HSMG local FDM solves run in FP32
     // captured chunk of CG iterations, replayed while the key matches
     bool use_graph = true;
     hipGraphExec_t graph = nullptr;
