@@ -79,6 +79,7 @@ int launch_ax_local(nekbone_problem *P, double *w, const double *u)
     A.w = w;
     A.u = u;
     A.g = P->d_g;
+    A.gcs = P->n;
     A.Drow = P->d_Drow;
     A.Dcol = P->d_D;
     A.iref = P->d_iref;
@@ -91,7 +92,8 @@ int launch_ax_local(nekbone_problem *P, double *w, const double *u)
     return timed_end(P, 0);
 }
 
-int launch_ax_cg(nekbone_problem *P, double *x, const double *r, int precond, int mbit)
+int launch_ax_cg(nekbone_problem *P, double *x, const double *r, int precond, int mbit,
+                 int rev)
 {
     const int mode = precond == NEKBONE_PRECOND_JACOBI ? 2 : 1;
     AxFn fn = lookup(P->nx1, P->axv.epb, P->arith == NEKBONE_ARITH_FMA, mode, P->axv.pf,
@@ -105,6 +107,7 @@ int launch_ax_cg(nekbone_problem *P, double *x, const double *r, int precond, in
     A.dinv = P->d_dinv;
     A.x = x;
     A.g = P->d_g;
+    A.gcs = P->n;
     A.Drow = P->d_Drow;
     A.Dcol = P->d_D;
     A.flag = P->d_flag;
@@ -113,6 +116,7 @@ int launch_ax_cg(nekbone_problem *P, double *x, const double *r, int precond, in
     A.scal = P->d_scal;
     A.part = P->d_part;
     A.nelt = P->nelt;
+    A.rev = rev;
     const int threads = round64(P->nx1 * P->nx1 * P->axv.epb);
     if (int rc = timed_begin(P, 0)) return rc;
     hipLaunchKernelGGL(fn, dim3(ax_blocks(P)), dim3(threads), 0, stream(), A);

@@ -28,10 +28,16 @@
 //    and wr = (g1 ur + g2 us) + g3 ut etc. (src/cg.f:156-163).  With
 //    FMA=false no contraction happens (the library is compiled with
 //    -ffp-contract=off), so results are bit-identical to the reference.
-//  * Geometric factors are element-blocked SoA, g[e][c][p]: the six loads
-//    of a t-line step are each one coalesced row.  PF=1 issues all of them
-//    at kernel entry (deeper memory pipelining, more VGPRs); PF=0 issues
-//    them in the geometric phase (higher occupancy).
+//  * Geometric factors are stored as six mesh-wide planes, g[c][e][p]
+//    (factor c of point p of element e at c*n + e*N3 + p): the six loads
+//    of a t-line step are each one coalesced row, and the whole launch
+//    reads seven long sequential streams (u and the six planes), which
+//    HBM serves faster than the same bytes blocked per element.  PF=1
+//    issues the loads at kernel entry (deeper memory pipelining, more
+//    VGPRs); PF=0 issues them in the geometric phase (higher occupancy);
+//    PF=2 is PF=0 compiled for 5 waves per SIMD (at most 96 VGPRs), the
+//    most that 16 KB of LDS per element allows at one element per
+//    workgroup.
 //
 // CG mode (MODE 1 = identity, 2 = Jacobi) fuses, per point of the
 // thread's t-line:
@@ -99,16 +105,19 @@ struct AxArgsT {
     const T *r;     // CG: residual
     const T *dinv;  // CG Jacobi
     T *x;           // CG: solution, updated one iteration late (XU)
-    const T *g;
+    const T *g;     // factor planes, g[c * gcs + e * N3 + p]
     const T *Drow, *Dcol;
     const uint8_t *flag;
     const int16_t *iref;  // reference position of each device position (N3)
-    const __half *gh;     // G16: geometric factors in FP16, g = gh * gsc[e][c]
-    const float *gsc;     // G16: per element and factor scale (6 per element)
+    const __half *gh;     // G16: geometric factors in FP16 (same planes),
+                          // g = gh * gsc[c * nelt + e]
+    const float *gsc;     // G16: per factor and element scale
     int mbit;
     const CgScalars *scal;
     double *part;        // CG: per-block pap partials
     int nelt;
+    int64_t gcs;         // distance between factor planes (local points n)
+    int rev;             // walk the elements downward (see sweep_block)
 };
 using AxArgs = AxArgsT<double>;
 
@@ -133,27 +142,31 @@ __device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
     const int64_t base = e * N3;
     const T *__restrict__ Drow = A_.Drow;
     const T *__restrict__ Dcol = A_.Dcol;
-    const T *ge = A_.g + e * 6 * N3;
+    const int64_t gcs = A_.gcs;
+    const T *ge = A_.g + e * N3;
     // G16 (FP32 inner solves only): factor c at point q is gh*scale
-    const __half *gh = G16 ? A_.gh + e * 6 * N3 : nullptr;
+    const __half *gh = G16 ? A_.gh + e * N3 : nullptr;
     T gsc[6] = {T(1), T(1), T(1), T(1), T(1), T(1)};
     if constexpr (G16) {
+        if (act) {
 #pragma unroll
-        for (int c = 0; c < 6; ++c) gsc[c] = T(A_.gsc[e * 6 + c]);
+            for (int c = 0; c < 6; ++c) gsc[c] = T(A_.gsc[c * int64_t(A_.nelt) + e]);
+        }
     }
     auto gload = [&](int c, int q) -> T {
         if constexpr (G16)
-            return T(__half2float(gh[c * N3 + q])) * gsc[c];
+            return T(__half2float(gh[c * gcs + q])) * gsc[c];
         else
-            return ge[c * N3 + q];
+            return ge[c * gcs + q];
     };
 
     if constexpr (MODE != 0) {
         if (A_.scal->done) return;  // converged tolerance run: nothing to do
     }
 
-    T gv[PF ? 6 : 1][PF ? NX : 1];
-    if constexpr (PF) {
+    constexpr bool PRE = PF == 1;
+    T gv[PRE ? 6 : 1][PRE ? NX : 1];
+    if constexpr (PRE) {
         if (act) {
 #pragma unroll
             for (int c = 0; c < 6; ++c)
@@ -249,7 +262,7 @@ __device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
             const T ur = B[q], us = A[q];
             const T ut = dot_row<NX, FMA>(Drow, k, uc);
             T g1, g2, g3, g4, g5, g6;
-            if constexpr (PF) {
+            if constexpr (PRE) {
                 g1 = gv[0][k], g2 = gv[1][k], g3 = gv[2][k];
                 g4 = gv[3][k], g5 = gv[4][k], g6 = gv[5][k];
             } else {
@@ -340,9 +353,10 @@ __device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
 }
 
 template <typename T, int NX, int EPB, bool FMA, int MODE, int PF, bool XU, bool G16 = false>
-__global__ void __launch_bounds__(round64(NX *NX *EPB)) k_ax(AxArgsT<T> A_)
+__global__ void __launch_bounds__(round64(NX *NX *EPB))
+    __attribute__((amdgpu_waves_per_eu(PF == 2 ? 5 : 1, 8))) k_ax(AxArgsT<T> A_)
 {
-    ax_body<T, NX, EPB, FMA, MODE, PF, XU, G16>(A_, blockIdx.x);
+    ax_body<T, NX, EPB, FMA, MODE, PF, XU, G16>(A_, sweep_block(blockIdx.x, gridDim.x, A_.rev));
 }
 
 // Batched form: problem q owns workgroups [start[q], start[q+1]); its
@@ -404,6 +418,7 @@ void reg_order(Table &t, bool full)
         reg_modes<NX, E, 1>(t);
         reg_modes<NX, 2 * E, 0>(t);
         reg_modes<NX, 2 * E, 1>(t);
+        if constexpr (E == 1) reg_modes<NX, E, 2>(t);
     }
 }
 

@@ -96,6 +96,7 @@ int prepare(nekbone_batch *B, int precond, const std::vector<int> &mbit)
             A.dinv = P->d_dinv;
             A.x = P->d_x;
             A.g = P->d_g;
+            A.gcs = P->n;
             A.Drow = P->d_Drow;
             A.Dcol = P->d_D;
             A.flag = P->d_flag;

@@ -80,8 +80,11 @@ __global__ void __launch_bounds__(kFinalThreads)
 
 // [XP] x += alpha p (add2s2, src/cg.f:63) when the element kernel does not
 // carry it; r += (-alpha) w (add2s2, src/cg.f:64); parts of rtr = (r*c)*r
-// (src/cg.f:66) and of the next rtz = (r*c)*z.  Four independent
-// elements per thread per sweep keep enough loads in flight.
+// (src/cg.f:66) and of the next rtz = (r*c)*z.  Logical block blk owns
+// one contiguous chunk of the vectors, so the kernel walks memory in
+// block order (either direction, see sweep_block) and each partial sums
+// the same points in the same order whatever the direction.  Four
+// independent points per thread per step keep enough loads in flight.
 template <int PRE, bool FMA, bool XP>
 __device__ __forceinline__ void update_body(double *__restrict__ r, const double *__restrict__ w,
                                             const double *__restrict__ dinv,
@@ -94,14 +97,16 @@ __device__ __forceinline__ void update_body(double *__restrict__ r, const double
     const double alpha = s->alpha, alphm = -alpha;
     double a0 = 0.0, a1 = 0.0;
     constexpr int U = 4;
-    const int64_t stride = int64_t(nblk) * kRedThreads * U;
-    for (int64_t i0 = int64_t(blk) * kRedThreads * U + threadIdx.x; i0 < n; i0 += stride) {
+    constexpr int64_t step = int64_t(kRedThreads) * U;
+    const int64_t cs = (n + int64_t(nblk) * step - 1) / (int64_t(nblk) * step) * step;
+    const int64_t beg = int64_t(blk) * cs, end = beg + cs < n ? beg + cs : n;
+    for (int64_t i0 = beg + threadIdx.x; i0 < end; i0 += step) {
         double rv[U], wv[U], dv[U], xv[U], pv[U];
         uint8_t fv[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int64_t i = i0 + int64_t(u) * kRedThreads;
-            if (i < n) {
+            if (i < end) {
                 rv[u] = r[i];
                 wv[u] = w[i];
                 fv[u] = flag[i];
@@ -115,7 +120,7 @@ __device__ __forceinline__ void update_body(double *__restrict__ r, const double
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int64_t i = i0 + int64_t(u) * kRedThreads;
-            if (i < n) {
+            if (i < end) {
                 if constexpr (XP) {
                     if constexpr (FMA)
                         x[i] = __builtin_fma(alpha, pv[u], xv[u]);
@@ -146,9 +151,10 @@ __global__ void __launch_bounds__(kRedThreads)
     k_update(double *__restrict__ r, const double *__restrict__ w,
              const double *__restrict__ dinv, const uint8_t *__restrict__ flag,
              double *__restrict__ x, const double *__restrict__ p, int64_t n,
-             double *__restrict__ part, int nblk, const CgScalars *__restrict__ s)
+             double *__restrict__ part, int nblk, const CgScalars *__restrict__ s, int rev)
 {
-    update_body<PRE, FMA, XP>(r, w, dinv, flag, x, p, n, part, nblk, s, blockIdx.x);
+    update_body<PRE, FMA, XP>(r, w, dinv, flag, x, p, n, part, nblk, s,
+                              sweep_block(blockIdx.x, nblk, rev));
 }
 
 template <int PRE>
@@ -378,8 +384,12 @@ static int iteration_fused(nekbone_problem *P, double *x, double *r, int precond
     const int64_t na = n_ax(P), ng = n_gs(P);
     const int nu = n_upd(P);
     double *part_upd = P->d_part + na + ng;
-    if (int rc = launch_ax_cg(P, x, r, precond, mbit)) return rc;
-    if (int rc = launch_gs_cg(P, P->d_w, P->d_p, mbit, P->d_part + na)) return rc;
+    // sweep directions: element kernel one way, gather-scatter back, update
+    // forward again; the next iteration starts in the other direction
+    const int rev = P->sweep & 1;
+    ++P->sweep;
+    if (int rc = launch_ax_cg(P, x, r, precond, mbit, rev)) return rc;
+    if (int rc = launch_gs_cg(P, P->d_w, P->d_p, mbit, P->d_part + na, !rev)) return rc;
     if (int rc = timed_begin(P, 4)) return rc;
     hipLaunchKernelGGL(k_pap_final, dim3(1), dim3(kFinalThreads), 0, stream(), P->d_part, na + ng,
                        P->d_scal, P->d_hist, P->hist_cap);
@@ -389,7 +399,8 @@ static int iteration_fused(nekbone_problem *P, double *x, double *r, int precond
     const bool xp = !P->axv.xu;  // x update here unless the element kernel carries it
 #define NB_UPD2(PRE, F, XP)                                                                  \
     hipLaunchKernelGGL((k_update<PRE, F, XP>), dim3(nu), dim3(kRedThreads), 0, stream(), r,    \
-                       P->d_w, P->d_dinv, P->d_flag, x, P->d_p, P->n, part_upd, nu, P->d_scal)
+                       P->d_w, P->d_dinv, P->d_flag, x, P->d_p, P->n, part_upd, nu, P->d_scal, \
+                       rev)
 #define NB_UPD(PRE, F, _)     \
     do {                      \
         if (xp)               \

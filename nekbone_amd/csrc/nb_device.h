@@ -8,6 +8,17 @@ namespace nb {
 constexpr int kRedThreads = 256;
 constexpr int kMaxRedBlocks = 1024;
 
+// Logical block of hardware workgroup i of nblk, walked upward (rev = 0)
+// or downward (rev = 1).  Workgroups start roughly in index order, so
+// consecutive CG kernels that alternate the direction each start on the
+// data their predecessor touched last, which the memory-side Infinity
+// Cache may still hold.  Per-block results are stored under the logical
+// block, so the direction changes no value.
+__device__ __forceinline__ int sweep_block(int i, int nblk, int rev)
+{
+    return rev ? nblk - 1 - i : i;
+}
+
 // Sum over the 64 lanes of a wave by a butterfly of cross-lane moves:
 // every lane ends with the same value, combined in an order fixed by the
 // lane indices.

@@ -261,6 +261,7 @@ struct GsArgs {
     const int32_t *m2, *m4, *m8, *roff, *ridx;
     int64_t n2, n4, n8, nr;
     int b2, b4, b8, br;  // workgroups per class
+    int rev;             // walk the groups downward (see sweep_block)
 };
 // Arguments of the fused CG gather-scatter of one problem.
 struct GsCgArgs {
@@ -329,6 +330,7 @@ struct MixedState {
 struct nekbone_problem {
     int nelt = 0, nx1 = 0, nelx = 0, nely = 0, nelz = 0;
     int64_t n = 0;
+    unsigned sweep = 0;  // fused CG iterations launched (sets the sweep direction)
     std::vector<int32_t> lglel;
     double dxm1[nb::kMaxNx * nb::kMaxNx] = {};
     double zgll[nb::kMaxNx] = {}, wgll[nb::kMaxNx] = {};
@@ -338,7 +340,7 @@ struct nekbone_problem {
     // device data
     double *d_D = nullptr;     // dxm1, nx1*nx1 column-major: Dcol[i*nx+m] = D(m,i)
     double *d_Drow = nullptr;  // row-major copy: Drow[i*nx+m] = D(i,m)
-    double *d_g = nullptr;     // geometric factors, element-blocked SoA g[e][c][p]
+    double *d_g = nullptr;     // geometric factors, six planes g[c][e][p] (c*n + e*nxyz + p)
     double *d_c = nullptr;     // inverse multiplicity
     double *d_dinv = nullptr;  // Jacobi
     uint8_t *d_flag = nullptr; // per-point flags (kFlag*), device order
@@ -395,11 +397,13 @@ int ax_blocks(const nekbone_problem *P);
 AxVariant ax_default_variant(int nx1);
 std::vector<AxVariant> ax_variants(int nx1);
 int launch_ax_local(nekbone_problem *P, double *w, const double *u);
-int launch_ax_cg(nekbone_problem *P, double *x, const double *r, int precond, int mbit);
+int launch_ax_cg(nekbone_problem *P, double *x, const double *r, int precond, int mbit,
+                 int rev = 0);
 int launch_gs(nekbone_problem *P, double *u, const CgScalars *guard);
 // Fused CG variant: finishes shared points (+0.1p, mask) and writes one
 // pap part per workgroup (gs_blocks of them) to part.
-int launch_gs_cg(nekbone_problem *P, double *w, const double *p, int mbit, double *part);
+int launch_gs_cg(nekbone_problem *P, double *w, const double *p, int mbit, double *part,
+                 int rev = 0);
 // FP32 gather-scatter (mixed-precision inner solve): plain, and the fused
 // CG variant (pap parts in FP64).
 int launch_gs32(nekbone_problem *P, float *u);

@@ -101,7 +101,7 @@ __global__ void __launch_bounds__(kRedThreads)
              const uint8_t *__restrict__ flag, int mbit, GsArgs G, double *__restrict__ part,
              const CgScalars *__restrict__ scal)
 {
-    gs_body<T, CG>(w, p, flag, mbit, G, part, scal, blockIdx.x);
+    gs_body<T, CG>(w, p, flag, mbit, G, part, scal, sweep_block(blockIdx.x, gridDim.x, G.rev));
 }
 
 // Batched fused CG gather-scatter: problem q owns workgroups
@@ -124,7 +124,7 @@ __global__ void __launch_bounds__(kRedThreads)
 
 GsArgs gs_args_map(const GsMap &M)
 {
-    GsArgs G{M.d_m2, M.d_m4, M.d_m8, M.d_roff, M.d_ridx, M.n2, M.n4, M.n8, M.nr, 0, 0, 0, 0};
+    GsArgs G{M.d_m2, M.d_m4, M.d_m8, M.d_roff, M.d_ridx, M.n2, M.n4, M.n8, M.nr, 0, 0, 0, 0, 0};
     auto nb = [](int64_t n) { return int((n + kRedThreads - 1) / kRedThreads); };
     G.b2 = nb(M.n2);
     G.b4 = nb(M.n4);
@@ -210,7 +210,7 @@ __global__ void __launch_bounds__(kRedThreads)
 // ---------------------------------------------------------------------
 
 // Brick factors g1=g4=g6=w_i w_j w_k, others 0 (src/prox_setup.f:27-52),
-// element-blocked SoA.
+// stored as six planes g[c][e][p] (factor c at c*n + e*nxyz + p).
 __global__ void k_brick_g(double *__restrict__ g, const double *__restrict__ wgll, int nx,
                           int64_t nelt)
 {
@@ -221,13 +221,14 @@ __global__ void k_brick_g(double *__restrict__ g, const double *__restrict__ wgl
     const int p = int(i - e * nxyz);
     const int a = p % nx, b = (p / nx) % nx, k = p / (nx * nx);
     const double v = wgll[a] * wgll[b] * wgll[k];
-    double *ge = g + e * 6 * nxyz;
+    const int64_t n = nxyz * nelt;
+    double *ge = g + e * nxyz;
     ge[p] = v;
-    ge[nxyz + p] = 0.0;
-    ge[2 * nxyz + p] = 0.0;
-    ge[3 * nxyz + p] = v;
-    ge[4 * nxyz + p] = 0.0;
-    ge[5 * nxyz + p] = v;
+    ge[n + p] = 0.0;
+    ge[2 * n + p] = 0.0;
+    ge[3 * n + p] = v;
+    ge[4 * n + p] = 0.0;
+    ge[5 * n + p] = v;
 }
 
 __device__ __forceinline__ double lerp_s(double a, double b, double s)
@@ -281,13 +282,14 @@ __global__ void k_deformed_g(double *__restrict__ g, const double *__restrict__ 
     const double C02 = yr * zs - ys * zr, C12 = xs * zr - xr * zs, C22 = xr * ys - xs * yr;
     const double det = (xr * C00 + xs * C01) + xt * C02;
     const double w3 = wgll[i] * wgll[j] * wgll[k];
-    double *ge = g + e * 6 * nxyz;
+    const int64_t n = nxyz * nelt;
+    double *ge = g + e * nxyz;
     ge[p] = (w3 * ((C00 * C00 + C10 * C10) + C20 * C20)) / det;
-    ge[nxyz + p] = (w3 * ((C00 * C01 + C10 * C11) + C20 * C21)) / det;
-    ge[2 * nxyz + p] = (w3 * ((C00 * C02 + C10 * C12) + C20 * C22)) / det;
-    ge[3 * nxyz + p] = (w3 * ((C01 * C01 + C11 * C11) + C21 * C21)) / det;
-    ge[4 * nxyz + p] = (w3 * ((C01 * C02 + C11 * C12) + C21 * C22)) / det;
-    ge[5 * nxyz + p] = (w3 * ((C02 * C02 + C12 * C12) + C22 * C22)) / det;
+    ge[n + p] = (w3 * ((C00 * C01 + C10 * C11) + C20 * C21)) / det;
+    ge[2 * n + p] = (w3 * ((C00 * C02 + C10 * C12) + C20 * C22)) / det;
+    ge[3 * n + p] = (w3 * ((C01 * C01 + C11 * C11) + C21 * C21)) / det;
+    ge[4 * n + p] = (w3 * ((C01 * C02 + C11 * C12) + C21 * C22)) / det;
+    ge[5 * n + p] = (w3 * ((C02 * C02 + C12 * C12) + C22 * C22)) / det;
 }
 
 // Local diagonal of A_L (Jacobi; not in the reference):
@@ -306,20 +308,21 @@ __global__ void k_jacobi_local(double *__restrict__ dl, const double *__restrict
     const int64_t e = gi / nxyz;
     const int p = int(gi - e * nxyz);
     const int i = p % nx, j = (p / nx) % nx, k = p / (nx * nx);
-    const double *ge = g + e * 6 * nxyz;
+    const int64_t n = nxyz * nelt;  // factor plane stride
+    const double *ge = g + e * nxyz;
     const int nx2 = nx * nx;
     double t1 = (D[0 + nx * i] * D[0 + nx * i]) * ge[0 + nx * j + nx2 * k];
-    double t2 = (D[0 + nx * j] * D[0 + nx * j]) * ge[3 * nxyz + i + nx2 * k];
-    double t3 = (D[0 + nx * k] * D[0 + nx * k]) * ge[5 * nxyz + i + nx * j];
+    double t2 = (D[0 + nx * j] * D[0 + nx * j]) * ge[3 * n + i + nx2 * k];
+    double t3 = (D[0 + nx * k] * D[0 + nx * k]) * ge[5 * n + i + nx * j];
     for (int m = 1; m < nx; ++m) {
         t1 = t1 + (D[m + nx * i] * D[m + nx * i]) * ge[m + nx * j + nx2 * k];
-        t2 = t2 + (D[m + nx * j] * D[m + nx * j]) * ge[3 * nxyz + i + nx * m + nx2 * k];
-        t3 = t3 + (D[m + nx * k] * D[m + nx * k]) * ge[5 * nxyz + i + nx * j + nx2 * m];
+        t2 = t2 + (D[m + nx * j] * D[m + nx * j]) * ge[3 * n + i + nx * m + nx2 * k];
+        t3 = t3 + (D[m + nx * k] * D[m + nx * k]) * ge[5 * n + i + nx * j + nx2 * m];
     }
     const double dii = D[i + nx * i], djj = D[j + nx * j], dkk = D[k + nx * k];
-    const double x2 = (2.0 * (dii * djj)) * ge[nxyz + p];
-    const double x3 = (2.0 * (dii * dkk)) * ge[2 * nxyz + p];
-    const double x5 = (2.0 * (djj * dkk)) * ge[4 * nxyz + p];
+    const double x2 = (2.0 * (dii * djj)) * ge[n + p];
+    const double x3 = (2.0 * (dii * dkk)) * ge[2 * n + p];
+    const double x5 = (2.0 * (djj * dkk)) * ge[4 * n + p];
     dl[e * nxyz + perm[p]] = ((((t1 + t2) + t3) + x2) + x3) + x5;
 }
 
@@ -362,7 +365,7 @@ __global__ void k_col2(double *__restrict__ a, const double *__restrict__ b, int
     if (i < n) a[i] = a[i] * b[i];
 }
 
-// AoS g(6,n) (reference layout) <-> element-blocked SoA g[e][c][p].
+// AoS g(6,n) (reference layout) <-> factor planes g[c][e][p].
 __global__ void k_aos_soa(const double *__restrict__ src, double *__restrict__ dst,
                           int64_t nxyz, int64_t n, int dir)
 {
@@ -370,8 +373,7 @@ __global__ void k_aos_soa(const double *__restrict__ src, double *__restrict__ d
     if (i >= 6 * n) return;
     const int64_t l = i / 6;
     const int c = int(i - l * 6);
-    const int64_t e = l / nxyz, p = l - e * nxyz;
-    const int64_t s = (e * 6 + c) * nxyz + p;
+    const int64_t s = c * n + l;  // factor planes: g[c][e][p]
     if (dir == 0)
         dst[s] = src[i];
     else
@@ -408,14 +410,16 @@ int launch_gs(nekbone_problem *P, double *u, const CgScalars *guard)
     return timed_end(P, 1);
 }
 
-int launch_gs_cg(nekbone_problem *P, double *w, const double *p, int mbit, double *part)
+int launch_gs_cg(nekbone_problem *P, double *w, const double *p, int mbit, double *part,
+                 int rev)
 {
     const int nb = gs_blocks(P);
     if (nb == 0) return NEKBONE_OK;
     if (int rc = timed_begin(P, 1)) return rc;
+    GsArgs G = gs_args(P);
+    G.rev = rev;
     hipLaunchKernelGGL((k_gs_cls<double, true>), dim3(nb), dim3(kRedThreads), 0, stream(), w, p,
-                       P->d_flag, mbit, gs_args(P), part,
-                       static_cast<const CgScalars *>(P->d_scal));
+                       P->d_flag, mbit, G, part, static_cast<const CgScalars *>(P->d_scal));
     NB_HIP(hipGetLastError());
     return timed_end(P, 1);
 }

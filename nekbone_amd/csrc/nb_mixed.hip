@@ -64,8 +64,8 @@ __global__ void k_to_f32(float *__restrict__ dst, const double *__restrict__ src
     if (i < n) dst[i] = float(src[i]);
 }
 
-// FP16 geometric factors: one workgroup per (element, factor) finds the
-// largest magnitude, stores it as the scale, and the factor divided by it
+// FP16 geometric factors: one workgroup per (factor, element) block of
+// the factor planes (block c*nelt + e) finds the largest magnitude, stores it as the scale, and the factor divided by it
 // rounded to half precision (relative error <= 2^-11 for every value above
 // 2^-14 of the largest).
 __global__ void __launch_bounds__(256)
@@ -327,6 +327,7 @@ int launch_ax32(nekbone_problem *P, int mode, int mbit, float *w, const float *u
     A.dinv = M->dinv;
     A.x = M->x;
     A.g = M->g;
+    A.gcs = P->n;
     A.Drow = M->Drow;
     A.Dcol = M->Dcol;
     A.flag = P->d_flag;
