@@ -231,6 +231,27 @@ static int finish_problem(nekbone_problem *P, const double *c_host)
             dst.push_back(int32_t(dev_index(P, idx[size_t(q)])));
         if (sz != 2 && sz != 4 && sz != 8) roff.push_back(int32_t(ridx.size()));
     }
+    // Within each fixed-size class the groups are ordered by the device
+    // position of their first member (the member order inside a group,
+    // which fixes the sum, is untouched).  On a box mesh a wave of pair
+    // groups then covers one face of an element, whose points sit
+    // contiguously in device order, against the matching contiguous face
+    // of the neighbour; edge and corner groups likewise.
+    auto order_class = [](std::vector<int32_t> &m, int S) {
+        const size_t ng = m.size() / size_t(S);
+        std::vector<size_t> ord(ng);
+        for (size_t g = 0; g < ng; ++g) ord[g] = g;
+        std::stable_sort(ord.begin(), ord.end(),
+                         [&](size_t a, size_t b) { return m[a * S] < m[b * S]; });
+        std::vector<int32_t> out(m.size());
+        for (size_t g = 0; g < ng; ++g)
+            std::copy(m.begin() + ptrdiff_t(ord[g] * S), m.begin() + ptrdiff_t(ord[g] * S + S),
+                      out.begin() + ptrdiff_t(g * S));
+        m.swap(out);
+    };
+    order_class(m2, 2);
+    order_class(m4, 4);
+    order_class(m8, 8);
     GsMap &G = P->gs;
     G.n2 = int64_t(m2.size()) / 2;
     G.n4 = int64_t(m4.size()) / 4;
