@@ -221,7 +221,9 @@ def hsmg_workload(nb, tol=1e-10):
     """HSMG-preconditioned FP64 CG on a 16x16x16-element box at N=7,
     deformed +-10% (seed 1), Dirichlet faces, RHS U[-1,1) seed 0, to a
     1e-10 relative residual: with the FP64 smoother (the reference's
-    arithmetic), with the FP32 local solves, and Jacobi CG beside them."""
+    arithmetic), with the FP32 local solves, each with the reference's
+    coarse step and with the FP64 direct coarse solve, and Jacobi CG
+    beside them."""
     import numpy as np
     P = nb.Problem(16, 16, 16, 8)
     P.set_mask_mode(nb.MASK_DIRICHLET)
@@ -236,10 +238,16 @@ def hsmg_workload(nb, tol=1e-10):
     out = {"elements": P.nelt, "nx1": 8, "local_points": P.n, "tol": tol,
            "geometry": "deformed +-10% vertices, seed 1", "setup_ms": setup_ms}
     xs = {}
-    for name, pre, fp32 in (("fp64_smoother", nb.PRECOND_HSMG, 0),
-                            ("fp32_smoother", nb.PRECOND_HSMG, 1),
-                            ("jacobi", nb.PRECOND_JACOBI, 0)):
+    for name, pre, fp32, coarse in (("fp64_smoother", nb.PRECOND_HSMG, 0, 0),
+                                    ("fp32_smoother", nb.PRECOND_HSMG, 1, 0),
+                                    ("fp64_smoother_direct_coarse", nb.PRECOND_HSMG, 0, 1),
+                                    ("fp32_smoother_direct_coarse", nb.PRECOND_HSMG, 1, 1),
+                                    ("jacobi", nb.PRECOND_JACOBI, 0, 0)):
         P.set_hsmg_fp32(fp32)
+        if coarse and "coarse_setup_ms" not in out:
+            t0 = time.perf_counter()
+            P.set_hsmg_coarse(1)
+            out["coarse_setup_ms"] = (time.perf_counter() - t0) * 1e3
         P.cg_tol_d(x, f, r, tol, 5000, pre)
         nb.sync()
         t0 = time.perf_counter()
@@ -249,12 +257,19 @@ def hsmg_workload(nb, tol=1e-10):
         out[name] = {"iters": it, "ms": ms, "ms_per_iter": ms / max(it, 1)}
         xs[name] = x.download()
     P.set_hsmg_fp32(0)
+    P.set_hsmg_coarse(0)
     d = xs["fp32_smoother"] - xs["fp64_smoother"]
     ref = xs["fp64_smoother"]
     out["fp32_vs_fp64_smoother_solution"] = {
         "l2_rel": float(np.linalg.norm(d) / np.linalg.norm(ref)),
         "max_rel": float(np.abs(d).max() / np.abs(ref).max())}
     out["fp32_smoother_speedup"] = out["fp64_smoother"]["ms"] / out["fp32_smoother"]["ms"]
+    d = xs["fp64_smoother_direct_coarse"] - xs["fp64_smoother"]
+    out["direct_coarse_vs_reference_coarse_solution"] = {
+        "l2_rel": float(np.linalg.norm(d) / np.linalg.norm(ref)),
+        "max_rel": float(np.abs(d).max() / np.abs(ref).max())}
+    out["direct_coarse_speedup"] = (out["fp64_smoother"]["ms"] /
+                                    out["fp64_smoother_direct_coarse"]["ms"])
     for v in (f, x, r):
         v.free()
     return out

@@ -146,6 +146,15 @@ int nekbone_hsmg_solve(nekbone_problem *P, double *z, double *r);
  * the outer CG stay FP64.  on = 0 (default): everything FP64.  The
  * reference has no FP32 path (it is this library's option). */
 int nekbone_set_hsmg_fp32(nekbone_problem *P, int on);
+/* HSMG coarse level (no counterpart in the reference, whose coarse step,
+ * hsmg_coarse_solve src/hsmg.f:945-957, only sums the restricted
+ * residual).  mode 0 (default): the reference's coarse step.  mode 1: an
+ * FP64 direct solve with the Galerkin projection of the operator onto the
+ * trilinear functions of the mesh vertices (A_c = sum_e J^T A_L J + 0.1
+ * sum_e J^T C_e J on the unmasked vertices), by a dense inverse formed
+ * once with rocSOLVER; at most 8192 coarse unknowns.  Requires
+ * nekbone_setup_hsmg; a new nekbone_setup_hsmg resets the mode to 0. */
+int nekbone_set_hsmg_coarse(nekbone_problem *P, int mode);
 
 /* ---------------------------------------------------------------------- */
 /* Operators (host arrays; *_d: device pointers on the library stream)     */

@@ -25,7 +25,7 @@ SOURCES = ["nb_ax_i10.hip", "nb_ax_i12.hip", "nb_ax_i8.hip", "nb_ax_i6.hip", "nb
            "nb_ax_ihigh.hip", "nb_setup.cpp", "nb_ax.hip", "nb_kernels.hip", "nb_cg.hip",
            "nb_api.hip",
            "nb_legacy.hip",
-           "nb_ax32_low.hip", "nb_ax32_high.hip", "nb_ax_ib.hip", "nb_mixed.hip", "nb_batch.hip", "nb_hsmg.hip", "nb_hsmg_setup.cpp"]
+           "nb_ax32_low.hip", "nb_ax32_high.hip", "nb_ax_ib.hip", "nb_mixed.hip", "nb_batch.hip", "nb_hsmg.hip", "nb_hsmg_setup.cpp", "nb_hsmg_coarse.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unused-function",
@@ -75,7 +75,8 @@ def build(verbose=False, jobs=None, incremental=False):
     with cf.ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(lambda s: _compile(s, verbose, incremental), srcs))
     tmp = lib_path() + ".tmp"
-    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs
+    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs + [
+        "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

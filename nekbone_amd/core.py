@@ -57,6 +57,7 @@ _SIGS = {
     "nekbone_setup_hsmg": (ctypes.c_int, [_vp]),
     "nekbone_hsmg_solve": (ctypes.c_int, [_vp, _dp, _dp]),
     "nekbone_set_hsmg_fp32": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "nekbone_set_hsmg_coarse": (ctypes.c_int, [_vp, ctypes.c_int]),
     "nekbone_get_dinv": (ctypes.c_int, [_vp, _dp]),
     "nekbone_ax": (ctypes.c_int, [_vp, _dp, _dp]),
     "nekbone_ax_d": (ctypes.c_int, [_vp, _vp, _vp]),
@@ -336,6 +337,11 @@ class Problem:
     def set_hsmg_fp32(self, on):
         """FP32 local FDM solves in the HSMG smoother (FP64 otherwise)."""
         _chk(lib().nekbone_set_hsmg_fp32(self.handle, int(bool(on))))
+
+    def set_hsmg_coarse(self, mode):
+        """HSMG coarse level: 0 the reference's (sum of the restricted
+        residual), 1 an FP64 direct solve of the order-1 operator."""
+        _chk(lib().nekbone_set_hsmg_coarse(self.handle, int(mode)))
 
     def hsmg_solve(self, r):
         """z = M^-1 r (h1mg_solve); returns z and r as the preconditioner

@@ -368,7 +368,7 @@ static int run_iterations(nekbone_problem *P, double *x, double *r, int precond,
             uintptr_t(P->d_hist), uintptr_t(P->hist_cap), uintptr_t(P->axv.epb),
             uintptr_t(P->axv.pf), uintptr_t(P->axv.xu), uintptr_t(P->arith),
             uintptr_t(P->fused), uintptr_t(P->mg), uintptr_t(P->d_dinv),
-            uintptr_t(P->hsmg_fp32)};
+            uintptr_t(P->hsmg_fp32), uintptr_t(P->hsmg_coarse)};
         if (!P->graph || P->graph_key != key) {
             drop_graph(P);
             hipGraph_t g = nullptr;

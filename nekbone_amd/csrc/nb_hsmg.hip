@@ -517,9 +517,12 @@ int launch_hsmg_apply(nekbone_problem *P, double *z, double *r)
         if (int rc = schwarz(P, L[l], L[l].d_e, L[l].d_r)) return rc;
         rcur = L[l].d_r;
     }
-    // coarse level: e1 = mask(QQ^T mask(J^T r2)) (hsmg_coarse_solve, :945-957)
+    // coarse level: e1 = mask(QQ^T mask(J^T r2)) (hsmg_coarse_solve, :945-957),
+    // or with the direct coarse solve e1 = Q A_c^-1 Q^T mask(J^T r2)
     if (int rc = rstr(P, L[1], L[2], L[1].d_e, rcur, true)) return rc;
     if (int rc = launch_gs_map(L[1].gs, L[1].d_e)) return rc;
+    if (P->hsmg_coarse)
+        if (int rc = hsmg_coarse_apply(P, L[1].d_e)) return rc;
     for (int l = 2; l <= lmax - 1; ++l)
         if (int rc = intp(P, L[l - 1], L[l], L[l].d_e, L[l - 1].d_e, l - 1 == 1)) return rc;
     if (int rc = intp(P, L[lmax - 1], L[lmax], z, L[lmax - 1].d_e, lmax - 1 == 1)) return rc;

@@ -287,8 +287,10 @@ void hsmg_free(nekbone_problem *P)
         dfree(L.d_eown);
     }
     dfree(M->d_z);
+    hsmg_coarse_free(M);
     delete M;
     P->mg = nullptr;
+    P->hsmg_coarse = 0;
 }
 
 int hsmg_build(nekbone_problem *P)

@@ -307,7 +307,18 @@ struct Hsmg {
     int lmax = 0;
     HsmgLevel L[4];  // 1..lmax
     double *d_z = nullptr;  // preconditioned residual of the CG (fine level)
+    // optional direct coarse solve on the order-1 level (nb_hsmg_coarse.hip)
+    int nc = 0;                   // unmasked mesh vertices (coarse unknowns)
+    double *d_ainv = nullptr;     // nc x nc inverse of the coarse operator (symmetric)
+    int32_t *d_cprim = nullptr;   // nc: a level-1 point holding each unknown
+    int32_t *d_cidx = nullptr;    // level-1 points: their unknown, -1 on the mask
+    double *d_xc = nullptr;       // nc: coarse solution
+    double *d_rc = nullptr;       // nc: coarse right-hand side
 };
+
+// Largest coarse problem the direct coarse solve accepts (its dense
+// inverse is 512 MB).
+constexpr int kMaxCoarse = 8192;
 
 // FP32 copies for the inner iterations of the mixed-precision CG:
 // operator data converted from the FP64 originals (rebuilt when the
@@ -374,6 +385,7 @@ struct nekbone_problem {
     nb::MixedState *mx = nullptr;
     nb::Hsmg *mg = nullptr;
     int hsmg_fp32 = 0;  // 1: the HSMG local FDM solves run in FP32
+    int hsmg_coarse = 0;  // 1: HSMG coarse level solved directly (FP64)
     // captured chunk of CG iterations, replayed while the key matches
     bool use_graph = true;
     hipGraphExec_t graph = nullptr;
@@ -457,6 +469,9 @@ void gs_map_free(GsMap &G);
 // HSMG (nb_hsmg_setup.cpp, nb_hsmg.hip)
 int hsmg_build(nekbone_problem *P);
 void hsmg_free(nekbone_problem *P);
+int hsmg_coarse_build(nekbone_problem *P);
+void hsmg_coarse_free(Hsmg *M);
+int hsmg_coarse_apply(nekbone_problem *P, double *e);
 int launch_hsmg_apply(nekbone_problem *P, double *z, double *r);
 // One CG iteration with an externally computed z = M^-1 r (r as the
 // preconditioner left it): rtz, beta, p, w = A p, pap, alpha, x, r, rtr.

@@ -982,6 +982,10 @@ struct orc_hsmg {
     int lmax;
     mglevel L[4]; /* 1..lmax */
     double *e, *w, *r, *work;
+    /* optional coarse solve on level 1 (orc_hsmg_set_coarse) */
+    int nc;
+    double *cainv;  /* nc x nc, column-major */
+    int32_t *ccidx; /* level-1 point -> unknown, -1 masked */
 };
 
 static void gs_from_glo(const int64_t *glo, int64_t n, gsmap *m)
@@ -1316,7 +1320,44 @@ static void hsmg_free(struct orc_hsmg *M)
     free(M->w);
     free(M->r);
     free(M->work);
+    free(M->cainv);
+    free(M->ccidx);
     free(M);
+}
+
+/* Replace the coarse step's result QQ^T r by Q Ainv Q^T r (the direct
+ * coarse solve the library offers; not in the reference).  cidx gives
+ * the unknown of every level-1 point (element-major, reference order),
+ * -1 on the mask; ainv is the caller's dense inverse. */
+int orc_hsmg_set_coarse(orc_problem *P, const double *ainv, int nc, const int32_t *cidx)
+{
+    struct orc_hsmg *M = P->mg;
+    if (!M || nc <= 0) return 1;
+    free(M->cainv);
+    free(M->ccidx);
+    M->nc = nc;
+    M->cainv = malloc(sizeof *M->cainv * (size_t)nc * (size_t)nc);
+    M->ccidx = malloc(sizeof *M->ccidx * (size_t)M->L[1].n);
+    memcpy(M->cainv, ainv, sizeof *M->cainv * (size_t)nc * (size_t)nc);
+    memcpy(M->ccidx, cidx, sizeof *M->ccidx * (size_t)M->L[1].n);
+    return 0;
+}
+
+static void coarse_direct(const struct orc_hsmg *M, double *e)
+{
+    const int nc = M->nc;
+    const int64_t n1 = M->L[1].n;
+    double *rg = calloc((size_t)nc, sizeof *rg), *xg = calloc((size_t)nc, sizeof *xg);
+    for (int64_t q = n1 - 1; q >= 0; --q) /* every copy holds the same sum */
+        if (M->ccidx[q] >= 0) rg[M->ccidx[q]] = e[q];
+    for (int j = 0; j < nc; ++j) {
+        double s = 0.0;
+        for (int i = 0; i < nc; ++i) s = s + M->cainv[(size_t)i + (size_t)nc * (size_t)j] * rg[i];
+        xg[j] = s;
+    }
+    for (int64_t q = 0; q < n1; ++q) e[q] = M->ccidx[q] >= 0 ? xg[M->ccidx[q]] : 0.0;
+    free(rg);
+    free(xg);
 }
 
 int orc_hsmg_levels(const orc_problem *P) { return P->mg ? P->mg->lmax : 0; }
@@ -1677,6 +1718,7 @@ void orc_hsmg_solve(orc_problem *P, double *z, double *rhs)
     mg_mask(&L[1], r);
     memcpy(e + is, r, sizeof *e * (size_t)L[1].n); /* hsmg_coarse_solve (:945-957) */
     gs_apply(&L[1].gs, e + is);
+    if (M->cainv) coarse_direct(M, e + is);
     mg_mask(&L[1], e + is);
     for (int l = 2; l <= lmax - 1; ++l) {
         int64_t im = is;

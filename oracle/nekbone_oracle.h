@@ -89,6 +89,12 @@ void orc_hsmg_solve(orc_problem *P, double *z, double *rhs);
  * level l (1 = coarsest). */
 int orc_hsmg_levels(const orc_problem *P);
 int orc_hsmg_nh(const orc_problem *P, int l);
+/* Coarse step with a caller-given dense inverse (checks the library's
+ * optional direct coarse solve; no counterpart in the reference):
+ * e = Q ainv Q^T r on level 1, cidx = unknown of each level-1 point
+ * (element-major, reference order), -1 on the mask.  Returns 0 on
+ * success. */
+int orc_hsmg_set_coarse(orc_problem *P, const double *ainv, int nc, const int32_t *cidx);
 
 /* Synthetic inputs. */
 void orc_rand_uniform(double *f, int64_t n, uint64_t seed); /* U[-1,1) */

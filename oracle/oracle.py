@@ -68,6 +68,9 @@ def lib():
         L.orc_hsmg_solve.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.orc_hsmg_levels.restype = ctypes.c_int
         L.orc_hsmg_levels.argtypes = [ctypes.c_void_p]
+        L.orc_hsmg_set_coarse.restype = ctypes.c_int
+        L.orc_hsmg_set_coarse.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int,
+                                          ctypes.POINTER(ctypes.c_int32)]
         L.orc_hsmg_nh.restype = ctypes.c_int
         L.orc_hsmg_nh.argtypes = [ctypes.c_void_p, ctypes.c_int]
         _LIB = L
@@ -216,6 +219,14 @@ class Problem:
     def hsmg_setup(self):
         if lib().orc_hsmg_setup(self._h) != 0:
             raise ValueError("HSMG needs nx1 >= 4")
+
+    def hsmg_set_coarse(self, ainv, cidx):
+        ainv = np.ascontiguousarray(ainv, dtype=np.float64)
+        cidx = np.ascontiguousarray(cidx, dtype=np.int32)
+        nc = ainv.shape[0]
+        if lib().orc_hsmg_set_coarse(self._h, ainv.ctypes.data_as(_dp), nc,
+                                     cidx.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))) != 0:
+            raise RuntimeError("orc_hsmg_set_coarse failed")
 
     def hsmg_levels(self):
         L = lib().orc_hsmg_levels(self._h)
