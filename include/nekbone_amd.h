@@ -262,6 +262,14 @@ int nekbone_d2h(nekbone_problem *P, double *dst, const double *src, int64_t coun
  * set_f post-processing f = c * dssum(f) (src/driver.f:96-97). */
 int nekbone_rand_uniform_d(nekbone_problem *P, double *dst, int64_t count, uint64_t seed);
 int nekbone_rhs_from_raw_d(nekbone_problem *P, double *f);
+/* The reference's right-hand side, set_f (src/driver.f:88-101).
+ * nekbone_set_f_raw: host only, raw[i-1] = sin(1e9 cos(1e9 (i*i))) for
+ * i = 1..n with i*i the wrapping 32-bit product of the reference's
+ * default INTEGER, sin/cos of the C library.  nekbone_set_f_d: the raw
+ * vector of the problem's n points, then dssum and the product with c,
+ * into device vector f (reference point order). */
+int nekbone_set_f_raw(int64_t n, double *raw);
+int nekbone_set_f_d(nekbone_problem *P, double *f);
 
 /* ---------------------------------------------------------------------- */
 /* Measurement                                                              */

@@ -6,11 +6,11 @@ include/nekbone_amd.h); this package is its host-side binding.
 from .core import (ARITH_EXACT, ARITH_FMA, KCLASS_AX, KCLASS_GS, KCLASS_POST, KCLASS_REDUCE,
                    KCLASS_UPDATE, LIB_PATH, MASK_DIRICHLET, MASK_REF, PRECOND_HSMG,
                    PRECOND_IDENTITY, PRECOND_JACOBI, Batch, DeviceVec, NekboneError, Problem,
-                   bw_probe, cubic, init, lib, sync)
+                   bw_probe, cubic, init, lib, set_f_raw, sync)
 
 __all__ = [
     "ARITH_EXACT", "ARITH_FMA", "KCLASS_AX", "KCLASS_GS", "KCLASS_POST", "KCLASS_REDUCE",
     "KCLASS_UPDATE", "LIB_PATH", "MASK_DIRICHLET", "MASK_REF", "PRECOND_HSMG",
     "PRECOND_IDENTITY", "PRECOND_JACOBI", "Batch", "DeviceVec", "NekboneError", "Problem",
-    "bw_probe", "cubic", "init", "lib", "sync",
+    "bw_probe", "cubic", "init", "lib", "set_f_raw", "sync",
 ]

@@ -85,6 +85,8 @@ _SIGS = {
     "nekbone_d2h": (ctypes.c_int, [_vp, _dp, _vp, ctypes.c_int64]),
     "nekbone_rand_uniform_d": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_uint64]),
     "nekbone_rhs_from_raw_d": (ctypes.c_int, [_vp, _vp]),
+    "nekbone_set_f_raw": (ctypes.c_int, [ctypes.c_int64, _dp]),
+    "nekbone_set_f_d": (ctypes.c_int, [_vp, _vp]),
     "nekbone_timing_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
     "nekbone_timing_get": (ctypes.c_int, [_vp, ctypes.c_int, _i64p, _dp]),
     "nekbone_timing_reset": (ctypes.c_int, [_vp]),
@@ -200,6 +202,14 @@ class Batch:
         _chk(lib().nekbone_batch_cg_tol_d(self.handle, ptrs(xs), ptrs(fs), ptrs(rs), tol, maxiter,
                                           precond, it, rtr))
         return list(it), list(rtr)
+
+
+def set_f_raw(n):
+    """The reference's raw right-hand side (set_f, src/driver.f:88-95)
+    for points 1..n, computed on the host."""
+    raw = np.empty(int(n))
+    _chk(lib().nekbone_set_f_raw(int(n), _p(raw)))
+    return raw
 
 
 def bw_probe(nbytes, reps=10):
@@ -404,6 +414,10 @@ class Problem:
 
     def rhs_from_raw_d(self, dv):
         _chk(lib().nekbone_rhs_from_raw_d(self.handle, dv.ptr))
+
+    def set_f_d(self, dv):
+        """dv = set_f's right-hand side (src/driver.f:88-101)."""
+        _chk(lib().nekbone_set_f_d(self.handle, dv.ptr))
 
     def cg_d(self, x, f, r, niter, precond=PRECOND_IDENTITY, want_hist=False):
         hist = np.empty(1 + 3 * niter) if want_hist else None

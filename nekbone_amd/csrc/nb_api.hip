@@ -6,7 +6,9 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <thread>
 #include <tuple>
+#include <vector>
 
 #include "nb_internal.h"
 
@@ -997,6 +999,42 @@ extern "C" int nekbone_rhs_from_raw_d(nekbone_problem *P, double *f)
     if (int rc = launch_gs(P, P->d_t2, nullptr)) return rc;
     if (int rc = launch_col2(P, P->d_t2, P->d_c)) return rc;
     return launch_to_ref(P, f, P->d_t2);
+}
+
+// set_f (src/driver.f:88-101): f(i) = sin(1e9*cos(1e9*(i*i))), i = 1..n,
+// where i*i is the reference's default-INTEGER product: it overflows for
+// i > 46340 and the flang-built reference wraps it (two's complement), as
+// the compiled reference's own output confirms.  sin and cos are the C
+// library's (the arguments reach 1e18, so the result depends on the
+// library's argument reduction; computed on the host, never on the
+// device).  Independent points, split over host threads.
+extern "C" int nekbone_set_f_raw(int64_t n, double *raw)
+{
+    if (n < 0 || (n > 0 && !raw)) return fail(NEKBONE_ERR_ARG, "set_f_raw: bad arguments");
+    if (n > (int64_t(1) << 31) - 1) return fail(NEKBONE_ERR_ARG, "set_f_raw: n >= 2^31");
+    const int nt = int(std::max<int64_t>(1, std::min<int64_t>(16, n / 65536)));
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t)
+        th.emplace_back([=] {
+            const int64_t a = n * t / nt, b = n * (t + 1) / nt;
+            for (int64_t l = a; l < b; ++l) {
+                const uint32_t i = uint32_t(l + 1);
+                const int32_t ii = int32_t(i * i);  // wraps like the reference's INTEGER product
+                raw[l] = std::sin(1.e9 * std::cos(1.e9 * double(ii)));
+            }
+        });
+    for (auto &x : th) x.join();
+    return NEKBONE_OK;
+}
+
+extern "C" int nekbone_set_f_d(nekbone_problem *P, double *f)
+{
+    if (int rc = check(P)) return rc;
+    if (!f) return fail(NEKBONE_ERR_ARG, "set_f: null vector");
+    std::vector<double> raw(static_cast<size_t>(P->n));
+    if (int rc = nekbone_set_f_raw(P->n, raw.data())) return rc;
+    NB_HIP(hipMemcpy(f, raw.data(), sizeof(double) * raw.size(), hipMemcpyHostToDevice));
+    return nekbone_rhs_from_raw_d(P, f);  // dssum, then col2 with c (:97-98)
 }
 
 // ======================================================================

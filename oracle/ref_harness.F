@@ -104,6 +104,11 @@ c        cmask(-1)>=0 and builds the mask list (src/cg.f:230-253).
          nm = int(cmask(0))
          call hkwr(trim(fout)//'.cmask', cmask(-1), nm+2)
 
+      elseif (task.eq.'setf') then
+c        The reference's own right-hand side, set_f (src/driver.f:88-101).
+         call set_f(f,c,n)
+         call hkwr(trim(fout)//'.f', f, n)
+
       elseif (task.eq.'dssum') then
          call hkrd(fin, p, n)
          call dssum(p)

@@ -99,10 +99,30 @@ def cg_case(tmp, variant, nel, nx1, niter, ncalls, seed, tag):
     return fx, secs, pre
 
 
+def setf_fixture(tmp):
+    """The reference's own right-hand side, set_f (src/driver.f:88-101),
+    on 8x8x8 elements at N=7 (262,144 points, so i*i overflows the
+    32-bit INTEGER from i = 46,341 on): a sample of the final f (after
+    dssum and col2) at fixed indices, with the indices."""
+    pre = os.path.join(tmp, "setf")
+    run("id", "setf", (8, 8, 8), 8, out=pre)
+    f = rd(pre, "f")
+    idx = np.unique(np.concatenate([np.arange(0, 2048), np.arange(46300, 46400),
+                                    np.arange(2048, f.size, 61), [f.size - 1]])).astype(np.int64)
+    return {"setf_b888_n8_idx": idx, "setf_b888_n8_f": f[idx],
+            "setf_b888_n8_sumsq": np.array([np.dot(f, f)])}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--big", action="store_true", help="also the 32^3, N=9 history (minutes)")
+    ap.add_argument("--setf", action="store_true",
+                    help="only write ref_setf.npz (the set_f right-hand side)")
     args = ap.parse_args()
+    if args.setf:
+        with tempfile.TemporaryDirectory() as tmp:
+            np.savez_compressed(os.path.join(HERE, "ref_setf.npz"), **setf_fixture(tmp))
+        return
     meta = {"generator": "tests/golden/make_golden.py", "source": "oracle/_ref refharness"}
     with tempfile.TemporaryDirectory() as tmp:
         fx = small_cases(tmp)
