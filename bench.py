@@ -207,14 +207,34 @@ def tol_workload(nb, nel, nx1, delta):
     xmh = xm.download()
     d = xmh - x64
     fp64_ms, mixed_ms = (tb - ta) * 1e3, (td - tc) * 1e3
-    return {
+    rtr0 = max(Q_rtr0(Q, fq), 1e-300)
+    out = {
         "fp64_ms": fp64_ms, "fp64_iters": iters, "mixed_ms": mixed_ms,
         "mixed_iters": m_it, "mixed_fp64_updates": m_upd, "speedup": fp64_ms / mixed_ms,
-        "mixed_final_rel_residual": (m_rtr / max(Q_rtr0(Q, fq), 1e-300)) ** 0.5,
+        "mixed_final_rel_residual": (m_rtr / rtr0) ** 0.5,
         "mixed_vs_fp64_solution": {
             "l2_rel": float(np.linalg.norm(d) / np.linalg.norm(x64)),
             "max_rel": float(np.abs(d).max() / np.abs(x64).max())},
         "geometry": "deformed +-10% vertices, seed 1", "delta": delta or 1e-3}
+    # the same solve with the FP16 factors, and with the FP16 matrix-core
+    # inner operator
+    for name, mode in (("mixed_g16", nb.MIXED_G16), ("mixed_f16_mfma", nb.MIXED_F16_MFMA)):
+        Q.set_mixed_inner(mode)
+        Q.cg_mixed_d(xm, fq, rq, 1e-10, 5000, nb.PRECOND_JACOBI, delta)
+        nb.sync()
+        t0 = time.perf_counter()
+        it2, upd2, rtr2 = Q.cg_mixed_d(xm, fq, rq, 1e-10, 5000, nb.PRECOND_JACOBI, delta)
+        nb.sync()
+        ms2 = (time.perf_counter() - t0) * 1e3
+        d2 = xm.download() - x64
+        out[name] = {"ms": ms2, "iters": it2, "fp64_updates": upd2,
+                     "speedup_vs_fp64": fp64_ms / ms2,
+                     "final_rel_residual": (rtr2 / rtr0) ** 0.5,
+                     "vs_fp64_solution": {
+                         "l2_rel": float(np.linalg.norm(d2) / np.linalg.norm(x64)),
+                         "max_rel": float(np.abs(d2).max() / np.abs(x64).max())}}
+    Q.set_mixed_inner(nb.MIXED_F32)
+    return out
 
 
 def hsmg_workload(nb, tol=1e-10):

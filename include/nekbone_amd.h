@@ -200,7 +200,8 @@ int nekbone_cg_tol_d(nekbone_problem *P, double *x, const double *f, double *r, 
  * gather-scatter in FP32; dot products accumulated in FP64); x is
  * accumulated in FP64 and the residual is recomputed in FP64 ("reliable
  * update", the search direction is kept) whenever the FP32 residual norm
- * has dropped by the factor delta since the last update (0 selects 1e-3)
+ * has dropped by the factor delta since the last update (0 selects 1e-3,
+ * or 3e-2 with an FP16 inner operator, see nekbone_set_mixed_inner)
  * or the FP32 iteration reports convergence.  x, f, r as for
  * nekbone_cg_tol_d.  *iters: inner iterations; *updates: FP64 residual
  * recomputations; *final_rtr: glsc3(r,c,r) of the returned x.  Requires
@@ -214,6 +215,20 @@ int nekbone_cg_mixed_d(nekbone_problem *P, double *x, const double *f, double *r
  * per point instead of 24); the FP64 residual updates still use the FP64
  * factors.  0 (default): FP32 factors. */
 int nekbone_set_mixed_g16(nekbone_problem *P, int on);
+/* Inner operator of the mixed solver (no counterpart in the reference):
+ * NEKBONE_MIXED_F32 (default) FP32 factors and FP32 contractions;
+ * NEKBONE_MIXED_G16 as nekbone_set_mixed_g16(P, 1);
+ * NEKBONE_MIXED_F16_MFMA FP16 factors and FP16 matrix-core contractions:
+ * per element, the input is scaled by its largest magnitude and rounded to
+ * FP16, the three gradient contractions run on MFMA (16x16x16, FP32
+ * accumulation), the geometric product in FP32, the products scaled by
+ * their largest magnitude and rounded to FP16 for the three transposed
+ * contractions on MFMA, w accumulated in FP32.  nx1 = 4..16.  The FP64
+ * residual updates and the stopping rule are unchanged. */
+#define NEKBONE_MIXED_F32 0
+#define NEKBONE_MIXED_G16 1
+#define NEKBONE_MIXED_F16_MFMA 2
+int nekbone_set_mixed_inner(nekbone_problem *P, int mode);
 /* FP32 element-kernel shape of the mixed solver (as nekbone_get_variant /
  * nekbone_set_variant), and its autotuner: times every compiled shape over
  * a few inner iterations and keeps the fastest for this problem. */

@@ -19,6 +19,7 @@ MASK_REF = 0
 MASK_DIRICHLET = 1
 ARITH_EXACT = 0
 ARITH_FMA = 1
+MIXED_F32, MIXED_G16, MIXED_F16_MFMA = 0, 1, 2
 
 KCLASS_AX, KCLASS_GS, KCLASS_POST, KCLASS_UPDATE, KCLASS_REDUCE = range(5)
 
@@ -58,6 +59,7 @@ _SIGS = {
     "nekbone_hsmg_solve": (ctypes.c_int, [_vp, _dp, _dp]),
     "nekbone_set_hsmg_fp32": (ctypes.c_int, [_vp, ctypes.c_int]),
     "nekbone_set_hsmg_coarse": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "nekbone_set_mixed_inner": (ctypes.c_int, [_vp, ctypes.c_int]),
     "nekbone_get_dinv": (ctypes.c_int, [_vp, _dp]),
     "nekbone_ax": (ctypes.c_int, [_vp, _dp, _dp]),
     "nekbone_ax_d": (ctypes.c_int, [_vp, _vp, _vp]),
@@ -456,6 +458,11 @@ class Problem:
 
     def set_mixed_g16(self, on):
         _chk(lib().nekbone_set_mixed_g16(self.handle, int(on)))
+
+    def set_mixed_inner(self, mode):
+        """Inner operator of the mixed solver: MIXED_F32, MIXED_G16 or
+        MIXED_F16_MFMA."""
+        _chk(lib().nekbone_set_mixed_inner(self.handle, int(mode)))
 
     def variant_f32(self):
         v = [ctypes.c_int() for _ in range(3)]

@@ -94,6 +94,93 @@ __device__ __forceinline__ T dot_row(const T *__restrict__ base, int row, const 
     return s;
 }
 
+// ---- FP16 matrix-core contractions (MF variant, FP32 storage) ---------
+// v_mfma_f32_16x16x16_f16, one wave: lane l holds A[l%16][4(l/16)+v],
+// B[4(l/16)+v][l%16] and C[4(l/16)+v][l%16], v = 0..3 (checked on the
+// device against a host product).
+typedef _Float16 mf_half4 __attribute__((ext_vector_type(4)));
+typedef float mf_float4 __attribute__((ext_vector_type(4)));
+
+// B fragment of the 16x16 matrix Bm[m][c] = D(c,m) (TR = false, the
+// gradient) or D(m,c) (TR = true, its transpose), zero outside NX x NX.
+// Drow[i*NX+m] = D(i,m).
+template <int NX, bool TR>
+__device__ __forceinline__ mf_half4 mf_dfrag(const float *__restrict__ Drow, int lane)
+{
+    mf_half4 b;
+    const int c = lane & 15;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int m = 4 * (lane >> 4) + v;
+        float d = 0.f;
+        if (m < NX && c < NX) d = TR ? Drow[m * NX + c] : Drow[c * NX + m];
+        b[v] = (_Float16)d;
+    }
+    return b;
+}
+
+// One 16-row tile of a contraction along the direction of stride SIG
+// (1: r, NX: s, NX^2: t) of an element held as FP16 in src (reference
+// point order): rows are the NX^2 lines of that direction, the line of
+// row rho starting at line_base(rho); out[point] = scale * sum_m
+// src[line + SIG m] Bm[m][c], written (ACC = false) or added (ACC =
+// true) at point line + SIG c.
+template <int NX, int SIG>
+__device__ __forceinline__ int mf_line_base(int rho)
+{
+    if constexpr (SIG == 1) return NX * rho;           // rho = j + NX k
+    else if constexpr (SIG == NX) return (rho % NX) + NX * NX * (rho / NX);  // rho = i + NX k
+    else return rho;                                    // rho = i + NX j
+}
+
+template <int NX, int SIG, bool ACC>
+__device__ __forceinline__ void mf_tile(float *__restrict__ out, const _Float16 *__restrict__ src,
+                                        mf_half4 bfrag, int tile, int lane, float scale)
+{
+    constexpr int N2 = NX * NX;
+    mf_half4 a;
+    const int ra = 16 * tile + (lane & 15);
+    const int la = ra < N2 ? mf_line_base<NX, SIG>(ra) : 0;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int m = 4 * (lane >> 4) + v;
+        a[v] = (ra < N2 && m < NX) ? src[la + SIG * m] : (_Float16)0.f;
+    }
+    mf_float4 c = {0.f, 0.f, 0.f, 0.f};
+    c = __builtin_amdgcn_mfma_f32_16x16x16f16(a, bfrag, c, 0, 0, 0);
+    const int col = lane & 15;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int rc = 16 * tile + 4 * (lane >> 4) + v;
+        if (rc < N2 && col < NX) {
+            const int q = mf_line_base<NX, SIG>(rc) + SIG * col;
+            if constexpr (ACC)
+                out[q] = out[q] + scale * c[v];
+            else
+                out[q] = scale * c[v];
+        }
+    }
+}
+
+// Largest |v| over the workgroup (every thread returns it): wave
+// butterflies of max, then the wave maxima through LDS.  One barrier
+// for the exchange, one so that the helper can run again.
+template <int NT>
+__device__ __forceinline__ float mf_block_max(float v)
+{
+    constexpr int NW = NT / 64;
+    __shared__ float sm[NW];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+    __syncthreads();
+    float m = sm[0];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) m = fmaxf(m, sm[w]);
+    __syncthreads();
+    return m;
+}
+
 // T is the storage and arithmetic type of the operator (double; float
 // for the inner solve of the mixed-precision CG).  Dot-product partials
 // and the CG scalars are FP64 for both.
@@ -123,13 +210,18 @@ using AxArgs = AxArgsT<double>;
 
 // Body of the element kernel for workgroup blk of one problem (shared by
 // the single-problem kernel and the batched one).
-template <typename T, int NX, int EPB, bool FMA, int MODE, int PF, bool XU, bool G16 = false>
+template <typename T, int NX, int EPB, bool FMA, int MODE, int PF, bool XU, bool G16 = false,
+          bool MF = false>
 __device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
 {
     constexpr int N2 = NX * NX, N3 = NX * NX * NX;
     constexpr int NT = round64(N2 * EPB);
+    static_assert(!MF || (EPB == 1 && G16 && sizeof(T) == 4 && NX <= 16),
+                  "FP16 matrix-core variant: FP32 storage, FP16 factors, one element per group");
     __shared__ T sA[EPB * N3];
     __shared__ T sB[EPB * N3];
+    __shared__ T sC[MF ? N3 : 1];          // MF: us, then the accumulated w
+    __shared__ _Float16 sH[MF ? N3 : 1];   // MF: FP16 input, then FP16 wt
 
     const int tid = threadIdx.x;
     const int el = tid / N2;
@@ -232,84 +324,159 @@ __device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
     }
     __syncthreads();
 
-    T uc[NX];  // the thread's t-line of the input
-    if (act) {
+    if constexpr (MF) {
+        // FP16 matrix-core operator (no counterpart in the reference; the
+        // inner operator of the mixed-precision CG).  The element's input,
+        // held in A (FP32), is scaled by its largest magnitude and rounded
+        // to FP16; the three gradient contractions run on MFMA with FP32
+        // accumulation; the geometric product is FP32; the three
+        // transposed contractions take the products scaled by their
+        // largest magnitude, rounded to FP16, and accumulate w in FP32.
+        const int lane = tid & 63, wv = tid >> 6;
+        constexpr int NW = NT / 64, TPC = (N2 + 15) / 16;  // waves, tiles per contraction
+        _Float16 *H = sH;  // FP16 input
+        float umax = 0.f;
+        T uc[NX];
+        if (act) {
 #pragma unroll
-        for (int k = 0; k < NX; ++k) uc[k] = A[a + NX * b + N2 * k];
-    }
-
-    T lr[NX], ls[NX];
-    if (act) {
-#pragma unroll
-        for (int m = 0; m < NX; ++m) lr[m] = A[m + NX * a + N2 * b];  // r-line (j,k)=(a,b)
-#pragma unroll
-        for (int m = 0; m < NX; ++m) ls[m] = A[a + NX * m + N2 * b];  // s-line (i,k)=(a,b)
-    }
-    __syncthreads();
-    if (act) {
-#pragma unroll
-        for (int i = 0; i < NX; ++i) B[i + NX * a + N2 * b] = dot_row<NX, FMA>(Drow, i, lr);
-#pragma unroll
-        for (int j = 0; j < NX; ++j) A[a + NX * j + N2 * b] = dot_row<NX, FMA>(Drow, j, ls);
-    }
-    __syncthreads();
-
-    T wt[NX];
-    if (act) {
-#pragma unroll
-        for (int k = 0; k < NX; ++k) {
-            const int q = a + NX * b + N2 * k;
-            const T ur = B[q], us = A[q];
-            const T ut = dot_row<NX, FMA>(Drow, k, uc);
-            T g1, g2, g3, g4, g5, g6;
-            if constexpr (PRE) {
-                g1 = gv[0][k], g2 = gv[1][k], g3 = gv[2][k];
-                g4 = gv[3][k], g5 = gv[4][k], g6 = gv[5][k];
-            } else {
-                g1 = gload(0, q), g2 = gload(1, q), g3 = gload(2, q);
-                g4 = gload(3, q), g5 = gload(4, q), g6 = gload(5, q);
+            for (int k = 0; k < NX; ++k) {
+                uc[k] = A[a + NX * b + N2 * k];
+                umax = fmaxf(umax, fabsf(uc[k]));
             }
-            T wr, ws;
-            if constexpr (FMA) {
-                wr = fma_t(g3, ut, fma_t(g2, us, g1 * ur));
-                ws = fma_t(g5, ut, fma_t(g4, us, g2 * ur));
+        }
+        const float su = mf_block_max<NT>(umax);
+        const float isu = su > 0.f ? 1.f / su : 1.f;
+        if (act) {
+#pragma unroll
+            for (int k = 0; k < NX; ++k) H[a + NX * b + N2 * k] = (_Float16)(uc[k] * isu);
+        }
+        __syncthreads();
+        const mf_half4 bg = mf_dfrag<NX, false>(A_.Drow, lane);
+        for (int t = wv; t < 3 * TPC; t += NW) {
+            if (t < TPC)
+                mf_tile<NX, 1, false>(sA, H, bg, t, lane, su);             // ur -> A
+            else if (t < 2 * TPC)
+                mf_tile<NX, NX, false>(sC, H, bg, t - TPC, lane, su);      // us -> C
+            else
+                mf_tile<NX, NX * NX, false>(sB, H, bg, t - 2 * TPC, lane, su);  // ut -> B
+        }
+        __syncthreads();
+        T wr[NX], ws[NX], wt[NX];
+        float wmax = 0.f;
+        if (act) {
+#pragma unroll
+            for (int k = 0; k < NX; ++k) {
+                const int q = a + NX * b + N2 * k;
+                const T ur = sA[q], us = sC[q], ut = sB[q];
+                const T g1 = gload(0, q), g2 = gload(1, q), g3 = gload(2, q);
+                const T g4 = gload(3, q), g5 = gload(4, q), g6 = gload(5, q);
+                wr[k] = fma_t(g3, ut, fma_t(g2, us, g1 * ur));
+                ws[k] = fma_t(g5, ut, fma_t(g4, us, g2 * ur));
                 wt[k] = fma_t(g6, ut, fma_t(g5, us, g3 * ur));
-            } else {
-                wr = g1 * ur + g2 * us + g3 * ut;
-                ws = g2 * ur + g4 * us + g5 * ut;
-                wt[k] = g3 * ur + g5 * us + g6 * ut;
+                wmax = fmaxf(wmax, fmaxf(fabsf(wr[k]), fmaxf(fabsf(ws[k]), fabsf(wt[k]))));
             }
-            B[q] = wr;
-            A[q] = ws;
         }
-    }
-    __syncthreads();
-
-    if (act) {
-        // r-line and s-line are each read and rewritten by their owner
-        // only, so no barrier is needed between the reads and the writes.
+        const float sw = mf_block_max<NT>(wmax);  // its barrier also ends every read above
+        const float isw = sw > 0.f ? 1.f / sw : 1.f;
+        _Float16 *HR = reinterpret_cast<_Float16 *>(sA);
+        _Float16 *HS = HR + N3;  // second half of A's bytes
+        _Float16 *HT = sH;
+        if (act) {
 #pragma unroll
-        for (int m = 0; m < NX; ++m) lr[m] = B[m + NX * a + N2 * b];
-#pragma unroll
-        for (int i = 0; i < NX; ++i) B[i + NX * a + N2 * b] = dot_row<NX, FMA>(Dcol, i, lr);
-#pragma unroll
-        for (int m = 0; m < NX; ++m) ls[m] = A[a + NX * m + N2 * b];
-#pragma unroll
-        for (int j = 0; j < NX; ++j) A[a + NX * j + N2 * b] = dot_row<NX, FMA>(Dcol, j, ls);
-    }
-    __syncthreads();
-
-    // w at the thread's t-line points, kept in place of the consumed
-    // partials (each entry is read and rewritten by its owner only)
-    if (act) {
-#pragma unroll
-        for (int k = 0; k < NX; ++k) {
-            const int q = a + NX * b + N2 * k;
-            A[q] = (B[q] + A[q]) + dot_row<NX, FMA>(Dcol, k, wt);
+            for (int k = 0; k < NX; ++k) {
+                const int q = a + NX * b + N2 * k;
+                HR[q] = (_Float16)(wr[k] * isw);
+                HS[q] = (_Float16)(ws[k] * isw);
+                HT[q] = (_Float16)(wt[k] * isw);
+            }
         }
-    }
-    __syncthreads();
+        __syncthreads();
+        const mf_half4 bt = mf_dfrag<NX, true>(A_.Drow, lane);
+        for (int t = wv; t < TPC; t += NW) mf_tile<NX, 1, false>(sC, HR, bt, t, lane, sw);
+        __syncthreads();
+        for (int t = wv; t < TPC; t += NW) mf_tile<NX, NX, true>(sC, HS, bt, t, lane, sw);
+        __syncthreads();
+        for (int t = wv; t < TPC; t += NW) mf_tile<NX, NX * NX, true>(sC, HT, bt, t, lane, sw);
+        __syncthreads();
+    } else {
+        T uc[NX];  // the thread's t-line of the input
+        if (act) {
+    #pragma unroll
+            for (int k = 0; k < NX; ++k) uc[k] = A[a + NX * b + N2 * k];
+        }
 
+        T lr[NX], ls[NX];
+        if (act) {
+    #pragma unroll
+            for (int m = 0; m < NX; ++m) lr[m] = A[m + NX * a + N2 * b];  // r-line (j,k)=(a,b)
+    #pragma unroll
+            for (int m = 0; m < NX; ++m) ls[m] = A[a + NX * m + N2 * b];  // s-line (i,k)=(a,b)
+        }
+        __syncthreads();
+        if (act) {
+    #pragma unroll
+            for (int i = 0; i < NX; ++i) B[i + NX * a + N2 * b] = dot_row<NX, FMA>(Drow, i, lr);
+    #pragma unroll
+            for (int j = 0; j < NX; ++j) A[a + NX * j + N2 * b] = dot_row<NX, FMA>(Drow, j, ls);
+        }
+        __syncthreads();
+
+        T wt[NX];
+        if (act) {
+    #pragma unroll
+            for (int k = 0; k < NX; ++k) {
+                const int q = a + NX * b + N2 * k;
+                const T ur = B[q], us = A[q];
+                const T ut = dot_row<NX, FMA>(Drow, k, uc);
+                T g1, g2, g3, g4, g5, g6;
+                if constexpr (PRE) {
+                    g1 = gv[0][k], g2 = gv[1][k], g3 = gv[2][k];
+                    g4 = gv[3][k], g5 = gv[4][k], g6 = gv[5][k];
+                } else {
+                    g1 = gload(0, q), g2 = gload(1, q), g3 = gload(2, q);
+                    g4 = gload(3, q), g5 = gload(4, q), g6 = gload(5, q);
+                }
+                T wr, ws;
+                if constexpr (FMA) {
+                    wr = fma_t(g3, ut, fma_t(g2, us, g1 * ur));
+                    ws = fma_t(g5, ut, fma_t(g4, us, g2 * ur));
+                    wt[k] = fma_t(g6, ut, fma_t(g5, us, g3 * ur));
+                } else {
+                    wr = g1 * ur + g2 * us + g3 * ut;
+                    ws = g2 * ur + g4 * us + g5 * ut;
+                    wt[k] = g3 * ur + g5 * us + g6 * ut;
+                }
+                B[q] = wr;
+                A[q] = ws;
+            }
+        }
+        __syncthreads();
+
+        if (act) {
+            // r-line and s-line are each read and rewritten by their owner
+            // only, so no barrier is needed between the reads and the writes.
+    #pragma unroll
+            for (int m = 0; m < NX; ++m) lr[m] = B[m + NX * a + N2 * b];
+    #pragma unroll
+            for (int i = 0; i < NX; ++i) B[i + NX * a + N2 * b] = dot_row<NX, FMA>(Dcol, i, lr);
+    #pragma unroll
+            for (int m = 0; m < NX; ++m) ls[m] = A[a + NX * m + N2 * b];
+    #pragma unroll
+            for (int j = 0; j < NX; ++j) A[a + NX * j + N2 * b] = dot_row<NX, FMA>(Dcol, j, ls);
+        }
+        __syncthreads();
+
+        // w at the thread's t-line points, kept in place of the consumed
+        // partials (each entry is read and rewritten by its owner only)
+        if (act) {
+    #pragma unroll
+            for (int k = 0; k < NX; ++k) {
+                const int q = a + NX * b + N2 * k;
+                A[q] = (B[q] + A[q]) + dot_row<NX, FMA>(Dcol, k, wt);
+            }
+        }
+        __syncthreads();
+    }
     double acc = 0.0;
     if (act) {
         // positions read again (cache hits) rather than kept live through
@@ -319,7 +486,7 @@ __device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
 #pragma unroll
         for (int m = 0; m < NX; ++m) {
             const int64_t q = base + ab + N2 * m;
-            const T wl = A[iref[ab + N2 * m]];
+            const T wl = MF ? sC[iref[ab + N2 * m]] : A[iref[ab + N2 * m]];
             if constexpr (MODE == 0) {
                 A_.w[q] = wl;
             } else {
@@ -352,11 +519,13 @@ __device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
     }
 }
 
-template <typename T, int NX, int EPB, bool FMA, int MODE, int PF, bool XU, bool G16 = false>
+template <typename T, int NX, int EPB, bool FMA, int MODE, int PF, bool XU, bool G16 = false,
+          bool MF = false>
 __global__ void __launch_bounds__(round64(NX *NX *EPB))
     __attribute__((amdgpu_waves_per_eu(PF == 2 ? 5 : 1, 8))) k_ax(AxArgsT<T> A_)
 {
-    ax_body<T, NX, EPB, FMA, MODE, PF, XU, G16>(A_, sweep_block(blockIdx.x, gridDim.x, A_.rev));
+    ax_body<T, NX, EPB, FMA, MODE, PF, XU, G16, MF>(A_,
+                                                   sweep_block(blockIdx.x, gridDim.x, A_.rev));
 }
 
 // Batched form: problem q owns workgroups [start[q], start[q+1]); its
@@ -478,9 +647,21 @@ void reg32_order(Table32 &t, bool full)
     }
 }
 
+// FP16 matrix-core kernels of the mixed solver (key g16 = 2): one element
+// per workgroup, factors in FP16; plain operator and the CG modes with
+// the x update fused.
+template <int NX>
+void reg32_mf(Table32 &t)
+{
+    t[Key32(NX, 1, 0, 0, 0, 2)] = k_ax<float, NX, 1, true, 0, 0, false, true, true>;
+    t[Key32(NX, 1, 1, 0, 1, 2)] = k_ax<float, NX, 1, true, 1, 0, true, true, true>;
+    t[Key32(NX, 1, 2, 0, 1, 2)] = k_ax<float, NX, 1, true, 2, 0, true, true, true>;
+}
+
 // Defined in the instantiation files nb_ax_i*.hip and nb_ax32_*.hip.
 void reg32_low(Table32 &t);
 void reg32_high(Table32 &t);
+void reg32_mf_all(Table32 &t);
 void regb_all(TableB &t);
 void reg_low(Table &t);
 void reg_high(Table &t);

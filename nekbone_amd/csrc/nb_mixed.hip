@@ -45,6 +45,7 @@ Table32 &table32()
         done = true;
         axk::reg32_low(t);
         axk::reg32_high(t);
+        axk::reg32_mf_all(t);
     }
     return t;
 }
@@ -437,7 +438,12 @@ extern "C" int nekbone_cg_mixed_d(nekbone_problem *P, double *x, const double *f
     if (int rc = check(P)) return rc;
     if (!(tol > 0.0)) return fail(NEKBONE_ERR_ARG, "cg_mixed: tol must be > 0");
     if (maxiter < 0) return fail(NEKBONE_ERR_ARG, "cg_mixed: maxiter < 0");
-    if (delta == 0.0) delta = 1e-3;
+    // default residual reduction between FP64 updates: 1e-3 for the FP32
+    // operator; 3e-2 for the FP16 ones, whose ~5e-4 relative operator error
+    // otherwise costs the inner iteration its convergence rate (measured at
+    // 32^3 N=9 deformed: 94 inner iterations with 3e-2 against 111 (FP16
+    // factors) and 427 (FP16 matrix cores) with 1e-3)
+    if (delta == 0.0) delta = (P->mx && P->mx->g16) ? 3e-2 : 1e-3;
     if (!(delta > 0.0 && delta < 1.0)) return fail(NEKBONE_ERR_ARG, "cg_mixed: need 0 < delta < 1");
     if (precond != NEKBONE_PRECOND_JACOBI && precond != NEKBONE_PRECOND_IDENTITY)
         return fail(NEKBONE_ERR_UNSUPPORTED, "cg_mixed: preconditioner not available");
@@ -485,21 +491,30 @@ extern "C" int nekbone_cg_mixed_d(nekbone_problem *P, double *x, const double *f
     return NEKBONE_OK;
 }
 
-extern "C" int nekbone_set_mixed_g16(nekbone_problem *P, int on)
+extern "C" int nekbone_set_mixed_inner(nekbone_problem *P, int mode)
 {
     if (int rc = check(P)) return rc;
+    if (mode < NEKBONE_MIXED_F32 || mode > NEKBONE_MIXED_F16_MFMA)
+        return fail(NEKBONE_ERR_ARG, "mixed: unknown inner operator");
     if (int rc = ensure_mixed(P, NEKBONE_PRECOND_IDENTITY)) return rc;
     MixedState *M = P->mx;
-    const int g16 = on ? 1 : 0;
-    if (g16 && !lookup32(P->nx1, M->axv.epb, 2, M->axv.pf, M->axv.xu, 1)) {
-        // fall back to a shape compiled with FP16 factors
+    if (mode != M->g16) {
+        // the first shape compiled for this operator: the default one, else
+        // one element per workgroup (the only shape of the matrix-core kernel)
         AxVariant v{axk::epb_for(P->nx1), 0, 1};
-        if (!lookup32(P->nx1, v.epb, 2, v.pf, v.xu, 1))
-            return fail(NEKBONE_ERR_UNSUPPORTED, "mixed: no FP16-factor kernel for this order");
+        if (!lookup32(P->nx1, v.epb, 2, v.pf, v.xu, mode)) v = AxVariant{1, 0, 1};
+        if (!lookup32(P->nx1, v.epb, 2, v.pf, v.xu, mode) ||
+            !lookup32(P->nx1, v.epb, 0, v.pf, 0, mode))
+            return fail(NEKBONE_ERR_UNSUPPORTED, "mixed: inner operator not built for this order");
         M->axv = v;
     }
-    M->g16 = g16;
+    M->g16 = mode;
     return ensure_mixed(P, NEKBONE_PRECOND_IDENTITY);
+}
+
+extern "C" int nekbone_set_mixed_g16(nekbone_problem *P, int on)
+{
+    return nekbone_set_mixed_inner(P, on ? NEKBONE_MIXED_G16 : NEKBONE_MIXED_F32);
 }
 
 extern "C" int nekbone_get_variant_f32(nekbone_problem *P, int *epb, int *pf, int *xu)

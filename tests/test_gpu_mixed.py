@@ -133,3 +133,64 @@ def test_cg_mixed_rejects_bad_arguments(nb):
         P.cg_mixed(f, 1e-10, 10, nb.PRECOND_IDENTITY, delta=2.0)
     with pytest.raises(nb.NekboneError):
         P.cg_mixed(f, -1.0, 10, nb.PRECOND_IDENTITY)
+
+
+@pytest.mark.parametrize("deformed", [False, True])
+@pytest.mark.parametrize("nel,nx1", [((2, 2, 2), 8), ((3, 2, 2), 10), ((2, 3, 1), 12),
+                                     ((1, 2, 1), 5), ((2, 1, 2), 16), ((2, 2, 1), 4)])
+def test_ax_local_f16_mfma_vs_fp64_oracle(nb, oracle, nel, nx1, deformed):
+    """FP16 matrix-core operator: within 2e-2 of max|w| of the FP64
+    oracle (FP16 unit roundoff 4.9e-4 on the scaled input, on D, on the
+    factors and on the scaled products, through sums of nx1 terms), and
+    not equal to the FP32 operator (the FP16 path ran).  A wrong index or
+    transposition in any of the six contractions gives errors of order
+    max|w|."""
+    _case("ax_local_f16_mfma %s %d deformed=%s" % (nel, nx1, deformed))
+    P, O = _pair(nb, oracle, nel, nx1, deformed, jacobi=False)
+    u32 = splitmix_uniform(P.n, 11).astype(np.float32)
+    w32 = P.ax_local_f32(u32)
+    P.set_mixed_inner(nb.MIXED_F16_MFMA)
+    w16 = P.ax_local_f32(u32)
+    u = u32.astype(np.float64)
+    n3 = nx1 ** 3
+    w64 = np.concatenate([O.ax_e(e, u[e * n3:(e + 1) * n3]) for e in range(O.nelt)])
+    scale = np.abs(w64).max()
+    err = np.abs(w16.astype(np.float64) - w64).max() / scale
+    print("f16 mfma operator: max err %.2e of max|w|" % err)
+    assert err <= 2e-2
+    assert np.abs(w16 - w32).max() > 1e-5 * scale
+    # a scaled input gives the scaled output (per-element scaling)
+    w16s = P.ax_local_f32((u32 * np.float32(2.0 ** -20)))
+    np.testing.assert_allclose(w16s.astype(np.float64) * 2.0 ** 20, w16, rtol=0,
+                               atol=1e-6 * scale)
+
+
+@pytest.mark.parametrize("nel,nx1", [((4, 3, 2), 8), ((3, 3, 3), 10)])
+def test_cg_mixed_f16_mfma_meets_fp64_criterion(nb, oracle, nel, nx1):
+    """The FP64 reliable updates make the FP16 inner operator reach the
+    FP64 criterion all the same (more inner iterations)."""
+    _case("cg_mixed f16 mfma %s %d" % (nel, nx1))
+    tol = 1e-10
+    P, O = _pair(nb, oracle, nel, nx1, True, jacobi=True)
+    P.set_mixed_inner(nb.MIXED_F16_MFMA)
+    f = O.rhs_from_raw(splitmix_uniform(O.n, 2))
+    x, r, it, nupd, rtr = P.cg_mixed(f, tol, 5000, 1)
+    c = O.c()
+    f0 = O.maskit(f)
+    rtrue = f0 - O.ax(x)
+    rr = float(np.sum(rtrue * c * rtrue))
+    rr0 = float(np.sum(f0 * c * f0))
+    assert rr <= tol ** 2 * rr0 * 1.0001
+    assert abs(rtr - rr) <= 1e-6 * rr
+    P.set_mixed_inner(nb.MIXED_F32)
+    _, _, it32, nupd32, _ = P.cg_mixed(f, tol, 5000, 1)
+    print("inner iterations: f16 mfma %d (+%d updates), f32 %d (+%d)" % (it, nupd, it32, nupd32))
+
+
+def test_mixed_inner_rejects_unknown_mode(nb):
+    P = nb.Problem(2, 2, 2, 8)
+    with pytest.raises(nb.NekboneError):
+        P.set_mixed_inner(3)
+    Q = nb.Problem(2, 2, 2, 3)
+    with pytest.raises(nb.NekboneError):
+        Q.set_mixed_inner(nb.MIXED_F16_MFMA)  # matrix-core kernels for nx1 >= 4
