@@ -181,6 +181,23 @@ __device__ __forceinline__ float mf_block_max(float v)
     return m;
 }
 
+// Jacobi diagonal storage: FP64 for the FP64 kernels; bfloat16 bits (the
+// FP32 exponent range, 8-bit significand) for the FP32 inner iterations of
+// the mixed solver, which only use it as a preconditioner.
+template <typename T>
+struct DinvOf {
+    using type = T;
+};
+template <>
+struct DinvOf<float> {
+    using type = uint16_t;
+};
+__device__ __forceinline__ double dinv_at(const double *d, int64_t i) { return d[i]; }
+__device__ __forceinline__ float dinv_at(const uint16_t *d, int64_t i)
+{
+    return __uint_as_float(uint32_t(d[i]) << 16);
+}
+
 // T is the storage and arithmetic type of the operator (double; float
 // for the inner solve of the mixed-precision CG).  Dot-product partials
 // and the CG scalars are FP64 for both.
@@ -190,7 +207,7 @@ struct AxArgsT {
     const T *u;     // MODE 0 input
     T *p;           // CG: direction, updated in place
     const T *r;     // CG: residual
-    const T *dinv;  // CG Jacobi
+    const typename DinvOf<T>::type *dinv;  // CG Jacobi
     T *x;           // CG: solution, updated one iteration late (XU)
     const T *g;     // factor planes, g[c * gcs + e * N3 + p]
     const T *Drow, *Dcol;
@@ -294,7 +311,7 @@ __device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
             }
             if constexpr (MODE == 2) {
 #pragma unroll
-                for (int m = 0; m < NX; ++m) z[m] = z[m] * A_.dinv[base + ab + N2 * m];
+                for (int m = 0; m < NX; ++m) z[m] = z[m] * dinv_at(A_.dinv, base + ab + N2 * m);
             }
             if constexpr (XU) {
                 if (A_.scal->xpending) {  // the previous iteration's alpha p

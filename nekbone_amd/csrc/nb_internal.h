@@ -325,7 +325,8 @@ constexpr int kMaxCoarse = 8192;
 // geometry or the Jacobi diagonal changes) and the FP32 work vectors,
 // all in the device point order.
 struct MixedState {
-    float *g = nullptr, *Drow = nullptr, *Dcol = nullptr, *dinv = nullptr;
+    float *g = nullptr, *Drow = nullptr, *Dcol = nullptr;
+    uint16_t *dinv = nullptr;  // Jacobi diagonal as bfloat16 bits
     float *x = nullptr, *r = nullptr, *p = nullptr, *w = nullptr;
     int geom_version = -1, dinv_version = -1;
     AxVariant axv;

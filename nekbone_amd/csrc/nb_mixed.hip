@@ -65,6 +65,16 @@ __global__ void k_to_f32(float *__restrict__ dst, const double *__restrict__ src
     if (i < n) dst[i] = float(src[i]);
 }
 
+// Jacobi diagonal for the inner iterations: FP64 -> FP32 -> bfloat16 bits,
+// round to nearest even (the values are positive and finite).
+__global__ void k_to_bf16(uint16_t *__restrict__ dst, const double *__restrict__ src, int64_t n)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t b = __float_as_uint(float(src[i]));
+    dst[i] = uint16_t((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
+}
+
 // FP16 geometric factors: one workgroup per (factor, element) block of
 // the factor planes (block c*nelt + e) finds the largest magnitude, stores it as the scale, and the factor divided by it
 // rounded to half precision (relative error <= 2^-11 for every value above
@@ -125,7 +135,7 @@ __global__ void k_mx_start(float *__restrict__ r32, float *__restrict__ x32,
 template <int PRE, bool XP>
 __global__ void __launch_bounds__(kRedThreads)
     k_mx_update(float *__restrict__ r, const float *__restrict__ w,
-                const float *__restrict__ dinv, const uint8_t *__restrict__ flag,
+                const uint16_t *__restrict__ dinv, const uint8_t *__restrict__ flag,
                 float *__restrict__ x, const float *__restrict__ p, int64_t n,
                 double *__restrict__ part, int nblk, const CgScalars *__restrict__ s)
 {
@@ -144,7 +154,7 @@ __global__ void __launch_bounds__(kRedThreads)
                 rv[u] = r[i];
                 wv[u] = w[i];
                 fv[u] = flag[i];
-                if constexpr (PRE == NEKBONE_PRECOND_JACOBI) dv[u] = dinv[i];
+                if constexpr (PRE == NEKBONE_PRECOND_JACOBI) dv[u] = axk::dinv_at(dinv, i);
                 if constexpr (XP) {
                     xv[u] = x[i];
                     pv[u] = p[i];
@@ -188,12 +198,13 @@ __global__ void k_mx_xflush(double *__restrict__ x64, float *__restrict__ x32,
 
 // Reliable update, residual: r64 = mask(f - (A_L x64 assembled + 0.1 x64))
 // (the FP64 operator of src/cg.f:96-125), r32 = r64, parts of
-// rtr = (r c r) and rtz = (r c M^-1 r).
+// rtr = (r c r) and rtz = (r c M^-1 r), with the inner iterations' own
+// (bfloat16) Jacobi diagonal so that beta stays consistent.
 template <int PRE>
 __global__ void __launch_bounds__(kRedThreads)
     k_mx_resid(double *__restrict__ r64, float *__restrict__ r32, const double *__restrict__ f,
                const double *__restrict__ w, const double *__restrict__ x,
-               const double *__restrict__ dinv, const uint8_t *__restrict__ flag, int mbit,
+               const uint16_t *__restrict__ dinv, const uint8_t *__restrict__ flag, int mbit,
                int64_t n, double *__restrict__ part, int nblk)
 {
     double a0 = 0.0, a1 = 0.0;
@@ -206,7 +217,8 @@ __global__ void __launch_bounds__(kRedThreads)
         r32[i] = float(v);
         const double rc = v * inv_mult(fl);
         a0 = a0 + rc * v;
-        if constexpr (PRE == NEKBONE_PRECOND_JACOBI) a1 = a1 + rc * (v * dinv[i]);
+        if constexpr (PRE == NEKBONE_PRECOND_JACOBI)
+            a1 = a1 + rc * (v * double(axk::dinv_at(dinv, i)));
     }
     block_sum2(a0, a1);
     if (threadIdx.x == 0) {
@@ -307,7 +319,7 @@ int ensure_mixed(nekbone_problem *P, int precond)
     if (precond == NEKBONE_PRECOND_JACOBI && M->dinv_version != P->dinv_version) {
         if (!M->dinv)
             if (int rc = dalloc(&M->dinv, P->n)) return rc;
-        hipLaunchKernelGGL(k_to_f32, dim3(grid(P->n)), dim3(kThreads), 0, stream(), M->dinv,
+        hipLaunchKernelGGL(k_to_bf16, dim3(grid(P->n)), dim3(kThreads), 0, stream(), M->dinv,
                            P->d_dinv, P->n);
         NB_HIP(hipGetLastError());
         M->dinv_version = P->dinv_version;
@@ -391,14 +403,15 @@ int reliable_update(nekbone_problem *P, int precond, int mbit)
     const int nb = reduce_blocks(P->n);
     if (precond == NEKBONE_PRECOND_JACOBI) {
         hipLaunchKernelGGL(k_mx_resid<NEKBONE_PRECOND_JACOBI>, dim3(nb), dim3(kRedThreads), 0,
-                           stream(), P->d_r, M->r, P->d_f, P->d_w, P->d_x, P->d_dinv, P->d_flag,
+                           stream(), P->d_r, M->r, P->d_f, P->d_w, P->d_x,
+                           static_cast<const uint16_t *>(M->dinv), P->d_flag,
                            mbit, P->n, P->d_part, nb);
         hipLaunchKernelGGL(k_mx_resid_final<NEKBONE_PRECOND_JACOBI>, dim3(1),
                            dim3(kFinalThreads), 0, stream(), P->d_part, nb, P->d_scal);
     } else {
         hipLaunchKernelGGL(k_mx_resid<NEKBONE_PRECOND_IDENTITY>, dim3(nb), dim3(kRedThreads), 0,
                            stream(), P->d_r, M->r, P->d_f, P->d_w, P->d_x,
-                           static_cast<const double *>(nullptr), P->d_flag, mbit, P->n,
+                           static_cast<const uint16_t *>(nullptr), P->d_flag, mbit, P->n,
                            P->d_part, nb);
         hipLaunchKernelGGL(k_mx_resid_final<NEKBONE_PRECOND_IDENTITY>, dim3(1),
                            dim3(kFinalThreads), 0, stream(), P->d_part, nb, P->d_scal);
@@ -420,7 +433,8 @@ void free_mixed(nekbone_problem *P)
 {
     if (!P->mx) return;
     MixedState *M = P->mx;
-    for (float **v : {&M->g, &M->Drow, &M->Dcol, &M->dinv, &M->x, &M->r, &M->p, &M->w}) dfree(*v);
+    for (float **v : {&M->g, &M->Drow, &M->Dcol, &M->x, &M->r, &M->p, &M->w}) dfree(*v);
+    dfree(M->dinv);
     dfree(M->gh);
     dfree(M->gsc);
     delete M;
