@@ -269,6 +269,55 @@ static int finish_problem(nekbone_problem *P, const double *c_host)
                     std::make_pair(&G.d_m8, &m8), std::make_pair(&G.d_roff, &roff),
                     std::make_pair(&G.d_ridx, &ridx)})
         if (int rc = up(pr.first, *pr.second)) return rc;
+    // Run form of the pair and quad classes (face and edge interiors of a
+    // box: runs of (nx-2)^2 and nx-2 groups).  The class's groups are split
+    // into maximal runs (every member one position past the previous
+    // group's); whole multiples of R go to the run form, placed first,
+    // the remainders stay in the member list.
+    auto split = [](std::vector<int32_t> &m, int S, int R, std::vector<int32_t> &runs) {
+        runs.clear();
+        std::vector<int32_t> rest;
+        const size_t ng = m.size() / size_t(S);
+        int64_t nrun = 0;
+        for (size_t g = 0; g < ng;) {
+            size_t len = 1;
+            while (g + len < ng) {
+                bool next = true;
+                for (int k = 0; k < S; ++k)
+                    next = next && m[(g + len) * S + k] == m[(g + len - 1) * S + k] + 1;
+                if (!next) break;
+                ++len;
+            }
+            const size_t full = R >= 2 ? len / size_t(R) * size_t(R) : 0;
+            for (size_t c = 0; c < full; c += size_t(R))
+                for (int k = 0; k < S; ++k) runs.push_back(m[(g + c) * S + k]);
+            for (size_t t = full; t < len; ++t)
+                for (int k = 0; k < S; ++k) rest.push_back(m[(g + t) * S + k]);
+            nrun += int64_t(full);
+            g += len;
+        }
+        m.swap(rest);
+        return nrun;
+    };
+    {
+        std::vector<int32_t> r2, r4;
+        const int R2 = (nx - 2) * (nx - 2), R4 = nx - 2;
+        const int64_t n2r = split(m2, 2, R2, r2), n4r = split(m4, 4, R4, r4);
+        dfree(G.d_m2);
+        dfree(G.d_m4);
+        if (int rc = up(&G.d_m2, m2)) return rc;
+        if (int rc = up(&G.d_m4, m4)) return rc;
+        if (n2r > 0) {
+            if (int rc = up(&G.d_m2r, r2)) return rc;
+            G.r2 = R2;
+            G.n2r = n2r;
+        }
+        if (n4r > 0) {
+            if (int rc = up(&G.d_m4r, r4)) return rc;
+            G.r4 = R4;
+            G.n4r = n4r;
+        }
+    }
 
     std::vector<double> drow(size_t(nx) * nx);
     for (int i = 0; i < nx; ++i)

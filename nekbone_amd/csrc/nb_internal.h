@@ -201,6 +201,17 @@ struct GsMap {
     int32_t *d_m8 = nullptr;                  // 8*n8
     int32_t *d_roff = nullptr;                // nr+1
     int32_t *d_ridx = nullptr;                // members of the other groups
+    // Run form of the pair and quad classes: groups that come in runs of
+    // r2 (r4) consecutive groups whose every member advances by one device
+    // position from group to group (the face and edge interiors of a box
+    // mesh in the vertex/edge/face/interior order) are stored as the first
+    // group of each run, and group t of a run has members first + t.  The
+    // first n2r (n4r) groups of the class are in run form, the rest are in
+    // the member lists d_m2 (d_m4).
+    int r2 = 0, r4 = 0;
+    int64_t n2r = 0, n4r = 0;
+    int32_t *d_m2r = nullptr;                 // 2 * n2r / r2
+    int32_t *d_m4r = nullptr;                 // 4 * n4r / r4
 };
 
 // Per-kernel-class HIP-event timing.  Event pairs are recorded on the
@@ -262,6 +273,9 @@ struct GsArgs {
     int64_t n2, n4, n8, nr;
     int b2, b4, b8, br;  // workgroups per class
     int rev;             // walk the groups downward (see sweep_block)
+    const int32_t *m2r, *m4r;  // run form of the pair / quad class (GsMap)
+    int r2, r4;
+    int64_t n2r, n4r;
 };
 // Arguments of the fused CG gather-scatter of one problem.
 struct GsCgArgs {

@@ -36,27 +36,50 @@ __device__ __forceinline__ void gs_finish(T *__restrict__ w, const T *__restrict
 }
 
 template <typename T, int S, bool CG>
-__device__ __forceinline__ void gs_group(T *__restrict__ w, const T *__restrict__ p,
-                                         const uint8_t *__restrict__ flag, int mbit,
-                                         const int32_t *__restrict__ mem, double &acc)
+__device__ __forceinline__ void gs_sum(T *__restrict__ w, const T *__restrict__ p,
+                                       const uint8_t *__restrict__ flag, int mbit,
+                                       const int (&m)[S], double &acc)
 {
-    int m[S];
-    if constexpr (S == 2) {
-        const int2 v = *reinterpret_cast<const int2 *>(mem);
-        m[0] = v.x, m[1] = v.y;
-    } else {
-#pragma unroll
-        for (int h = 0; h < S / 4; ++h) {
-            const int4 v = reinterpret_cast<const int4 *>(mem)[h];
-            m[4 * h] = v.x, m[4 * h + 1] = v.y, m[4 * h + 2] = v.z, m[4 * h + 3] = v.w;
-        }
-    }
     T t = w[m[0]];
 #pragma unroll
     for (int q = 1; q < S; ++q) t += w[m[q]];
     const double c = 1. / double(S);
 #pragma unroll
     for (int q = 0; q < S; ++q) gs_finish<T, CG>(w, p, flag, mbit, m[q], t, c, acc);
+}
+
+// Group g of a class of size S: the first nrun groups are in run form
+// (runs of R; group t of a run has the run's first members + t), the
+// rest in the member list.
+template <typename T, int S, bool CG>
+__device__ __forceinline__ void gs_group(T *__restrict__ w, const T *__restrict__ p,
+                                         const uint8_t *__restrict__ flag, int mbit,
+                                         const int32_t *__restrict__ list,
+                                         const int32_t *__restrict__ runs, int R, int64_t nrun,
+                                         int64_t g, double &acc)
+{
+    int m[S];
+    const int32_t *mem;
+    int t = 0;
+    if (g < nrun) {
+        const int64_t run = g / R;
+        t = int(g - run * R);
+        mem = runs + S * run;
+    } else {
+        mem = list + S * (g - nrun);
+    }
+    if constexpr (S == 2) {
+        const int2 v = *reinterpret_cast<const int2 *>(mem);
+        m[0] = v.x + t, m[1] = v.y + t;
+    } else {
+#pragma unroll
+        for (int h = 0; h < S / 4; ++h) {
+            const int4 v = reinterpret_cast<const int4 *>(mem)[h];
+            m[4 * h] = v.x + t, m[4 * h + 1] = v.y + t, m[4 * h + 2] = v.z + t,
+                  m[4 * h + 3] = v.w + t;
+        }
+    }
+    gs_sum<T, S, CG>(w, p, flag, mbit, m, acc);
 }
 
 template <typename T, bool CG>
@@ -70,13 +93,13 @@ __device__ __forceinline__ void gs_body(T *__restrict__ w, const T *__restrict__
     double acc = 0.0;
     if (blk < G.b2) {
         const int64_t g = int64_t(blk) * kRedThreads + threadIdx.x;
-        if (g < G.n2) gs_group<T, 2, CG>(w, p, flag, mbit, G.m2 + 2 * g, acc);
+        if (g < G.n2) gs_group<T, 2, CG>(w, p, flag, mbit, G.m2, G.m2r, G.r2, G.n2r, g, acc);
     } else if ((blk -= G.b2) < G.b4) {
         const int64_t g = int64_t(blk) * kRedThreads + threadIdx.x;
-        if (g < G.n4) gs_group<T, 4, CG>(w, p, flag, mbit, G.m4 + 4 * g, acc);
+        if (g < G.n4) gs_group<T, 4, CG>(w, p, flag, mbit, G.m4, G.m4r, G.r4, G.n4r, g, acc);
     } else if ((blk -= G.b4) < G.b8) {
         const int64_t g = int64_t(blk) * kRedThreads + threadIdx.x;
-        if (g < G.n8) gs_group<T, 8, CG>(w, p, flag, mbit, G.m8 + 8 * g, acc);
+        if (g < G.n8) gs_group<T, 8, CG>(w, p, flag, mbit, G.m8, nullptr, 1, 0, g, acc);
     } else {
         blk -= G.b8;
         const int64_t g = int64_t(blk) * kRedThreads + threadIdx.x;
@@ -124,7 +147,8 @@ __global__ void __launch_bounds__(kRedThreads)
 
 GsArgs gs_args_map(const GsMap &M)
 {
-    GsArgs G{M.d_m2, M.d_m4, M.d_m8, M.d_roff, M.d_ridx, M.n2, M.n4, M.n8, M.nr, 0, 0, 0, 0, 0};
+    GsArgs G{M.d_m2, M.d_m4, M.d_m8, M.d_roff, M.d_ridx, M.n2, M.n4, M.n8, M.nr, 0, 0, 0, 0, 0,
+             M.d_m2r, M.d_m4r, M.r2, M.r4, M.n2r, M.n4r};
     auto nb = [](int64_t n) { return int((n + kRedThreads - 1) / kRedThreads); };
     G.b2 = nb(M.n2);
     G.b4 = nb(M.n4);
@@ -490,6 +514,10 @@ int gs_map_upload(GsMap &G, const std::vector<int32_t> &off, const std::vector<i
 
 void gs_map_free(GsMap &G)
 {
+    dfree(G.d_m2r);
+    dfree(G.d_m4r);
+    G.r2 = G.r4 = 0;
+    G.n2r = G.n4r = 0;
     dfree(G.d_m2);
     dfree(G.d_m4);
     dfree(G.d_m8);
