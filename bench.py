@@ -196,9 +196,11 @@ def tol_workload(nb, nel, nx1, delta):
     nb.sync()
     tb = time.perf_counter()
     x64 = xq.download()
-    # the first call builds the FP32 copies
+    # the first call builds the FP32 copies; the autotuner picks the FP32
+    # element-kernel shape (as nekbone_autotune does for the FP64 one)
     xm = Q.vec()
     Q.cg_mixed_d(xm, fq, rq, 1e-10, 5000, nb.PRECOND_JACOBI, delta)
+    Q.autotune_mixed(nb.PRECOND_JACOBI)
     nb.sync()
     tc = time.perf_counter()
     m_it, m_upd, m_rtr = Q.cg_mixed_d(xm, fq, rq, 1e-10, 5000, nb.PRECOND_JACOBI, delta)
@@ -218,9 +220,11 @@ def tol_workload(nb, nel, nx1, delta):
         "geometry": "deformed +-10% vertices, seed 1", "delta": delta or 1e-3}
     # the same solve with the FP16 factors, and with the FP16 matrix-core
     # inner operator
+    out["mixed_variant_f32"] = list(Q.variant_f32())
     for name, mode in (("mixed_g16", nb.MIXED_G16), ("mixed_f16_mfma", nb.MIXED_F16_MFMA)):
         Q.set_mixed_inner(mode)
         Q.cg_mixed_d(xm, fq, rq, 1e-10, 5000, nb.PRECOND_JACOBI, delta)
+        Q.autotune_mixed(nb.PRECOND_JACOBI)
         nb.sync()
         t0 = time.perf_counter()
         it2, upd2, rtr2 = Q.cg_mixed_d(xm, fq, rq, 1e-10, 5000, nb.PRECOND_JACOBI, delta)
@@ -228,12 +232,17 @@ def tol_workload(nb, nel, nx1, delta):
         ms2 = (time.perf_counter() - t0) * 1e3
         d2 = xm.download() - x64
         out[name] = {"ms": ms2, "iters": it2, "fp64_updates": upd2,
+                     "variant_f32": list(Q.variant_f32()),
                      "speedup_vs_fp64": fp64_ms / ms2,
                      "final_rel_residual": (rtr2 / rtr0) ** 0.5,
                      "vs_fp64_solution": {
                          "l2_rel": float(np.linalg.norm(d2) / np.linalg.norm(x64)),
                          "max_rel": float(np.abs(d2).max() / np.abs(x64).max())}}
     Q.set_mixed_inner(nb.MIXED_F32)
+    best = min((("mixed_f32", out["mixed_ms"]),) +
+               tuple((k, out[k]["ms"]) for k in ("mixed_g16", "mixed_f16_mfma")),
+               key=lambda kv: kv[1])
+    out["best_mixed"] = {"inner": best[0], "ms": best[1], "speedup": fp64_ms / best[1]}
     return out
 
 

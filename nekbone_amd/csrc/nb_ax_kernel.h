@@ -35,9 +35,9 @@
 //    HBM serves faster than the same bytes blocked per element.  PF=1
 //    issues the loads at kernel entry (deeper memory pipelining, more
 //    VGPRs); PF=0 issues them in the geometric phase (higher occupancy);
-//    PF=2 is PF=0 compiled for 5 waves per SIMD (at most 96 VGPRs), the
-//    most that 16 KB of LDS per element allows at one element per
-//    workgroup.
+//    PF=2 is PF=0 compiled for 5 waves per SIMD in FP64 (at most 96
+//    VGPRs, the most that 16 KB of LDS per element allows at one element
+//    per workgroup) and for 8 in FP32 (at most 64 VGPRs).
 //
 // CG mode (MODE 1 = identity, 2 = Jacobi) fuses, per point of the
 // thread's t-line:
@@ -209,11 +209,11 @@ struct AxArgsT {
     const T *r;     // CG: residual
     const typename DinvOf<T>::type *dinv;  // CG Jacobi
     T *x;           // CG: solution, updated one iteration late (XU)
-    const T *g;     // factor planes, g[c * gcs + e * N3 + p]
+    const T *g;     // FP64: planes g[c * gcs + e * N3 + p]; FP32: g[6 * (e * N3 + p) + c]
     const T *Drow, *Dcol;
     const uint8_t *flag;
     const int16_t *iref;  // reference position of each device position (N3)
-    const __half *gh;     // G16: geometric factors in FP16 (same planes),
+    const __half *gh;     // G16: geometric factors in FP16 (six per point),
                           // g = gh * gsc[c * nelt + e]
     const float *gsc;     // G16: per factor and element scale
     int mbit;
@@ -251,10 +251,16 @@ __device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
     const int64_t base = e * N3;
     const T *__restrict__ Drow = A_.Drow;
     const T *__restrict__ Dcol = A_.Dcol;
+    // Factor layouts: FP64, six mesh-wide planes (factor c of point q at
+    // ge[c*gcs + q]); FP32 and FP16 (inner solves of the mixed CG), the six
+    // factors of a point contiguous (g[6*(e*N3 + q) + c]), read with one
+    // 8-byte load per two factors (FP32) or one 4-byte load (FP16), which
+    // the narrow types need to stream near the HBM rate.
+    constexpr bool AOS = sizeof(T) == 4;
     const int64_t gcs = A_.gcs;
-    const T *ge = A_.g + e * N3;
+    const T *ge = A_.g + (AOS ? 6 : 1) * e * N3;
     // G16 (FP32 inner solves only): factor c at point q is gh*scale
-    const __half *gh = G16 ? A_.gh + e * N3 : nullptr;
+    const __half *gh = G16 ? A_.gh + 6 * e * N3 : nullptr;
     T gsc[6] = {T(1), T(1), T(1), T(1), T(1), T(1)};
     if constexpr (G16) {
         if (act) {
@@ -262,11 +268,28 @@ __device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
             for (int c = 0; c < 6; ++c) gsc[c] = T(A_.gsc[c * int64_t(A_.nelt) + e]);
         }
     }
-    auto gload = [&](int c, int q) -> T {
-        if constexpr (G16)
-            return T(__half2float(gh[c * gcs + q])) * gsc[c];
-        else
-            return ge[c * gcs + q];
+    // the six factors of point q of the element
+    auto gload6 = [&](int q, T(&g)[6]) {
+        if constexpr (G16) {
+            const uint32_t *p = reinterpret_cast<const uint32_t *>(gh + 6 * q);
+#pragma unroll
+            for (int h = 0; h < 3; ++h) {
+                const uint32_t v = p[h];
+                g[2 * h] = T(__half2float(__ushort_as_half(uint16_t(v & 0xffffu)))) * gsc[2 * h];
+                g[2 * h + 1] = T(__half2float(__ushort_as_half(uint16_t(v >> 16)))) * gsc[2 * h + 1];
+            }
+        } else if constexpr (AOS) {
+            const float2 *p = reinterpret_cast<const float2 *>(ge + 6 * q);
+#pragma unroll
+            for (int h = 0; h < 3; ++h) {
+                const float2 v = p[h];
+                g[2 * h] = v.x;
+                g[2 * h + 1] = v.y;
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) g[c] = ge[c * gcs + q];
+        }
     };
 
     if constexpr (MODE != 0) {
@@ -274,13 +297,11 @@ __device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
     }
 
     constexpr bool PRE = PF == 1;
-    T gv[PRE ? 6 : 1][PRE ? NX : 1];
+    T gv[PRE ? NX : 1][6];
     if constexpr (PRE) {
         if (act) {
 #pragma unroll
-            for (int c = 0; c < 6; ++c)
-#pragma unroll
-                for (int k = 0; k < NX; ++k) gv[c][k] = gload(c, a + NX * b + N2 * k);
+            for (int k = 0; k < NX; ++k) gload6(a + NX * b + N2 * k, gv[k]);
         }
     }
 
@@ -385,8 +406,9 @@ __device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
             for (int k = 0; k < NX; ++k) {
                 const int q = a + NX * b + N2 * k;
                 const T ur = sA[q], us = sC[q], ut = sB[q];
-                const T g1 = gload(0, q), g2 = gload(1, q), g3 = gload(2, q);
-                const T g4 = gload(3, q), g5 = gload(4, q), g6 = gload(5, q);
+                T gq[6];
+                gload6(q, gq);
+                const T g1 = gq[0], g2 = gq[1], g3 = gq[2], g4 = gq[3], g5 = gq[4], g6 = gq[5];
                 wr[k] = fma_t(g3, ut, fma_t(g2, us, g1 * ur));
                 ws[k] = fma_t(g5, ut, fma_t(g4, us, g2 * ur));
                 wt[k] = fma_t(g6, ut, fma_t(g5, us, g3 * ur));
@@ -418,41 +440,41 @@ __device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
     } else {
         T uc[NX];  // the thread's t-line of the input
         if (act) {
-    #pragma unroll
+#pragma unroll
             for (int k = 0; k < NX; ++k) uc[k] = A[a + NX * b + N2 * k];
         }
 
         T lr[NX], ls[NX];
         if (act) {
-    #pragma unroll
+#pragma unroll
             for (int m = 0; m < NX; ++m) lr[m] = A[m + NX * a + N2 * b];  // r-line (j,k)=(a,b)
-    #pragma unroll
+#pragma unroll
             for (int m = 0; m < NX; ++m) ls[m] = A[a + NX * m + N2 * b];  // s-line (i,k)=(a,b)
         }
         __syncthreads();
         if (act) {
-    #pragma unroll
+#pragma unroll
             for (int i = 0; i < NX; ++i) B[i + NX * a + N2 * b] = dot_row<NX, FMA>(Drow, i, lr);
-    #pragma unroll
+#pragma unroll
             for (int j = 0; j < NX; ++j) A[a + NX * j + N2 * b] = dot_row<NX, FMA>(Drow, j, ls);
         }
         __syncthreads();
 
         T wt[NX];
         if (act) {
-    #pragma unroll
+#pragma unroll
             for (int k = 0; k < NX; ++k) {
                 const int q = a + NX * b + N2 * k;
                 const T ur = B[q], us = A[q];
                 const T ut = dot_row<NX, FMA>(Drow, k, uc);
-                T g1, g2, g3, g4, g5, g6;
+                T gq[6];
                 if constexpr (PRE) {
-                    g1 = gv[0][k], g2 = gv[1][k], g3 = gv[2][k];
-                    g4 = gv[3][k], g5 = gv[4][k], g6 = gv[5][k];
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) gq[c] = gv[k][c];
                 } else {
-                    g1 = gload(0, q), g2 = gload(1, q), g3 = gload(2, q);
-                    g4 = gload(3, q), g5 = gload(4, q), g6 = gload(5, q);
+                    gload6(q, gq);
                 }
+                const T g1 = gq[0], g2 = gq[1], g3 = gq[2], g4 = gq[3], g5 = gq[4], g6 = gq[5];
                 T wr, ws;
                 if constexpr (FMA) {
                     wr = fma_t(g3, ut, fma_t(g2, us, g1 * ur));
@@ -472,13 +494,13 @@ __device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
         if (act) {
             // r-line and s-line are each read and rewritten by their owner
             // only, so no barrier is needed between the reads and the writes.
-    #pragma unroll
+#pragma unroll
             for (int m = 0; m < NX; ++m) lr[m] = B[m + NX * a + N2 * b];
-    #pragma unroll
+#pragma unroll
             for (int i = 0; i < NX; ++i) B[i + NX * a + N2 * b] = dot_row<NX, FMA>(Dcol, i, lr);
-    #pragma unroll
+#pragma unroll
             for (int m = 0; m < NX; ++m) ls[m] = A[a + NX * m + N2 * b];
-    #pragma unroll
+#pragma unroll
             for (int j = 0; j < NX; ++j) A[a + NX * j + N2 * b] = dot_row<NX, FMA>(Dcol, j, ls);
         }
         __syncthreads();
@@ -486,7 +508,7 @@ __device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
         // w at the thread's t-line points, kept in place of the consumed
         // partials (each entry is read and rewritten by its owner only)
         if (act) {
-    #pragma unroll
+#pragma unroll
             for (int k = 0; k < NX; ++k) {
                 const int q = a + NX * b + N2 * k;
                 A[q] = (B[q] + A[q]) + dot_row<NX, FMA>(Dcol, k, wt);
@@ -539,7 +561,8 @@ __device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
 template <typename T, int NX, int EPB, bool FMA, int MODE, int PF, bool XU, bool G16 = false,
           bool MF = false>
 __global__ void __launch_bounds__(round64(NX *NX *EPB))
-    __attribute__((amdgpu_waves_per_eu(PF == 2 ? 5 : 1, 8))) k_ax(AxArgsT<T> A_)
+    __attribute__((amdgpu_waves_per_eu(PF == 2 ? (sizeof(T) == 4 ? 8 : 5) : 1, 8)))
+    k_ax(AxArgsT<T> A_)
 {
     ax_body<T, NX, EPB, FMA, MODE, PF, XU, G16, MF>(A_,
                                                    sweep_block(blockIdx.x, gridDim.x, A_.rev));
@@ -661,6 +684,8 @@ void reg32_order(Table32 &t, bool full)
         reg32_modes<NX, E, 1>(t);
         reg32_modes<NX, 2 * E, 0>(t);
         reg32_modes<NX, 2 * E, 1>(t);
+        reg32_modes<NX, E, 2>(t);
+        reg32_modes<NX, 2 * E, 2>(t);
     }
 }
 

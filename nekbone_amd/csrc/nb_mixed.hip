@@ -76,12 +76,13 @@ __global__ void k_to_bf16(uint16_t *__restrict__ dst, const double *__restrict__
 }
 
 // FP16 geometric factors: one workgroup per (factor, element) block of
-// the factor planes (block c*nelt + e) finds the largest magnitude, stores it as the scale, and the factor divided by it
-// rounded to half precision (relative error <= 2^-11 for every value above
-// 2^-14 of the largest).
+// the FP64 factor planes (block c*nelt + e) finds the largest magnitude,
+// stores it as the scale, and writes the factor divided by it, rounded to
+// half precision (relative error <= 2^-11 for every value above 2^-14 of
+// the largest).
 __global__ void __launch_bounds__(256)
     k_to_g16(__half *__restrict__ gh, float *__restrict__ gsc, const double *__restrict__ g,
-             int n3)
+             int n3, int64_t nelt)
 {
     const int64_t base = int64_t(blockIdx.x) * n3;
     double m = 0.0;
@@ -96,8 +97,22 @@ __global__ void __launch_bounds__(256)
     const double scale = sm[0] > 0.0 ? sm[0] : 1.0;
     if (threadIdx.x == 0) gsc[blockIdx.x] = float(scale);
     const double inv = 1.0 / double(float(scale));
+    // six factors per point: factor c of point q of element e at
+    // gh[6*(e*n3 + q) + c]
+    const int64_t c = blockIdx.x / nelt, e = blockIdx.x - c * nelt;
     for (int q = threadIdx.x; q < n3; q += blockDim.x)
-        gh[base + q] = __float2half(float(g[base + q] * inv));
+        gh[6 * (e * n3 + q) + c] = __float2half(float(g[base + q] * inv));
+}
+
+// FP32 factors for the inner iterations, six per point:
+// dst[6*l + c] = src[c*n + l] (l = e*n3 + q).
+__global__ void k_g_to_f32_points(float *__restrict__ dst, const double *__restrict__ src,
+                                  int64_t n)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= 6 * n) return;
+    const int64_t l = i / 6, c = i - l * 6;
+    dst[i] = float(src[c * n + l]);
 }
 
 // Reference point order <-> device order for float vectors (perm gives
@@ -306,13 +321,13 @@ int ensure_mixed(nekbone_problem *P, int precond)
         const int n3 = P->nx1 * P->nx1 * P->nx1;
         hipLaunchKernelGGL(k_to_g16, dim3(6 * P->nelt), dim3(256), 0, stream(),
                            reinterpret_cast<__half *>(M->gh), M->gsc,
-                           static_cast<const double *>(P->d_g), n3);
+                           static_cast<const double *>(P->d_g), n3, int64_t(P->nelt));
         NB_HIP(hipGetLastError());
         M->g16_version = P->geom_version;
     }
     if (M->geom_version != P->geom_version) {
-        hipLaunchKernelGGL(k_to_f32, dim3(grid(6 * P->n)), dim3(kThreads), 0, stream(), M->g,
-                           P->d_g, 6 * P->n);
+        hipLaunchKernelGGL(k_g_to_f32_points, dim3(grid(6 * P->n)), dim3(kThreads), 0, stream(),
+                           M->g, P->d_g, P->n);
         NB_HIP(hipGetLastError());
         M->geom_version = P->geom_version;
     }
@@ -453,11 +468,15 @@ extern "C" int nekbone_cg_mixed_d(nekbone_problem *P, double *x, const double *f
     if (!(tol > 0.0)) return fail(NEKBONE_ERR_ARG, "cg_mixed: tol must be > 0");
     if (maxiter < 0) return fail(NEKBONE_ERR_ARG, "cg_mixed: maxiter < 0");
     // default residual reduction between FP64 updates: 1e-3 for the FP32
-    // operator; 3e-2 for the FP16 ones, whose ~5e-4 relative operator error
-    // otherwise costs the inner iteration its convergence rate (measured at
-    // 32^3 N=9 deformed: 94 inner iterations with 3e-2 against 111 (FP16
-    // factors) and 427 (FP16 matrix cores) with 1e-3)
-    if (delta == 0.0) delta = (P->mx && P->mx->g16) ? 3e-2 : 1e-3;
+    // operator; 1e-2 (FP16 factors) and 3e-2 (FP16 matrix cores) for the
+    // FP16 ones, whose ~5e-4 relative operator error otherwise costs the
+    // inner iteration its convergence rate (measured at 32^3 N=9 deformed:
+    // 94 inner iterations with these, against 111 and 427 with 1e-3)
+    if (delta == 0.0)
+        delta = !P->mx ? 1e-3
+                : P->mx->g16 == NEKBONE_MIXED_G16 ? 1e-2
+                : P->mx->g16 == NEKBONE_MIXED_F16_MFMA ? 3e-2
+                : 1e-3;
     if (!(delta > 0.0 && delta < 1.0)) return fail(NEKBONE_ERR_ARG, "cg_mixed: need 0 < delta < 1");
     if (precond != NEKBONE_PRECOND_JACOBI && precond != NEKBONE_PRECOND_IDENTITY)
         return fail(NEKBONE_ERR_UNSUPPORTED, "cg_mixed: preconditioner not available");
