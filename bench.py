@@ -115,7 +115,7 @@ def splitmix_u01(n, seed):
     return (z >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
 
 
-def batched_workload(nb, count, seed=2024):
+def batched_workload(nb, count, seed=2024, graph=True):
     """count problems: E_k = round(exp(U[ln 64, ln 8192])) elements in the
     box cubic(E_k), N_k uniform in {5, 7, 9, 11}, brick geometry,
     Dirichlet faces, RHS U[-1,1) seed k -> dssum -> *c, Jacobi to 1e-8.
@@ -135,6 +135,9 @@ def batched_workload(nb, count, seed=2024):
         probs.append(P), fs.append(f), xs.append(x)
     nb.sync()
     B = nb.Batch(probs)
+    B.set_graph(graph)
+    for P in probs:
+        P.set_graph(graph)
     B.autotune(nb.PRECOND_JACOBI)
     tol, maxit = 1e-8, 5000
     B.cg_tol_d(xs, fs, [None] * count, tol, maxit)
@@ -327,6 +330,9 @@ def main():
                     help="skip the HSMG-preconditioned solve on 16x16x16 elements, N=7")
     ap.add_argument("--only", choices=("tol", "batch", "hsmg"), default=None,
                     help="run only this part (no timed CG steps) and print its record")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch every kernel directly instead of replaying HIP graphs "
+                         "(for tracing tools)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tol", action="store_true")
     args = ap.parse_args()
@@ -348,12 +354,13 @@ def main():
     nx1, niter = args.nx1, args.niter
     if args.only:
         part = {"tol": lambda: tol_workload(nb, nel, nx1, args.delta),
-                "batch": lambda: batched_workload(nb, args.batch),
+                "batch": lambda: batched_workload(nb, args.batch, graph=not args.no_graph),
                 "hsmg": lambda: hsmg_workload(nb)}[args.only]()
         print(json.dumps({args.only: part}))
         return
     P = nb.Problem(*nel, nx1)
     n = P.n
+    P.set_graph(not args.no_graph)
     P.set_mask_mode(nb.MASK_DIRICHLET)
     P.setup_jacobi()
     f, x, r = P.vec(), P.vec(), P.vec()
@@ -423,7 +430,7 @@ def main():
 
     if args.batch > 0:
         print("bench: batched solves", file=sys.stderr, flush=True)
-        extra["batched"] = batched_workload(nb, args.batch)
+        extra["batched"] = batched_workload(nb, args.batch, graph=not args.no_graph)
 
     cpu = None
     if not args.no_cpu:

@@ -258,6 +258,9 @@ int nekbone_batch_size(const nekbone_batch *B);
  * the vector update.  nekbone_batch_autotune times both and keeps the
  * faster; *best_ms is its time per batched iteration. */
 int nekbone_batch_set_xu(nekbone_batch *B, int xu);
+/* 1 (default): chunks of ten iterations are captured once into a HIP graph
+ * and replayed; 0: every kernel is launched directly (for tracing tools). */
+int nekbone_batch_set_graph(nekbone_batch *B, int on);
 int nekbone_batch_autotune(nekbone_batch *B, int precond, double *best_ms);
 /* CG to a relative residual for every problem k: x[k], f[k], r[k] as for
  * nekbone_cg_tol_d (x, r may be NULL); stops when every problem has met

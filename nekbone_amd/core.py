@@ -60,6 +60,7 @@ _SIGS = {
     "nekbone_set_hsmg_fp32": (ctypes.c_int, [_vp, ctypes.c_int]),
     "nekbone_set_hsmg_coarse": (ctypes.c_int, [_vp, ctypes.c_int]),
     "nekbone_set_mixed_inner": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "nekbone_batch_set_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
     "nekbone_get_dinv": (ctypes.c_int, [_vp, _dp]),
     "nekbone_ax": (ctypes.c_int, [_vp, _dp, _dp]),
     "nekbone_ax_d": (ctypes.c_int, [_vp, _vp, _vp]),
@@ -188,6 +189,9 @@ class Batch:
 
     def set_xu(self, xu):
         _chk(lib().nekbone_batch_set_xu(self.handle, int(xu)))
+
+    def set_graph(self, on):
+        _chk(lib().nekbone_batch_set_graph(self.handle, int(bool(on))))
 
     def autotune(self, precond=None):
         precond = PRECOND_JACOBI if precond is None else precond

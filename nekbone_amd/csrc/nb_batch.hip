@@ -37,6 +37,7 @@ struct nekbone_batch {
     // captured chunk of iterations
     hipGraphExec_t exec = nullptr;
     int exec_precond = -1, exec_chunk = 0, exec_xu = -1;
+    int use_graph = 1;  // replay chunks of iterations from a captured graph
 };
 
 namespace nb {
@@ -278,6 +279,14 @@ extern "C" int nekbone_batch_destroy(nekbone_batch *B)
 
 extern "C" int nekbone_batch_size(const nekbone_batch *B) { return B ? int(B->P.size()) : -1; }
 
+extern "C" int nekbone_batch_set_graph(nekbone_batch *B, int on)
+{
+    if (!B) return fail(NEKBONE_ERR_ARG, "batch: null");
+    B->use_graph = on ? 1 : 0;
+    if (!B->use_graph) drop_graph(B);
+    return NEKBONE_OK;
+}
+
 extern "C" int nekbone_batch_set_xu(nekbone_batch *B, int xu)
 {
     if (!B) return fail(NEKBONE_ERR_ARG, "batch: null");
@@ -311,12 +320,17 @@ extern "C" int nekbone_batch_cg_tol_d(nekbone_batch *B, double *const *x, const 
     }
     if (int rc = prepare(B, precond, mbit)) return rc;
     const int chunk = 10;
-    if (int rc = ensure_graph(B, precond, chunk)) return rc;
+    if (B->use_graph)
+        if (int rc = ensure_graph(B, precond, chunk)) return rc;
     std::vector<CgScalars> s;
     int done_iters = 0;
     while (done_iters < maxiter) {
-        if (maxiter - done_iters >= chunk) {
+        if (B->use_graph && maxiter - done_iters >= chunk) {
             NB_HIP(hipGraphLaunch(B->exec, stream()));
+            done_iters += chunk;
+        } else if (maxiter - done_iters >= chunk) {
+            for (int i = 0; i < chunk; ++i)
+                if (int rc = launch_iteration(B, precond)) return rc;
             done_iters += chunk;
         } else {
             for (; done_iters < maxiter; ++done_iters)
