@@ -201,7 +201,7 @@ int nekbone_cg_tol_d(nekbone_problem *P, double *x, const double *f, double *r, 
  * accumulated in FP64 and the residual is recomputed in FP64 ("reliable
  * update", the search direction is kept) whenever the FP32 residual norm
  * has dropped by the factor delta since the last update (0 selects 1e-3,
- * 1e-2 with FP16 factors, 3e-2 with the FP16 matrix-core operator; see
+ * 5e-3 with FP16 factors, 3e-2 with the FP16 matrix-core operator; see
  * nekbone_set_mixed_inner)
  * or the FP32 iteration reports convergence.  x, f, r as for
  * nekbone_cg_tol_d.  *iters: inner iterations; *updates: FP64 residual

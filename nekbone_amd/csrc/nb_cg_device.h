@@ -18,19 +18,28 @@ __device__ __forceinline__ double inv_mult(uint8_t fl)
     return m == 1 ? 1.0 : m == 2 ? 0.5 : m == 4 ? 0.25 : m == 8 ? 0.125 : 1. / double(m);
 }
 
-// Fixed-order sum of a strided slice with four interleaved accumulators
-// (four loads in flight per thread), combined as (s0+s1)+(s2+s3).
+// Fixed-order sum of a strided slice with sixteen interleaved accumulators
+// (sixteen loads in flight per thread: the single-block final kernels sum
+// up to ~60k partials, and their time is the memory latency of the
+// rounds of loads), combined pairwise in a fixed tree.
 template <int NT>
 __device__ __forceinline__ double slice_sum(const double *__restrict__ a, int64_t na)
 {
-    double s[4] = {0.0, 0.0, 0.0, 0.0};
-    int64_t i = threadIdx.x;
-    for (; i + 3 * NT < na; i += 4 * NT) {
+    constexpr int U = 16;
+    double s[U];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) s[u] = s[u] + a[i + u * NT];
+    for (int u = 0; u < U; ++u) s[u] = 0.0;
+    int64_t i = threadIdx.x;
+    for (; i + (U - 1) * NT < na; i += U * NT) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) s[u] = s[u] + a[i + u * NT];
     }
     for (int u = 0; i < na; i += NT, ++u) s[u] = s[u] + a[i];
-    return (s[0] + s[1]) + (s[2] + s[3]);
+#pragma unroll
+    for (int h = U / 2; h > 0; h >>= 1)
+#pragma unroll
+        for (int u = 0; u < h; ++u) s[u] = s[u] + s[u + h];
+    return s[0];
 }
 
 // Sum of a partials region in a fixed order by one block.
