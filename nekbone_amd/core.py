@@ -60,6 +60,7 @@ _SIGS = {
     "nekbone_set_hsmg_fp32": (ctypes.c_int, [_vp, ctypes.c_int]),
     "nekbone_set_hsmg_coarse": (ctypes.c_int, [_vp, ctypes.c_int]),
     "nekbone_set_mixed_inner": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "nekbone_set_mixed_refresh": (ctypes.c_int, [_vp, ctypes.c_int]),
     "nekbone_batch_set_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
     "nekbone_get_dinv": (ctypes.c_int, [_vp, _dp]),
     "nekbone_ax": (ctypes.c_int, [_vp, _dp, _dp]),
@@ -467,6 +468,10 @@ class Problem:
         """Inner operator of the mixed solver: MIXED_F32, MIXED_G16 or
         MIXED_F16_MFMA."""
         _chk(lib().nekbone_set_mixed_inner(self.handle, int(mode)))
+
+    def set_mixed_refresh(self, fused):
+        """1: fused FP64 residual refresh (default); 0: separate passes."""
+        _chk(lib().nekbone_set_mixed_refresh(self.handle, int(fused)))
 
     def variant_f32(self):
         v = [ctypes.c_int() for _ in range(3)]

@@ -222,6 +222,18 @@ struct AxArgsT {
     int nelt;
     int64_t gcs;         // distance between factor planes (local points n)
     int rev;             // walk the elements downward (see sweep_block)
+    // MODE 3 (the FP64 residual refresh of the mixed-precision CG): x = x
+    // (+) the FP32 increment x32 (+ alpha p32 when pending), x32 = 0, and
+    // for points not shared with another element r = mask(f - (A_L x +
+    // 0.1 x)) into r (FP64) and r32, with the parts of (r,c,r) and of
+    // (r,c,dinv16 r) in part[blk] and part[nrtz + blk].
+    float *x32;
+    const float *p32;
+    float *r32;
+    T *r64;
+    const uint16_t *dinv16;  // null: identity preconditioner
+    const double *f;
+    int64_t nrtz;
 };
 using AxArgs = AxArgsT<double>;
 
@@ -292,7 +304,7 @@ __device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
         }
     };
 
-    if constexpr (MODE != 0) {
+    if constexpr (MODE == 1 || MODE == 2) {
         if (A_.scal->done) return;  // converged tolerance run: nothing to do
     }
 
@@ -316,11 +328,25 @@ __device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
         for (int m = 0; m < NX; ++m) ref[m] = A_.iref[ab + N2 * m];
     }
 
-    T pd[MODE != 0 ? NX : 1];  // CG: the new direction at the linear points
+    T pd[MODE != 0 ? NX : 1];  // CG: the new direction; MODE 3: x, at the linear points
     if (act) {
         if constexpr (MODE == 0) {
 #pragma unroll
             for (int m = 0; m < NX; ++m) A[ref[m]] = A_.u[base + ab + N2 * m];
+        } else if constexpr (MODE == 3) {
+            const bool xp = A_.scal->xpending != 0;
+            const float al = float(A_.scal->alpha);
+#pragma unroll
+            for (int m = 0; m < NX; ++m) {
+                const int64_t q = base + ab + N2 * m;
+                float xi = A_.x32[q];
+                if (xp) xi = __builtin_fmaf(al, A_.p32[q], xi);
+                const T xv = A_.x[q] + T(xi);
+                A_.x[q] = xv;
+                A_.x32[q] = 0.0f;
+                pd[m] = xv;
+                A[ref[m]] = xv;
+            }
         } else {
             const T beta = T(A_.scal->beta);
             T po[NX], z[NX];
@@ -516,7 +542,7 @@ __device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
         }
         __syncthreads();
     }
-    double acc = 0.0;
+    double acc = 0.0, acc2 = 0.0;
     if (act) {
         // positions read again (cache hits) rather than kept live through
         // the contractions
@@ -528,6 +554,18 @@ __device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
             const T wl = MF ? sC[iref[ab + N2 * m]] : A[iref[ab + N2 * m]];
             if constexpr (MODE == 0) {
                 A_.w[q] = wl;
+            } else if constexpr (MODE == 3) {
+                const uint8_t fl = A_.flag[q];
+                if (fl & kFlagShared) {
+                    A_.w[q] = wl;  // finished by the gather-scatter kernel
+                } else {
+                    double v = A_.f[q] - (double(wl) + 0.1 * double(pd[m]));
+                    if (fl & A_.mbit) v = 0.0;
+                    A_.r32[q] = float(v);
+                    A_.r64[q] = T(v);
+                    acc = acc + v * v;
+                    if (A_.dinv16) acc2 = acc2 + v * (v * double(dinv_at(A_.dinv16, q)));
+                }
             } else {
                 const uint8_t fl = A_.flag[q];
                 if (fl & kFlagShared) {
@@ -541,7 +579,7 @@ __device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
             }
         }
     }
-    if constexpr (MODE != 0) {
+    if constexpr (MODE == 1 || MODE == 2) {
         // deterministic block sum of the pap partials: wave butterflies,
         // then the wave sums in wave order
         constexpr int NW = NT / 64;
@@ -554,6 +592,26 @@ __device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
 #pragma unroll
             for (int w = 1; w < NW; ++w) s = s + red[w];
             A_.part[blk] = s;
+        }
+    } else if constexpr (MODE == 3) {
+        constexpr int NW = NT / 64;
+        __shared__ double red[2][NW];
+        acc = wave_sum(acc);
+        acc2 = wave_sum(acc2);
+        if ((tid & 63) == 0) {
+            red[0][tid >> 6] = acc;
+            red[1][tid >> 6] = acc2;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double s = red[0][0], s2 = red[1][0];
+#pragma unroll
+            for (int w = 1; w < NW; ++w) {
+                s = s + red[0][w];
+                s2 = s2 + red[1][w];
+            }
+            A_.part[blk] = s;
+            A_.part[A_.nrtz + blk] = s2;
         }
     }
 }
@@ -616,12 +674,22 @@ void reg_modes(Table &t)
     t[Key(NX, EPB, 1, 2, PF, 1)] = k_ax<double, NX, EPB, true, 2, PF, true>;
 }
 
+// FP64 residual refresh of the mixed-precision CG (MODE 3): the default
+// shape, exact arithmetic.  Key (nx, epb, 0, 3, 0, 0).
+template <int NX>
+void reg_refresh(Table &t)
+{
+    constexpr int E = epb_for(NX);
+    t[Key(NX, E, 0, 3, 0, 0)] = k_ax<double, NX, E, false, 3, 0, false>;
+}
+
 // Orders the benchmark configurations use get the full variant space;
 // the others get the default shape.
 template <int NX>
 void reg_order(Table &t, bool full)
 {
     constexpr int E = epb_for(NX);
+    reg_refresh<NX>(t);
     reg_modes<NX, E, 0>(t);
     if (full) {
         reg_modes<NX, E, 1>(t);

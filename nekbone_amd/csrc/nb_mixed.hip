@@ -291,6 +291,8 @@ int ensure_mixed(nekbone_problem *P, int precond)
         if (int rc = dalloc(&M->Dcol, int64_t(nx) * nx)) return rc;
         for (float **v : {&M->x, &M->r, &M->p, &M->w})
             if (int rc = dalloc(v, P->n)) return rc;
+        if (int rc = dalloc(&M->rpart, 2 * int64_t(ax_refresh_blocks(P) + gs_blocks(P))))
+            return rc;
         std::vector<float> dr(size_t(nx) * nx), dc(size_t(nx) * nx);
         for (int i = 0; i < nx; ++i)
             for (int m = 0; m < nx; ++m) {
@@ -410,6 +412,29 @@ int iteration32(nekbone_problem *P, int precond, int mbit)
 int reliable_update(nekbone_problem *P, int precond, int mbit)
 {
     MixedState *M = P->mx;
+    const uint16_t *d16 =
+        precond == NEKBONE_PRECOND_JACOBI ? static_cast<const uint16_t *>(M->dinv) : nullptr;
+    if (M->refresh) {
+        // fused form: the element kernel flushes x and finishes the points
+        // no other element shares; the gather-scatter kernel the rest
+        const int nax = ax_refresh_blocks(P);
+        const int nall = nax + gs_blocks(P);
+        const int rc = launch_ax_refresh(P, M->x, M->p, M->r, d16, mbit, M->rpart, nall);
+        if (rc == NEKBONE_OK) {
+            if (int rc2 = launch_gs_refresh(P, P->d_w, P->d_x, P->d_f, P->d_r, M->r, d16, mbit,
+                                            M->rpart + nax, nall))
+                return rc2;
+            if (precond == NEKBONE_PRECOND_JACOBI)
+                hipLaunchKernelGGL(k_mx_resid_final<NEKBONE_PRECOND_JACOBI>, dim3(1),
+                                   dim3(kFinalThreads), 0, stream(), M->rpart, nall, P->d_scal);
+            else
+                hipLaunchKernelGGL(k_mx_resid_final<NEKBONE_PRECOND_IDENTITY>, dim3(1),
+                                   dim3(kFinalThreads), 0, stream(), M->rpart, nall, P->d_scal);
+            NB_HIP(hipGetLastError());
+            return NEKBONE_OK;
+        }
+        if (rc != NEKBONE_ERR_UNSUPPORTED) return rc;
+    }
     hipLaunchKernelGGL(k_mx_xflush, dim3(grid(P->n)), dim3(kThreads), 0, stream(), P->d_x, M->x,
                        M->p, P->n, static_cast<const CgScalars *>(P->d_scal));
     NB_HIP(hipGetLastError());
@@ -452,6 +477,7 @@ void free_mixed(nekbone_problem *P)
     dfree(M->dinv);
     dfree(M->gh);
     dfree(M->gsc);
+    dfree(M->rpart);
     delete M;
     P->mx = nullptr;
 }
@@ -522,6 +548,15 @@ extern "C" int nekbone_cg_mixed_d(nekbone_problem *P, double *x, const double *f
     if (updates) *updates = nupd;
     if (final_rtr) *final_rtr = s.rtr;
     if (s.status) return fail(NEKBONE_ERR_NUMERIC, "cg_mixed: non-positive or non-finite pap");
+    return NEKBONE_OK;
+}
+
+extern "C" int nekbone_set_mixed_refresh(nekbone_problem *P, int fused)
+{
+    if (int rc = check(P)) return rc;
+    if (fused != 0 && fused != 1) return fail(NEKBONE_ERR_ARG, "mixed: refresh must be 0 or 1");
+    if (int rc = ensure_mixed(P, NEKBONE_PRECOND_IDENTITY)) return rc;
+    P->mx->refresh = fused;
     return NEKBONE_OK;
 }
 

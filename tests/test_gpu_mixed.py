@@ -194,3 +194,37 @@ def test_mixed_inner_rejects_unknown_mode(nb):
     Q = nb.Problem(2, 2, 2, 3)
     with pytest.raises(nb.NekboneError):
         Q.set_mixed_inner(nb.MIXED_F16_MFMA)  # matrix-core kernels for nx1 >= 4
+
+
+@pytest.mark.parametrize("precond", [1, 0])
+@pytest.mark.parametrize("nel,nx1,deformed", [((3, 2, 2), 9, True), ((3, 1, 1), 5, True),
+                                              ((2, 2, 3), 7, False), ((2, 2, 2), 12, True)])
+def test_cg_mixed_fused_refresh_matches_separate_passes(nb, oracle, nel, nx1, deformed, precond):
+    """The fused FP64 residual refresh (element kernel + gather-scatter
+    refresh) against the separate x flush / operator / gather-scatter /
+    residual passes: the residuals are computed by the same arithmetic in
+    the same order, so only the dot-product partial sums are ordered
+    differently.  Same inner-iteration and update count, x and r within
+    1e-12 of their magnitude, and both final rtr the FP64 residual's."""
+    _case("cg_mixed fused refresh %s %d deformed=%s precond=%d" % (nel, nx1, deformed, precond))
+    P, O = _pair(nb, oracle, nel, nx1, deformed, jacobi=bool(precond))
+    f = O.rhs_from_raw(splitmix_uniform(O.n, 8))
+    out = {}
+    for fused in (1, 0):
+        P.set_mixed_refresh(fused)
+        out[fused] = P.cg_mixed(f, 1e-10, 3000, precond)
+    (xa, ra, ita, nua, rtra), (xb, rb, itb, nub, rtrb) = out[1], out[0]
+    assert (ita, nua) == (itb, nub) and nua >= 1
+    np.testing.assert_allclose(xa, xb, rtol=0, atol=1e-12 * np.abs(xb).max())
+    np.testing.assert_allclose(ra, rb, rtol=0, atol=1e-12 * np.abs(f).max())
+    f0 = O.maskit(f)
+    for x, rtr in ((xa, rtra), (xb, rtrb)):
+        rtrue = f0 - O.ax(x)
+        assert abs(rtr - float(np.sum(rtrue * O.c() * rtrue))) <= 1e-6 * rtr
+
+
+def test_mixed_refresh_rejects_bad_value(nb):
+    _case("mixed refresh bad value")
+    P = nb.Problem(2, 2, 2, 6)
+    with pytest.raises(nb.NekboneError):
+        P.set_mixed_refresh(2)
