@@ -230,13 +230,12 @@ int nekbone_set_mixed_g16(nekbone_problem *P, int on);
 #define NEKBONE_MIXED_G16 1
 #define NEKBONE_MIXED_F16_MFMA 2
 int nekbone_set_mixed_inner(nekbone_problem *P, int mode);
-/* How the mixed solver recomputes the FP64 residual (no counterpart in the
- * reference): 1 (default) fused, the element kernel flushes x and finishes
- * the points no other element shares while the gather-scatter kernel
- * finishes the shared ones (exact arithmetic only; with
- * NEKBONE_ARITH_FMA the unfused form runs); 0 unfused, x flush, element
- * operator, gather-scatter and residual as separate passes.  Both give the
- * same residual up to the order of the dot-product partial sums. */
+/* How the mixed solver's FP64 residual refresh folds the FP32 increment
+ * into x (no counterpart in the reference): 1 (default) inside the FP64
+ * element kernel that applies A_L to the new x; 0 in a pass of its own
+ * before it (also what runs with NEKBONE_ARITH_FMA).  The arithmetic is the
+ * same, so the solves agree bit for bit; 1 saves one pass over x, x32 and
+ * p32 per refresh. */
 int nekbone_set_mixed_refresh(nekbone_problem *P, int fused);
 /* FP32 element-kernel shape of the mixed solver (as nekbone_get_variant /
  * nekbone_set_variant), and its autotuner: times every compiled shape over

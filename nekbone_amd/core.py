@@ -470,7 +470,8 @@ class Problem:
         _chk(lib().nekbone_set_mixed_inner(self.handle, int(mode)))
 
     def set_mixed_refresh(self, fused):
-        """1: fused FP64 residual refresh (default); 0: separate passes."""
+        """1 (default): the FP64 residual refresh flushes x inside the
+        element kernel; 0: in a pass of its own."""
         _chk(lib().nekbone_set_mixed_refresh(self.handle, int(fused)))
 
     def variant_f32(self):

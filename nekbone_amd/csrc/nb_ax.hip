@@ -124,13 +124,7 @@ int launch_ax_cg(nekbone_problem *P, double *x, const double *r, int precond, in
     return timed_end(P, 0);
 }
 
-int ax_refresh_blocks(const nekbone_problem *P)
-{
-    return (P->nelt + axk::epb_for(P->nx1) - 1) / axk::epb_for(P->nx1);
-}
-
-int launch_ax_refresh(nekbone_problem *P, float *x32, const float *p32, float *r32,
-                      const uint16_t *dinv16, int mbit, double *part, int64_t nrtz)
+int launch_ax_flush(nekbone_problem *P, float *x32, const float *p32)
 {
     const int epb = axk::epb_for(P->nx1);
     AxFn fn = lookup(P->nx1, epb, false, 3, 0, 0);
@@ -142,21 +136,14 @@ int launch_ax_refresh(nekbone_problem *P, float *x32, const float *p32, float *r
     A.gcs = P->n;
     A.Drow = P->d_Drow;
     A.Dcol = P->d_D;
-    A.flag = P->d_flag;
     A.iref = P->d_iref;
-    A.mbit = mbit;
     A.scal = P->d_scal;
-    A.part = part;
     A.nelt = P->nelt;
     A.x32 = x32;
     A.p32 = p32;
-    A.r32 = r32;
-    A.r64 = P->d_r;
-    A.dinv16 = dinv16;
-    A.f = P->d_f;
-    A.nrtz = nrtz;
     const int threads = round64(P->nx1 * P->nx1 * epb);
-    hipLaunchKernelGGL(fn, dim3(ax_refresh_blocks(P)), dim3(threads), 0, stream(), A);
+    const int blocks = (P->nelt + epb - 1) / epb;
+    hipLaunchKernelGGL(fn, dim3(blocks), dim3(threads), 0, stream(), A);
     NB_HIP(hipGetLastError());
     return NEKBONE_OK;
 }

@@ -222,18 +222,11 @@ struct AxArgsT {
     int nelt;
     int64_t gcs;         // distance between factor planes (local points n)
     int rev;             // walk the elements downward (see sweep_block)
-    // MODE 3 (the FP64 residual refresh of the mixed-precision CG): x = x
-    // (+) the FP32 increment x32 (+ alpha p32 when pending), x32 = 0, and
-    // for points not shared with another element r = mask(f - (A_L x +
-    // 0.1 x)) into r (FP64) and r32, with the parts of (r,c,r) and of
-    // (r,c,dinv16 r) in part[blk] and part[nrtz + blk].
+    // MODE 3 (first pass of the FP64 residual refresh of the mixed-
+    // precision CG): x = x + the FP32 increment x32 (+ alpha p32 when
+    // pending), x32 = 0, then w = A_L x at every point as in MODE 0.
     float *x32;
     const float *p32;
-    float *r32;
-    T *r64;
-    const uint16_t *dinv16;  // null: identity preconditioner
-    const double *f;
-    int64_t nrtz;
 };
 using AxArgs = AxArgsT<double>;
 
@@ -327,8 +320,20 @@ __device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
 #pragma unroll
         for (int m = 0; m < NX; ++m) ref[m] = A_.iref[ab + N2 * m];
     }
+    // the points' flags, loaded with the vectors (four to a register) so
+    // that the epilogue does not wait for a fresh memory round trip
+    // measured: FP64 gains, FP32 loses
+    constexpr bool FLPRE = (MODE == 1 || MODE == 2) && sizeof(T) == 8;
+    uint32_t flp[FLPRE ? (NX + 3) / 4 : 1] = {};
+    if constexpr (FLPRE) {
+        if (act) {
+#pragma unroll
+            for (int m = 0; m < NX; ++m)
+                flp[m / 4] |= uint32_t(A_.flag[base + ab + N2 * m]) << (8 * (m % 4));
+        }
+    }
 
-    T pd[MODE != 0 ? NX : 1];  // CG: the new direction; MODE 3: x, at the linear points
+    T pd[MODE == 1 || MODE == 2 ? NX : 1];  // CG: the new direction at the linear points
     if (act) {
         if constexpr (MODE == 0) {
 #pragma unroll
@@ -344,7 +349,6 @@ __device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
                 const T xv = A_.x[q] + T(xi);
                 A_.x[q] = xv;
                 A_.x32[q] = 0.0f;
-                pd[m] = xv;
                 A[ref[m]] = xv;
             }
         } else {
@@ -542,7 +546,7 @@ __device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
         }
         __syncthreads();
     }
-    double acc = 0.0, acc2 = 0.0;
+    double acc = 0.0;
     if (act) {
         // positions read again (cache hits) rather than kept live through
         // the contractions
@@ -552,22 +556,10 @@ __device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
         for (int m = 0; m < NX; ++m) {
             const int64_t q = base + ab + N2 * m;
             const T wl = MF ? sC[iref[ab + N2 * m]] : A[iref[ab + N2 * m]];
-            if constexpr (MODE == 0) {
+            if constexpr (MODE == 0 || MODE == 3) {
                 A_.w[q] = wl;
-            } else if constexpr (MODE == 3) {
-                const uint8_t fl = A_.flag[q];
-                if (fl & kFlagShared) {
-                    A_.w[q] = wl;  // finished by the gather-scatter kernel
-                } else {
-                    double v = A_.f[q] - (double(wl) + 0.1 * double(pd[m]));
-                    if (fl & A_.mbit) v = 0.0;
-                    A_.r32[q] = float(v);
-                    A_.r64[q] = T(v);
-                    acc = acc + v * v;
-                    if (A_.dinv16) acc2 = acc2 + v * (v * double(dinv_at(A_.dinv16, q)));
-                }
             } else {
-                const uint8_t fl = A_.flag[q];
+                const uint8_t fl = FLPRE ? uint8_t(flp[m / 4] >> (8 * (m % 4))) : A_.flag[q];
                 if (fl & kFlagShared) {
                     A_.w[q] = wl;  // finished by the gather-scatter kernel
                 } else {
@@ -592,26 +584,6 @@ __device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
 #pragma unroll
             for (int w = 1; w < NW; ++w) s = s + red[w];
             A_.part[blk] = s;
-        }
-    } else if constexpr (MODE == 3) {
-        constexpr int NW = NT / 64;
-        __shared__ double red[2][NW];
-        acc = wave_sum(acc);
-        acc2 = wave_sum(acc2);
-        if ((tid & 63) == 0) {
-            red[0][tid >> 6] = acc;
-            red[1][tid >> 6] = acc2;
-        }
-        __syncthreads();
-        if (tid == 0) {
-            double s = red[0][0], s2 = red[1][0];
-#pragma unroll
-            for (int w = 1; w < NW; ++w) {
-                s = s + red[0][w];
-                s2 = s2 + red[1][w];
-            }
-            A_.part[blk] = s;
-            A_.part[A_.nrtz + blk] = s2;
         }
     }
 }
@@ -674,8 +646,8 @@ void reg_modes(Table &t)
     t[Key(NX, EPB, 1, 2, PF, 1)] = k_ax<double, NX, EPB, true, 2, PF, true>;
 }
 
-// FP64 residual refresh of the mixed-precision CG (MODE 3): the default
-// shape, exact arithmetic.  Key (nx, epb, 0, 3, 0, 0).
+// x flush + FP64 operator of the mixed-precision CG's residual refresh
+// (MODE 3): the default shape, exact arithmetic.  Key (nx, epb, 0, 3, 0, 0).
 template <int NX>
 void reg_refresh(Table &t)
 {

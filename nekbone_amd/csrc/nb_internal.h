@@ -349,10 +349,9 @@ struct MixedState {
     uint16_t *gh = nullptr;
     float *gsc = nullptr;
     int g16_version = -1;
-    // reliable updates by the fused refresh kernels (element kernel in
-    // refresh mode + gather-scatter refresh); rpart holds their partials
+    // 1: the FP64 residual refresh flushes x inside the element kernel;
+    // 0: in a pass of its own
     int refresh = 1;
-    double *rpart = nullptr;
 };
 
 }  // namespace nb
@@ -435,14 +434,10 @@ int launch_gs(nekbone_problem *P, double *u, const CgScalars *guard);
 // pap part per workgroup (gs_blocks of them) to part.
 int launch_gs_cg(nekbone_problem *P, double *w, const double *p, int mbit, double *part,
                  int rev = 0);
-int launch_gs_refresh(nekbone_problem *P, const double *w, const double *x, const double *f,
-                      double *r64, float *r32, const uint16_t *dinv16, int mbit, double *part,
-                      int64_t nrtz);
-// Element kernel in the mixed solver's FP64 residual-refresh mode (nb_ax.hip);
+// First pass of the mixed solver's FP64 residual refresh (nb_ax.hip):
+// x += x32 (+ alpha p32 when pending), x32 = 0, w = A_L x;
 // NEKBONE_ERR_UNSUPPORTED when no such kernel is compiled for the shape.
-int launch_ax_refresh(nekbone_problem *P, float *x32, const float *p32, float *r32,
-                      const uint16_t *dinv16, int mbit, double *part, int64_t nrtz);
-int ax_refresh_blocks(const nekbone_problem *P);
+int launch_ax_flush(nekbone_problem *P, float *x32, const float *p32);
 // FP32 gather-scatter (mixed-precision inner solve): plain, and the fused
 // CG variant (pap parts in FP64).
 int launch_gs32(nekbone_problem *P, float *u);

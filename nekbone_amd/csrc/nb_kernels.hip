@@ -48,36 +48,6 @@ __device__ __forceinline__ void gs_sum(T *__restrict__ w, const T *__restrict__ 
     for (int q = 0; q < S; ++q) gs_finish<T, CG>(w, p, flag, mbit, m[q], t, c, acc);
 }
 
-// Members of group g of a class of size S: the first nrun groups are in
-// run form (runs of R; group t of a run has the run's first members + t),
-// the rest in the member list.
-template <int S>
-__device__ __forceinline__ void gs_members(const int32_t *__restrict__ list,
-                                           const int32_t *__restrict__ runs, int R, int64_t nrun,
-                                           int64_t g, int (&m)[S])
-{
-    const int32_t *mem;
-    int t = 0;
-    if (g < nrun) {
-        const int64_t run = g / R;
-        t = int(g - run * R);
-        mem = runs + S * run;
-    } else {
-        mem = list + S * (g - nrun);
-    }
-    if constexpr (S == 2) {
-        const int2 v = *reinterpret_cast<const int2 *>(mem);
-        m[0] = v.x + t, m[1] = v.y + t;
-    } else {
-#pragma unroll
-        for (int h = 0; h < S / 4; ++h) {
-            const int4 v = reinterpret_cast<const int4 *>(mem)[h];
-            m[4 * h] = v.x + t, m[4 * h + 1] = v.y + t, m[4 * h + 2] = v.z + t,
-                  m[4 * h + 3] = v.w + t;
-        }
-    }
-}
-
 // Group g of a class of size S: the first nrun groups are in run form
 // (runs of R; group t of a run has the run's first members + t), the
 // rest in the member list.
@@ -155,95 +125,6 @@ __global__ void __launch_bounds__(kRedThreads)
              const CgScalars *__restrict__ scal)
 {
     gs_body<T, CG>(w, p, flag, mbit, G, part, scal, sweep_block(blockIdx.x, gridDim.x, G.rev));
-}
-
-// FP64 residual refresh of the mixed-precision CG on the shared points,
-// after the element kernel's refresh mode has left A_L x there: the group
-// sum t (the reference's order), then per member r = mask(f - (t + 0.1 x))
-// into r64 and r32, and the parts of (r,c,r) and (r,c,dinv16 r) with c =
-// 1/(group size) (dinv16: bfloat16 bits, null for the identity).
-template <int S>
-__device__ __forceinline__ void gs_refresh_group(const double *__restrict__ w,
-                                                 const double *__restrict__ x,
-                                                 const double *__restrict__ f,
-                                                 double *__restrict__ r64, float *__restrict__ r32,
-                                                 const uint16_t *__restrict__ dinv16,
-                                                 const uint8_t *__restrict__ flag, int mbit,
-                                                 const int (&m)[S], double &a0, double &a1)
-{
-    double t = w[m[0]];
-#pragma unroll
-    for (int q = 1; q < S; ++q) t += w[m[q]];
-    const double c = 1. / double(S);
-#pragma unroll
-    for (int q = 0; q < S; ++q) {
-        double v = f[m[q]] - (t + 0.1 * x[m[q]]);
-        if (flag[m[q]] & mbit) v = 0.0;
-        r64[m[q]] = v;
-        r32[m[q]] = float(v);
-        const double rc = v * c;
-        a0 = a0 + rc * v;
-        if (dinv16)
-            a1 = a1 + rc * (v * double(__uint_as_float(uint32_t(dinv16[m[q]]) << 16)));
-    }
-}
-
-__global__ void __launch_bounds__(kRedThreads)
-    k_gs_refresh(GsArgs G, const double *__restrict__ w, const double *__restrict__ x,
-                 const double *__restrict__ f, double *__restrict__ r64, float *__restrict__ r32,
-                 const uint16_t *__restrict__ dinv16, const uint8_t *__restrict__ flag, int mbit,
-                 double *__restrict__ part, int64_t nrtz)
-{
-    int blk = blockIdx.x;
-    const int blk0 = blk;
-    double a0 = 0.0, a1 = 0.0;
-    if (blk < G.b2) {
-        const int64_t g = int64_t(blk) * kRedThreads + threadIdx.x;
-        if (g < G.n2) {
-            int m[2];
-            gs_members<2>(G.m2, G.m2r, G.r2, G.n2r, g, m);
-            gs_refresh_group<2>(w, x, f, r64, r32, dinv16, flag, mbit, m, a0, a1);
-        }
-    } else if ((blk -= G.b2) < G.b4) {
-        const int64_t g = int64_t(blk) * kRedThreads + threadIdx.x;
-        if (g < G.n4) {
-            int m[4];
-            gs_members<4>(G.m4, G.m4r, G.r4, G.n4r, g, m);
-            gs_refresh_group<4>(w, x, f, r64, r32, dinv16, flag, mbit, m, a0, a1);
-        }
-    } else if ((blk -= G.b4) < G.b8) {
-        const int64_t g = int64_t(blk) * kRedThreads + threadIdx.x;
-        if (g < G.n8) {
-            int m[8];
-            gs_members<8>(G.m8, nullptr, 1, 0, g, m);
-            gs_refresh_group<8>(w, x, f, r64, r32, dinv16, flag, mbit, m, a0, a1);
-        }
-    } else {
-        blk -= G.b8;
-        const int64_t g = int64_t(blk) * kRedThreads + threadIdx.x;
-        if (g < G.nr) {
-            const int a = G.roff[g], b = G.roff[g + 1];
-            double t = w[G.ridx[a]];
-            for (int q = a + 1; q < b; ++q) t += w[G.ridx[q]];
-            const double c = 1. / double(b - a);
-            for (int q = a; q < b; ++q) {
-                const int mq = G.ridx[q];
-                double v = f[mq] - (t + 0.1 * x[mq]);
-                if (flag[mq] & mbit) v = 0.0;
-                r64[mq] = v;
-                r32[mq] = float(v);
-                const double rc = v * c;
-                a0 = a0 + rc * v;
-                if (dinv16)
-                    a1 = a1 + rc * (v * double(__uint_as_float(uint32_t(dinv16[mq]) << 16)));
-            }
-        }
-    }
-    block_sum2(a0, a1);
-    if (threadIdx.x == 0) {
-        part[blk0] = a0;
-        part[nrtz + blk0] = a1;
-    }
 }
 
 // Batched fused CG gather-scatter: problem q owns workgroups
@@ -565,19 +446,6 @@ int launch_gs_cg(nekbone_problem *P, double *w, const double *p, int mbit, doubl
                        P->d_flag, mbit, G, part, static_cast<const CgScalars *>(P->d_scal));
     NB_HIP(hipGetLastError());
     return timed_end(P, 1);
-}
-
-int launch_gs_refresh(nekbone_problem *P, const double *w, const double *x, const double *f,
-                      double *r64, float *r32, const uint16_t *dinv16, int mbit, double *part,
-                      int64_t nrtz)
-{
-    const int nb = gs_blocks(P);
-    if (nb == 0) return NEKBONE_OK;
-    hipLaunchKernelGGL(k_gs_refresh, dim3(nb), dim3(kRedThreads), 0, stream(), gs_args(P), w, x,
-                       f, r64, r32, dinv16, static_cast<const uint8_t *>(P->d_flag), mbit, part,
-                       nrtz);
-    NB_HIP(hipGetLastError());
-    return NEKBONE_OK;
 }
 
 GsCgArgs gs_cg_args(nekbone_problem *P, double *w, const double *p, int mbit, double *part)

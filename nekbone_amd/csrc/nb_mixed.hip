@@ -223,17 +223,36 @@ __global__ void __launch_bounds__(kRedThreads)
                int64_t n, double *__restrict__ part, int nblk)
 {
     double a0 = 0.0, a1 = 0.0;
-    const int64_t stride = int64_t(gridDim.x) * kRedThreads;
-    for (int64_t i = int64_t(blockIdx.x) * kRedThreads + threadIdx.x; i < n; i += stride) {
-        const uint8_t fl = flag[i];
-        double v = f[i] - (w[i] + 0.1 * x[i]);
-        if (fl & mbit) v = 0.0;
-        r64[i] = v;
-        r32[i] = float(v);
-        const double rc = v * inv_mult(fl);
-        a0 = a0 + rc * v;
-        if constexpr (PRE == NEKBONE_PRECOND_JACOBI)
-            a1 = a1 + rc * (v * double(axk::dinv_at(dinv, i)));
+    constexpr int U = 4;  // points per thread with their loads in flight together
+    const int64_t stride = int64_t(gridDim.x) * kRedThreads * U;
+    for (int64_t i0 = int64_t(blockIdx.x) * kRedThreads * U + threadIdx.x; i0 < n; i0 += stride) {
+        double fv[U], wv[U], xv[U];
+        float dv[U];
+        uint8_t fl[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t i = i0 + int64_t(u) * kRedThreads;
+            if (i < n) {
+                fv[u] = f[i];
+                wv[u] = w[i];
+                xv[u] = x[i];
+                fl[u] = flag[i];
+                if constexpr (PRE == NEKBONE_PRECOND_JACOBI) dv[u] = axk::dinv_at(dinv, i);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t i = i0 + int64_t(u) * kRedThreads;
+            if (i < n) {
+                double v = fv[u] - (wv[u] + 0.1 * xv[u]);
+                if (fl[u] & mbit) v = 0.0;
+                r64[i] = v;
+                r32[i] = float(v);
+                const double rc = v * inv_mult(fl[u]);
+                a0 = a0 + rc * v;
+                if constexpr (PRE == NEKBONE_PRECOND_JACOBI) a1 = a1 + rc * (v * double(dv[u]));
+            }
+        }
     }
     block_sum2(a0, a1);
     if (threadIdx.x == 0) {
@@ -242,22 +261,12 @@ __global__ void __launch_bounds__(kRedThreads)
     }
 }
 
-template <int NT>
-__device__ __forceinline__ double fixed_sum(const double *__restrict__ a, int64_t na)
-{
-    double s = 0.0;
-    for (int64_t i = threadIdx.x; i < na; i += NT) s = s + a[i];
-    return s;
-}
-
-// The replaced residual becomes the recurrence's: rtr, rtr_upd, and the
-// rtz of this iteration (beta against the previous rtz), convergence.
 template <int PRE>
 __global__ void __launch_bounds__(kFinalThreads)
     k_mx_resid_final(const double *__restrict__ part, int nblk, CgScalars *__restrict__ s)
 {
-    double rtr = fixed_sum<kFinalThreads>(part, nblk);
-    double rtz = fixed_sum<kFinalThreads>(part + nblk, nblk);
+    double rtr = slice_sum<kFinalThreads>(part, nblk);
+    double rtz = slice_sum<kFinalThreads>(part + nblk, nblk);
     block_sum2<kFinalThreads>(rtr, rtz);
     if (threadIdx.x == 0) {
         if (PRE != NEKBONE_PRECOND_JACOBI) rtz = rtr;
@@ -291,8 +300,6 @@ int ensure_mixed(nekbone_problem *P, int precond)
         if (int rc = dalloc(&M->Dcol, int64_t(nx) * nx)) return rc;
         for (float **v : {&M->x, &M->r, &M->p, &M->w})
             if (int rc = dalloc(v, P->n)) return rc;
-        if (int rc = dalloc(&M->rpart, 2 * int64_t(ax_refresh_blocks(P) + gs_blocks(P))))
-            return rc;
         std::vector<float> dr(size_t(nx) * nx), dc(size_t(nx) * nx);
         for (int i = 0; i < nx; ++i)
             for (int m = 0; m < nx; ++m) {
@@ -412,35 +419,16 @@ int iteration32(nekbone_problem *P, int precond, int mbit)
 int reliable_update(nekbone_problem *P, int precond, int mbit)
 {
     MixedState *M = P->mx;
-    const uint16_t *d16 =
-        precond == NEKBONE_PRECOND_JACOBI ? static_cast<const uint16_t *>(M->dinv) : nullptr;
-    if (M->refresh) {
-        // fused form: the element kernel flushes x and finishes the points
-        // no other element shares; the gather-scatter kernel the rest
-        const int nax = ax_refresh_blocks(P);
-        const int nall = nax + gs_blocks(P);
-        const int rc = launch_ax_refresh(P, M->x, M->p, M->r, d16, mbit, M->rpart, nall);
-        if (rc == NEKBONE_OK) {
-            if (int rc2 = launch_gs_refresh(P, P->d_w, P->d_x, P->d_f, P->d_r, M->r, d16, mbit,
-                                            M->rpart + nax, nall))
-                return rc2;
-            if (precond == NEKBONE_PRECOND_JACOBI)
-                hipLaunchKernelGGL(k_mx_resid_final<NEKBONE_PRECOND_JACOBI>, dim3(1),
-                                   dim3(kFinalThreads), 0, stream(), M->rpart, nall, P->d_scal);
-            else
-                hipLaunchKernelGGL(k_mx_resid_final<NEKBONE_PRECOND_IDENTITY>, dim3(1),
-                                   dim3(kFinalThreads), 0, stream(), M->rpart, nall, P->d_scal);
-            NB_HIP(hipGetLastError());
-            return NEKBONE_OK;
-        }
-        if (rc != NEKBONE_ERR_UNSUPPORTED) return rc;
+    int rc = M->refresh ? launch_ax_flush(P, M->x, M->p) : NEKBONE_ERR_UNSUPPORTED;
+    if (rc == NEKBONE_ERR_UNSUPPORTED) {
+        hipLaunchKernelGGL(k_mx_xflush, dim3(grid(P->n)), dim3(kThreads), 0, stream(), P->d_x,
+                           M->x, M->p, P->n, static_cast<const CgScalars *>(P->d_scal));
+        NB_HIP(hipGetLastError());
+        rc = launch_ax_local(P, P->d_w, P->d_x);
     }
-    hipLaunchKernelGGL(k_mx_xflush, dim3(grid(P->n)), dim3(kThreads), 0, stream(), P->d_x, M->x,
-                       M->p, P->n, static_cast<const CgScalars *>(P->d_scal));
-    NB_HIP(hipGetLastError());
-    if (int rc = launch_ax_local(P, P->d_w, P->d_x)) return rc;
+    if (rc) return rc;
     if (int rc = launch_gs(P, P->d_w, nullptr)) return rc;
-    const int nb = reduce_blocks(P->n);
+    const int nb = upd_blocks(P->n);
     if (precond == NEKBONE_PRECOND_JACOBI) {
         hipLaunchKernelGGL(k_mx_resid<NEKBONE_PRECOND_JACOBI>, dim3(nb), dim3(kRedThreads), 0,
                            stream(), P->d_r, M->r, P->d_f, P->d_w, P->d_x,
@@ -477,7 +465,6 @@ void free_mixed(nekbone_problem *P)
     dfree(M->dinv);
     dfree(M->gh);
     dfree(M->gsc);
-    dfree(M->rpart);
     delete M;
     P->mx = nullptr;
 }
