@@ -110,6 +110,10 @@ int prepare(nekbone_batch *B, int precond, const std::vector<int> &mbit)
             axs.push_back(axs.back() + cg_ax_blocks(P));
             gs.push_back(gs_cg_args(P, P->d_w, P->d_p, mbit[size_t(k)],
                                     P->d_part + cg_ax_blocks(P)));
+            if (cg_fold_partials(P)) {  // as in the single-problem iteration
+                gs.back().G.pre = P->d_part;
+                gs.back().G.npre = cg_ax_blocks(P);
+            }
             gss.push_back(gss.back() + cg_gs_blocks(P));
         }
         G.nblk_ax = axs.back();

@@ -389,10 +389,16 @@ static int iteration_fused(nekbone_problem *P, double *x, double *r, int precond
     const int rev = P->sweep & 1;
     ++P->sweep;
     if (int rc = launch_ax_cg(P, x, r, precond, mbit, rev)) return rc;
-    if (int rc = launch_gs_cg(P, P->d_w, P->d_p, mbit, P->d_part + na, !rev)) return rc;
+    // the element kernel's partials are folded into the gather-scatter
+    // kernel's when it has workgroups enough, so the final sum reads fewer
+    const bool fold = cg_fold_partials(P);
+    if (int rc = launch_gs_cg(P, P->d_w, P->d_p, mbit, P->d_part + na, !rev,
+                              fold ? P->d_part : nullptr, fold ? na : 0))
+        return rc;
     if (int rc = timed_begin(P, 4)) return rc;
-    hipLaunchKernelGGL(k_pap_final, dim3(1), dim3(kFinalThreads), 0, stream(), P->d_part, na + ng,
-                       P->d_scal, P->d_hist, P->hist_cap);
+    hipLaunchKernelGGL(k_pap_final, dim3(1), dim3(kFinalThreads), 0, stream(),
+                       fold ? P->d_part + na : P->d_part, fold ? ng : na + ng, P->d_scal,
+                       P->d_hist, P->hist_cap);
     NB_HIP(hipGetLastError());
     if (int rc = timed_end(P, 4)) return rc;
     if (int rc = timed_begin(P, 3)) return rc;
@@ -563,9 +569,10 @@ CgBatchArgs cg_batch_args(nekbone_problem *P, double *x, double *r)
     A.x = x;
     A.p = P->d_p;
     A.n = P->n;
-    A.part = P->d_part;
-    A.npap = n_ax(P) + n_gs(P);
-    A.part_upd = P->d_part + A.npap;
+    const bool fold = cg_fold_partials(P);  // as in the single-problem iteration
+    A.part = fold ? P->d_part + n_ax(P) : P->d_part;
+    A.npap = fold ? n_gs(P) : n_ax(P) + n_gs(P);
+    A.part_upd = P->d_part + n_ax(P) + n_gs(P);
     A.nupd = n_upd(P);
     A.scal = P->d_scal;
     A.hist = P->d_hist;
@@ -609,6 +616,7 @@ int launch_cg_batch_tail(const CgBatchArgs *d_args, const int *d_ustart, int npr
 }
 
 int cg_ax_blocks(const nekbone_problem *P) { return int(n_ax(P)); }
+bool cg_fold_partials(const nekbone_problem *P) { return n_gs(P) * kRedThreads >= n_ax(P); }
 int cg_gs_blocks(const nekbone_problem *P) { return int(n_gs(P)); }
 
 int launch_cg_finish(nekbone_problem *P, double *x)

@@ -276,6 +276,11 @@ struct GsArgs {
     const int32_t *m2r, *m4r;  // run form of the pair / quad class (GsMap)
     int r2, r4;
     int64_t n2r, n4r;
+    // CG form: partial sums of the element kernel folded into this
+    // kernel's (logical workgroup b adds pre[b*kRedThreads + thread]), so
+    // the final sum reads this kernel's partials only; null: none
+    const double *pre = nullptr;
+    int64_t npre = 0;
 };
 // Arguments of the fused CG gather-scatter of one problem.
 struct GsCgArgs {
@@ -433,7 +438,7 @@ int launch_gs(nekbone_problem *P, double *u, const CgScalars *guard);
 // Fused CG variant: finishes shared points (+0.1p, mask) and writes one
 // pap part per workgroup (gs_blocks of them) to part.
 int launch_gs_cg(nekbone_problem *P, double *w, const double *p, int mbit, double *part,
-                 int rev = 0);
+                 int rev = 0, const double *pre = nullptr, int64_t npre = 0);
 // First pass of the mixed solver's FP64 residual refresh (nb_ax.hip):
 // x += x32 (+ alpha p32 when pending), x32 = 0, w = A_L x;
 // NEKBONE_ERR_UNSUPPORTED when no such kernel is compiled for the shape.
@@ -441,7 +446,8 @@ int launch_ax_flush(nekbone_problem *P, float *x32, const float *p32);
 // FP32 gather-scatter (mixed-precision inner solve): plain, and the fused
 // CG variant (pap parts in FP64).
 int launch_gs32(nekbone_problem *P, float *u);
-int launch_gs_cg32(nekbone_problem *P, float *w, const float *p, int mbit, double *part);
+int launch_gs_cg32(nekbone_problem *P, float *w, const float *p, int mbit, double *part,
+                   const double *pre = nullptr, int64_t npre = 0);
 
 int gs_blocks(const nekbone_problem *P);
 int launch_post(nekbone_problem *P, double *w, const double *p, int mbit, bool pap);
@@ -481,6 +487,9 @@ CgBatchArgs cg_batch_args(nekbone_problem *P, double *x, double *r);
 int launch_cg_batch_tail(const CgBatchArgs *d_args, const int *d_ustart, int nprob, int nupd_total,
                          int precond, bool fma, bool xp, int set_xpending);
 int cg_ax_blocks(const nekbone_problem *P);
+// whether a fused iteration folds the element kernel's partial sums into
+// the gather-scatter kernel's (it has workgroups enough)
+bool cg_fold_partials(const nekbone_problem *P);
 int cg_gs_blocks(const nekbone_problem *P);
 void free_mixed(nekbone_problem *P);
 // gather-scatter with an explicit map (HSMG levels)

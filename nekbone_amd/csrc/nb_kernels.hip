@@ -112,6 +112,10 @@ __device__ __forceinline__ void gs_body(T *__restrict__ w, const T *__restrict__
         }
     }
     if constexpr (CG) {
+        if (G.pre) {
+            const int64_t j = int64_t(blk0) * kRedThreads + threadIdx.x;
+            if (j < G.npre) acc = acc + G.pre[j];
+        }
         double dummy = 0.0;
         block_sum2(acc, dummy);
         if (threadIdx.x == 0) part[blk0] = acc;
@@ -435,13 +439,17 @@ int launch_gs(nekbone_problem *P, double *u, const CgScalars *guard)
 }
 
 int launch_gs_cg(nekbone_problem *P, double *w, const double *p, int mbit, double *part,
-                 int rev)
+                 int rev, const double *pre, int64_t npre)
 {
     const int nb = gs_blocks(P);
     if (nb == 0) return NEKBONE_OK;
+    if (pre && int64_t(nb) * kRedThreads < npre)
+        return fail(NEKBONE_ERR_STATE, "gs: too few workgroups to fold the partials");
     if (int rc = timed_begin(P, 1)) return rc;
     GsArgs G = gs_args(P);
     G.rev = rev;
+    G.pre = pre;
+    G.npre = npre;
     hipLaunchKernelGGL((k_gs_cls<double, true>), dim3(nb), dim3(kRedThreads), 0, stream(), w, p,
                        P->d_flag, mbit, G, part, static_cast<const CgScalars *>(P->d_scal));
     NB_HIP(hipGetLastError());
@@ -525,14 +533,19 @@ void gs_map_free(GsMap &G)
     dfree(G.d_ridx);
 }
 
-int launch_gs_cg32(nekbone_problem *P, float *w, const float *p, int mbit, double *part)
+int launch_gs_cg32(nekbone_problem *P, float *w, const float *p, int mbit, double *part,
+                   const double *pre, int64_t npre)
 {
     const int nb = gs_blocks(P);
     if (nb == 0) return NEKBONE_OK;
+    if (pre && int64_t(nb) * kRedThreads < npre)
+        return fail(NEKBONE_ERR_STATE, "gs: too few workgroups to fold the partials");
     if (int rc = timed_begin(P, 1)) return rc;
+    GsArgs G = gs_args(P);
+    G.pre = pre;
+    G.npre = npre;
     hipLaunchKernelGGL((k_gs_cls<float, true>), dim3(nb), dim3(kRedThreads), 0, stream(), w, p,
-                       P->d_flag, mbit, gs_args(P), part,
-                       static_cast<const CgScalars *>(P->d_scal));
+                       P->d_flag, mbit, G, part, static_cast<const CgScalars *>(P->d_scal));
     NB_HIP(hipGetLastError());
     return timed_end(P, 1);
 }

@@ -390,8 +390,12 @@ int iteration32(nekbone_problem *P, int precond, int mbit)
     const int nu = upd_blocks(P->n);
     double *part_upd = P->d_part + na + ng;
     if (int rc = launch_ax32(P, jac ? 2 : 1, mbit, M->w, nullptr)) return rc;
-    if (int rc = launch_gs_cg32(P, M->w, M->p, mbit, P->d_part + na)) return rc;
-    if (int rc = launch_pap_final(P, P->d_part, na + ng)) return rc;
+    const bool fold = ng * kRedThreads >= na;  // as in the FP64 iteration
+    if (int rc = launch_gs_cg32(P, M->w, M->p, mbit, P->d_part + na, fold ? P->d_part : nullptr,
+                                fold ? na : 0))
+        return rc;
+    if (int rc = launch_pap_final(P, fold ? P->d_part + na : P->d_part, fold ? ng : na + ng))
+        return rc;
     if (int rc = timed_begin(P, 3)) return rc;
     const bool xp = !M->axv.xu;
 #define NB_MXU(PRE, XP)                                                                         \
