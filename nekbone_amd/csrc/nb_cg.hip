@@ -83,8 +83,9 @@ __global__ void __launch_bounds__(kFinalThreads)
 // (src/cg.f:66) and of the next rtz = (r*c)*z.  Logical block blk owns
 // one contiguous chunk of the vectors, so the kernel walks memory in
 // block order (either direction, see sweep_block) and each partial sums
-// the same points in the same order whatever the direction.  Four
-// independent points per thread per step keep enough loads in flight.
+// the same points in the same order whatever the direction.  Eight
+// independent points per thread per step keep enough loads in flight
+// (measured 1% faster than four at 32^3 elements, N=9).
 template <int PRE, bool FMA, bool XP>
 __device__ __forceinline__ void update_body(double *__restrict__ r, const double *__restrict__ w,
                                             const double *__restrict__ dinv,
@@ -96,7 +97,7 @@ __device__ __forceinline__ void update_body(double *__restrict__ r, const double
     if (s->done) return;
     const double alpha = s->alpha, alphm = -alpha;
     double a0 = 0.0, a1 = 0.0;
-    constexpr int U = 4;
+    constexpr int U = 8;
     constexpr int64_t step = int64_t(kRedThreads) * U;
     const int64_t cs = (n + int64_t(nblk) * step - 1) / (int64_t(nblk) * step) * step;
     const int64_t beg = int64_t(blk) * cs, end = beg + cs < n ? beg + cs : n;
