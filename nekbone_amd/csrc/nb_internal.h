@@ -346,6 +346,7 @@ constexpr int kMaxCoarse = 8192;
 struct MixedState {
     float *g = nullptr, *Drow = nullptr, *Dcol = nullptr;
     uint16_t *dinv = nullptr;  // Jacobi diagonal as bfloat16 bits
+    int mult_in_dinv = 0;      // its last two bits hold log2(multiplicity)
     float *x = nullptr, *r = nullptr, *p = nullptr, *w = nullptr;
     int geom_version = -1, dinv_version = -1;
     AxVariant axv;

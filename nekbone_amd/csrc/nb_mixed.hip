@@ -67,12 +67,25 @@ __global__ void k_to_f32(float *__restrict__ dst, const double *__restrict__ src
 
 // Jacobi diagonal for the inner iterations: FP64 -> FP32 -> bfloat16 bits,
 // round to nearest even (the values are positive and finite).
-__global__ void k_to_bf16(uint16_t *__restrict__ dst, const double *__restrict__ src, int64_t n)
+__global__ void k_to_bf16(uint16_t *__restrict__ dst, const double *__restrict__ src,
+                          const uint8_t *__restrict__ flag, int64_t n)
 {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t b = __float_as_uint(float(src[i]));
-    dst[i] = uint16_t((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
+    if (!flag) {
+        dst[i] = uint16_t((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
+        return;
+    }
+    // multiplicity form: five mantissa bits (round to nearest even), the
+    // last two bits of the bfloat16 word log2 of the point's multiplicity
+    // (1, 2, 4 or 8), so the vector update needs no flag byte; every use
+    // of the diagonal reads the same perturbed value (relative change
+    // < 2^-5), so the preconditioner stays one fixed diagonal
+    const uint32_t t = (b + 0x1FFFFu + ((b >> 18) & 1u)) >> 18;
+    const int m = flag[i] >> kMultShift;
+    const uint32_t code = m >= 8 ? 3u : m >= 4 ? 2u : m >= 2 ? 1u : 0u;
+    dst[i] = uint16_t((t << 2) | code);
 }
 
 // FP16 geometric factors: one workgroup per (factor, element) block of
@@ -147,7 +160,7 @@ __global__ void k_mx_start(float *__restrict__ r32, float *__restrict__ x32,
 
 // r32 -= alpha w32; [XP] x32 += alpha p32; parts of rtr = (r c r) and of
 // rtz = (r c z), z = M^-1 r, accumulated in FP64.
-template <int PRE, bool XP>
+template <int PRE, bool XP, bool MB = false>
 __global__ void __launch_bounds__(kRedThreads)
     k_mx_update(float *__restrict__ r, const float *__restrict__ w,
                 const uint16_t *__restrict__ dinv, const uint8_t *__restrict__ flag,
@@ -168,8 +181,15 @@ __global__ void __launch_bounds__(kRedThreads)
             if (i < n) {
                 rv[u] = r[i];
                 wv[u] = w[i];
-                fv[u] = flag[i];
-                if constexpr (PRE == NEKBONE_PRECOND_JACOBI) dv[u] = axk::dinv_at(dinv, i);
+                if constexpr (MB) {
+                    // multiplicity form of the diagonal (k_to_bf16): c from its last bits
+                    const uint32_t h = dinv[i];
+                    dv[u] = __uint_as_float(h << 16);
+                    fv[u] = uint8_t(1u << (h & 3u)) << kMultShift;
+                } else {
+                    fv[u] = flag[i];
+                    if constexpr (PRE == NEKBONE_PRECOND_JACOBI) dv[u] = axk::dinv_at(dinv, i);
+                }
                 if constexpr (XP) {
                     xv[u] = x[i];
                     pv[u] = p[i];
@@ -343,8 +363,11 @@ int ensure_mixed(nekbone_problem *P, int precond)
     if (precond == NEKBONE_PRECOND_JACOBI && M->dinv_version != P->dinv_version) {
         if (!M->dinv)
             if (int rc = dalloc(&M->dinv, P->n)) return rc;
+        // multiplicities 1, 2, 4, 8 only (no group outside the pair, quad and
+        // octet classes): the diagonal carries them
+        M->mult_in_dinv = P->gs.nr == 0 ? 1 : 0;
         hipLaunchKernelGGL(k_to_bf16, dim3(grid(P->n)), dim3(kThreads), 0, stream(), M->dinv,
-                           P->d_dinv, P->n);
+                           P->d_dinv, M->mult_in_dinv ? P->d_flag : nullptr, P->n);
         NB_HIP(hipGetLastError());
         M->dinv_version = P->dinv_version;
     }
@@ -398,19 +421,25 @@ int iteration32(nekbone_problem *P, int precond, int mbit)
         return rc;
     if (int rc = timed_begin(P, 3)) return rc;
     const bool xp = !M->axv.xu;
-#define NB_MXU(PRE, XP)                                                                         \
-    hipLaunchKernelGGL((k_mx_update<PRE, XP>), dim3(nu), dim3(kRedThreads), 0, stream(), M->r, \
-                       M->w, M->dinv, P->d_flag, M->x, M->p, P->n, part_upd, nu, P->d_scal)
-    if (jac) {
+#define NB_MXU(PRE, XP, MB)                                                                 \
+    hipLaunchKernelGGL((k_mx_update<PRE, XP, MB>), dim3(nu), dim3(kRedThreads), 0, stream(), \
+                       M->r, M->w, M->dinv, P->d_flag, M->x, M->p, P->n, part_upd, nu,        \
+                       P->d_scal)
+    if (jac && M->mult_in_dinv) {
         if (xp)
-            NB_MXU(NEKBONE_PRECOND_JACOBI, true);
+            NB_MXU(NEKBONE_PRECOND_JACOBI, true, true);
         else
-            NB_MXU(NEKBONE_PRECOND_JACOBI, false);
+            NB_MXU(NEKBONE_PRECOND_JACOBI, false, true);
+    } else if (jac) {
+        if (xp)
+            NB_MXU(NEKBONE_PRECOND_JACOBI, true, false);
+        else
+            NB_MXU(NEKBONE_PRECOND_JACOBI, false, false);
     } else {
         if (xp)
-            NB_MXU(NEKBONE_PRECOND_IDENTITY, true);
+            NB_MXU(NEKBONE_PRECOND_IDENTITY, true, false);
         else
-            NB_MXU(NEKBONE_PRECOND_IDENTITY, false);
+            NB_MXU(NEKBONE_PRECOND_IDENTITY, false, false);
     }
 #undef NB_MXU
     NB_HIP(hipGetLastError());
