@@ -48,17 +48,14 @@ __device__ __forceinline__ void gs_sum(T *__restrict__ w, const T *__restrict__ 
     for (int q = 0; q < S; ++q) gs_finish<T, CG>(w, p, flag, mbit, m[q], t, c, acc);
 }
 
-// Group g of a class of size S: the first nrun groups are in run form
-// (runs of R; group t of a run has the run's first members + t), the
-// rest in the member list.
-template <typename T, int S, bool CG>
-__device__ __forceinline__ void gs_group(T *__restrict__ w, const T *__restrict__ p,
-                                         const uint8_t *__restrict__ flag, int mbit,
-                                         const int32_t *__restrict__ list,
-                                         const int32_t *__restrict__ runs, int R, int64_t nrun,
-                                         int64_t g, double &acc)
+// Members of group g of a class of size S: the first nrun groups are in
+// run form (runs of R; group t of a run has the run's first members + t),
+// the rest in the member list.
+template <int S>
+__device__ __forceinline__ void gs_members(const int32_t *__restrict__ list,
+                                           const int32_t *__restrict__ runs, int R, int64_t nrun,
+                                           int64_t g, int (&m)[S])
 {
-    int m[S];
     const int32_t *mem;
     int t = 0;
     if (g < nrun) {
@@ -79,7 +76,69 @@ __device__ __forceinline__ void gs_group(T *__restrict__ w, const T *__restrict_
                   m[4 * h + 3] = v.w + t;
         }
     }
+}
+
+template <typename T, int S, bool CG>
+__device__ __forceinline__ void gs_group(T *__restrict__ w, const T *__restrict__ p,
+                                         const uint8_t *__restrict__ flag, int mbit,
+                                         const int32_t *__restrict__ list,
+                                         const int32_t *__restrict__ runs, int R, int64_t nrun,
+                                         int64_t g, double &acc)
+{
+    int m[S];
+    gs_members<S>(list, runs, R, nrun, g, m);
     gs_sum<T, S, CG>(w, p, flag, mbit, m, acc);
+}
+
+// Two pairs per thread (groups g and g + kRedThreads of the block's
+// range): every load of both is issued before the first store, since the
+// compiler may not move a load of the second pair above a store of the
+// first.  Each pair is summed and finished as gs_sum does.
+template <typename T, bool CG>
+__device__ __forceinline__ void gs_two_pairs(T *__restrict__ w, const T *__restrict__ p,
+                                             const uint8_t *__restrict__ flag, int mbit,
+                                             const GsArgs &G, int64_t g0, double &acc)
+{
+    const int64_t gg[2] = {g0, g0 + kRedThreads};
+    int m[2][2] = {{0, 0}, {0, 0}};
+    bool v[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        v[h] = gg[h] < G.n2;
+        if (v[h]) gs_members<2>(G.m2, G.m2r, G.r2, G.n2r, gg[h], m[h]);
+    }
+    T a[2][2], pm[2][2];
+    uint8_t fl[2][2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        if (v[h]) {
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                a[h][q] = w[m[h][q]];
+                if constexpr (CG) {
+                    pm[h][q] = p[m[h][q]];
+                    fl[h][q] = flag[m[h][q]];
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        if (!v[h]) continue;
+        T t = a[h][0];
+        t += a[h][1];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            if constexpr (CG) {
+                T vq = t + T(0.1) * pm[h][q];
+                if (fl[h][q] & mbit) vq = T(0);
+                w[m[h][q]] = vq;
+                acc = acc + double(vq) * 0.5 * double(pm[h][q]);
+            } else {
+                w[m[h][q]] = t;
+            }
+        }
+    }
 }
 
 template <typename T, bool CG>
@@ -92,8 +151,8 @@ __device__ __forceinline__ void gs_body(T *__restrict__ w, const T *__restrict__
     int blk = blk0;
     double acc = 0.0;
     if (blk < G.b2) {
-        const int64_t g = int64_t(blk) * kRedThreads + threadIdx.x;
-        if (g < G.n2) gs_group<T, 2, CG>(w, p, flag, mbit, G.m2, G.m2r, G.r2, G.n2r, g, acc);
+        gs_two_pairs<T, CG>(w, p, flag, mbit, G, int64_t(blk) * (2 * kRedThreads) + threadIdx.x,
+                            acc);
     } else if ((blk -= G.b2) < G.b4) {
         const int64_t g = int64_t(blk) * kRedThreads + threadIdx.x;
         if (g < G.n4) gs_group<T, 4, CG>(w, p, flag, mbit, G.m4, G.m4r, G.r4, G.n4r, g, acc);
@@ -154,7 +213,7 @@ GsArgs gs_args_map(const GsMap &M)
     GsArgs G{M.d_m2, M.d_m4, M.d_m8, M.d_roff, M.d_ridx, M.n2, M.n4, M.n8, M.nr, 0, 0, 0, 0, 0,
              M.d_m2r, M.d_m4r, M.r2, M.r4, M.n2r, M.n4r};
     auto nb = [](int64_t n) { return int((n + kRedThreads - 1) / kRedThreads); };
-    G.b2 = nb(M.n2);
+    G.b2 = int((M.n2 + 2 * kRedThreads - 1) / (2 * kRedThreads));  // two pairs per thread
     G.b4 = nb(M.n4);
     G.b8 = nb(M.n8);
     G.br = nb(M.nr);
