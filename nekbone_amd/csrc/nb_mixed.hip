@@ -144,23 +144,27 @@ __global__ void k_perm32(float *__restrict__ dst, const float *__restrict__ src,
         dst[i] = src[d];
 }
 
-// Start of the inner iterations: r32 = r64, x32 = p32 = 0; the FP64
-// residual of the initial guess is the first reliable residual.
+// Start of the inner iterations: r32 = r64 (with the Jacobi diagonal d:
+// z32 = d r64, the vector the inner iterations carry), x32 = p32 = 0; the
+// FP64 residual of the initial guess is the first reliable residual.
 __global__ void k_mx_start(float *__restrict__ r32, float *__restrict__ x32,
                            float *__restrict__ p32, const double *__restrict__ r64, int64_t n,
-                           CgScalars *__restrict__ s)
+                           CgScalars *__restrict__ s, const uint16_t *__restrict__ dinv)
 {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i == 0) s->rtr_upd = s->rtr;
     if (i >= n) return;
-    r32[i] = float(r64[i]);
+    r32[i] = dinv ? float(r64[i] * double(axk::dinv_at(dinv, i))) : float(r64[i]);
     x32[i] = 0.0f;
     p32[i] = 0.0f;
 }
 
 // r32 -= alpha w32; [XP] x32 += alpha p32; parts of rtr = (r c r) and of
 // rtz = (r c z), z = M^-1 r, accumulated in FP64.
-template <int PRE, bool XP, bool MB = false>
+// [ZF] (Jacobi) the vector is z = M^-1 r instead of r: z -= alpha (d w),
+// and the dot products take r = z / d (d the diagonal as stored), so the
+// element kernel reads z alone instead of r and the diagonal.
+template <int PRE, bool XP, bool MB = false, bool ZF = false>
 __global__ void __launch_bounds__(kRedThreads)
     k_mx_update(float *__restrict__ r, const float *__restrict__ w,
                 const uint16_t *__restrict__ dinv, const uint8_t *__restrict__ flag,
@@ -201,12 +205,22 @@ __global__ void __launch_bounds__(kRedThreads)
             const int64_t i = i0 + int64_t(u) * kRedThreads;
             if (i < n) {
                 if constexpr (XP) x[i] = __builtin_fmaf(alpha, pv[u], xv[u]);
-                const float ri = __builtin_fmaf(-alpha, wv[u], rv[u]);
-                r[i] = ri;
-                const double rd = double(ri);
-                const double rc = rd * inv_mult(fv[u]);
-                a0 = a0 + rc * rd;
-                if constexpr (PRE == NEKBONE_PRECOND_JACOBI) a1 = a1 + rc * double(ri * dv[u]);
+                if constexpr (ZF) {
+                    const float zi = __builtin_fmaf(-alpha, dv[u] * wv[u], rv[u]);
+                    r[i] = zi;
+                    const double rd = double(zi / dv[u]);
+                    const double rc = rd * inv_mult(fv[u]);
+                    a0 = a0 + rc * rd;
+                    a1 = a1 + rc * double(zi);
+                } else {
+                    const float ri = __builtin_fmaf(-alpha, wv[u], rv[u]);
+                    r[i] = ri;
+                    const double rd = double(ri);
+                    const double rc = rd * inv_mult(fv[u]);
+                    a0 = a0 + rc * rd;
+                    if constexpr (PRE == NEKBONE_PRECOND_JACOBI)
+                        a1 = a1 + rc * double(ri * dv[u]);
+                }
             }
         }
     }
@@ -267,7 +281,11 @@ __global__ void __launch_bounds__(kRedThreads)
                 double v = fv[u] - (wv[u] + 0.1 * xv[u]);
                 if (fl[u] & mbit) v = 0.0;
                 r64[i] = v;
-                r32[i] = float(v);
+                // Jacobi: the inner iterations carry z = d r
+                if constexpr (PRE == NEKBONE_PRECOND_JACOBI)
+                    r32[i] = float(v * double(dv[u]));
+                else
+                    r32[i] = float(v);
                 const double rc = v * inv_mult(fl[u]);
                 a0 = a0 + rc * v;
                 if constexpr (PRE == NEKBONE_PRECOND_JACOBI) a1 = a1 + rc * (v * double(dv[u]));
@@ -412,7 +430,8 @@ int iteration32(nekbone_problem *P, int precond, int mbit)
     const int64_t na = ax32_blocks(P), ng = gs_blocks(P);
     const int nu = upd_blocks(P->n);
     double *part_upd = P->d_part + na + ng;
-    if (int rc = launch_ax32(P, jac ? 2 : 1, mbit, M->w, nullptr)) return rc;
+    // Jacobi runs in z form: the element kernel's identity mode reads z
+    if (int rc = launch_ax32(P, 1, mbit, M->w, nullptr)) return rc;
     const bool fold = ng * kRedThreads >= na;  // as in the FP64 iteration
     if (int rc = launch_gs_cg32(P, M->w, M->p, mbit, P->d_part + na, fold ? P->d_part : nullptr,
                                 fold ? na : 0))
@@ -421,25 +440,25 @@ int iteration32(nekbone_problem *P, int precond, int mbit)
         return rc;
     if (int rc = timed_begin(P, 3)) return rc;
     const bool xp = !M->axv.xu;
-#define NB_MXU(PRE, XP, MB)                                                                 \
-    hipLaunchKernelGGL((k_mx_update<PRE, XP, MB>), dim3(nu), dim3(kRedThreads), 0, stream(), \
-                       M->r, M->w, M->dinv, P->d_flag, M->x, M->p, P->n, part_upd, nu,        \
+#define NB_MXU(PRE, XP, MB, ZF)                                                               \
+    hipLaunchKernelGGL((k_mx_update<PRE, XP, MB, ZF>), dim3(nu), dim3(kRedThreads), 0, stream(), \
+                       M->r, M->w, M->dinv, P->d_flag, M->x, M->p, P->n, part_upd, nu,          \
                        P->d_scal)
     if (jac && M->mult_in_dinv) {
         if (xp)
-            NB_MXU(NEKBONE_PRECOND_JACOBI, true, true);
+            NB_MXU(NEKBONE_PRECOND_JACOBI, true, true, true);
         else
-            NB_MXU(NEKBONE_PRECOND_JACOBI, false, true);
+            NB_MXU(NEKBONE_PRECOND_JACOBI, false, true, true);
     } else if (jac) {
         if (xp)
-            NB_MXU(NEKBONE_PRECOND_JACOBI, true, false);
+            NB_MXU(NEKBONE_PRECOND_JACOBI, true, false, true);
         else
-            NB_MXU(NEKBONE_PRECOND_JACOBI, false, false);
+            NB_MXU(NEKBONE_PRECOND_JACOBI, false, false, true);
     } else {
         if (xp)
-            NB_MXU(NEKBONE_PRECOND_IDENTITY, true, false);
+            NB_MXU(NEKBONE_PRECOND_IDENTITY, true, false, false);
         else
-            NB_MXU(NEKBONE_PRECOND_IDENTITY, false, false);
+            NB_MXU(NEKBONE_PRECOND_IDENTITY, false, false, false);
     }
 #undef NB_MXU
     NB_HIP(hipGetLastError());
@@ -539,7 +558,9 @@ extern "C" int nekbone_cg_mixed_d(nekbone_problem *P, double *x, const double *f
     const int mbit = next_mask_bit(P);
     MixedState *M = P->mx;
     hipLaunchKernelGGL(k_mx_start, dim3(grid(P->n)), dim3(kThreads), 0, stream(), M->r, M->x,
-                       M->p, P->d_r, P->n, P->d_scal);
+                       M->p, P->d_r, P->n, P->d_scal,
+                       precond == NEKBONE_PRECOND_JACOBI ? static_cast<const uint16_t *>(M->dinv)
+                                                         : nullptr);
     NB_HIP(hipGetLastError());
 
     const double delta2 = delta * delta;
@@ -641,7 +662,7 @@ extern "C" int nekbone_autotune_mixed(nekbone_problem *P, int precond, double *b
     for (const auto &kv : table32()) {
         int nx, epb, mode, pf, xu, g16;
         std::tie(nx, epb, mode, pf, xu, g16) = kv.first;
-        if (nx == P->nx1 && mode == 2 && g16 == M->g16) cands.push_back(AxVariant{epb, pf, xu});
+        if (nx == P->nx1 && mode == 1 && g16 == M->g16) cands.push_back(AxVariant{epb, pf, xu});
     }
     const AxVariant keep = M->axv;
     const int mbit = P->mask_mode == NEKBONE_MASK_DIRICHLET ? 1 : 2;
@@ -658,7 +679,10 @@ extern "C" int nekbone_autotune_mixed(nekbone_problem *P, int precond, double *b
         rc = launch_cg_init(P, P->d_x, P->d_r, P->d_f, 1, precond, 0.0);
         if (!rc) {
             hipLaunchKernelGGL(k_mx_start, dim3(grid(P->n)), dim3(kThreads), 0, stream(), M->r,
-                               M->x, M->p, P->d_r, P->n, P->d_scal);
+                               M->x, M->p, P->d_r, P->n, P->d_scal,
+                               precond == NEKBONE_PRECOND_JACOBI
+                                   ? static_cast<const uint16_t *>(M->dinv)
+                                   : nullptr);
             if (hipGetLastError() != hipSuccess) rc = NEKBONE_ERR_HIP;
         }
         for (int i = 0; i < warm && !rc; ++i) rc = iteration32(P, precond, mbit);
