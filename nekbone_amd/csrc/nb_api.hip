@@ -453,10 +453,11 @@ static int run_cg(nekbone_problem *P, double *x, const double *f, double *r, int
     if (niter < 0) return fail(NEKBONE_ERR_ARG, "cg: niter < 0");
     if (int rc = ensure_hist(P, niter)) return rc;
     const int mbit0 = next_mask_bit(P);  // maskit(r), src/cg.f:40
-    if (int rc = launch_cg_init(P, x, r, f, mbit0, precond, tol2)) return rc;
+    if (int rc = launch_cg_init(P, x, r, f, mbit0, precond, tol2, cg_zform(P, precond)))
+        return rc;
     const int mbit = next_mask_bit(P);  // maskit in ax, src/cg.f:119 (state is now fixed)
     if (int rc = run_iterations(P, x, r, precond, mbit, niter)) return rc;
-    return launch_cg_finish(P, x);
+    return launch_cg_finish(P, x, r, cg_zform(P, precond));
 }
 
 }  // namespace nb
@@ -770,7 +771,7 @@ extern "C" int nekbone_autotune(nekbone_problem *P, int precond, double *best_ms
     int rc = NEKBONE_OK;
     for (const AxVariant &v : cands) {
         P->axv = v;
-        rc = launch_cg_init(P, P->d_x, P->d_r, P->d_f, 1, precond, 0.0);
+        rc = launch_cg_init(P, P->d_x, P->d_r, P->d_f, 1, precond, 0.0, cg_zform(P, precond));
         for (int i = 0; i < warm && !rc; ++i) rc = launch_cg_iteration(P, P->d_x, P->d_r, precond, mbit);
         if (!rc && hipEventRecord(e0, stream()) != hipSuccess) rc = NEKBONE_ERR_HIP;
         for (int i = 0; i < reps && !rc; ++i) rc = launch_cg_iteration(P, P->d_x, P->d_r, precond, mbit);
@@ -967,7 +968,8 @@ extern "C" int nekbone_cg_tol_d(nekbone_problem *P, double *x, const double *f, 
     if (int rc = ensure_hist(P, maxiter)) return rc;
     if (int rc = launch_to_dev(P, P->d_f, f)) return rc;
     const int mbit0 = next_mask_bit(P);
-    if (int rc = launch_cg_init(P, P->d_x, P->d_r, P->d_f, mbit0, precond, tol * tol))
+    if (int rc = launch_cg_init(P, P->d_x, P->d_r, P->d_f, mbit0, precond, tol * tol,
+                                cg_zform(P, precond)))
         return rc;
     const int mbit = next_mask_bit(P);
     // Iterations are queued in chunks; kernels of an already converged
@@ -982,7 +984,7 @@ extern "C" int nekbone_cg_tol_d(nekbone_problem *P, double *x, const double *f, 
         NB_HIP(hipStreamSynchronize(stream()));
         if (s.done) break;
     }
-    if (int rc = launch_cg_finish(P, P->d_x)) return rc;
+    if (int rc = launch_cg_finish(P, P->d_x, P->d_r, cg_zform(P, precond))) return rc;
     if (x)
         if (int rc = launch_to_ref(P, x, P->d_x)) return rc;
     if (r)

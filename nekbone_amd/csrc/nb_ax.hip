@@ -53,7 +53,7 @@ std::vector<AxVariant> ax_variants(int nx1)
     for (const auto &kv : table()) {
         int nx, epb, fma, mode, pf, xu;
         std::tie(nx, epb, fma, mode, pf, xu) = kv.first;
-        if (nx == nx1 && fma == 0 && mode == 2) out.push_back(AxVariant{epb, pf, xu});
+        if (nx == nx1 && fma == 0 && mode == 1) out.push_back(AxVariant{epb, pf, xu});
     }
     return out;
 }

@@ -78,7 +78,9 @@ void drop_graph(nekbone_batch *B)
 // update arguments.
 int prepare(nekbone_batch *B, int precond, const std::vector<int> &mbit)
 {
-    const int mode = precond == NEKBONE_PRECOND_JACOBI ? 2 : 1;
+    // Jacobi runs in z form (see cg_zform): the element kernel's identity
+    // mode reads z
+    const int mode = 1;
     for (auto &G : B->groups) {
         G.fn = nullptr;
         auto it = tableb().find(axk::Key(G.nx1, G.v.epb, B->fma ? 1 : 0, mode, G.v.pf, B->xu));
@@ -318,7 +320,8 @@ extern "C" int nekbone_batch_cg_tol_d(nekbone_batch *B, double *const *x, const 
         if (!f || !f[k]) return fail(NEKBONE_ERR_ARG, "batch: null right-hand side");
         if (int rc = launch_to_dev(P, P->d_f, f[k])) return rc;
         const int mbit0 = next_mask_bit(P);
-        if (int rc = launch_cg_init(P, P->d_x, P->d_r, P->d_f, mbit0, precond, tol * tol))
+        if (int rc = launch_cg_init(P, P->d_x, P->d_r, P->d_f, mbit0, precond, tol * tol,
+                                    precond == NEKBONE_PRECOND_JACOBI))
             return rc;
         mbit[k] = next_mask_bit(P);
     }
@@ -347,7 +350,8 @@ extern "C" int nekbone_batch_cg_tol_d(nekbone_batch *B, double *const *x, const 
     }
     for (size_t k = 0; k < np; ++k) {
         nekbone_problem *P = B->P[k];
-        if (int rc = launch_cg_finish(P, P->d_x)) return rc;
+        if (int rc = launch_cg_finish(P, P->d_x, P->d_r, precond == NEKBONE_PRECOND_JACOBI))
+            return rc;
         if (x && x[k])
             if (int rc = launch_to_ref(P, x[k], P->d_x)) return rc;
         if (r && r[k])
@@ -390,7 +394,9 @@ extern "C" int nekbone_batch_autotune(nekbone_batch *B, int precond, double *bes
         for (size_t k = 0; k < np && !rc; ++k) {
             nekbone_problem *P = B->P[k];
             rc = launch_rand(P->d_f, P->n, 12345 + k);
-            if (!rc) rc = launch_cg_init(P, P->d_x, P->d_r, P->d_f, 1, precond, 0.0);
+            if (!rc)
+                rc = launch_cg_init(P, P->d_x, P->d_r, P->d_f, 1, precond, 0.0,
+                                    precond == NEKBONE_PRECOND_JACOBI);
         }
         if (!rc) rc = prepare(B, precond, mbit);
         for (int i = 0; i < 2 && !rc; ++i) rc = launch_iteration(B, precond);

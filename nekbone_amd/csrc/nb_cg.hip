@@ -23,7 +23,7 @@ namespace nb {
 namespace {
 
 // x = 0, r = mask(f) (src/cg.f:38-40); parts of glsc3(r,c,r) and of
-// glsc3(r,c,z) with z = M^-1 r.
+// glsc3(r,c,z) with z = M^-1 r.  PRE = kPreJacobiZ stores z in place of r.
 template <int PRE>
 __global__ void __launch_bounds__(kRedThreads)
     k_cg_init(double *__restrict__ x, double *__restrict__ r, const double *__restrict__ f,
@@ -36,10 +36,15 @@ __global__ void __launch_bounds__(kRedThreads)
     for (int64_t i = int64_t(blockIdx.x) * kRedThreads + threadIdx.x; i < n; i += stride) {
         x[i] = 0.0;
         const double v = (flag[i] & mbit) ? 0.0 : f[i];
-        r[i] = v;
         const double rc = v * c[i];
         a0 = a0 + rc * v;
-        if constexpr (PRE == NEKBONE_PRECOND_JACOBI) a1 = a1 + rc * (v * dinv[i]);
+        if constexpr (pre_jacobi(PRE)) {
+            const double z = v * dinv[i];
+            a1 = a1 + rc * z;
+            r[i] = PRE == kPreJacobiZ ? z : v;
+        } else {
+            r[i] = v;
+        }
     }
     block_sum2(a0, a1);
     if (threadIdx.x == 0) {
@@ -59,7 +64,7 @@ __global__ void __launch_bounds__(kFinalThreads)
         s->rtr0 = rtr;
         s->rtr = rtr;
         s->rtz1 = 1.0;  // src/cg.f:36
-        s->rtz_next = (PRE == NEKBONE_PRECOND_JACOBI) ? rtz : rtr;
+        s->rtz_next = pre_jacobi(PRE) ? rtz : rtr;
         s->iter = 0;
         s->alpha = 0.0;
         s->done = (tol2 > 0.0 && rtr == 0.0) ? 1 : 0;
@@ -111,7 +116,7 @@ __device__ __forceinline__ void update_body(double *__restrict__ r, const double
                 rv[u] = r[i];
                 wv[u] = w[i];
                 fv[u] = flag[i];
-                if constexpr (PRE == NEKBONE_PRECOND_JACOBI) dv[u] = dinv[i];
+                if constexpr (pre_jacobi(PRE)) dv[u] = dinv[i];
                 if constexpr (XP) {
                     xv[u] = x[i];
                     pv[u] = p[i];
@@ -128,15 +133,29 @@ __device__ __forceinline__ void update_body(double *__restrict__ r, const double
                     else
                         x[i] = xv[u] + alpha * pv[u];
                 }
-                double ri;
-                if constexpr (FMA)
-                    ri = __builtin_fma(alphm, wv[u], rv[u]);
-                else
-                    ri = rv[u] + alphm * wv[u];
-                r[i] = ri;
-                const double rc = ri * inv_mult(fv[u]);
-                a0 = a0 + rc * ri;
-                if constexpr (PRE == NEKBONE_PRECOND_JACOBI) a1 = a1 + rc * (ri * dv[u]);
+                if constexpr (PRE == kPreJacobiZ) {
+                    // z form: rv holds z; z -= alpha dinv w, r = z / dinv
+                    double zi;
+                    if constexpr (FMA)
+                        zi = __builtin_fma(alphm, dv[u] * wv[u], rv[u]);
+                    else
+                        zi = rv[u] + alphm * (dv[u] * wv[u]);
+                    r[i] = zi;
+                    const double ri = zi / dv[u];
+                    const double rc = ri * inv_mult(fv[u]);
+                    a0 = a0 + rc * ri;
+                    a1 = a1 + rc * zi;
+                } else {
+                    double ri;
+                    if constexpr (FMA)
+                        ri = __builtin_fma(alphm, wv[u], rv[u]);
+                    else
+                        ri = rv[u] + alphm * wv[u];
+                    r[i] = ri;
+                    const double rc = ri * inv_mult(fv[u]);
+                    a0 = a0 + rc * ri;
+                    if constexpr (PRE == NEKBONE_PRECOND_JACOBI) a1 = a1 + rc * (ri * dv[u]);
+                }
             }
         }
     }
@@ -341,11 +360,21 @@ int64_t cg_partials_needed(const nekbone_problem *P)
     return std::max<int64_t>(P->nelt + n_gs(P) + 2 * n_upd(P), 2 * int64_t(kMaxRedBlocks));
 }
 
+bool cg_zform(const nekbone_problem *P, int precond)
+{
+    return precond == NEKBONE_PRECOND_JACOBI && P->c_is_mult && P->fused;
+}
+
 int launch_cg_init(nekbone_problem *P, double *x, double *r, const double *f, int mbit,
-                   int precond, double tol2)
+                   int precond, double tol2, bool zform)
 {
     const int nb = reduce_blocks(P->n);
-    if (precond == NEKBONE_PRECOND_JACOBI) {
+    if (precond == NEKBONE_PRECOND_JACOBI && zform) {
+        hipLaunchKernelGGL(k_cg_init<kPreJacobiZ>, dim3(nb), dim3(kRedThreads), 0, stream(), x,
+                           r, f, P->d_c, P->d_dinv, P->d_flag, mbit, P->n, P->d_part, nb);
+        hipLaunchKernelGGL(k_cg_init_final<NEKBONE_PRECOND_JACOBI>, dim3(1), dim3(kFinalThreads),
+                           0, stream(), P->d_part, nb, P->d_scal, P->d_hist, P->hist_cap, tol2);
+    } else if (precond == NEKBONE_PRECOND_JACOBI) {
         hipLaunchKernelGGL(k_cg_init<NEKBONE_PRECOND_JACOBI>, dim3(nb), dim3(kRedThreads), 0,
                            stream(), x, r, f, P->d_c, P->d_dinv, P->d_flag, mbit, P->n,
                            P->d_part, nb);
@@ -389,7 +418,9 @@ static int iteration_fused(nekbone_problem *P, double *x, double *r, int precond
     // forward again; the next iteration starts in the other direction
     const int rev = P->sweep & 1;
     ++P->sweep;
-    if (int rc = launch_ax_cg(P, x, r, precond, mbit, rev)) return rc;
+    // Jacobi runs in z form (r holds z = M^-1 r): the element kernel's
+    // identity mode reads z alone
+    if (int rc = launch_ax_cg(P, x, r, NEKBONE_PRECOND_IDENTITY, mbit, rev)) return rc;
     // the element kernel's partials are folded into the gather-scatter
     // kernel's when it has workgroups enough, so the final sum reads fewer
     const bool fold = cg_fold_partials(P);
@@ -415,7 +446,17 @@ static int iteration_fused(nekbone_problem *P, double *x, double *r, int precond
         else                  \
             NB_UPD2(PRE, F, false); \
     } while (0)
-    NB_PRE_FMA(NB_UPD, 0);
+    if (jac) {
+        if (fma)
+            NB_UPD(kPreJacobiZ, true, 0);
+        else
+            NB_UPD(kPreJacobiZ, false, 0);
+    } else {
+        if (fma)
+            NB_UPD(NEKBONE_PRECOND_IDENTITY, true, 0);
+        else
+            NB_UPD(NEKBONE_PRECOND_IDENTITY, false, 0);
+    }
 #undef NB_UPD
 #undef NB_UPD2
     NB_HIP(hipGetLastError());
@@ -594,9 +635,9 @@ int launch_cg_batch_tail(const CgBatchArgs *d_args, const int *d_ustart, int npr
     const bool jac = precond == NEKBONE_PRECOND_JACOBI;
     if (jac) {
         if (fma) {
-            if (xp) NB_UB(NEKBONE_PRECOND_JACOBI, true, true); else NB_UB(NEKBONE_PRECOND_JACOBI, true, false);
+            if (xp) NB_UB(kPreJacobiZ, true, true); else NB_UB(kPreJacobiZ, true, false);
         } else {
-            if (xp) NB_UB(NEKBONE_PRECOND_JACOBI, false, true); else NB_UB(NEKBONE_PRECOND_JACOBI, false, false);
+            if (xp) NB_UB(kPreJacobiZ, false, true); else NB_UB(kPreJacobiZ, false, false);
         }
     } else {
         if (fma) {
@@ -620,8 +661,20 @@ int cg_ax_blocks(const nekbone_problem *P) { return int(n_ax(P)); }
 bool cg_fold_partials(const nekbone_problem *P) { return n_gs(P) * kRedThreads >= n_ax(P); }
 int cg_gs_blocks(const nekbone_problem *P) { return int(n_gs(P)); }
 
-int launch_cg_finish(nekbone_problem *P, double *x)
+// z form: r = z / dinv in place.
+__global__ void k_z_to_r(double *__restrict__ r, const double *__restrict__ dinv, int64_t n)
 {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) r[i] = r[i] / dinv[i];
+}
+
+int launch_cg_finish(nekbone_problem *P, double *x, double *r, bool zform)
+{
+    if (zform) {
+        hipLaunchKernelGGL(k_z_to_r, dim3(grid1d(P->n, 256)), dim3(256), 0, stream(), r,
+                           static_cast<const double *>(P->d_dinv), P->n);
+        NB_HIP(hipGetLastError());
+    }
     if (!(P->c_is_mult && P->fused) || !P->axv.xu) return NEKBONE_OK;
     if (P->arith == NEKBONE_ARITH_FMA)
         hipLaunchKernelGGL(k_x_finish<true>, dim3(grid1d(P->n, 256)), dim3(256), 0, stream(), x,

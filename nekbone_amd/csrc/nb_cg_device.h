@@ -10,6 +10,13 @@ namespace nb {
 
 constexpr int kFinalThreads = 1024;
 
+// Template value of PRE for the fused Jacobi CG in z form: the vector the
+// iterations carry in place of r is z = M^-1 r (updated as z -= alpha
+// M^-1 w), so the element kernel reads z alone; r = z / dinv where the dot
+// products need it and once at the end.
+constexpr int kPreJacobiZ = 3;
+constexpr bool pre_jacobi(int pre) { return pre == NEKBONE_PRECOND_JACOBI || pre == kPreJacobiZ; }
+
 // c = 1/multiplicity (exactly the reference's 1./c(i)); the common hex
 // multiplicities are powers of two, exact without a division.
 __device__ __forceinline__ double inv_mult(uint8_t fl)
@@ -100,7 +107,7 @@ __device__ __forceinline__ void update_final_body(const double *__restrict__ par
         if (h < hcap) hist[h] = rtr;
         s->iter += 1;
         if (set_xpending) s->xpending = 1;
-        s->rtz_next = (PRE == NEKBONE_PRECOND_JACOBI) ? rtz : rtr;
+        s->rtz_next = pre_jacobi(PRE) ? rtz : rtr;
         if (s->tol2 > 0.0 && rtr <= s->tol2 * s->rtr0) s->done = 1;
         prepare_next(s, hist, hcap);
     }

@@ -469,10 +469,15 @@ int launch_to_ref(nekbone_problem *P, double *dst, const double *src);
 int upload_dev(nekbone_problem *P, double *dst, const double *src);
 int download_ref(nekbone_problem *P, double *dst, const double *src);
 // CG building blocks
+// zform (fused Jacobi CG only, see cg_zform): r receives z = M^-1 r
 int launch_cg_init(nekbone_problem *P, double *x, double *r, const double *f, int mbit,
-                   int precond, double tol2);
+                   int precond, double tol2, bool zform = false);
+// whether the fused Jacobi CG runs in z form (the vector r holds z = M^-1 r
+// between launch_cg_init and launch_cg_finish)
+bool cg_zform(const nekbone_problem *P, int precond);
 int launch_cg_iteration(nekbone_problem *P, double *x, double *r, int precond, int mbit);
-int launch_cg_finish(nekbone_problem *P, double *x);
+// after the last iteration: the pending x update, and r = z / dinv in z form
+int launch_cg_finish(nekbone_problem *P, double *x, double *r, bool zform);
 int64_t cg_partials_needed(const nekbone_problem *P);
 // Single-block scalar kernels of the fused iteration, shared with the
 // mixed-precision solver: alpha from npart pap parts; rtr / next rtz from
