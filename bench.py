@@ -391,13 +391,13 @@ def main():
     print("bench: %.3f ms/solve" % ms_step, file=sys.stderr, flush=True)
 
     # Dominant kernel: the element operator as launched inside CG, with the
-    # direction update fused in.  Compulsory streams per local point:
-    # reads r, dinv, p (8 B each), g (48 B), the point flag byte (1 B);
-    # writes p, w (8 B each); with the deferred x update also reads and
-    # writes x (16 B).
+    # direction update fused in (Jacobi in z form: the CG carries
+    # z = M^-1 r).  Compulsory streams per local point: reads z, p (8 B
+    # each), g (48 B), the point flag byte (1 B); writes p, w (8 B each);
+    # with the deferred x update also reads and writes x (16 B).
     epb, pf, xu = P.variant()
-    bpp = 89 + (16 if xu else 0)
-    kname = "k_ax<%d,%d,false,2,%d,%d>" % (nx1, epb, pf, xu)
+    bpp = 81 + (16 if xu else 0)
+    kname = "k_ax<%d,%d,false,1,%d,%d>" % (nx1, epb, pf, xu)
     ax_avg_s = ax_ms / max(launches, 1) / 1e3
     bytes_fused = float(bpp) * n
     flops_ax = (19.0 + 12.0 * nx1) * nx1 ** 3 * P.nelt
@@ -406,7 +406,7 @@ def main():
                 "traffic": None, "kernel": kname,
                 "bytes_per_point": bpp, "avg_us": ax_avg_s * 1e6, "launches": launches,
                 "gflops": flops_ax / ax_avg_s / 1e9,
-                "fused": "ax_e + Jacobi p update" + (" + deferred x update" if xu else "")}
+                "fused": "ax_e + Jacobi p update (z form)" + (" + deferred x update" if xu else "")}
     roofline["traffic"] = args.traffic if args.traffic is not None else \
         measured_traffic(kname, "%dx%dx%d" % nel)
 

@@ -92,19 +92,15 @@ int launch_ax_local(nekbone_problem *P, double *w, const double *u)
     return timed_end(P, 0);
 }
 
-int launch_ax_cg(nekbone_problem *P, double *x, const double *r, int precond, int mbit,
-                 int rev)
+int launch_ax_cg(nekbone_problem *P, double *x, const double *z, int mbit, int rev)
 {
-    const int mode = precond == NEKBONE_PRECOND_JACOBI ? 2 : 1;
-    AxFn fn = lookup(P->nx1, P->axv.epb, P->arith == NEKBONE_ARITH_FMA, mode, P->axv.pf,
+    AxFn fn = lookup(P->nx1, P->axv.epb, P->arith == NEKBONE_ARITH_FMA, 1, P->axv.pf,
                      P->axv.xu);
     if (!fn) return fail(NEKBONE_ERR_UNSUPPORTED, "ax: no kernel for this order/variant");
-    if (mode == 2 && !P->d_dinv) return fail(NEKBONE_ERR_STATE, "ax: Jacobi not set up");
     AxArgs A{};
     A.w = P->d_w;
     A.p = P->d_p;
-    A.r = r;
-    A.dinv = P->d_dinv;
+    A.r = z;
     A.x = x;
     A.g = P->d_g;
     A.gcs = P->n;

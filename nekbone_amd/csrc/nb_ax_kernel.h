@@ -39,11 +39,13 @@
 //    VGPRs, the most that 16 KB of LDS per element allows at one element
 //    per workgroup) and for 8 in FP32 (at most 64 VGPRs).
 //
-// CG mode (MODE 1 = identity, 2 = Jacobi) fuses, per point of the
-// thread's t-line:
+// CG mode (MODE 1) fuses, per point of the thread's t-line:
 //   prologue  [XU] x += alpha_prev p_prev     (add2s2, src/cg.f:63, one
 //                                              iteration late)
-//             p = beta p + z,  z = r or r*dinv (add2s1, src/cg.f:56)
+//             p = beta p + z                  (add2s1, src/cg.f:56; z is
+//                                              the vector the CG carries:
+//                                              r with the identity, M^-1 r
+//                                              with Jacobi)
 //   epilogue  points not shared with another element are final after
 //             the element operator: w = w + 0.1 p (src/cg.f:118), mask
 //             (src/cg.f:119), and their part of pap = sum (w c) p
@@ -181,18 +183,9 @@ __device__ __forceinline__ float mf_block_max(float v)
     return m;
 }
 
-// Jacobi diagonal storage: FP64 for the FP64 kernels; bfloat16 bits (the
-// FP32 exponent range, 8-bit significand) for the FP32 inner iterations of
-// the mixed solver, which only use it as a preconditioner.
-template <typename T>
-struct DinvOf {
-    using type = T;
-};
-template <>
-struct DinvOf<float> {
-    using type = uint16_t;
-};
-__device__ __forceinline__ double dinv_at(const double *d, int64_t i) { return d[i]; }
+// Jacobi diagonal of the FP32 inner iterations of the mixed solver:
+// bfloat16 bits (the FP32 exponent range, 8-bit significand), only ever
+// used as a preconditioner.
 __device__ __forceinline__ float dinv_at(const uint16_t *d, int64_t i)
 {
     return __uint_as_float(uint32_t(d[i]) << 16);
@@ -206,8 +199,7 @@ struct AxArgsT {
     T *w;
     const T *u;     // MODE 0 input
     T *p;           // CG: direction, updated in place
-    const T *r;     // CG: residual
-    const typename DinvOf<T>::type *dinv;  // CG Jacobi
+    const T *r;     // CG: z (r with the identity, M^-1 r with Jacobi)
     T *x;           // CG: solution, updated one iteration late (XU)
     const T *g;     // FP64: planes g[c * gcs + e * N3 + p]; FP32: g[6 * (e * N3 + p) + c]
     const T *Drow, *Dcol;
@@ -297,7 +289,7 @@ __device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
         }
     };
 
-    if constexpr (MODE == 1 || MODE == 2) {
+    if constexpr (MODE == 1) {
         if (A_.scal->done) return;  // converged tolerance run: nothing to do
     }
 
@@ -323,7 +315,7 @@ __device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
     // the points' flags, loaded with the vectors (four to a register) so
     // that the epilogue does not wait for a fresh memory round trip
     // measured: FP64 gains, FP32 loses
-    constexpr bool FLPRE = (MODE == 1 || MODE == 2) && sizeof(T) == 8;
+    constexpr bool FLPRE = MODE == 1 && sizeof(T) == 8;
     uint32_t flp[FLPRE ? (NX + 3) / 4 : 1] = {};
     if constexpr (FLPRE) {
         if (act) {
@@ -333,7 +325,7 @@ __device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
         }
     }
 
-    T pd[MODE == 1 || MODE == 2 ? NX : 1];  // CG: the new direction at the linear points
+    T pd[MODE == 1 ? NX : 1];  // CG: the new direction at the linear points
     if (act) {
         if constexpr (MODE == 0) {
 #pragma unroll
@@ -359,10 +351,6 @@ __device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
                 const int64_t q = base + ab + N2 * m;
                 po[m] = A_.p[q];
                 z[m] = A_.r[q];
-            }
-            if constexpr (MODE == 2) {
-#pragma unroll
-                for (int m = 0; m < NX; ++m) z[m] = z[m] * dinv_at(A_.dinv, base + ab + N2 * m);
             }
             if constexpr (XU) {
                 if (A_.scal->xpending) {  // the previous iteration's alpha p
@@ -571,7 +559,7 @@ __device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
             }
         }
     }
-    if constexpr (MODE == 1 || MODE == 2) {
+    if constexpr (MODE == 1) {
         // deterministic block sum of the pap partials: wave butterflies,
         // then the wave sums in wave order
         constexpr int NW = NT / 64;
@@ -638,12 +626,8 @@ void reg_modes(Table &t)
     t[Key(NX, EPB, 1, 0, PF, 0)] = k_ax<double, NX, EPB, true, 0, PF, false>;
     t[Key(NX, EPB, 0, 1, PF, 0)] = k_ax<double, NX, EPB, false, 1, PF, false>;
     t[Key(NX, EPB, 1, 1, PF, 0)] = k_ax<double, NX, EPB, true, 1, PF, false>;
-    t[Key(NX, EPB, 0, 2, PF, 0)] = k_ax<double, NX, EPB, false, 2, PF, false>;
-    t[Key(NX, EPB, 1, 2, PF, 0)] = k_ax<double, NX, EPB, true, 2, PF, false>;
     t[Key(NX, EPB, 0, 1, PF, 1)] = k_ax<double, NX, EPB, false, 1, PF, true>;
     t[Key(NX, EPB, 1, 1, PF, 1)] = k_ax<double, NX, EPB, true, 1, PF, true>;
-    t[Key(NX, EPB, 0, 2, PF, 1)] = k_ax<double, NX, EPB, false, 2, PF, true>;
-    t[Key(NX, EPB, 1, 2, PF, 1)] = k_ax<double, NX, EPB, true, 2, PF, true>;
 }
 
 // x flush + FP64 operator of the mixed-precision CG's residual refresh
@@ -671,7 +655,7 @@ void reg_order(Table &t, bool full)
     }
 }
 
-// Batched CG kernels (double, modes 1 and 2), same keys as the table.
+// Batched CG kernels (double, mode 1), same keys as the table.
 using AxFnB = void (*)(const AxArgs *, const int *, int);
 using TableB = std::map<Key, AxFnB>;
 
@@ -679,13 +663,9 @@ template <int NX, int EPB, int PF>
 void regb_modes(TableB &t)
 {
     t[Key(NX, EPB, 0, 1, PF, 0)] = k_ax_batch<double, NX, EPB, false, 1, PF, false>;
-    t[Key(NX, EPB, 0, 2, PF, 0)] = k_ax_batch<double, NX, EPB, false, 2, PF, false>;
     t[Key(NX, EPB, 0, 1, PF, 1)] = k_ax_batch<double, NX, EPB, false, 1, PF, true>;
-    t[Key(NX, EPB, 0, 2, PF, 1)] = k_ax_batch<double, NX, EPB, false, 2, PF, true>;
     t[Key(NX, EPB, 1, 1, PF, 0)] = k_ax_batch<double, NX, EPB, true, 1, PF, false>;
-    t[Key(NX, EPB, 1, 2, PF, 0)] = k_ax_batch<double, NX, EPB, true, 2, PF, false>;
     t[Key(NX, EPB, 1, 1, PF, 1)] = k_ax_batch<double, NX, EPB, true, 1, PF, true>;
-    t[Key(NX, EPB, 1, 2, PF, 1)] = k_ax_batch<double, NX, EPB, true, 2, PF, true>;
 }
 
 template <int NX>
@@ -707,12 +687,9 @@ void reg32_modes(Table32 &t)
 {
     t[Key32(NX, EPB, 0, PF, 0, 0)] = k_ax<float, NX, EPB, true, 0, PF, false>;
     t[Key32(NX, EPB, 1, PF, 0, 0)] = k_ax<float, NX, EPB, true, 1, PF, false>;
-    t[Key32(NX, EPB, 2, PF, 0, 0)] = k_ax<float, NX, EPB, true, 2, PF, false>;
     t[Key32(NX, EPB, 1, PF, 1, 0)] = k_ax<float, NX, EPB, true, 1, PF, true>;
-    t[Key32(NX, EPB, 2, PF, 1, 0)] = k_ax<float, NX, EPB, true, 2, PF, true>;
     t[Key32(NX, EPB, 0, PF, 0, 1)] = k_ax<float, NX, EPB, true, 0, PF, false, true>;
     t[Key32(NX, EPB, 1, PF, 1, 1)] = k_ax<float, NX, EPB, true, 1, PF, true, true>;
-    t[Key32(NX, EPB, 2, PF, 1, 1)] = k_ax<float, NX, EPB, true, 2, PF, true, true>;
 }
 
 template <int NX>
@@ -737,7 +714,6 @@ void reg32_mf(Table32 &t)
 {
     t[Key32(NX, 1, 0, 0, 0, 2)] = k_ax<float, NX, 1, true, 0, 0, false, true, true>;
     t[Key32(NX, 1, 1, 0, 1, 2)] = k_ax<float, NX, 1, true, 1, 0, true, true, true>;
-    t[Key32(NX, 1, 2, 0, 1, 2)] = k_ax<float, NX, 1, true, 2, 0, true, true, true>;
 }
 
 // Defined in the instantiation files nb_ax_i*.hip and nb_ax32_*.hip.

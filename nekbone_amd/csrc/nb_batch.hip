@@ -96,7 +96,6 @@ int prepare(nekbone_batch *B, int precond, const std::vector<int> &mbit)
             A.w = P->d_w;
             A.p = P->d_p;
             A.r = P->d_r;
-            A.dinv = P->d_dinv;
             A.x = P->d_x;
             A.g = P->d_g;
             A.gcs = P->n;

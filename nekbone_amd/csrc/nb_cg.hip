@@ -420,7 +420,7 @@ static int iteration_fused(nekbone_problem *P, double *x, double *r, int precond
     ++P->sweep;
     // Jacobi runs in z form (r holds z = M^-1 r): the element kernel's
     // identity mode reads z alone
-    if (int rc = launch_ax_cg(P, x, r, NEKBONE_PRECOND_IDENTITY, mbit, rev)) return rc;
+    if (int rc = launch_ax_cg(P, x, r, mbit, rev)) return rc;
     // the element kernel's partials are folded into the gather-scatter
     // kernel's when it has workgroups enough, so the final sum reads fewer
     const bool fold = cg_fold_partials(P);

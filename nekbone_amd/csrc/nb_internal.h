@@ -433,8 +433,9 @@ int ax_blocks(const nekbone_problem *P);
 AxVariant ax_default_variant(int nx1);
 std::vector<AxVariant> ax_variants(int nx1);
 int launch_ax_local(nekbone_problem *P, double *w, const double *u);
-int launch_ax_cg(nekbone_problem *P, double *x, const double *r, int precond, int mbit,
-                 int rev = 0);
+// element kernel of the fused CG iteration; z is the vector the CG carries
+// (r with the identity, M^-1 r with Jacobi in z form)
+int launch_ax_cg(nekbone_problem *P, double *x, const double *z, int mbit, int rev = 0);
 int launch_gs(nekbone_problem *P, double *u, const CgScalars *guard);
 // Fused CG variant: finishes shared points (+0.1p, mask) and writes one
 // pap part per workgroup (gs_blocks of them) to part.

@@ -353,8 +353,8 @@ int ensure_mixed(nekbone_problem *P, int precond)
         M->axv = ax_default_variant(nx);
         M->axv.xu = 1;
         M->axv.pf = 1;
-        if (!lookup32(nx, M->axv.epb, 2, M->axv.pf, 1)) M->axv.pf = 0;
-        if (!lookup32(nx, M->axv.epb, 2, M->axv.pf, 1)) {
+        if (!lookup32(nx, M->axv.epb, 1, M->axv.pf, 1)) M->axv.pf = 0;
+        if (!lookup32(nx, M->axv.epb, 1, M->axv.pf, 1)) {
             M->axv.epb = axk::epb_for(nx);
             M->axv.pf = 0;
         }
@@ -402,7 +402,6 @@ int launch_ax32(nekbone_problem *P, int mode, int mbit, float *w, const float *u
     A.u = u;
     A.p = M->p;
     A.r = M->r;
-    A.dinv = M->dinv;
     A.x = M->x;
     A.g = M->g;
     A.gcs = P->n;
@@ -612,8 +611,8 @@ extern "C" int nekbone_set_mixed_inner(nekbone_problem *P, int mode)
         // the first shape compiled for this operator: the default one, else
         // one element per workgroup (the only shape of the matrix-core kernel)
         AxVariant v{axk::epb_for(P->nx1), 0, 1};
-        if (!lookup32(P->nx1, v.epb, 2, v.pf, v.xu, mode)) v = AxVariant{1, 0, 1};
-        if (!lookup32(P->nx1, v.epb, 2, v.pf, v.xu, mode) ||
+        if (!lookup32(P->nx1, v.epb, 1, v.pf, v.xu, mode)) v = AxVariant{1, 0, 1};
+        if (!lookup32(P->nx1, v.epb, 1, v.pf, v.xu, mode) ||
             !lookup32(P->nx1, v.epb, 0, v.pf, 0, mode))
             return fail(NEKBONE_ERR_UNSUPPORTED, "mixed: inner operator not built for this order");
         M->axv = v;
@@ -641,7 +640,7 @@ extern "C" int nekbone_set_variant_f32(nekbone_problem *P, int epb, int pf, int 
 {
     if (int rc = check(P)) return rc;
     if (int rc = ensure_mixed(P, NEKBONE_PRECOND_IDENTITY)) return rc;
-    if (!lookup32(P->nx1, epb, 2, pf, xu ? 1 : 0, P->mx->g16) ||
+    if (!lookup32(P->nx1, epb, 1, pf, xu ? 1 : 0, P->mx->g16) ||
         !lookup32(P->nx1, epb, 0, pf, 0, P->mx->g16))
         return fail(NEKBONE_ERR_UNSUPPORTED, "set_variant_f32: shape not compiled for this order");
     P->mx->axv = AxVariant{epb, pf, xu ? 1 : 0};
