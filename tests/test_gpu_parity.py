@@ -302,3 +302,53 @@ def test_graph_replay_matches_direct_launches(nb, oracle, precond):
     for a, b in ((res[0], res[2]), (res[1], res[3]), (res[0], res[1])):
         for u, v in zip(a, b):
             assert np.array_equal(u, v)
+
+
+@pytest.mark.parametrize("precond", [0, 1])
+def test_cg_fma_arithmetic_vs_oracle(nb, oracle, precond):
+    """NEKBONE_ARITH_FMA through the whole fused CG (element kernel and
+    vector updates with fused multiply-adds; Jacobi in z form): the
+    rounding differs from the oracle's left-to-right sums in the last
+    bits only, so the history stays within 1e-10 and x within 1e-11 of
+    max|x| over 60 iterations."""
+    _case("cg fma precond=%d" % precond)
+    nel, nx1 = (4, 3, 3), 9
+    P = nb.Problem(*nel, nx1)
+    O = oracle.Problem(*nel, nx1)
+    P.set_mask_mode(nb.MASK_DIRICHLET)
+    O.set_mask_mode(oracle.MASK_DIRICHLET)
+    P.set_geometry_deformed(0.1, 4)
+    O.deformed_g(0.1, 4)
+    if precond:
+        P.setup_jacobi()
+        O.setup_jacobi()
+    P.set_arith(nb.ARITH_FMA)
+    f = O.rhs_from_raw(splitmix_uniform(O.n, 5))
+    xo, ro, ho = O.cg(f, 60, precond)
+    x, r, h = P.cg(f, 60, precond)
+    np.testing.assert_allclose(h, ho, rtol=1e-10, atol=0)
+    np.testing.assert_allclose(x, xo, rtol=0, atol=1e-11 * np.abs(xo).max())
+    np.testing.assert_allclose(r, ro, rtol=0, atol=1e-11 * np.abs(f).max())
+
+
+@pytest.mark.parametrize("niter", [0, 1, 2])
+def test_cg_jacobi_few_iterations(nb, oracle, niter):
+    """Jacobi CG with 0, 1 and 2 iterations: the z form's conversion back
+    to r at the end (r = z/dinv) returns the residual of the oracle's r
+    form to rounding (at niter = 0, mask(f))."""
+    _case("cg jacobi niter=%d" % niter)
+    nel, nx1 = (3, 2, 2), 8
+    P = nb.Problem(*nel, nx1)
+    O = oracle.Problem(*nel, nx1)
+    P.set_mask_mode(nb.MASK_DIRICHLET)
+    O.set_mask_mode(oracle.MASK_DIRICHLET)
+    P.setup_jacobi()
+    O.setup_jacobi()
+    f = O.rhs_from_raw(splitmix_uniform(O.n, 6))
+    xo, ro, ho = O.cg(f, niter, oracle.PRECOND_JACOBI)
+    x, r, h = P.cg(f, niter, nb.PRECOND_JACOBI)
+    np.testing.assert_allclose(h, ho, rtol=1e-13, atol=0)
+    np.testing.assert_allclose(r, ro, rtol=0, atol=1e-14 * np.abs(f).max())
+    np.testing.assert_allclose(x, xo, rtol=0, atol=1e-14 * max(np.abs(xo).max(), 1e-300))
+    if niter == 0:
+        assert not np.any(x)
