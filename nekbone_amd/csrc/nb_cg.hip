@@ -407,7 +407,10 @@ int launch_cg_init(nekbone_problem *P, double *x, double *r, const double *f, in
         }                                                            \
     } while (0)
 
-static int iteration_fused(nekbone_problem *P, double *x, double *r, int precond, int mbit)
+// zin: the vector the element kernel reads in place of r (an explicit
+// z = M^-1 r from the HSMG preconditioner); null: r itself.
+static int iteration_fused(nekbone_problem *P, double *x, double *r, int precond, int mbit,
+                           const double *zin = nullptr)
 {
     const bool jac = precond == NEKBONE_PRECOND_JACOBI;
     const bool fma = P->arith == NEKBONE_ARITH_FMA;
@@ -420,7 +423,7 @@ static int iteration_fused(nekbone_problem *P, double *x, double *r, int precond
     ++P->sweep;
     // Jacobi runs in z form (r holds z = M^-1 r): the element kernel's
     // identity mode reads z alone
-    if (int rc = launch_ax_cg(P, x, r, mbit, rev)) return rc;
+    if (int rc = launch_ax_cg(P, x, zin ? zin : r, mbit, rev)) return rc;
     // the element kernel's partials are folded into the gather-scatter
     // kernel's when it has workgroups enough, so the final sum reads fewer
     const bool fold = cg_fold_partials(P);
@@ -567,6 +570,11 @@ int launch_cg_iteration_z(nekbone_problem *P, double *x, double *r, const double
     hipLaunchKernelGGL(k_rtz_final, dim3(1), dim3(kFinalThreads), 0, stream(),
                        static_cast<const double *>(P->d_part), nb, P->d_scal, P->d_hist,
                        P->hist_cap);
+    NB_HIP(hipGetLastError());
+    // with c = 1/multiplicity the rest is the fused identity iteration with
+    // the element kernel reading z (p = z + beta p fused into it)
+    if (P->c_is_mult && P->fused)
+        return iteration_fused(P, x, r, NEKBONE_PRECOND_IDENTITY, mbit, z);
     if (fma)
         hipLaunchKernelGGL(k_pupdate_z<true>, dim3(grid1d(P->n, 256)), dim3(256), 0, stream(),
                            P->d_p, z, P->n, static_cast<const CgScalars *>(P->d_scal));
