@@ -352,3 +352,28 @@ def test_cg_jacobi_few_iterations(nb, oracle, niter):
     np.testing.assert_allclose(x, xo, rtol=0, atol=1e-14 * max(np.abs(xo).max(), 1e-300))
     if niter == 0:
         assert not np.any(x)
+
+
+@pytest.mark.parametrize("mask,nel,nx1,niter", [("ref", (4, 3, 2), 8, 30),
+                                                ("dirichlet", (3, 1, 1), 5, 12),
+                                                ("ref", (2, 2, 1), 3, 30)])
+def test_cg_jacobi_masks_and_odd_sizes(nb, oracle, mask, nel, nx1, niter):
+    """Jacobi CG (z form) with the reference's mask list and with local
+    point counts that are not multiples of the update's per-thread step
+    (3 x 125 and 4 x 27 points), against the oracle.  The 3x1x1 N=4 box
+    has 99 unknowns; it runs 12 iterations, before its residual reaches
+    roundoff, where relative history comparisons stop meaning anything."""
+    _case("cg jacobi mask=%s %s %d" % (mask, nel, nx1))
+    P = nb.Problem(*nel, nx1)
+    O = oracle.Problem(*nel, nx1)
+    if mask == "dirichlet":
+        P.set_mask_mode(nb.MASK_DIRICHLET)
+        O.set_mask_mode(oracle.MASK_DIRICHLET)
+    P.setup_jacobi()
+    O.setup_jacobi()
+    f = O.rhs_from_raw(splitmix_uniform(O.n, 7))
+    xo, ro, ho = O.cg(f, niter, oracle.PRECOND_JACOBI)
+    x, r, h = P.cg(f, niter, nb.PRECOND_JACOBI)
+    np.testing.assert_allclose(h, ho, rtol=HIST_RTOL, atol=0)
+    np.testing.assert_allclose(x, xo, rtol=0, atol=1e-12 * np.abs(xo).max())
+    np.testing.assert_allclose(r, ro, rtol=0, atol=1e-12 * np.abs(f).max())
