@@ -96,7 +96,7 @@ def Q_rtr0(Q, fq):
     residual (a zero-iteration tolerance solve reports it)."""
     import nekbone_amd as nb
     x, r = Q.vec(), Q.vec()
-    _, rtr0 = Q.cg_tol_d(x, fq, r, 1e-300, 0, nb.PRECOND_JACOBI)
+    _, rtr0 = Q.cg_tol_d(x, fq, r, 1e-300, 0, nb.PRECOND_JACOBI, maxiter_ok=True)
     x.free()
     r.free()
     return rtr0

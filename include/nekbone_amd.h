@@ -34,7 +34,10 @@ enum {
     NEKBONE_ERR_HIP = 2,      /* HIP runtime error                      */
     NEKBONE_ERR_STATE = 3,    /* call order (e.g. no Jacobi set up)     */
     NEKBONE_ERR_NUMERIC = 4,  /* non-finite or non-positive CG scalar   */
-    NEKBONE_ERR_UNSUPPORTED = 5
+    NEKBONE_ERR_UNSUPPORTED = 5,
+    /* a tolerance solve stopped at maxiter with rtr > tol^2 rtr_0; its
+     * outputs (x, r, iteration count, final rtr) are written all the same */
+    NEKBONE_NOT_CONVERGED = 6
 };
 
 enum { NEKBONE_PRECOND_IDENTITY = 0, NEKBONE_PRECOND_JACOBI = 1, NEKBONE_PRECOND_HSMG = 2 };
@@ -188,7 +191,8 @@ int nekbone_cg_d(nekbone_problem *P, double *x, const double *f, double *r, int 
                  int precond, double *hist);
 /* Run to a relative residual: stops after the first iteration with
  * rtr <= tol^2 * rtr_0 (rtr_0 the initial glsc3(r,c,r)), or at maxiter.
- * *iters receives the iteration count. */
+ * *iters receives the iteration count.  Returns NEKBONE_NOT_CONVERGED
+ * when it stopped at maxiter without meeting tol. */
 int nekbone_cg_tol_d(nekbone_problem *P, double *x, const double *f, double *r, double tol,
                      int maxiter, int precond, int *iters, double *final_rtr);
 
@@ -205,7 +209,9 @@ int nekbone_cg_tol_d(nekbone_problem *P, double *x, const double *f, double *r, 
  * nekbone_set_mixed_inner)
  * or the FP32 iteration reports convergence.  x, f, r as for
  * nekbone_cg_tol_d.  *iters: inner iterations; *updates: FP64 residual
- * recomputations; *final_rtr: glsc3(r,c,r) of the returned x.  Requires
+ * recomputations; *final_rtr: glsc3(r,c,r) of the returned x.  Returns
+ * NEKBONE_NOT_CONVERGED when maxiter inner iterations ran without the FP64
+ * residual meeting tol (outputs written, as for nekbone_cg_tol_d).  Requires
  * c = 1/multiplicity.  The FP32 copies of g, D and dinv are made on first
  * use and remade after the geometry or the Jacobi diagonal changes. */
 int nekbone_cg_mixed_d(nekbone_problem *P, double *x, const double *f, double *r, double tol,
@@ -271,7 +277,8 @@ int nekbone_batch_set_graph(nekbone_batch *B, int on);
 int nekbone_batch_autotune(nekbone_batch *B, int precond, double *best_ms);
 /* CG to a relative residual for every problem k: x[k], f[k], r[k] as for
  * nekbone_cg_tol_d (x, r may be NULL); stops when every problem has met
- * tol or after maxiter iterations.  iters[k], final_rtr[k] may be NULL. */
+ * tol or after maxiter iterations.  iters[k], final_rtr[k] may be NULL.
+ * Returns NEKBONE_NOT_CONVERGED when some problem did not meet tol. */
 int nekbone_batch_cg_tol_d(nekbone_batch *B, double *const *x, const double *const *f,
                            double *const *r, double tol, int maxiter, int precond, int *iters,
                            double *final_rtr);

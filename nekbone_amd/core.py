@@ -118,6 +118,15 @@ class NekboneError(RuntimeError):
     pass
 
 
+class NotConverged(NekboneError):
+    """A tolerance solve stopped at maxiter without meeting tol
+    (NEKBONE_NOT_CONVERGED); its outputs were written all the same."""
+    pass
+
+
+NOT_CONVERGED = 6
+
+
 def lib():
     """Load the HIP library (fails loudly if it is not built)."""
     global _LIB
@@ -135,10 +144,14 @@ def lib():
     return _LIB
 
 
-def _chk(rc):
-    if rc != 0:
-        msg = lib().nekbone_last_error().decode(errors="replace")
-        raise NekboneError("nekbone error %d: %s" % (rc, msg))
+def _chk(rc, maxiter_ok=False):
+    """Raise on a non-zero status; maxiter_ok accepts NEKBONE_NOT_CONVERGED
+    (a tolerance solve that ran to maxiter on purpose)."""
+    if rc == 0 or (maxiter_ok and rc == NOT_CONVERGED):
+        return
+    msg = lib().nekbone_last_error().decode(errors="replace")
+    cls = NotConverged if rc == NOT_CONVERGED else NekboneError
+    raise cls("nekbone error %d: %s" % (rc, msg))
 
 
 def _p(a):
@@ -200,14 +213,14 @@ class Batch:
         _chk(lib().nekbone_batch_autotune(self.handle, precond, ctypes.byref(ms)))
         return ms.value
 
-    def cg_tol_d(self, xs, fs, rs, tol, maxiter, precond=None):
+    def cg_tol_d(self, xs, fs, rs, tol, maxiter, precond=None, maxiter_ok=False):
         precond = PRECOND_JACOBI if precond is None else precond
         n = len(self.problems)
         ptrs = lambda vs: (_vp * n)(*[(v.ptr if v is not None else None) for v in vs])
         it = (ctypes.c_int * n)()
         rtr = (ctypes.c_double * n)()
         _chk(lib().nekbone_batch_cg_tol_d(self.handle, ptrs(xs), ptrs(fs), ptrs(rs), tol, maxiter,
-                                          precond, it, rtr))
+                                          precond, it, rtr), maxiter_ok)
         return list(it), list(rtr)
 
 
@@ -432,30 +445,34 @@ class Problem:
                                 None if hist is None else _p(hist)))
         return hist
 
-    def cg_tol_d(self, x, f, r, tol, maxiter, precond=PRECOND_IDENTITY):
+    def cg_tol_d(self, x, f, r, tol, maxiter, precond=PRECOND_IDENTITY, maxiter_ok=False):
+        """CG to a relative residual on device vectors; returns (iterations,
+        final rtr).  Raises NotConverged at maxiter unless maxiter_ok."""
         it = ctypes.c_int()
         rtr = ctypes.c_double()
         _chk(lib().nekbone_cg_tol_d(self.handle, x.ptr, f.ptr, r.ptr, tol, maxiter, precond,
-                                    ctypes.byref(it), ctypes.byref(rtr)))
+                                    ctypes.byref(it), ctypes.byref(rtr)), maxiter_ok)
         return it.value, rtr.value
 
-    def cg_mixed_d(self, x, f, r, tol, maxiter, precond=PRECOND_JACOBI, delta=0.0):
+    def cg_mixed_d(self, x, f, r, tol, maxiter, precond=PRECOND_JACOBI, delta=0.0,
+                   maxiter_ok=False):
         """Mixed-precision CG to a relative residual on device vectors;
-        returns (inner iterations, FP64 residual updates, final rtr)."""
+        returns (inner iterations, FP64 residual updates, final rtr).
+        Raises NotConverged at maxiter unless maxiter_ok."""
         it, nu = ctypes.c_int(), ctypes.c_int()
         rtr = ctypes.c_double()
         _chk(lib().nekbone_cg_mixed_d(self.handle, x.ptr, f.ptr, r.ptr, tol, maxiter, precond,
                                       delta, ctypes.byref(it), ctypes.byref(nu),
-                                      ctypes.byref(rtr)))
+                                      ctypes.byref(rtr)), maxiter_ok)
         return it.value, nu.value, rtr.value
 
-    def cg_mixed(self, f, tol, maxiter, precond=PRECOND_JACOBI, delta=0.0):
+    def cg_mixed(self, f, tol, maxiter, precond=PRECOND_JACOBI, delta=0.0, maxiter_ok=False):
         """Host-array form: returns x, r (reference order), iterations,
         updates, final rtr."""
         fv, xv, rv = self.vec(), self.vec(), self.vec()
         try:
             fv.upload(f)
-            it, nu, rtr = self.cg_mixed_d(xv, fv, rv, tol, maxiter, precond, delta)
+            it, nu, rtr = self.cg_mixed_d(xv, fv, rv, tol, maxiter, precond, delta, maxiter_ok)
             return xv.download(), rv.download(), it, nu, rtr
         finally:
             for v in (fv, xv, rv):

@@ -70,6 +70,33 @@ int ensure_init()
     return nekbone_init(g_device < 0 ? 0 : g_device);
 }
 
+int h2d(void *dst, const void *src, size_t bytes)
+{
+    if (bytes == 0) return NEKBONE_OK;
+    if (int rc = ensure_init()) return rc;
+    NB_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream()));
+    NB_HIP(hipStreamSynchronize(stream()));
+    return NEKBONE_OK;
+}
+
+int d2h(void *dst, const void *src, size_t bytes)
+{
+    if (bytes == 0) return NEKBONE_OK;
+    if (int rc = ensure_init()) return rc;
+    NB_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream()));
+    NB_HIP(hipStreamSynchronize(stream()));
+    return NEKBONE_OK;
+}
+
+int dzero(void *dst, size_t bytes)
+{
+    if (bytes == 0) return NEKBONE_OK;
+    if (int rc = ensure_init()) return rc;
+    NB_HIP(hipMemsetAsync(dst, 0, bytes, stream()));
+    NB_HIP(hipStreamSynchronize(stream()));
+    return NEKBONE_OK;
+}
+
 int next_mask_bit(nekbone_problem *P)
 {
     if (P->mask_mode == NEKBONE_MASK_DIRICHLET) return 1;
@@ -208,14 +235,12 @@ static int finish_problem(nekbone_problem *P, const double *c_host)
         return fail(NEKBONE_ERR_STATE, "internal: device point order is not a permutation");
     boundary_first_perm(nx, P->perm);
     if (int rc = dalloc(&P->d_perm, int64_t(P->perm.size()))) return rc;
-    NB_HIP(hipMemcpy(P->d_perm, P->perm.data(), sizeof(int32_t) * P->perm.size(),
-                     hipMemcpyHostToDevice));
+    if (int rc = h2d(P->d_perm, P->perm.data(), sizeof(int32_t) * P->perm.size())) return rc;
     {
         std::vector<int16_t> iref(P->perm.size());
         for (size_t p = 0; p < P->perm.size(); ++p) iref[size_t(P->perm[p])] = int16_t(p);
         if (int rc = dalloc(&P->d_iref, int64_t(iref.size()))) return rc;
-        NB_HIP(hipMemcpy(P->d_iref, iref.data(), sizeof(int16_t) * iref.size(),
-                         hipMemcpyHostToDevice));
+        if (int rc = h2d(P->d_iref, iref.data(), sizeof(int16_t) * iref.size())) return rc;
     }
 
     // Groups keep the reference's member order (ascending reference local
@@ -261,9 +286,7 @@ static int finish_problem(nekbone_problem *P, const double *c_host)
     G.nr = int64_t(roff.size()) - 1;
     auto up = [](int32_t **d, const std::vector<int32_t> &h) -> int {
         if (int rc = dalloc(d, int64_t(h.size()))) return rc;
-        if (!h.empty())
-            NB_HIP(hipMemcpy(*d, h.data(), sizeof(int32_t) * h.size(), hipMemcpyHostToDevice));
-        return NEKBONE_OK;
+        return h2d(*d, h.data(), sizeof(int32_t) * h.size());
     };
     for (auto pr : {std::make_pair(&G.d_m2, &m2), std::make_pair(&G.d_m4, &m4),
                     std::make_pair(&G.d_m8, &m8), std::make_pair(&G.d_roff, &roff),
@@ -324,8 +347,8 @@ static int finish_problem(nekbone_problem *P, const double *c_host)
         for (int m = 0; m < nx; ++m) drow[size_t(i * nx + m)] = P->dxm1[i + nx * m];
     if (int rc = dalloc(&P->d_D, int64_t(nx) * nx)) return rc;
     if (int rc = dalloc(&P->d_Drow, int64_t(nx) * nx)) return rc;
-    NB_HIP(hipMemcpy(P->d_D, P->dxm1, sizeof(double) * nx * nx, hipMemcpyHostToDevice));
-    NB_HIP(hipMemcpy(P->d_Drow, drow.data(), sizeof(double) * nx * nx, hipMemcpyHostToDevice));
+    if (int rc = h2d(P->d_D, P->dxm1, sizeof(double) * nx * nx)) return rc;
+    if (int rc = h2d(P->d_Drow, drow.data(), sizeof(double) * nx * nx)) return rc;
     if (int rc = dalloc(&P->d_g, 6 * n)) return rc;
     if (int rc = dalloc(&P->d_c, n)) return rc;
     for (double **v : {&P->d_x, &P->d_r, &P->d_p, &P->d_w, &P->d_f, &P->d_t, &P->d_t2})
@@ -333,16 +356,17 @@ static int finish_problem(nekbone_problem *P, const double *c_host)
     P->part_cap = cg_partials_needed(P);
     if (int rc = dalloc(&P->d_part, P->part_cap)) return rc;
     if (int rc = dalloc(&P->d_scal, 1)) return rc;
-    NB_HIP(hipMemset(P->d_scal, 0, sizeof(CgScalars)));
+    if (int rc = dzero(P->d_scal, sizeof(CgScalars))) return rc;
+    if (int rc = dzero(P->d_part, sizeof(double) * size_t(P->part_cap))) return rc;
     for (double *v : {P->d_x, P->d_r, P->d_p, P->d_w, P->d_f, P->d_t, P->d_t2})
-        NB_HIP(hipMemset(v, 0, sizeof(double) * size_t(n)));
+        if (int rc = dzero(v, sizeof(double) * size_t(n))) return rc;
 
     std::vector<uint8_t> flag, flag_dev(static_cast<size_t>(n));
     std::vector<int32_t> mult;
     build_flags(P, off, idx, flag, mult);
     for (int64_t l = 0; l < n; ++l) flag_dev[size_t(dev_index(P, l))] = flag[size_t(l)];
     if (int rc = dalloc(&P->d_flag, n)) return rc;
-    NB_HIP(hipMemcpy(P->d_flag, flag_dev.data(), size_t(n), hipMemcpyHostToDevice));
+    if (int rc = h2d(P->d_flag, flag_dev.data(), size_t(n))) return rc;
 
     if (c_host) {
         if (int rc = upload_dev(P, P->d_c, c_host)) return rc;
@@ -625,7 +649,10 @@ extern "C" int nekbone_set_geometry(nekbone_problem *P, const double *gxyz)
     if (!gxyz) return fail(NEKBONE_ERR_ARG, "set_geometry: null g");
     double *tmp = nullptr;
     if (int rc = dalloc(&tmp, 6 * P->n)) return rc;
-    NB_HIP(hipMemcpy(tmp, gxyz, sizeof(double) * 6 * size_t(P->n), hipMemcpyHostToDevice));
+    if (int rc = h2d(tmp, gxyz, sizeof(double) * 6 * size_t(P->n))) {
+        dfree(tmp);
+        return rc;
+    }
     int rc = launch_aos_to_soa(P, tmp, P->d_g, 0);
     (void)hipStreamSynchronize(stream());
     dfree(tmp);
@@ -641,8 +668,7 @@ extern "C" int nekbone_get_geometry(nekbone_problem *P, double *gxyz)
     if (int rc = dalloc(&tmp, 6 * P->n)) return rc;
     int rc = launch_aos_to_soa(P, tmp, P->d_g, 1);
     if (rc == NEKBONE_OK) {
-        NB_HIP(hipStreamSynchronize(stream()));
-        NB_HIP(hipMemcpy(gxyz, tmp, sizeof(double) * 6 * size_t(P->n), hipMemcpyDeviceToHost));
+        rc = d2h(gxyz, tmp, sizeof(double) * 6 * size_t(P->n));
     }
     dfree(tmp);
     return rc;
@@ -672,7 +698,10 @@ extern "C" int nekbone_set_geometry_deformed(nekbone_problem *P, double amp, uin
                 V[size_t(3 * v + 1)] = 2.0 * b + scale * V[size_t(3 * v + 1)];
                 V[size_t(3 * v + 2)] = 2.0 * c + scale * V[size_t(3 * v + 2)];
             }
-    NB_HIP(hipMemcpy(d_v, V.data(), sizeof(double) * V.size(), hipMemcpyHostToDevice));
+    if (int rc = h2d(d_v, V.data(), sizeof(double) * V.size())) {
+        dfree(d_v);
+        return rc;
+    }
     int rc = launch_deformed_g(P, d_v);
     (void)hipStreamSynchronize(stream());
     dfree(d_v);
@@ -994,6 +1023,8 @@ extern "C" int nekbone_cg_tol_d(nekbone_problem *P, double *x, const double *f, 
     if (iters) *iters = s.iter;
     if (final_rtr) *final_rtr = s.rtr;
     if (s.status) return fail(NEKBONE_ERR_NUMERIC, "cg: non-positive or non-finite pap");
+    if (!s.done)
+        return fail(NEKBONE_NOT_CONVERGED, "cg_tol: stopped at maxiter with rtr > tol^2 rtr0");
     return NEKBONE_OK;
 }
 
@@ -1004,14 +1035,17 @@ extern "C" int nekbone_alloc_vec(nekbone_problem *P, double **dptr)
 {
     if (int rc = check(P)) return rc;
     if (int rc = dalloc(dptr, P->n)) return rc;
-    NB_HIP(hipMemset(*dptr, 0, sizeof(double) * size_t(P->n)));
+    if (int rc = dzero(*dptr, sizeof(double) * size_t(P->n))) {
+        dfree(*dptr);
+        return rc;
+    }
     return NEKBONE_OK;
 }
 
 extern "C" int nekbone_free_vec(nekbone_problem *P, double *dptr)
 {
     (void)P;
-    NB_HIP(hipFree(dptr));
+    dfree(dptr);
     return NEKBONE_OK;
 }
 
@@ -1084,7 +1118,7 @@ extern "C" int nekbone_set_f_d(nekbone_problem *P, double *f)
     if (!f) return fail(NEKBONE_ERR_ARG, "set_f: null vector");
     std::vector<double> raw(static_cast<size_t>(P->n));
     if (int rc = nekbone_set_f_raw(P->n, raw.data())) return rc;
-    NB_HIP(hipMemcpy(f, raw.data(), sizeof(double) * raw.size(), hipMemcpyHostToDevice));
+    if (int rc = h2d(f, raw.data(), sizeof(double) * raw.size())) return rc;
     return nekbone_rhs_from_raw_d(P, f);  // dssum, then col2 with c (:97-98)
 }
 

@@ -62,8 +62,9 @@ int upload(T **d, const std::vector<T> &h)
 {
     if (!*d)
         if (int rc = dalloc(d, int64_t(h.size()))) return rc;
-    if (!h.empty()) NB_HIP(hipMemcpy(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
-    return NEKBONE_OK;
+    // stream-ordered: kernels of an earlier replay that read the table
+    // finish before it is rewritten
+    return h2d(*d, h.data(), sizeof(T) * h.size());
 }
 
 void drop_graph(nekbone_batch *B)
@@ -358,12 +359,15 @@ extern "C" int nekbone_batch_cg_tol_d(nekbone_batch *B, double *const *x, const 
     }
     if (int rc = read_done(B, s)) return rc;
     int status = 0;
+    bool all = true;
     for (size_t k = 0; k < np; ++k) {
         if (iters) iters[k] = s[k].iter;
         if (final_rtr) final_rtr[k] = s[k].rtr;
         status |= s[k].status;
+        all = all && s[k].done;
     }
     if (status) return fail(NEKBONE_ERR_NUMERIC, "batch: non-positive or non-finite pap");
+    if (!all) return fail(NEKBONE_NOT_CONVERGED, "batch: a problem stopped at maxiter unconverged");
     return NEKBONE_OK;
 }
 

@@ -140,7 +140,7 @@ int hsmg_coarse_build(nekbone_problem *P)
     std::vector<int64_t> glo;
     box_numbering(glo, 2, P->nelx, P->nely, P->nelz, P->lglel);
     std::vector<uint8_t> bnd(static_cast<size_t>(n1));
-    NB_HIP(hipMemcpy(bnd.data(), M->L[1].d_bnd, size_t(n1), hipMemcpyDeviceToHost));
+    if (int rc = d2h(bnd.data(), M->L[1].d_bnd, size_t(n1))) return rc;
     std::map<int64_t, int32_t> id;
     std::vector<int32_t> cidx(static_cast<size_t>(n1), -1), prim;
     for (int64_t q = 0; q < n1; ++q) {
@@ -178,8 +178,7 @@ int hsmg_coarse_build(nekbone_problem *P)
     if (!rc) rc = dalloc(&d_jq, int64_t(n3));
     if (!rc) rc = dalloc(&d_ae, int64_t(nelt) * 64);
     hipError_t err = hipSuccess;
-    if (!rc)
-        err = hipMemcpy(d_j, jh.data(), sizeof(double) * jh.size(), hipMemcpyHostToDevice);
+    if (!rc) rc = h2d(d_j, jh.data(), sizeof(double) * jh.size());
     for (int q = 0; q < 8 && !rc && err == hipSuccess; ++q) {
         for (int p = 0; p < n3; ++p) jcol[size_t(p)] = jh[size_t(p) * 8 + size_t(q)];
         err = hipMemcpyAsync(d_jq, jcol.data(), sizeof(double) * jcol.size(),
@@ -222,7 +221,7 @@ int hsmg_coarse_build(nekbone_problem *P)
         }
 
     if (int rc = dalloc(&M->d_ainv, int64_t(nc) * nc)) return rc;
-    NB_HIP(hipMemcpy(M->d_ainv, A.data(), sizeof(double) * A.size(), hipMemcpyHostToDevice));
+    if (int rc = h2d(M->d_ainv, A.data(), sizeof(double) * A.size())) return rc;
     rocblas_handle h = nullptr;
     if (rocblas_create_handle(&h) != rocblas_status_success)
         return fail(NEKBONE_ERR_HIP, "hsmg coarse: rocblas handle");
@@ -230,7 +229,7 @@ int hsmg_coarse_build(nekbone_problem *P)
     rocblas_int *d_info = nullptr;
     int info = 0;
     rocblas_status st = rocblas_status_success;
-    if (hipMalloc(&d_info, sizeof(rocblas_int)) != hipSuccess) st = rocblas_status_memory_error;
+    if (dalloc(&d_info, 1) != NEKBONE_OK) st = rocblas_status_memory_error;
     if (st == rocblas_status_success)
         st = rocsolver_dpotrf(h, rocblas_fill_lower, nc, M->d_ainv, nc, d_info);
     if (st == rocblas_status_success &&
@@ -244,7 +243,7 @@ int hsmg_coarse_build(nekbone_problem *P)
              hipSuccess ||
          hipStreamSynchronize(stream()) != hipSuccess || info != 0))
         st = rocblas_status_internal_error;
-    if (d_info) (void)hipFree(d_info);
+    dfree(d_info);
     rocblas_destroy_handle(h);
     if (st != rocblas_status_success)
         return fail(NEKBONE_ERR_HIP, "hsmg coarse: Cholesky inverse of the coarse operator failed");
@@ -256,10 +255,8 @@ int hsmg_coarse_build(nekbone_problem *P)
     if (int rc = dalloc(&M->d_cidx, n1)) return rc;
     if (int rc = dalloc(&M->d_xc, int64_t(nc))) return rc;
     if (int rc = dalloc(&M->d_rc, int64_t(nc))) return rc;
-    NB_HIP(hipMemcpy(M->d_cprim, prim.data(), sizeof(int32_t) * prim.size(),
-                     hipMemcpyHostToDevice));
-    NB_HIP(hipMemcpy(M->d_cidx, cidx.data(), sizeof(int32_t) * cidx.size(),
-                     hipMemcpyHostToDevice));
+    if (int rc = h2d(M->d_cprim, prim.data(), sizeof(int32_t) * prim.size())) return rc;
+    if (int rc = h2d(M->d_cidx, cidx.data(), sizeof(int32_t) * cidx.size())) return rc;
     M->nc = nc;
     NB_HIP(hipStreamSynchronize(stream()));
     return NEKBONE_OK;

@@ -233,8 +233,7 @@ template <class T>
 int upload(T **d, const std::vector<T> &h)
 {
     if (int rc = dalloc(d, int64_t(h.size()))) return rc;
-    if (!h.empty()) NB_HIP(hipMemcpy(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
-    return NEKBONE_OK;
+    return h2d(*d, h.data(), sizeof(T) * h.size());
 }
 
 // Ghost point of an extended element: face f (x-, x+, y-, y+, z-, z+) at
@@ -558,7 +557,7 @@ int hsmg_build(nekbone_problem *P)
         if (int rc = upload(&L.d_sw, sw)) return rc;
     }
     if (int rc = dalloc(&M->d_z, P->n)) return rc;
-    NB_HIP(hipMemset(M->d_z, 0, sizeof(double) * size_t(P->n)));
+    if (int rc = dzero(M->d_z, sizeof(double) * size_t(P->n))) return rc;
     return NEKBONE_OK;
 }
 

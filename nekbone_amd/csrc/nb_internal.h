@@ -35,6 +35,19 @@ int fail(int code, const std::string &msg);
                                                    hipGetErrorString(_e));         \
     } while (0)
 
+// The library's stream.  Every device operation of the library, copies and
+// fills included, is issued on it.  It is created non-blocking, so the
+// legacy null stream (plain hipMemcpy / hipMemset) is not ordered against
+// it: a null-stream fill of a new vector may land after a later copy into
+// that vector on this stream.
+hipStream_t stream();
+
+// Stream-ordered transfers and fills on stream(); each returns once the
+// operation has completed (the host buffer may be reused at once).
+int h2d(void *dst, const void *src, size_t bytes);
+int d2h(void *dst, const void *src, size_t bytes);
+int dzero(void *dst, size_t bytes);
+
 // Device allocation helpers (at least one element is allocated).
 template <class T>
 int dalloc(T **p, int64_t count)
@@ -45,10 +58,15 @@ int dalloc(T **p, int64_t count)
     return NEKBONE_OK;
 }
 
+// Frees after the library stream has drained, so that no kernel still in
+// flight reads or writes the memory.
 template <class T>
 void dfree(T *&p)
 {
-    if (p) (void)hipFree(p);
+    if (p) {
+        (void)hipStreamSynchronize(stream());
+        (void)hipFree(p);
+    }
     p = nullptr;
 }
 
@@ -423,7 +441,6 @@ inline int check(const nekbone_problem *P)
     if (!P) return fail(NEKBONE_ERR_ARG, "null problem");
     return NEKBONE_OK;
 }
-hipStream_t stream();
 int ensure_init();
 // Mask bit to apply for the next maskit call, advancing the state.
 int next_mask_bit(nekbone_problem *P);

@@ -571,9 +571,7 @@ int gs_map_upload(GsMap &G, const std::vector<int32_t> &off, const std::vector<i
     G.nr = int64_t(roff.size()) - 1;
     auto up = [](int32_t **d, const std::vector<int32_t> &h) -> int {
         if (int rc = dalloc(d, int64_t(h.size()))) return rc;
-        if (!h.empty())
-            NB_HIP(hipMemcpy(*d, h.data(), sizeof(int32_t) * h.size(), hipMemcpyHostToDevice));
-        return NEKBONE_OK;
+        return h2d(*d, h.data(), sizeof(int32_t) * h.size());
     };
     for (auto pr : {std::make_pair(&G.d_m2, &m2), std::make_pair(&G.d_m4, &m4),
                     std::make_pair(&G.d_m8, &m8), std::make_pair(&G.d_roff, &roff),

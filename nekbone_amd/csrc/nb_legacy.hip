@@ -54,10 +54,9 @@ int scratch(int64_t count, double **out)
 {
     LegacyState &L = S();
     if (L.scratch_len < count) {
-        if (L.d_scratch) (void)hipFree(L.d_scratch);
-        L.d_scratch = nullptr;
+        dfree(L.d_scratch);
         L.scratch_len = 0;
-        NB_HIP(hipMalloc(&L.d_scratch, sizeof(double) * size_t(count)));
+        if (int rc = dalloc(&L.d_scratch, count)) return rc;
         L.scratch_len = count;
     }
     *out = L.d_scratch;
@@ -300,24 +299,24 @@ static void grad_common(bool transpose, double *o0, double *o1, double *o2, doub
     if (rc) return report(rc, "local_grad3");
     double *dD = d + 7 * nxyz, *dDt = dD + nx * nx;
     const size_t eb = sizeof(double) * size_t(nxyz), db = sizeof(double) * size_t(nx * nx);
-    bool ok = hipMemcpy(dD, D, db, hipMemcpyHostToDevice) == hipSuccess &&
-              hipMemcpy(dDt, Dt, db, hipMemcpyHostToDevice) == hipSuccess;
+    bool ok = h2d(dD, D, db) == NEKBONE_OK &&
+              h2d(dDt, Dt, db) == NEKBONE_OK;
     if (!transpose) {
-        ok = ok && hipMemcpy(d, in0, eb, hipMemcpyHostToDevice) == hipSuccess;
+        ok = ok && h2d(d, in0, eb) == NEKBONE_OK;
         hipLaunchKernelGGL(k_grad3, dim3(unsigned((nxyz + 255) / 256)), dim3(256), 0, stream(),
                            d + nxyz, d + 2 * nxyz, d + 3 * nxyz, d, dD, dDt, nx);
         (void)hipStreamSynchronize(stream());
-        ok = ok && hipMemcpy(o0, d + nxyz, eb, hipMemcpyDeviceToHost) == hipSuccess &&
-             hipMemcpy(o1, d + 2 * nxyz, eb, hipMemcpyDeviceToHost) == hipSuccess &&
-             hipMemcpy(o2, d + 3 * nxyz, eb, hipMemcpyDeviceToHost) == hipSuccess;
+        ok = ok && d2h(o0, d + nxyz, eb) == NEKBONE_OK &&
+             d2h(o1, d + 2 * nxyz, eb) == NEKBONE_OK &&
+             d2h(o2, d + 3 * nxyz, eb) == NEKBONE_OK;
     } else {
-        ok = ok && hipMemcpy(d + nxyz, in0, eb, hipMemcpyHostToDevice) == hipSuccess &&
-             hipMemcpy(d + 2 * nxyz, in1, eb, hipMemcpyHostToDevice) == hipSuccess &&
-             hipMemcpy(d + 3 * nxyz, in2, eb, hipMemcpyHostToDevice) == hipSuccess;
+        ok = ok && h2d(d + nxyz, in0, eb) == NEKBONE_OK &&
+             h2d(d + 2 * nxyz, in1, eb) == NEKBONE_OK &&
+             h2d(d + 3 * nxyz, in2, eb) == NEKBONE_OK;
         hipLaunchKernelGGL(k_grad3_t, dim3(unsigned((nxyz + 255) / 256)), dim3(256), 0,
                            stream(), d, d + nxyz, d + 2 * nxyz, d + 3 * nxyz, dD, dDt, nx);
         (void)hipStreamSynchronize(stream());
-        ok = ok && hipMemcpy(out, d, eb, hipMemcpyDeviceToHost) == hipSuccess;
+        ok = ok && d2h(out, d, eb) == NEKBONE_OK;
     }
     if (!ok) report(fail(NEKBONE_ERR_HIP, "local_grad3 copy"), "local_grad3");
 }
@@ -367,13 +366,12 @@ extern "C" double glsc3_(const double *a, const double *b, const double *mult, c
     nekbone_problem tmp;  // carries only n and the partials for the reduction
     tmp.n = nn;
     tmp.d_part = d + 3 * nn;
-    bool ok = hipMemcpy(d, a, bytes, hipMemcpyHostToDevice) == hipSuccess &&
-              hipMemcpy(d + nn, b, bytes, hipMemcpyHostToDevice) == hipSuccess &&
-              hipMemcpy(d + 2 * nn, mult, bytes, hipMemcpyHostToDevice) == hipSuccess;
+    bool ok = h2d(d, a, bytes) == NEKBONE_OK &&
+              h2d(d + nn, b, bytes) == NEKBONE_OK &&
+              h2d(d + 2 * nn, mult, bytes) == NEKBONE_OK;
     ok = ok && launch_glsc3(&tmp, d, d + nn, d + 2 * nn, d + 3 * nn + 2 * 1024) == NEKBONE_OK;
     (void)hipStreamSynchronize(stream());
-    ok = ok && hipMemcpy(&out, d + 3 * nn + 2 * 1024, sizeof(double), hipMemcpyDeviceToHost) ==
-                   hipSuccess;
+    ok = ok && d2h(&out, d + 3 * nn + 2 * 1024, sizeof(double)) == NEKBONE_OK;
     tmp.d_part = nullptr;
     if (!ok) report(fail(NEKBONE_ERR_HIP, "glsc3_"), "glsc3_");
     return out;
@@ -398,20 +396,12 @@ extern "C" void gs_setup_(int *handle, const int64_t *glo_num, const int *n, con
     gs_groups(glo_num, *n, off, idx);
     h.map.ngroups = int64_t(off.size()) - 1;
     h.map.nshared = int64_t(idx.size());
-    if (!rc && hipMalloc(&h.map.d_off, sizeof(int32_t) * off.size()) != hipSuccess)
-        rc = NEKBONE_ERR_HIP;
-    if (!rc && hipMalloc(&h.map.d_idx, sizeof(int32_t) * std::max<size_t>(idx.size(), 1)) !=
-                   hipSuccess)
-        rc = NEKBONE_ERR_HIP;
-    if (!rc && hipMalloc(&h.d_u, sizeof(double) * size_t(std::max<int64_t>(h.n, 1))) !=
-                   hipSuccess)
-        rc = NEKBONE_ERR_HIP;
+    if (!rc) rc = dalloc(&h.map.d_off, int64_t(off.size()));
+    if (!rc) rc = dalloc(&h.map.d_idx, int64_t(idx.size()));
+    if (!rc) rc = dalloc(&h.d_u, int64_t(h.n));
     if (!rc) {
-        (void)hipMemcpy(h.map.d_off, off.data(), sizeof(int32_t) * off.size(),
-                        hipMemcpyHostToDevice);
-        if (!idx.empty())
-            (void)hipMemcpy(h.map.d_idx, idx.data(), sizeof(int32_t) * idx.size(),
-                            hipMemcpyHostToDevice);
+        rc = h2d(h.map.d_off, off.data(), sizeof(int32_t) * off.size());
+        if (!rc) rc = h2d(h.map.d_idx, idx.data(), sizeof(int32_t) * idx.size());
     }
     if (rc) {
         report(fail(rc, "gs_setup_: device allocation"), "gs_setup_");
@@ -436,12 +426,12 @@ extern "C" void gs_op_(const int *handle, double *u, const int *dom, const int *
                       "gs_op_");
     GsHandle &h = it->second;
     const size_t bytes = sizeof(double) * size_t(h.n);
-    bool ok = hipMemcpy(h.d_u, u, bytes, hipMemcpyHostToDevice) == hipSuccess;
+    bool ok = h2d(h.d_u, u, bytes) == NEKBONE_OK;
     if (h.map.ngroups > 0)
         hipLaunchKernelGGL(k_gs_plain, dim3(unsigned((h.map.ngroups + 255) / 256)), dim3(256), 0,
                            stream(), h.d_u, h.map.d_off, h.map.d_idx, h.map.ngroups);
     (void)hipStreamSynchronize(stream());
-    ok = ok && hipMemcpy(u, h.d_u, bytes, hipMemcpyDeviceToHost) == hipSuccess;
+    ok = ok && d2h(u, h.d_u, bytes) == NEKBONE_OK;
     if (!ok) report(fail(NEKBONE_ERR_HIP, "gs_op_ copy"), "gs_op_");
 }
 
@@ -451,8 +441,8 @@ extern "C" void gs_free_(const int *handle)
     auto it = L.gs.find(*handle);
     if (it == L.gs.end()) return;
     GsHandle &h = it->second;
-    if (h.map.d_off) (void)hipFree(h.map.d_off);
-    if (h.map.d_idx) (void)hipFree(h.map.d_idx);
-    if (h.d_u) (void)hipFree(h.d_u);
+    dfree(h.map.d_off);
+    dfree(h.map.d_idx);
+    dfree(h.d_u);
     L.gs.erase(it);
 }

@@ -344,8 +344,8 @@ int ensure_mixed(nekbone_problem *P, int precond)
                 dr[size_t(i * nx + m)] = float(P->dxm1[i + nx * m]);  // D(i,m)
                 dc[size_t(i * nx + m)] = float(P->dxm1[m + nx * i]);  // D(m,i)
             }
-        NB_HIP(hipMemcpy(M->Drow, dr.data(), sizeof(float) * dr.size(), hipMemcpyHostToDevice));
-        NB_HIP(hipMemcpy(M->Dcol, dc.data(), sizeof(float) * dc.size(), hipMemcpyHostToDevice));
+        if (int rc = h2d(M->Drow, dr.data(), sizeof(float) * dr.size())) return rc;
+        if (int rc = h2d(M->Dcol, dc.data(), sizeof(float) * dc.size())) return rc;
         // default FP32 shape: the FP64 default's elements per workgroup
         // with the geometric factors prefetched at entry (measured fastest
         // at N=9 on MI355X; FP32 leaves the registers for it), where that
@@ -588,6 +588,9 @@ extern "C" int nekbone_cg_mixed_d(nekbone_problem *P, double *x, const double *f
     if (updates) *updates = nupd;
     if (final_rtr) *final_rtr = s.rtr;
     if (s.status) return fail(NEKBONE_ERR_NUMERIC, "cg_mixed: non-positive or non-finite pap");
+    if (!s.done)
+        return fail(NEKBONE_NOT_CONVERGED,
+                    "cg_mixed: stopped at maxiter with the FP64 rtr > tol^2 rtr0");
     return NEKBONE_OK;
 }
 

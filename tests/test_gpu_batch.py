@@ -74,11 +74,11 @@ def test_batch_repeat_and_maxiter(nb, oracle):
     assert a == b
     assert all(np.array_equal(x.download(), y) for x, y in zip(xs, xa))
     for m in (3, 17):
-        its, _ = B.cg_tol_d(xs, fs, [None] * 4, 1e-30, m)
+        its, _ = B.cg_tol_d(xs, fs, [None] * 4, 1e-30, m, maxiter_ok=True)
         assert its == [m] * 4
         for k, (P, f) in enumerate(probs):
             x1, r1 = P.vec(), P.vec()
-            it1, _ = P.cg_tol_d(x1, fs[k], r1, 1e-30, m, nb.PRECOND_JACOBI)
+            it1, _ = P.cg_tol_d(x1, fs[k], r1, 1e-30, m, nb.PRECOND_JACOBI, maxiter_ok=True)
             assert np.array_equal(xs[k].download(), x1.download())
     ms = B.autotune()
     assert ms > 0

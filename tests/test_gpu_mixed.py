@@ -83,7 +83,9 @@ def test_cg_mixed_meets_fp64_criterion(nb, oracle, nel, nx1, deformed, precond):
     P, O = _pair(nb, oracle, nel, nx1, deformed, jacobi=bool(precond))
     f = O.rhs_from_raw(splitmix_uniform(O.n, 2))
     x, r, it, nupd, rtr = P.cg_mixed(f, tol, 3000, precond)
-    assert nupd >= 1 and it > 0
+    print("mixed: %d inner iterations, %d FP64 updates, final rtr %.3e" % (it, nupd, rtr),
+          flush=True)
+    assert nupd >= 1 and 0 < it < 3000
     c = O.c()
     f0 = O.maskit(f)
     rtrue = f0 - O.ax(x)
@@ -107,7 +109,7 @@ def test_cg_mixed_maxiter_and_geometry_change(nb, oracle):
     P, O = _pair(nb, oracle, (3, 2, 2), 8, True, gseed=1)
     f = O.rhs_from_raw(splitmix_uniform(O.n, 6))
     # a tolerance it cannot reach: stops at maxiter with x64 and r64 current
-    x, r, it, nupd, rtr = P.cg_mixed(f, 1e-30, 25, 1)
+    x, r, it, nupd, rtr = P.cg_mixed(f, 1e-30, 25, 1, maxiter_ok=True)
     assert it == 25 and nupd >= 1
     f0 = O.maskit(f)
     rtrue = f0 - O.ax(x)
