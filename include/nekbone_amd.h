@@ -368,6 +368,25 @@ void gs_setup_(int *handle, const int64_t *glo_num, const int *n, const int *com
 void gs_op_(const int *handle, double *u, const int *dom, const int *op, const int *transpose);
 void gs_free_(const int *handle);
 
+/* ---------------------------------------------------------------------- */
+/* Allocation-free entry point.  No counterpart in the reference: it is for */
+/* callers that cannot allocate device memory (code under HIP graph         */
+/* capture, tight serving loops, harnesses that forbid hipMalloc).          */
+/* ---------------------------------------------------------------------- */
+/* *result = sum a_i*b_i*mult_i over n points, the value glsc3_ and
+ * nekbone_glsc3 return for the same data (bit-identical: same kernels,
+ * same block partials).  a, b, mult, result are device pointers; the sum
+ * is written to device memory in the library stream's order and the host
+ * is not synchronised (nekbone_sync or a stream wait makes it visible).
+ * The block partials live in a static device array of 1024 doubles (8 KiB
+ * per GPU, held until the process ends), so the call allocates nothing;
+ * nekbone_init must have run.  0 <= n <= 2^31 - 1, else NEKBONE_ERR_ARG.
+ * Calls on one GPU share its array: calls through the library stream run
+ * in order, but two overlapping ones (e.g. replays of a captured call on
+ * two streams at once) race. */
+int nekbone_glsc3_noalloc_d(const double *a, const double *b, const double *mult, int64_t n,
+                            double *result);
+
 #ifdef __cplusplus
 }
 #endif

@@ -109,6 +109,8 @@ _SIGS = {
     "nekbone_batch_cg_tol_d": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                               ctypes.POINTER(_vp), ctypes.c_double, ctypes.c_int,
                                               ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp]),
+    "nekbone_glsc3_noalloc_d": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int64, _vp]),
+    "glsc3_": (ctypes.c_double, [_dp, _dp, _dp, ctypes.POINTER(ctypes.c_int)]),
 }
 
 _LIB = None
@@ -171,6 +173,14 @@ def init(device=0):
 
 def sync():
     _chk(lib().nekbone_sync())
+
+
+def glsc3_noalloc_d(a, b, mult, n, result):
+    """sum a*b*mult over n points of device memory into result[0] (device),
+    without allocating (nekbone_glsc3_noalloc_d); a, b, mult, result are
+    DeviceVec objects or raw device addresses."""
+    ptr = lambda v: v.ptr if isinstance(v, DeviceVec) else v
+    _chk(lib().nekbone_glsc3_noalloc_d(ptr(a), ptr(b), ptr(mult), int(n), ptr(result)))
 
 
 def cubic(np_):

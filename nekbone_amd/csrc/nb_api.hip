@@ -501,6 +501,7 @@ extern "C" int nekbone_init(int device)
     if (device < 0 || device >= count)
         return fail(NEKBONE_ERR_ARG, "nekbone_init: no such device");
     NB_HIP(hipSetDevice(device));
+    if (int rc = noalloc_setup(device)) return rc;
     if (g_stream) NB_HIP(hipStreamDestroy(g_stream));
     NB_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
     g_device = device;

@@ -473,6 +473,14 @@ int launch_post(nekbone_problem *P, double *w, const double *p, int mbit, bool p
 int launch_mask(nekbone_problem *P, double *w, int mbit);
 int launch_glsc3(nekbone_problem *P, const double *a, const double *b, const double *m,
                  double *d_out);
+// glsc3 of n points with explicit partials: reduce_blocks(n) <= kMaxRedBlocks
+// block partials in part, the sum in d_out (both device memory).
+int launch_glsc3_raw(const double *a, const double *b, const double *m, int64_t n,
+                     double *part, double *d_out);
+// Allocation-free entry points (nb_noalloc.hip): per-device addresses of
+// their static device scratch, looked up once per device at nekbone_init
+// with that device current.
+int noalloc_setup(int device);
 int launch_brick_g(nekbone_problem *P);
 int launch_deformed_g(nekbone_problem *P, const double *d_vert);
 int launch_jacobi_local(nekbone_problem *P, double *dl);

@@ -273,26 +273,15 @@ __global__ void __launch_bounds__(kRedThreads)
     if (threadIdx.x == 0) part[blockIdx.x] = acc;
 }
 
-// Sum of nblk partials (two interleaved streams) in a fixed order.
-__device__ __forceinline__ void sum_partials(const double *__restrict__ part, int nblk,
-                                             double &s0, double &s1)
-{
-    double a0 = 0.0, a1 = 0.0;
-    for (int i = threadIdx.x; i < nblk; i += kRedThreads) {
-        a0 = a0 + part[i];
-        a1 = a1 + part[kMaxRedBlocks + i];
-    }
-    block_sum2(a0, a1);
-    s0 = a0;
-    s1 = a1;
-}
-
+// Sum of nblk partials in a fixed order: thread t adds partials t,
+// t + kRedThreads, ..., then the block sum.
 __global__ void __launch_bounds__(kRedThreads)
     k_final_scalar(const double *__restrict__ part, int nblk, double *__restrict__ out)
 {
-    double s0, s1;
-    sum_partials(part, nblk, s0, s1);
-    if (threadIdx.x == 0) *out = s0;
+    double a0 = 0.0, dummy = 0.0;
+    for (int i = threadIdx.x; i < nblk; i += kRedThreads) a0 = a0 + part[i];
+    block_sum2(a0, dummy);
+    if (threadIdx.x == 0) *out = a0;
 }
 
 // ---------------------------------------------------------------------
@@ -645,16 +634,22 @@ int launch_mask(nekbone_problem *P, double *w, int mbit)
     return NEKBONE_OK;
 }
 
+int launch_glsc3_raw(const double *a, const double *b, const double *m, int64_t n,
+                     double *part, double *d_out)
+{
+    const int nb = reduce_blocks(n);
+    hipLaunchKernelGGL(k_glsc3_part, dim3(nb), dim3(kRedThreads), 0, stream(), a, b, m, n,
+                       part);
+    hipLaunchKernelGGL(k_final_scalar, dim3(1), dim3(kRedThreads), 0, stream(),
+                       static_cast<const double *>(part), nb, d_out);
+    NB_HIP(hipGetLastError());
+    return NEKBONE_OK;
+}
+
 int launch_glsc3(nekbone_problem *P, const double *a, const double *b, const double *m,
                  double *d_out)
 {
-    const int nb = reduce_blocks(P->n);
-    hipLaunchKernelGGL(k_glsc3_part, dim3(nb), dim3(kRedThreads), 0, stream(), a, b, m, P->n,
-                       P->d_part);
-    hipLaunchKernelGGL(k_final_scalar, dim3(1), dim3(kRedThreads), 0, stream(), P->d_part, nb,
-                       d_out);
-    NB_HIP(hipGetLastError());
-    return NEKBONE_OK;
+    return launch_glsc3_raw(a, b, m, P->n, P->d_part, d_out);
 }
 
 int launch_brick_g(nekbone_problem *P)
