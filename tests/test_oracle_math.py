@@ -240,3 +240,96 @@ def test_rand_uniform_splitmix64_known_answer(oracle):
     prev = (s - 0x9E3779B97F4A7C15) % (1 << 64)
     assert np.array_equal(oracle.rand_uniform(4, s)[:3], oracle.rand_uniform(4, prev)[1:4])
     assert np.array_equal(splitmix_uniform(1000, 7), oracle.rand_uniform(1000, 7))
+
+
+def _deformed_nodes(oracle, P, amp, seed):
+    """Physical coordinates (n, 3) of every GLL node of the deformed box,
+    computed here: vertex (a, b, c) of the width-2 grid moved by
+    2 amp U (three values per vertex of the seeded generator, vertex
+    index a + vx (b + vy c)), and each element mapped trilinearly from its
+    8 vertices (lexicographic elements, x fastest)."""
+    vx, vy, vz = P.nelx + 1, P.nely + 1, P.nelz + 1
+    U = oracle.rand_uniform(3 * vx * vy * vz, seed).reshape(vz, vy, vx, 3)
+    c, b, a = np.meshgrid(np.arange(vz), np.arange(vy), np.arange(vx), indexing="ij")
+    V = 2.0 * np.stack([a, b, c], -1) + (2.0 * amp) * U          # V[c, b, a, d]
+    nx = P.nx1
+    z, _ = oracle.zwgll(nx)
+    h = np.stack([0.5 * (1.0 - z), 0.5 * (1.0 + z)])            # h[corner, node]
+    X = np.empty((P.nelt, nx, nx, nx, 3))                       # X[e, k, j, i, d]
+    for e in range(P.nelt):
+        ex, ey, ez = e % P.nelx, (e // P.nelx) % P.nely, e // (P.nelx * P.nely)
+        C = V[ez:ez + 2, ey:ey + 2, ex:ex + 2]                   # C[cc, bb, aa, d]
+        X[e] = np.einsum("ck,bj,ai,cbad->kjid", h, h, h, C)
+    return X.reshape(-1, 3)
+
+
+def _g_from_jacobian(oracle, P, X, transpose=False):
+    """g (n, 6) from a Jacobian found by spectral differentiation of the
+    node coordinates (exact: the map is trilinear) and numpy's inverse:
+    G = w_i w_j w_k det(J) J^-1 J^-T, factors (rr, rs, rt, ss, st, tt)."""
+    nx = P.nx1
+    D = P.dxm1().reshape(nx, nx, order="F")
+    _, w = oracle.zwgll(nx)
+    Xe = X.reshape(P.nelt, nx, nx, nx, 3)
+    dr = np.einsum("im,ekjmd->ekjid", D, Xe)
+    ds = np.einsum("jm,ekmid->ekjid", D, Xe)
+    dt = np.einsum("km,emjid->ekjid", D, Xe)
+    J = np.stack([dr, ds, dt], -1)                              # J[..., d, a] = dx_d / dr_a
+    if transpose:
+        J = np.swapaxes(J, -1, -2)
+    Ji = np.linalg.inv(J)
+    W = np.einsum("k,j,i->kji", w, w, w)[None, :, :, :]
+    G = (W * np.linalg.det(J))[..., None, None] * (Ji @ np.swapaxes(Ji, -1, -2))
+    G = G.reshape(-1, 3, 3)
+    return np.stack([G[:, 0, 0], G[:, 0, 1], G[:, 0, 2], G[:, 1, 1], G[:, 1, 2], G[:, 2, 2]], -1)
+
+
+@pytest.mark.parametrize("nel,nx1", [((3, 2, 2), 3), ((2, 2, 2), 5), ((2, 1, 2), 8),
+                                     ((1, 2, 1), 10)])
+def test_deformed_g_against_numpy_jacobian(oracle, nel, nx1):
+    """The oracle's deformed factors against an independent computation
+    (node coordinates, spectral Jacobian, numpy inverse and determinant).
+    A transposed Jacobian gives different factors."""
+    P = oracle.Problem(*nel, nx1)
+    P.deformed_g(0.1, 17)
+    X = _deformed_nodes(oracle, P, 0.1, 17)
+    g = P.g().reshape(-1, 6)
+    gn = _g_from_jacobian(oracle, P, X)
+    scale = np.abs(g).max()
+    np.testing.assert_allclose(g, gn, rtol=0, atol=1e-12 * scale)
+    assert np.abs(_g_from_jacobian(oracle, P, X, transpose=True) - g).max() > 1e-3 * scale
+
+
+@pytest.mark.parametrize("nel,nx1", [((3, 2, 2), 3), ((2, 2, 2), 5), ((2, 1, 2), 8),
+                                     ((1, 2, 1), 10), ((2, 2, 1), 12)])
+def test_deformed_operator_annihilates_linear_functions(oracle, nel, nx1):
+    """For u = a . x linear in the physical coordinates, the element
+    operator's rows at element-interior nodes are zero: the exact row is
+    the integral of grad(phi_i) . a over the element, which vanishes since
+    phi_i is zero on the element's faces, and the GLL rule is exact here
+    for N >= 2 (det(J) J^-1 = cof(J)^T has degree 2 in the differentiated
+    coordinate and 1 in the others for a trilinear map, and the Piola
+    identity makes its rows divergence-free).  A transposed Jacobian in
+    the factors breaks this by orders of magnitude."""
+    P = oracle.Problem(*nel, nx1)
+    P.deformed_g(0.1, 23)
+    X = _deformed_nodes(oracle, P, 0.1, 23)
+    u = X @ np.array([0.3, -0.7, 1.1])
+    n3 = nx1 ** 3
+    inner = np.zeros((nx1, nx1, nx1), bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    inner = inner.reshape(-1)
+
+    def rows(Q):
+        w = np.concatenate([Q.ax_e(e, u[e * n3:(e + 1) * n3]) for e in range(Q.nelt)])
+        return w.reshape(Q.nelt, n3)
+
+    w = rows(P)
+    scale = np.abs(w).max()
+    assert scale > 1e-2
+    assert np.abs(w[:, inner]).max() <= 1e-12 * scale
+    # the same check with the factors of a transposed Jacobian (a planted
+    # mistake, on the oracle) fails
+    Q = oracle.Problem(*nel, nx1)
+    Q.set_g(_g_from_jacobian(oracle, Q, X, transpose=True).reshape(-1))
+    assert np.abs(rows(Q)[:, inner]).max() > 1e-4 * scale
