@@ -333,11 +333,19 @@ void nekbone_setup_(const int *nelt, const int *nx1, const int *nelx, const int 
                     const int *nelz, const int32_t *lglel, const double *dxm1, const int *gsh);
 /* Select the preconditioner solvem_/cg_ apply: 0 = identity (the
  * reference's default build, src/hsmg_dum.f:9-15), nonzero = Jacobi.
- * No counterpart in the reference, which has no Jacobi option. */
+ * No counterpart in the reference, which has no Jacobi option.  The
+ * choice holds for later meshes; each nekbone_setup_ starts from it, and
+ * h1mg_setup_ switches the bound mesh to HSMG. */
 void nekbone_use_jacobi_(const int *on);
-/* src/cg.f:2-84 */
+/* src/cg.f:2-84.  Prints the reference's two residual lines
+ * ('cg:',i4,1p4e12.4; src/cg.f:43,76) on stdout.  g and c are uploaded
+ * when their address or contents differ from the previous call's. */
 void cg_(double *x, const double *f, const double *g, const double *c, double *r, double *w,
          double *p, double *z, const int *n, const int *niter, double *flop_cg);
+/* src/semhat.f:2-37: a, c, d (0:n,0:n) column-major, b and z (0:n); w is
+ * the reference's work array (unused).  Called by the reference's
+ * proxy_setup (src/prox_setup.f:9). */
+void semhat_(double *a, double *b, double *c, double *d, double *z, double *w, const int *n);
 /* src/cg.f:86-94 (identity, or Jacobi after nekbone_setup_jacobi) */
 void solvem_(double *z, double *r, const int *n);
 /* src/hsmg.f:3-29; after it solvem_ and cg_ use the HSMG preconditioner,

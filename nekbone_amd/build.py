@@ -84,7 +84,41 @@ def build(verbose=False, jobs=None, incremental=False):
     if r.returncode != 0:
         raise RuntimeError("link failed\n%s\n%s" % (r.stdout, r.stderr))
     os.replace(tmp, lib_path())
+    build_fortran_host(verbose)
     return lib_path()
+
+
+FORTRAN = os.path.join(HERE, "fortran")
+HOST = os.path.join(LIBDIR, "nekbone_host")
+
+
+def flang():
+    """The image's flang, or None."""
+    for cand in (os.environ.get("FLANG"), "/opt/rocm/llvm/bin/flang", shutil.which("flang")):
+        if cand and os.path.exists(cand):
+            return cand
+    return None
+
+
+def build_fortran_host(verbose=False):
+    """Link the Fortran 2003 example host (nekbone_amd/fortran) against the
+    library: lib/nekbone_host, found next to libnekbone_amd.so at run time.
+    Returns its path, or None when no flang is installed."""
+    fc = flang()
+    if fc is None:
+        return None
+    mod = os.path.join(BUILD, "fortran")
+    os.makedirs(mod, exist_ok=True)
+    cmd = [fc, "-O2", "-J", mod, "-o", HOST + ".tmp",
+           os.path.join(FORTRAN, "nekbone_amd_if.f90"), os.path.join(FORTRAN, "nekbone_host.f90"),
+           "-L" + LIBDIR, "-lnekbone_amd", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("fortran host build failed\n%s\n%s" % (r.stdout, r.stderr))
+    os.replace(HOST + ".tmp", HOST)
+    return HOST
 
 
 def clean():

@@ -4,6 +4,8 @@
 // (/dimn/, /dxyz/, /mymask/, /comm_handles/).  Host arrays in and out;
 // all arithmetic runs on the device through the same kernels as the
 // explicit API.
+#include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -26,11 +28,41 @@ struct GsHandle {
     double *d_u = nullptr;
 };
 
+// Content fingerprint of a host array: a 64-bit multiply-xorshift hash of
+// every word in four interleaved lanes (one streaming pass).  The legacy
+// entry points take g and c on every call, as the reference does; they
+// upload them again only when the address or the contents changed, so an
+// in-place update by the caller is not missed.
+uint64_t fingerprint(const double *a, int64_t count)
+{
+    uint64_t h[4] = {0x243F6A8885A308D3ull, 0x13198A2E03707344ull, 0xA4093822299F31D0ull,
+                     0x082EFA98EC4E6C89ull};
+    const uint64_t *w = reinterpret_cast<const uint64_t *>(a);
+    int64_t i = 0;
+    for (; i + 4 <= count; i += 4)
+        for (int l = 0; l < 4; ++l) {
+            uint64_t v = (h[l] ^ w[i + l]) * 0x9E3779B97F4A7C15ull;
+            h[l] = v ^ (v >> 29);
+        }
+    for (; i < count; ++i) {
+        uint64_t v = (h[0] ^ w[i]) * 0x9E3779B97F4A7C15ull;
+        h[0] = v ^ (v >> 29);
+    }
+    uint64_t out = uint64_t(count);
+    for (int l = 0; l < 4; ++l) {
+        uint64_t v = (out ^ h[l]) * 0xBF58476D1CE4E5B9ull;
+        out = v ^ (v >> 31);
+    }
+    return out;
+}
+
 struct LegacyState {
     nekbone_problem *P = nullptr;
-    const double *g_ptr = nullptr;  // host g last uploaded
-    const double *c_ptr = nullptr;  // host c last uploaded
+    const double *g_ptr = nullptr;  // host g last uploaded, and its fingerprint
+    const double *c_ptr = nullptr;  // host c last uploaded, and its fingerprint
+    uint64_t g_fp = 0, c_fp = 0;
     int precond = NEKBONE_PRECOND_IDENTITY;
+    bool jacobi_on = false;  // nekbone_use_jacobi_: kept across meshes
     std::map<int, GsHandle> gs;
     int next_handle = 1;
     nekbone_problem *elem = nullptr;  // one-element problem for ax_e_
@@ -127,21 +159,40 @@ __global__ void k_mul(double *dst, const double *a, const double *b, int64_t n)
     if (i < n) dst[i] = a[i] * b[i];
 }
 
-// Upload g and c when the caller hands a different array than last time.
+// Upload g and c when the caller hands a different array than last time,
+// or the same array with different contents.
 int sync_geometry(const double *g, const double *c)
 {
     LegacyState &L = S();
-    if (g && g != L.g_ptr) {
-        if (int rc = nekbone_set_geometry(L.P, g)) return rc;
-        L.g_ptr = g;
-        if (L.precond == NEKBONE_PRECOND_JACOBI)
-            if (int rc = nekbone_setup_jacobi(L.P)) return rc;
+    if (g) {
+        const uint64_t fp = fingerprint(g, 6 * L.P->n);
+        if (g != L.g_ptr || fp != L.g_fp) {
+            if (int rc = nekbone_set_geometry(L.P, g)) return rc;
+            L.g_ptr = g;
+            L.g_fp = fp;
+            if (L.precond == NEKBONE_PRECOND_JACOBI)
+                if (int rc = nekbone_setup_jacobi(L.P)) return rc;
+        }
     }
-    if (c && c != L.c_ptr) {
-        if (int rc = set_c_host(L.P, c)) return rc;
-        L.c_ptr = c;
+    if (c) {
+        const uint64_t fp = fingerprint(c, L.P->n);
+        if (c != L.c_ptr || fp != L.c_fp) {
+            if (int rc = set_c_host(L.P, c)) return rc;
+            L.c_ptr = c;
+            L.c_fp = fp;
+        }
     }
     return NEKBONE_OK;
+}
+
+// The reference's residual line, format ('cg:',i4,1p4e12.4)
+// (src/cg.f:43,76); nvals of rnorm, alpha, beta, pap.
+void cg_line(int iter, const double *v, int nvals)
+{
+    std::printf("cg:%4d", iter);
+    for (int q = 0; q < nvals; ++q) std::printf("%12.4E", v[q]);
+    std::printf("\n");
+    std::fflush(stdout);
 }
 
 int bound(const char *where)
@@ -177,14 +228,21 @@ extern "C" void nekbone_setup_(const int *nelt, const int *nx1, const int *nelx,
     if (L.P) nekbone_problem_destroy(L.P);
     L.P = nullptr;
     L.g_ptr = L.c_ptr = nullptr;
+    L.g_fp = L.c_fp = 0;
     int rc = nekbone_problem_create(&L.P, *nelt, *nx1, *nelx, *nely, *nelz, lglel, dxm1,
                                     it->second.glo.data(), nullptr, nullptr);
+    // a new mesh starts with the solveM of the build: identity, or Jacobi
+    // when selected; HSMG again only after h1mg_setup_ for this mesh, as
+    // src/driver.f:46 calls it per mesh
+    L.precond = L.jacobi_on ? NEKBONE_PRECOND_JACOBI : NEKBONE_PRECOND_IDENTITY;
+    if (!rc && L.jacobi_on) rc = nekbone_setup_jacobi(L.P);
     report(rc, "nekbone_setup_");
 }
 
 extern "C" void nekbone_use_jacobi_(const int *on)
 {
     LegacyState &L = S();
+    L.jacobi_on = *on != 0;
     L.precond = (*on) ? NEKBONE_PRECOND_JACOBI : NEKBONE_PRECOND_IDENTITY;
     if (L.P && L.precond == NEKBONE_PRECOND_JACOBI)
         report(nekbone_setup_jacobi(L.P), "nekbone_use_jacobi_");
@@ -198,8 +256,23 @@ extern "C" void cg_(double *x, const double *f, const double *g, const double *c
     int rc = bound("cg_");
     if (!rc && *n != L.P->n) rc = fail(NEKBONE_ERR_ARG, "cg_: n differs from the bound mesh");
     if (!rc) rc = sync_geometry(g, c);
-    if (!rc) rc = nekbone_cg(L.P, x, f, r, *niter, L.precond, nullptr);
+    std::vector<double> hist(size_t(1 + 3 * std::max(*niter, 0)));
+    if (!rc) rc = nekbone_cg(L.P, x, f, r, *niter, L.precond, hist.data());
     if (!rc) {
+        // src/cg.f:43 and :76: the initial residual norm, then after the
+        // loop iter = niter+1 with the last rnorm, alpha, beta and pap
+        const double r0 = std::sqrt(hist[0]);
+        cg_line(0, &r0, 1);
+        const int m = *niter;
+        double v[4] = {r0, 0.0, 0.0, 0.0};
+        if (m >= 1) {
+            const double rtz1 = hist[size_t(3 * m - 2)], pap = hist[size_t(3 * m - 1)];
+            v[0] = std::sqrt(hist[size_t(3 * m)]);
+            v[1] = rtz1 / pap;
+            v[2] = m == 1 ? 0.0 : rtz1 / hist[size_t(3 * m - 5)];
+            v[3] = pap;
+        }
+        cg_line(m + 1, v, 4);
         if (w) rc = download_ref(L.P, w, L.P->d_w);
         if (!rc && p) rc = download_ref(L.P, p, L.P->d_p);
         (void)z;  // z is scratch in the reference; the fused path never stores it
@@ -445,4 +518,18 @@ extern "C" void gs_free_(const int *handle)
     dfree(h.map.d_idx);
     dfree(h.d_u);
     L.gs.erase(it);
+}
+
+// semhat (src/semhat.f:2-37): the 1-D spectral-element matrices of order
+// n, column-major (0:n,0:n): a = Laplacian, b = GLL weights, c = b*d,
+// d = derivative matrix, z = GLL points; w is the reference's work array
+// (0:2n), unused here.  The reference host's proxy_setup (src/prox_setup.f:
+// 2-25) copies d to dxm1, z to zgm1, b to wxm1.
+extern "C" void semhat_(double *a, double *b, double *c, double *d, double *z, double *w,
+                        const int *n)
+{
+    (void)w;
+    if (!n || *n < 1 || *n + 1 > kMaxNx)
+        return report(fail(NEKBONE_ERR_ARG, "semhat_: need 1 <= n <= 15"), "semhat_");
+    semhat(a, b, c, d, z, *n);
 }

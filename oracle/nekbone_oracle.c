@@ -11,6 +11,7 @@
 #include "nekbone_oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -33,6 +34,7 @@ struct orc_problem {
     int32_t *mfull, *mref;
     int mask_mode;
     int mask_built; /* 0: cmask(-1)>=0 (next maskit builds the list) */
+    int exact_dots; /* 1: orc_cg's dot products correctly rounded (orc_glsc3_exact) */
     double *wk;     /* scratch n */
     struct orc_hsmg *mg; /* HSMG preconditioner, NULL until orc_hsmg_setup */
 };
@@ -725,6 +727,57 @@ double orc_glsc3(const double *a, const double *b, const double *m, int64_t n)
     return tmp;
 }
 
+/* Neumaier's compensated sum step (Neumaier 1974, the improved
+ * Kahan-Babuska summation): s + c carries the running sum. */
+static void neumaier_add(double *s, double *c, double t)
+{
+    double u = *s + t;
+    if (fabs(*s) >= fabs(t))
+        *c += (*s - u) + t;
+    else
+        *c += (t - u) + *s;
+    *s = u;
+}
+
+/* The sum of glsc3's rounded terms (a_i b_i) m_i, correctly rounded to
+ * within a unit in the last place (compensated summation; its error is
+ * 2 eps |sum| + n eps^2 sum |t_i|).  Not the reference's value: it takes
+ * the summation order out of the comparison (the reference's sequential
+ * sum of n terms errs by up to n eps sum |t_i|). */
+double orc_glsc3_exact(const double *a, const double *b, const double *m, int64_t n)
+{
+    enum { NT = 256 };
+    double ps[NT] = {0}, pc[NT] = {0};
+    int nt = 1;
+#pragma omp parallel if (n > 1000000)
+    {
+#pragma omp single
+        nt = omp_get_num_threads() < NT ? omp_get_num_threads() : NT;
+        int t = omp_get_thread_num();
+        if (t < nt) {
+            int64_t lo = n * t / nt, hi = n * (t + 1) / nt;
+            double s = 0.0, c = 0.0;
+            for (int64_t i = lo; i < hi; ++i) neumaier_add(&s, &c, a[i] * b[i] * m[i]);
+            ps[t] = s;
+            pc[t] = c;
+        }
+    }
+    double s = 0.0, c = 0.0;
+    for (int t = 0; t < nt; ++t) {
+        neumaier_add(&s, &c, ps[t]);
+        neumaier_add(&s, &c, pc[t]);
+    }
+    return s + c;
+}
+
+void orc_set_exact_dots(orc_problem *P, int on) { P->exact_dots = on ? 1 : 0; }
+
+static double cg_dot(const orc_problem *P, const double *a, const double *b, const double *m,
+                     int64_t n)
+{
+    return P->exact_dots ? orc_glsc3_exact(a, b, m, n) : orc_glsc3(a, b, m, n);
+}
+
 /* ------------------------------------------------------------------ */
 /* Jacobi (not in the reference).  Local diagonal of A_L = sum_ab       */
 /* D_a^T G_ab D_b at point (i,j,k):                                      */
@@ -797,7 +850,7 @@ void orc_cg(orc_problem *P, double *x, const double *f, double *r, int niter, in
         r[i] = f[i];
     }
     orc_maskit(P, r);
-    hist[h++] = orc_glsc3(r, c, r, n);
+    hist[h++] = cg_dot(P, r, c, r, n);
     for (int iter = 1; iter <= niter; ++iter) {
         /* solveM (src/cg.f:86-94): identity copy (src/hsmg_dum.f:9-15)
          * or Jacobi z = r * dinv. */
@@ -811,14 +864,14 @@ void orc_cg(orc_problem *P, double *x, const double *f, double *r, int niter, in
             for (int64_t i = 0; i < n; ++i) z[i] = r[i];
         }
         rtz2 = rtz1;
-        rtz1 = orc_glsc3(r, c, z, n);
+        rtz1 = cg_dot(P, r, c, z, n);
         hist[h++] = rtz1;
         beta = rtz1 / rtz2;
         if (iter == 1) beta = 0.0;
 #pragma omp parallel for schedule(static) if (n > 1000000)
         for (int64_t i = 0; i < n; ++i) p[i] = beta * p[i] + z[i]; /* add2s1 */
         orc_ax(P, w, p);
-        pap = orc_glsc3(w, c, p, n);
+        pap = cg_dot(P, w, c, p, n);
         hist[h++] = pap;
         alpha = rtz1 / pap;
         alphm = -alpha;
@@ -827,7 +880,7 @@ void orc_cg(orc_problem *P, double *x, const double *f, double *r, int niter, in
             x[i] = x[i] + alpha * p[i]; /* add2s2 */
             r[i] = r[i] + alphm * w[i];
         }
-        rtr = orc_glsc3(r, c, r, n);
+        rtr = cg_dot(P, r, c, r, n);
         hist[h++] = rtr;
     }
     free(w);

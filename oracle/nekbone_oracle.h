@@ -65,6 +65,10 @@ void orc_ax_e(const orc_problem *P, int e, double *w, const double *u);
 void orc_ax(orc_problem *P, double *w, const double *u);     /* cg.f:96-125 */
 void orc_maskit(orc_problem *P, double *w);                  /* cg.f:214-258 */
 double orc_glsc3(const double *a, const double *b, const double *m, int64_t n);
+/* the same terms, summed with compensation (correct to an ulp) */
+double orc_glsc3_exact(const double *a, const double *b, const double *m, int64_t n);
+/* 1: orc_cg sums its dot products with orc_glsc3_exact */
+void orc_set_exact_dots(orc_problem *P, int on);
 
 /* Jacobi preconditioner (not in the reference): d = QQ^T diag(A_L) + 0.1,
  * dinv = 1/d.  diag(A_L) evaluated in the fixed order documented in the

@@ -55,6 +55,9 @@ def lib():
         L.orc_ax.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.orc_glsc3.restype = ctypes.c_double
         L.orc_glsc3.argtypes = [_dp, _dp, _dp, ctypes.c_int64]
+        L.orc_glsc3_exact.restype = ctypes.c_double
+        L.orc_glsc3_exact.argtypes = [_dp, _dp, _dp, ctypes.c_int64]
+        L.orc_set_exact_dots.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_setup_jacobi.argtypes = [ctypes.c_void_p]
         L.orc_cg.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp]
         L.orc_rand_uniform.argtypes = [_dp, ctypes.c_int64, ctypes.c_uint64]
@@ -170,6 +173,11 @@ class Problem:
         lib().orc_mask_lists(self._h, full.ctypes.data_as(_i32p), ref.ctypes.data_as(_i32p))
         return full[:nf], ref[:nr]
 
+    def set_exact_dots(self, on):
+        """cg with correctly rounded dot products (compensated sums of the
+        reference's rounded terms) instead of the sequential sums."""
+        lib().orc_set_exact_dots(self._h, int(on))
+
     def set_mask_mode(self, mode):
         lib().orc_set_mask_mode(self._h, mode)
 
@@ -252,3 +260,8 @@ class Problem:
 def glsc3(a, b, m):
     a, b, m = (np.ascontiguousarray(v, dtype=np.float64) for v in (a, b, m))
     return lib().orc_glsc3(_p(a), _p(b), _p(m), a.size)
+
+
+def glsc3_exact(a, b, m):
+    a, b, m = (np.ascontiguousarray(v, dtype=np.float64) for v in (a, b, m))
+    return lib().orc_glsc3_exact(_p(a), _p(b), _p(m), a.size)

@@ -83,3 +83,100 @@ def test_batch_repeat_and_maxiter(nb, oracle):
     ms = B.autotune()
     assert ms > 0
     B.close()
+
+
+def _true_rel_residual(P, f, x):
+    """sqrt(glsc3(r,c,r) / glsc3(f0,c,f0)) for r = mask(f) - A x, with the
+    library's FP64 operator (bit-exact against the reference's ax)."""
+    f0 = P.maskit(f)
+    r = f0 - P.ax(x)
+    c = P.c()
+    return float(np.sqrt(np.sum(r * c * r) / np.sum(f0 * c * f0)))
+
+
+def test_batch_vs_oracle(nb, oracle):
+    """Every problem of a mixed-order batch solved to 1e-8: the true
+    residual of its x, recomputed by the oracle's FP64 operator, meets the
+    tolerance; and for two problems the iterates after a fixed number of
+    iterations (10 and 37) against the oracle's CG (x to 1e-12 of max|x|,
+    the final rtr to 1e-12 of the oracle's history entry)."""
+    _case("batch vs oracle")
+    specs = SPECS
+    probs = _problems(nb, oracle, specs)
+    B = nb.Batch([p for p, _ in probs])
+    tol = 1e-8
+    fs, xs = [], []
+    for P, f in probs:
+        fv, xv = P.vec(), P.vec()
+        fv.upload(f)
+        fs.append(fv), xs.append(xv)
+    its, _ = B.cg_tol_d(xs, fs, [None] * len(probs), tol, 400, nb.PRECOND_JACOBI)
+    Os = []
+    for k, ((E, nx1, deformed, mask), (P, f)) in enumerate(zip(specs, probs)):
+        O = oracle.Problem(*nb.cubic(E), nx1)
+        O.set_mask_mode(mask)
+        if deformed:
+            O.deformed_g(0.1, 10 + k)
+        O.setup_jacobi()
+        Os.append(O)
+        O.reset_mask()
+        x = xs[k].download()
+        f0 = O.maskit(f)
+        O.reset_mask()
+        O.maskit(f)  # the reference mask: every later maskit applies the list
+        r = f0 - O.ax(x)
+        c = O.c()
+        rel = float(np.sqrt(np.sum(r * c * r) / np.sum(f0 * c * f0)))
+        assert rel <= tol * 1.01, (k, rel, its[k])
+    for m in (10, 37):
+        for P, _ in probs:
+            P.reset_mask()
+        its, rtrs = B.cg_tol_d(xs, fs, [None] * len(probs), 1e-30, m, nb.PRECOND_JACOBI,
+                               maxiter_ok=True)
+        for k in (0, 3):
+            O = Os[k]
+            O.reset_mask()
+            xo, _, ho = O.cg(probs[k][1], m, oracle.PRECOND_JACOBI)
+            x = xs[k].download()
+            np.testing.assert_allclose(x, xo, rtol=0, atol=1e-12 * np.abs(xo).max())
+            np.testing.assert_allclose(rtrs[k], ho[-1], rtol=1e-12)
+    B.close()
+
+
+def test_batch_full_size_workload(nb):
+    """The batched workload at its full size: 64 problems, element counts
+    log-uniform in [64, 8192] (box from cubic), orders drawn from
+    {5, 7, 9, 11}, Dirichlet faces, RHS c * dssum(U[-1,1) seed k), Jacobi
+    to 1e-8.  Every problem converges before maxiter, and the true
+    residual of its x (recomputed with the FP64 operator) meets 1e-8."""
+    _case("batch full size: 64 problems")
+    import math
+    u = (splitmix_uniform(128, 77) + 1.0) / 2.0
+    probs, fs, xs = [], [], []
+    for k in range(64):
+        E = int(round(math.exp(math.log(64) + u[2 * k] * (math.log(8192) - math.log(64)))))
+        nx1 = (6, 8, 10, 12)[min(3, int(4 * u[2 * k + 1]))]
+        P = nb.Problem(*nb.cubic(E), nx1)
+        P.set_mask_mode(nb.MASK_DIRICHLET)
+        P.setup_jacobi()
+        f, x = P.vec(), P.vec()
+        P.rand_uniform_d(f, k)
+        P.rhs_from_raw_d(f)
+        probs.append(P), fs.append(f), xs.append(x)
+    npts = sum(P.n for P in probs)
+    print("batch full size: %d local points, E %d..%d" %
+          (npts, min(P.nelt for P in probs), max(P.nelt for P in probs)), flush=True)
+    assert max(P.nelt for P in probs) > 2048 and min(P.nelt for P in probs) < 256
+    B = nb.Batch(probs)
+    tol = 1e-8
+    its, _ = B.cg_tol_d(xs, fs, [None] * 64, tol, 2000, nb.PRECOND_JACOBI)
+    assert all(0 < it < 2000 for it in its)
+    worst = 0.0
+    for P, f, x in zip(probs, fs, xs):
+        worst = max(worst, _true_rel_residual(P, f.download(), x.download()))
+    print("batch full size: iterations %d..%d, worst true relative residual %.3e" %
+          (min(its), max(its), worst), flush=True)
+    assert worst <= tol * 1.01
+    B.close()
+    for P in probs:
+        P.close()

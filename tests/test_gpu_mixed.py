@@ -228,3 +228,57 @@ def test_mixed_refresh_rejects_bad_value(nb):
     P = nb.Problem(2, 2, 2, 6)
     with pytest.raises(nb.NekboneError):
         P.set_mixed_refresh(2)
+
+
+@pytest.mark.parametrize("deformed", [False, True])
+@pytest.mark.parametrize("nel,nx1", [((2, 2, 2), 8), ((3, 2, 2), 10), ((2, 3, 1), 12),
+                                     ((1, 1, 1), 5), ((2, 1, 2), 16)])
+def test_ax_local_g16_vs_fp64_oracle(nb, oracle, nel, nx1, deformed):
+    """FP16 geometric factors (one FP32 scale per element and factor, then
+    each factor rounded to half precision: relative error <= 2^-11 above
+    2^-14 of the largest, absolute 2^-25 of it below), FP32 contractions:
+    within 5e-3 of max|w| of the FP64 oracle, and not equal to the FP32
+    operator (the FP16 factors were read).  A wrong scale index or factor
+    slot gives errors of order max|w|."""
+    _case("ax_local_g16 %s %d deformed=%s" % (nel, nx1, deformed))
+    P, O = _pair(nb, oracle, nel, nx1, deformed, jacobi=False)
+    u32 = splitmix_uniform(P.n, 11).astype(np.float32)
+    w32 = P.ax_local_f32(u32)
+    P.set_mixed_inner(nb.MIXED_G16)
+    w16 = P.ax_local_f32(u32)
+    u = u32.astype(np.float64)
+    n3 = nx1 ** 3
+    w64 = np.concatenate([O.ax_e(e, u[e * n3:(e + 1) * n3]) for e in range(O.nelt)])
+    scale = np.abs(w64).max()
+    err = np.abs(w16.astype(np.float64) - w64).max() / scale
+    print("g16 operator: max err %.2e of max|w|" % err)
+    assert err <= 5e-3
+    assert np.abs(w16 - w32).max() > 1e-7 * scale
+
+
+@pytest.mark.parametrize("precond", [1, 0])
+@pytest.mark.parametrize("nel,nx1", [((4, 3, 2), 8), ((3, 3, 3), 10)])
+def test_cg_mixed_g16_meets_fp64_criterion(nb, oracle, nel, nx1, precond):
+    """The FP16-factor inner operator (the fastest one at 32^3 N=9) on a
+    deformed mesh: the returned x meets the FP64 criterion with r
+    recomputed by the oracle, before maxiter, and the reported final rtr
+    is that residual's."""
+    _case("cg_mixed g16 %s %d precond=%d" % (nel, nx1, precond))
+    tol = 1e-10
+    P, O = _pair(nb, oracle, nel, nx1, True, jacobi=bool(precond))
+    P.set_mixed_inner(nb.MIXED_G16)
+    f = O.rhs_from_raw(splitmix_uniform(O.n, 2))
+    x, r, it, nupd, rtr = P.cg_mixed(f, tol, 5000, precond)
+    print("g16 mixed: %d inner iterations, %d FP64 updates" % (it, nupd), flush=True)
+    assert nupd >= 1 and 0 < it < 5000
+    c = O.c()
+    f0 = O.maskit(f)
+    rtrue = f0 - O.ax(x)
+    rr = float(np.sum(rtrue * c * rtrue))
+    rr0 = float(np.sum(f0 * c * f0))
+    assert rr <= tol ** 2 * rr0 * 1.0001
+    assert abs(rtr - rr) <= 1e-6 * rr
+    xs = _converged_oracle(O, f, precond)
+    err = np.abs(x - xs).max() / np.abs(xs).max()
+    print("g16 mixed: max error vs converged FP64 solution %.2e of max|x|" % err)
+    assert err <= 1e-8

@@ -333,3 +333,22 @@ def test_deformed_operator_annihilates_linear_functions(oracle, nel, nx1):
     Q = oracle.Problem(*nel, nx1)
     Q.set_g(_g_from_jacobian(oracle, Q, X, transpose=True).reshape(-1))
     assert np.abs(rows(Q)[:, inner]).max() > 1e-4 * scale
+
+
+def test_glsc3_exact_is_correctly_rounded(oracle):
+    """orc_glsc3_exact against math.fsum (exactly rounded) of the same
+    rounded terms (a b) m, on sums with heavy cancellation; the sequential
+    glsc3 is measurably off on the same data."""
+    import math
+    rng = np.random.default_rng(5)
+    worst_seq = 0.0
+    for n in (1, 7, 1000, 200003):
+        a = rng.standard_normal(n) * 10.0 ** rng.integers(-8, 8, n)
+        b = rng.standard_normal(n)
+        m = rng.choice([1.0, 0.5, 0.25, 0.125], n)
+        t = (a * b) * m
+        exact = math.fsum(t)
+        got = oracle.glsc3_exact(a, b, m)
+        assert abs(got - exact) <= 2.0 * np.spacing(abs(exact)) + 1e-300
+        worst_seq = max(worst_seq, abs(oracle.glsc3(a, b, m) - exact) / np.sum(np.abs(t)))
+    assert worst_seq > 1e-17
