@@ -36,6 +36,7 @@ program nekbone_host
   real(c_double) :: ah(256), bh(16), ch(256), dh(256), zh(16), wh(32), flop_cg, wt, s
   character(len=16) :: task
   character(len=512) :: fin, fout
+  character(len=1600) :: line
   integer(8) :: t0, t1, rate
 
   read(*, *) nelx, nely, nelz, nx1, niter
@@ -62,8 +63,13 @@ program nekbone_host
   c = 1.0d0 / c
 
   do
-    read(*, *, iostat=ios) task, fin, fout
+    ! whole line, split at blanks (list-directed input would end the
+    ! record at the first '/' of a path)
+    read(*, '(a)', iostat=ios) line
     if (ios /= 0) exit
+    call next_word(line, task)
+    call next_word(line, fin)
+    call next_word(line, fout)
     if (task == 'end') exit
     select case (trim(task))
     case ('cg')
@@ -111,6 +117,19 @@ program nekbone_host
   write(*, '(a)') 'nekbone_host: done'
 
 contains
+
+  ! First blank-separated word of s into w; s loses it.
+  subroutine next_word(s, w)
+    character(len=*), intent(inout) :: s
+    character(len=*), intent(out) :: w
+    integer :: i0, i1
+    s = adjustl(s)
+    i1 = index(s, ' ')
+    if (i1 == 0) i1 = len(s) + 1
+    i0 = 1
+    w = s(i0:i1 - 1)
+    s = s(i1:)
+  end subroutine
 
   subroutine bind_mesh()
     call nekbone_setup(nelt, nx1, nelx, nely, nelz, lglel, dh, gsh)
