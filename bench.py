@@ -76,19 +76,88 @@ def cpu_baseline(nel, nx1, niter_sample):
                     "sample": "flang-compiled reference cg (src/cg.f, identity M^-1, its own "
                               "mask) on the full %dx%dx%d N=%d mesh, %d iterations, %.1f s" %
                               (nel[0], nel[1], nel[2], nx1 - 1, niter_sample, secs)}
+    return cpu_all_cores(nel, nx1, niter_sample)
+
+
+def cpu_reference_ranks(nel, nx1, niter_sample, ranks):
+    """The flang-compiled reference on `ranks` host cores at once: one
+    process per core, each on its slab of the mesh (nelx x nely x nelz/ranks
+    elements, as an MPI decomposition along z gives it), without the halo
+    exchange between slabs (the build has no MPI), all started together.
+    Throughput = total points x iterations / the slowest process's time of
+    the timed cg call, an upper bound for the reference on these cores."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "c1", "refharness")
+    if not os.path.exists(ref) or nel[2] % ranks:
+        return None
+    sub = (nel[0], nel[1], nel[2] // ranks)
+    n = sub[0] * sub[1] * sub[2] * nx1 ** 3
+    tmp = tempfile.mkdtemp(prefix="nb_cpur_")
+    atexit.register(shutil.rmtree, tmp, True)
     from oracle import oracle as O
+    O.rand_uniform(n, 0).tofile(os.path.join(tmp, "f.bin"))
+
+    def pre():
+        resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    procs = []
+    for k in range(ranks):
+        inp = "bench\n%d %d %d %d %d 1\n%s\n%s\n" % (sub[0], sub[1], sub[2], nx1, niter_sample,
+                                                    os.path.join(tmp, "f.bin"),
+                                                    os.path.join(tmp, "o%d" % k))
+        p = subprocess.Popen([ref], stdin=subprocess.PIPE, stdout=subprocess.PIPE,
+                             preexec_fn=pre, text=True, env=env)
+        p.stdin.write(inp)
+        p.stdin.close()
+        procs.append(p)
+    secs = []
+    for p in procs:
+        out = p.stdout.read()
+        rc = p.wait(timeout=900)
+        m = re.search(r"HK cg call\s+1 niter\s+(\d+) n\s+(\d+) seconds\s+(\S+)", out)
+        if rc != 0 or not m:
+            return None
+        secs.append(float(m.group(3)))
+    print("cpu_baseline: %d reference processes, cg seconds %.2f..%.2f" %
+          (ranks, min(secs), max(secs)), file=sys.stderr, flush=True)
+    return {"value": ranks * n * niter_sample / max(secs), "unit": "DOF*iter/s", "cores": ranks,
+            "kind": "reference",
+            "sample": "%d flang-compiled reference processes at once, one per core, each cg "
+                      "(identity M^-1, its own mask) on a %dx%dx%d-element slab of the "
+                      "%dx%dx%d N=%d mesh (no halo exchange), %d iterations, slowest %.1f s" %
+                      (ranks, sub[0], sub[1], sub[2], nel[0], nel[1], nel[2], nx1 - 1,
+                       niter_sample, max(secs))}
+
+
+def host_cores():
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_all_cores(nel, nx1, niter_sample):
+    """The GPU line's own workload on every host core: the oracle's FP64
+    Jacobi CG (Dirichlet faces, the same RHS) with OpenMP over elements in
+    ax and over points in the vector updates, and its dot products summed
+    per thread (compensated) and combined, on a bounded sample of
+    iterations on the full mesh."""
+    from oracle import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     Q = O.Problem(*nel, nx1)
     Q.set_mask_mode(O.MASK_DIRICHLET)
     Q.setup_jacobi()
+    Q.set_exact_dots(1)
     f = Q.rhs_from_raw(O.rand_uniform(Q.n, 0))
+    Q.cg(f, 1, O.PRECOND_JACOBI)  # warm-up: pages touched once
     t0 = time.perf_counter()
     Q.cg(f, niter_sample, O.PRECOND_JACOBI)
     secs = time.perf_counter() - t0
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": Q.n * niter_sample / secs, "unit": "DOF*iter/s", "cores": threads,
             "kind": "port",
-            "sample": "oracle Jacobi CG on the full mesh, %d iterations, %.1f s" %
-                      (niter_sample, secs)}
+            "sample": "oracle FP64 Jacobi CG (the GPU line's workload) on the full %dx%dx%d N=%d "
+                      "mesh, OpenMP on %d threads, %d iterations, %.1f s" %
+                      (nel[0], nel[1], nel[2], nx1 - 1, threads, niter_sample, secs)}
 
 
 def Q_rtr0(Q, fq):
@@ -439,6 +508,13 @@ def main():
     if not args.no_cpu:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
         cpu = cpu_baseline(nel, nx1, args.cpu_iters)
+        if cpu.get("kind") == "reference":
+            print("bench: cpu baseline, all cores", file=sys.stderr, flush=True)
+            ranks = min(int(os.environ.get("OMP_NUM_THREADS", host_cores())), host_cores())
+            while ranks > 1 and nel[2] % ranks:
+                ranks -= 1
+            allc = cpu_reference_ranks(nel, nx1, args.cpu_iters, ranks) if ranks > 1 else None
+            extra["cpu_baseline_all_cores"] = allc or cpu_all_cores(nel, nx1, args.cpu_iters)
 
     rec = {
         "metric": "CG DOF*iter/s + time to 1e-10 rel. residual, mixed vs FP64, N=9, 32^3 elems",
