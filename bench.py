@@ -208,6 +208,7 @@ def batched_workload(nb, count, seed=2024, graph=True):
     for P in probs:
         P.set_graph(graph)
     B.autotune(nb.PRECOND_JACOBI)
+    path = B.path()
     tol, maxit = 1e-8, 5000
     B.cg_tol_d(xs, fs, [None] * count, tol, maxit)
     nb.sync()
@@ -230,6 +231,8 @@ def batched_workload(nb, count, seed=2024, graph=True):
     return {"problems": count, "local_points": npts, "tol": tol,
             "orders": sorted(set(P.nx1 - 1 for P in probs)),
             "iters_min": min(its), "iters_max": max(its), "same_iters_as_single": its == its1,
+            "element_kernel_path": ("work queue, tile set %d" % path[0]) if path[0] >= 0
+            else "one launch per order group", "x_update_in_element_kernel": bool(path[1]),
             "batched_ms": tb * 1e3, "one_at_a_time_ms": ts * 1e3, "speedup": ts / tb,
             "DOF_iter_per_s": work / tb}
 

@@ -128,6 +128,12 @@ int nekbone_set_graph(nekbone_problem *P, int on);
  * arithmetic in this process; *best_ms = its time per iteration. */
 int nekbone_autotune(nekbone_problem *P, int precond, double *best_ms);
 int nekbone_get_variant(const nekbone_problem *P, int *epb, int *pf, int *xu);
+/* The autotuner's choices on disk (JSON, "nekbone_amd tuning 1"): save
+ * writes every shape found in this process; load takes them back (shapes
+ * not compiled for their order are skipped), so problems created later
+ * start with them without tuning again.  *loaded may be NULL. */
+int nekbone_tuning_save(const char *path);
+int nekbone_tuning_load(const char *path, int *loaded);
 int nekbone_set_variant(nekbone_problem *P, int epb, int pf, int xu);
 
 /* Jacobi preconditioner (the reference has none): dinv = 1/(QQ^T
@@ -275,6 +281,16 @@ int nekbone_batch_set_xu(nekbone_batch *B, int xu);
  * and replayed; 0: every kernel is launched directly (for tracing tools). */
 int nekbone_batch_set_graph(nekbone_batch *B, int on);
 int nekbone_batch_autotune(nekbone_batch *B, int precond, double *best_ms);
+/* Element-kernel path: tileset = -1 launches the element kernel once per
+ * order group; 0 .. ntilesets-1 runs the problems of orders 6, 8, 10, 12
+ * (N = 5, 7, 9, 11) in one persistent launch of the work-queue kernel over
+ * (problem, element tile) items with that compiled tile set (elements per
+ * workgroup per order), the other orders per group.  The default is -1
+ * (measured faster on the 64-problem workload); nekbone_batch_autotune
+ * picks the fastest path and shapes.  Every problem's values are those of its single
+ * solve with the same shape (the batch sets it on the problem). */
+int nekbone_batch_set_queue(nekbone_batch *B, int tileset);
+int nekbone_batch_get_path(const nekbone_batch *B, int *tileset, int *xu, int *ntilesets);
 /* CG to a relative residual for every problem k: x[k], f[k], r[k] as for
  * nekbone_cg_tol_d (x, r may be NULL); stops when every problem has met
  * tol or after maxiter iterations.  iters[k], final_rtr[k] may be NULL.

@@ -7,12 +7,12 @@ from .core import (ARITH_EXACT, ARITH_FMA, MIXED_F16_MFMA, MIXED_F32, MIXED_G16,
                    KCLASS_GS, KCLASS_POST, KCLASS_REDUCE, KCLASS_UPDATE, LIB_PATH, MASK_DIRICHLET,
                    MASK_REF, NOT_CONVERGED, PRECOND_HSMG, PRECOND_IDENTITY, PRECOND_JACOBI, Batch,
                    DeviceVec, NekboneError, NotConverged, Problem, bw_probe, cubic, init, lib,
-                   set_f_raw, sync)
+                   set_f_raw, sync, tuning_load, tuning_save)
 
 __all__ = [
     "ARITH_EXACT", "ARITH_FMA", "MIXED_F16_MFMA", "MIXED_F32", "MIXED_G16", "KCLASS_AX",
     "KCLASS_GS", "KCLASS_POST", "KCLASS_REDUCE", "KCLASS_UPDATE", "LIB_PATH", "MASK_DIRICHLET",
     "MASK_REF", "NOT_CONVERGED", "PRECOND_HSMG", "PRECOND_IDENTITY", "PRECOND_JACOBI", "Batch",
     "DeviceVec", "NekboneError", "NotConverged", "Problem", "bw_probe", "cubic", "init", "lib",
-    "set_f_raw", "sync",
+    "set_f_raw", "sync", "tuning_load", "tuning_save",
 ]

@@ -25,7 +25,7 @@ SOURCES = ["nb_ax_i10.hip", "nb_ax_i12.hip", "nb_ax_i8.hip", "nb_ax_i6.hip", "nb
            "nb_ax_ihigh.hip", "nb_setup.cpp", "nb_ax.hip", "nb_kernels.hip", "nb_cg.hip",
            "nb_api.hip",
            "nb_legacy.hip",
-           "nb_ax32_low.hip", "nb_ax32_high.hip", "nb_ax32_mf.hip", "nb_ax_ib.hip", "nb_mixed.hip", "nb_batch.hip", "nb_hsmg.hip", "nb_hsmg_setup.cpp", "nb_hsmg_coarse.hip",
+           "nb_ax32_low.hip", "nb_ax32_high.hip", "nb_ax32_mf.hip", "nb_ax_ib.hip", "nb_ax_iq.hip", "nb_mixed.hip", "nb_batch.hip", "nb_hsmg.hip", "nb_hsmg_setup.cpp", "nb_hsmg_coarse.hip",
            "nb_noalloc.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

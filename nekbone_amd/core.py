@@ -106,10 +106,14 @@ _SIGS = {
     "nekbone_batch_size": (ctypes.c_int, [_vp]),
     "nekbone_batch_set_xu": (ctypes.c_int, [_vp, ctypes.c_int]),
     "nekbone_batch_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "nekbone_batch_set_queue": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "nekbone_batch_get_path": (ctypes.c_int, [_vp] + [ctypes.POINTER(ctypes.c_int)] * 3),
     "nekbone_batch_cg_tol_d": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                               ctypes.POINTER(_vp), ctypes.c_double, ctypes.c_int,
                                               ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp]),
     "nekbone_glsc3_noalloc_d": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int64, _vp]),
+    "nekbone_tuning_save": (ctypes.c_int, [ctypes.c_char_p]),
+    "nekbone_tuning_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]),
     "glsc3_": (ctypes.c_double, [_dp, _dp, _dp, ctypes.POINTER(ctypes.c_int)]),
 }
 
@@ -183,6 +187,18 @@ def glsc3_noalloc_d(a, b, mult, n, result):
     _chk(lib().nekbone_glsc3_noalloc_d(ptr(a), ptr(b), ptr(mult), int(n), ptr(result)))
 
 
+def tuning_save(path):
+    """Write the element-kernel shapes the autotuner found (JSON)."""
+    _chk(lib().nekbone_tuning_save(os.fsencode(path)))
+
+
+def tuning_load(path):
+    """Read a tuning file; returns the number of entries taken."""
+    n = ctypes.c_int()
+    _chk(lib().nekbone_tuning_load(os.fsencode(path), ctypes.byref(n)))
+    return n.value
+
+
 def cubic(np_):
     """Box factorization of np_ elements (cubic, src/driver.f:196-247)."""
     m = [ctypes.c_int() for _ in range(3)]
@@ -216,6 +232,18 @@ class Batch:
 
     def set_graph(self, on):
         _chk(lib().nekbone_batch_set_graph(self.handle, int(bool(on))))
+
+    def set_queue(self, tileset):
+        """-1: one element-kernel launch per order group; k >= 0: the
+        work-queue kernel with compiled tile set k."""
+        _chk(lib().nekbone_batch_set_queue(self.handle, int(tileset)))
+
+    def path(self):
+        """(tile set or -1, x update in the element kernel, number of tile
+        sets compiled)."""
+        v = [ctypes.c_int() for _ in range(3)]
+        _chk(lib().nekbone_batch_get_path(self.handle, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
 
     def autotune(self, precond=None):
         precond = PRECOND_JACOBI if precond is None else precond

@@ -828,6 +828,62 @@ extern "C" int nekbone_autotune(nekbone_problem *P, int precond, double *best_ms
     return NEKBONE_OK;
 }
 
+// Tuning cache on disk: the element-kernel shapes nekbone_autotune found,
+// as JSON, one entry per (order, preconditioner, arithmetic); -1/-1 is the
+// shape later problems of that order start with.
+extern "C" int nekbone_tuning_save(const char *path)
+{
+    if (!path) return fail(NEKBONE_ERR_ARG, "tuning_save: null path");
+    FILE *fp = std::fopen(path, "w");
+    if (!fp) return fail(NEKBONE_ERR_ARG, std::string("tuning_save: cannot write ") + path);
+    std::fprintf(fp, "{\"format\": \"nekbone_amd tuning 1\", \"element_kernel\": [");
+    bool first = true;
+    for (const auto &kv : tuned_cache()) {
+        int nx1, precond, arith;
+        std::tie(nx1, precond, arith) = kv.first;
+        std::fprintf(fp, "%s\n  {\"nx1\": %d, \"precond\": %d, \"arith\": %d, \"epb\": %d, "
+                         "\"pf\": %d, \"xu\": %d}",
+                     first ? "" : ",", nx1, precond, arith, kv.second.epb, kv.second.pf,
+                     kv.second.xu);
+        first = false;
+    }
+    std::fprintf(fp, "\n]}\n");
+    const bool ok = std::fclose(fp) == 0;
+    return ok ? NEKBONE_OK : fail(NEKBONE_ERR_ARG, "tuning_save: write failed");
+}
+
+// Reads what nekbone_tuning_save wrote; shapes not compiled for their order
+// are skipped.  *loaded: the number of entries taken.
+extern "C" int nekbone_tuning_load(const char *path, int *loaded)
+{
+    if (!path) return fail(NEKBONE_ERR_ARG, "tuning_load: null path");
+    FILE *fp = std::fopen(path, "r");
+    if (!fp) return fail(NEKBONE_ERR_ARG, std::string("tuning_load: cannot read ") + path);
+    std::string text;
+    char buf[4096];
+    for (size_t got; (got = std::fread(buf, 1, sizeof buf, fp)) > 0;) text.append(buf, got);
+    std::fclose(fp);
+    if (text.find("\"nekbone_amd tuning 1\"") == std::string::npos)
+        return fail(NEKBONE_ERR_ARG, "tuning_load: not a tuning file");
+    int count = 0;
+    for (size_t at = text.find('{', 1); at != std::string::npos; at = text.find('{', at + 1)) {
+        int nx1, precond, arith, epb, pf, xu;
+        if (std::sscanf(text.c_str() + at,
+                        "{\"nx1\": %d, \"precond\": %d, \"arith\": %d, \"epb\": %d, \"pf\": %d, "
+                        "\"xu\": %d}",
+                        &nx1, &precond, &arith, &epb, &pf, &xu) != 6)
+            continue;
+        bool compiled = false;
+        for (const AxVariant &v : ax_variants(nx1))
+            compiled = compiled || (v.epb == epb && v.pf == pf && v.xu == xu);
+        if (!compiled) continue;
+        tuned_cache()[std::make_tuple(nx1, precond, arith)] = AxVariant{epb, pf, xu};
+        ++count;
+    }
+    if (loaded) *loaded = count;
+    return NEKBONE_OK;
+}
+
 extern "C" int nekbone_set_graph(nekbone_problem *P, int on)
 {
     if (int rc = check(P)) return rc;

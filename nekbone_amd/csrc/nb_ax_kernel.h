@@ -56,6 +56,7 @@
 
 #include <map>
 #include <tuple>
+#include <vector>
 
 #include <hip/hip_fp16.h>
 
@@ -223,17 +224,21 @@ struct AxArgsT {
 using AxArgs = AxArgsT<double>;
 
 // Body of the element kernel for workgroup blk of one problem (shared by
-// the single-problem kernel and the batched one).
+// the single-problem kernel, the batched ones and the work-queue kernel).
+// sA, sB: two LDS planes of EPB*NX^3 values; NTB: threads of the
+// workgroup (0: round64(NX^2*EPB), the kernel's own shape; more when a
+// work-queue kernel runs several orders with one block size; the extra
+// threads own no line).
 template <typename T, int NX, int EPB, bool FMA, int MODE, int PF, bool XU, bool G16 = false,
-          bool MF = false>
-__device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
+          bool MF = false, int NTB = 0>
+__device__ __forceinline__ void ax_core(const AxArgsT<T> &A_, const int blk, T *__restrict__ sA,
+                                        T *__restrict__ sB)
 {
     constexpr int N2 = NX * NX, N3 = NX * NX * NX;
-    constexpr int NT = round64(N2 * EPB);
-    static_assert(!MF || (EPB == 1 && G16 && sizeof(T) == 4 && NX <= 16),
+    constexpr int NT = NTB ? NTB : round64(N2 * EPB);
+    static_assert(NT >= N2 * EPB && NT % 64 == 0, "workgroup too small for its elements");
+    static_assert(!MF || (EPB == 1 && G16 && sizeof(T) == 4 && NX <= 16 && NTB == 0),
                   "FP16 matrix-core variant: FP32 storage, FP16 factors, one element per group");
-    __shared__ T sA[EPB * N3];
-    __shared__ T sB[EPB * N3];
     __shared__ T sC[MF ? N3 : 1];          // MF: us, then the accumulated w
     __shared__ _Float16 sH[MF ? N3 : 1];   // MF: FP16 input, then FP16 wt
 
@@ -578,6 +583,16 @@ __device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
 
 template <typename T, int NX, int EPB, bool FMA, int MODE, int PF, bool XU, bool G16 = false,
           bool MF = false>
+__device__ __forceinline__ void ax_body(const AxArgsT<T> &A_, const int blk)
+{
+    constexpr int N3 = NX * NX * NX;
+    __shared__ T sA[EPB * N3];
+    __shared__ T sB[EPB * N3];
+    ax_core<T, NX, EPB, FMA, MODE, PF, XU, G16, MF>(A_, blk, sA, sB);
+}
+
+template <typename T, int NX, int EPB, bool FMA, int MODE, int PF, bool XU, bool G16 = false,
+          bool MF = false>
 __global__ void __launch_bounds__(round64(NX *NX *EPB))
     __attribute__((amdgpu_waves_per_eu(PF == 2 ? (sizeof(T) == 4 ? 8 : 5) : 1, 8)))
     k_ax(AxArgsT<T> A_)
@@ -605,6 +620,68 @@ __global__ void __launch_bounds__(round64(NX *NX *EPB))
     const AxArgsT<T> A = args[lo];
     ax_body<T, NX, EPB, FMA, MODE, PF, XU>(A, b - start[lo]);
 }
+
+// Work-queue form of the batched element kernel: one launch for the
+// problems of several orders.  Item i of the queue is (problem, workgroup
+// of that problem's own launch); workgroup b takes items b, b + gridDim.x,
+// b + 2 gridDim.x, ... (persistent), and the queue lists the highest
+// orders first so that the longest items start first.  Each item runs the
+// body of the problem's own kernel shape (E6..E12 elements per workgroup
+// at orders 6, 8, 10, 12: the batched workload's), so every value is the
+// one the problem's single solve gives with that shape.  NTB threads per
+// workgroup (enough for every order's lines; the rest own none); the two
+// LDS planes come from the dynamic allocation, 2 * max(E * NX^3) doubles.
+// One kernel carries the register allocation of its largest body (161-233
+// VGPRs uncapped against 96-128 for one order alone); capped at 128 (4
+// waves per SIMD) it spills 25-33 registers and ran fastest of the caps
+// measured (3 waves per SIMD and uncapped were slower).
+struct QItem {
+    int prob, blk;
+};
+
+template <int NTB, bool FMA, bool XU, int E6, int E8, int E10, int E12>
+__global__ void __launch_bounds__(NTB) __attribute__((amdgpu_waves_per_eu(4, 8)))
+    k_ax_queue(const AxArgsT<double> *__restrict__ args, const int *__restrict__ pnx,
+               const QItem *__restrict__ items, int nitems)
+{
+    extern __shared__ double qlds[];
+    for (int i = blockIdx.x; i < nitems; i += gridDim.x) {
+        const QItem q = items[i];
+        const AxArgsT<double> A = args[q.prob];
+        switch (pnx[q.prob]) {
+        case 6:
+            ax_core<double, 6, E6, FMA, 1, 0, XU, false, false, NTB>(A, q.blk, qlds,
+                                                                     qlds + E6 * 216);
+            break;
+        case 8:
+            ax_core<double, 8, E8, FMA, 1, 0, XU, false, false, NTB>(A, q.blk, qlds,
+                                                                     qlds + E8 * 512);
+            break;
+        case 10:
+            ax_core<double, 10, E10, FMA, 1, 0, XU, false, false, NTB>(A, q.blk, qlds,
+                                                                       qlds + E10 * 1000);
+            break;
+        case 12:
+            ax_core<double, 12, E12, FMA, 1, 0, XU, false, false, NTB>(A, q.blk, qlds,
+                                                                       qlds + E12 * 1728);
+            break;
+        default:
+            break;
+        }
+        __syncthreads();  // the next item writes the LDS planes again
+    }
+}
+
+using AxFnQ = void (*)(const AxArgsT<double> *, const int *, const QItem *, int);
+// A compiled tile set of the work-queue kernel: threads per workgroup and
+// elements per workgroup at orders 6, 8, 10, 12; fn[fma][xu].
+struct QShape {
+    int ntb, e[4];
+    AxFnQ fn[2][2];
+};
+constexpr int kQOrders[4] = {6, 8, 10, 12};
+// Defined in nb_ax_iq.hip.
+void regq_all(std::vector<QShape> &t);
 
 // Default elements per workgroup for each NX: >= 2 waves per group and
 // LDS well inside 160 KB.
@@ -673,6 +750,9 @@ void regb_order(TableB &t)
 {
     constexpr int E = epb_for(NX);
     regb_modes<NX, E, 0>(t);
+    // the batched workload's orders also get the doubled shape (the
+    // single-problem table has it too), for the batch autotuner
+    if constexpr (NX == 6 || NX == 8 || NX == 10 || NX == 12) regb_modes<NX, 2 * E, 0>(t);
 }
 
 // FP32 kernels (inner iterations of the mixed-precision CG), always with

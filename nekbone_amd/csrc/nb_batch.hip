@@ -1,13 +1,20 @@
 // Batched CG: many independent problems of different sizes and orders
 // solved together.  The reference's driver runs one mesh at a time
-// (src/driver.f:28-79); here every iteration of every problem in the batch
-// is one launch per kernel per order group (element kernel, gather-scatter)
-// plus one launch for all problems (pap/alpha, r update with rtr and rtz,
-// rtr/beta).  Each problem keeps its own workgroups, partial sums and
-// device scalars, so every value is the one its single-problem solve
-// produces.  Chunks of iterations are captured once into a HIP graph and
-// replayed; a problem that has converged keeps its state (its kernels
-// return at once) until the whole batch is done.
+// (src/driver.f:28-79); here an iteration of the whole batch is
+//   * the element kernel: one persistent launch over a device queue of
+//     (problem, element tile) items for the orders 6, 8, 10, 12 (the work-
+//     queue kernel, k_ax_queue), or one launch per order group; other
+//     orders always launch per group;
+//   * one gather-scatter launch for all problems;
+//   * one launch each for all problems of pap/alpha, the vector update with
+//     rtr and rtz, and rtr/beta.
+// The autotuner picks the element-kernel path, the tile shapes per order
+// and the x-update placement.  Each problem keeps its own workgroups,
+// partial sums and device scalars, so every value is the one its single-
+// problem solve produces with the same shape.  Chunks of iterations are
+// captured once into a HIP graph and replayed; a problem that has
+// converged keeps its state (its kernels return at once) until the whole
+// batch is done.
 #include <algorithm>
 #include <cmath>
 #include <map>
@@ -21,11 +28,9 @@ struct nekbone_batch {
         nb::AxVariant v;
         std::vector<int> members;
         nb::axk::AxFnB fn = nullptr;
-        int nblk_ax = 0, nblk_gs = 0, threads = 0;
+        int nblk_ax = 0, threads = 0;
         nb::axk::AxArgs *d_ax = nullptr;
         int *d_axstart = nullptr;
-        nb::GsCgArgs *d_gs = nullptr;
-        int *d_gsstart = nullptr;
     };
     std::vector<nekbone_problem *> P;
     std::vector<Group> groups;
@@ -34,9 +39,22 @@ struct nekbone_batch {
     nb::CgBatchArgs *d_args = nullptr;
     int *d_ustart = nullptr;
     int nupd_total = 0;
-    // captured chunk of iterations
+    // work-queue element kernel: index of its compiled tile set, -1 for
+    // per-group launches
+    int queue = -1;
+    nb::axk::AxArgs *d_qargs = nullptr;
+    int *d_qnx = nullptr;
+    nb::axk::QItem *d_qitems = nullptr;
+    int nqitems = 0, qgrid = 0, qprobs = 0, qitems_cap = 0;
+    size_t qlds = 0;
+    // gather-scatter of every problem in one launch
+    nb::GsCgArgs *d_gsall = nullptr;
+    int *d_gsstart = nullptr;
+    int nblk_gs = 0;
+    // captured chunk of iterations; shapes counts changes of the shapes
     hipGraphExec_t exec = nullptr;
-    int exec_precond = -1, exec_chunk = 0, exec_xu = -1;
+    int exec_precond = -1, exec_chunk = 0, exec_shapes = -1;
+    int shapes = 0;
     int use_graph = 1;  // replay chunks of iterations from a captured graph
 };
 
@@ -54,6 +72,38 @@ axk::TableB &tableb()
     return t;
 }
 
+const std::vector<axk::QShape> &tableq()
+{
+    static std::vector<axk::QShape> t;
+    static bool done = false;
+    if (!done) {
+        done = true;
+        axk::regq_all(t);
+    }
+    return t;
+}
+
+// slot of an order in the work-queue kernel's tile set, -1 if it has none
+int qslot(int nx1)
+{
+    for (int i = 0; i < 4; ++i)
+        if (axk::kQOrders[i] == nx1) return i;
+    return -1;
+}
+
+bool in_queue(const nekbone_batch *B, int nx1) { return B->queue >= 0 && qslot(nx1) >= 0; }
+
+// Elements per workgroup of every member as the current path runs it; the
+// problems carry the same shape, so a single solve gives the same values.
+void apply_shapes(nekbone_batch *B)
+{
+    for (auto &G : B->groups) {
+        const int epb = in_queue(B, G.nx1) ? tableq()[size_t(B->queue)].e[qslot(G.nx1)] : G.v.epb;
+        for (int k : G.members) B->P[size_t(k)]->axv = AxVariant{epb, 0, B->xu};
+    }
+    ++B->shapes;
+}
+
 // Argument tables keep their device address for the life of the batch (a
 // captured graph refers to it); their sizes depend only on the membership,
 // which is fixed at creation, so later solves rewrite them in place.
@@ -69,63 +119,135 @@ int upload(T **d, const std::vector<T> &h)
 
 void drop_graph(nekbone_batch *B)
 {
-    if (B->exec) (void)hipGraphExecDestroy(B->exec);
+    if (B->exec) {
+        // no replay of it may still be running
+        (void)hipStreamSynchronize(stream());
+        (void)hipGraphExecDestroy(B->exec);
+    }
     B->exec = nullptr;
     B->exec_precond = -1;
 }
 
-// Argument tables of one solve: per group the element-kernel and
-// gather-scatter arguments of its members, per problem the scalar and
-// update arguments.
+// Element-kernel arguments of problem k for the CG mode.
+axk::AxArgs ax_args(nekbone_batch *B, int k, int mbit)
+{
+    nekbone_problem *P = B->P[size_t(k)];
+    axk::AxArgs A{};
+    A.w = P->d_w;
+    A.p = P->d_p;
+    A.r = P->d_r;
+    A.x = P->d_x;
+    A.g = P->d_g;
+    A.gcs = P->n;
+    A.Drow = P->d_Drow;
+    A.Dcol = P->d_D;
+    A.flag = P->d_flag;
+    A.iref = P->d_iref;
+    A.mbit = mbit;
+    A.scal = P->d_scal;
+    A.part = P->d_part;
+    A.nelt = P->nelt;
+    return A;
+}
+
+int device_cus()
+{
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+            return 0;
+        cus = prop.multiProcessorCount;
+    }
+    return cus;
+}
+
+// Argument tables of one solve: the element-kernel arguments of the work
+// queue and of the per-group launches, the gather-scatter arguments of
+// every problem, and per problem the scalar and update arguments.  The
+// members' shapes (apply_shapes) must be current.
 int prepare(nekbone_batch *B, int precond, const std::vector<int> &mbit)
 {
+    (void)precond;
     // Jacobi runs in z form (see cg_zform): the element kernel's identity
     // mode reads z
     const int mode = 1;
     for (auto &G : B->groups) {
         G.fn = nullptr;
+        if (in_queue(B, G.nx1)) continue;
         auto it = tableb().find(axk::Key(G.nx1, G.v.epb, B->fma ? 1 : 0, mode, G.v.pf, B->xu));
         if (it == tableb().end())
             return fail(NEKBONE_ERR_UNSUPPORTED, "batch: no batched kernel for this order");
         G.fn = it->second;
         std::vector<axk::AxArgs> ax;
-        std::vector<GsCgArgs> gs;
-        std::vector<int> axs{0}, gss{0};
+        std::vector<int> axs{0};
         for (int k : G.members) {
-            nekbone_problem *P = B->P[size_t(k)];
-            axk::AxArgs A{};
-            A.w = P->d_w;
-            A.p = P->d_p;
-            A.r = P->d_r;
-            A.x = P->d_x;
-            A.g = P->d_g;
-            A.gcs = P->n;
-            A.Drow = P->d_Drow;
-            A.Dcol = P->d_D;
-            A.flag = P->d_flag;
-            A.iref = P->d_iref;
-            A.mbit = mbit[size_t(k)];
-            A.scal = P->d_scal;
-            A.part = P->d_part;
-            A.nelt = P->nelt;
-            ax.push_back(A);
-            axs.push_back(axs.back() + cg_ax_blocks(P));
-            gs.push_back(gs_cg_args(P, P->d_w, P->d_p, mbit[size_t(k)],
-                                    P->d_part + cg_ax_blocks(P)));
-            if (cg_fold_partials(P)) {  // as in the single-problem iteration
-                gs.back().G.pre = P->d_part;
-                gs.back().G.npre = cg_ax_blocks(P);
-            }
-            gss.push_back(gss.back() + cg_gs_blocks(P));
+            ax.push_back(ax_args(B, k, mbit[size_t(k)]));
+            axs.push_back(axs.back() + cg_ax_blocks(B->P[size_t(k)]));
         }
         G.nblk_ax = axs.back();
-        G.nblk_gs = gss.back();
         G.threads = axk::round64(G.nx1 * G.nx1 * G.v.epb);
         if (int rc = upload(&G.d_ax, ax)) return rc;
         if (int rc = upload(&G.d_axstart, axs)) return rc;
-        if (int rc = upload(&G.d_gs, gs)) return rc;
-        if (int rc = upload(&G.d_gsstart, gss)) return rc;
     }
+    B->nqitems = 0;
+    if (B->queue >= 0) {
+        const axk::QShape &Q = tableq()[size_t(B->queue)];
+        std::vector<axk::AxArgs> qa;
+        std::vector<int> qnx;
+        std::vector<axk::QItem> items;
+        size_t lds = 0;
+        for (int slot = 3; slot >= 0; --slot)  // highest order first
+            for (auto &G : B->groups) {
+                if (G.nx1 != axk::kQOrders[slot]) continue;
+                const size_t n3 = size_t(G.nx1) * G.nx1 * G.nx1;
+                lds = std::max(lds, 2 * size_t(Q.e[slot]) * n3 * sizeof(double));
+                for (int k : G.members) {
+                    const int q = int(qa.size());
+                    qa.push_back(ax_args(B, k, mbit[size_t(k)]));
+                    qnx.push_back(G.nx1);
+                    for (int b = 0; b < cg_ax_blocks(B->P[size_t(k)]); ++b)
+                        items.push_back(axk::QItem{q, b});
+                }
+            }
+        // the argument tables keep their size for the life of the batch
+        // (the queue's membership is fixed); the item list is reallocated
+        // when a tile set changes the number of workgroups (the captured
+        // graph is re-captured then, see apply_shapes)
+        if (B->d_qitems && int(items.size()) != B->qitems_cap) {
+            dfree(B->d_qitems);
+            B->qitems_cap = 0;
+        }
+        if (int rc = upload(&B->d_qargs, qa)) return rc;
+        if (int rc = upload(&B->d_qnx, qnx)) return rc;
+        if (int rc = upload(&B->d_qitems, items)) return rc;
+        B->qitems_cap = int(items.size());
+        B->qprobs = int(qa.size());
+        B->nqitems = int(items.size());
+        B->qlds = lds;
+        int per_cu = 0;
+        NB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void *>(Q.fn[B->fma ? 1 : 0][B->xu]), Q.ntb, lds));
+        const int cus = device_cus();
+        if (per_cu < 1 || cus < 1)
+            return fail(NEKBONE_ERR_UNSUPPORTED, "batch: work-queue kernel does not fit a CU");
+        B->qgrid = std::max(1, std::min(B->nqitems, per_cu * cus));
+    }
+    std::vector<GsCgArgs> gs;
+    std::vector<int> gss{0};
+    for (size_t k = 0; k < B->P.size(); ++k) {
+        nekbone_problem *P = B->P[k];
+        gs.push_back(gs_cg_args(P, P->d_w, P->d_p, mbit[k], P->d_part + cg_ax_blocks(P)));
+        if (cg_fold_partials(P)) {  // as in the single-problem iteration
+            gs.back().G.pre = P->d_part;
+            gs.back().G.npre = cg_ax_blocks(P);
+        }
+        gss.push_back(gss.back() + cg_gs_blocks(P));
+    }
+    B->nblk_gs = gss.back();
+    if (int rc = upload(&B->d_gsall, gs)) return rc;
+    if (int rc = upload(&B->d_gsstart, gss)) return rc;
     std::vector<CgBatchArgs> args;
     std::vector<int> us{0};
     for (nekbone_problem *P : B->P) {
@@ -140,21 +262,30 @@ int prepare(nekbone_batch *B, int precond, const std::vector<int> &mbit)
 int launch_iteration(nekbone_batch *B, int precond)
 {
     const int np = int(B->P.size());
+    if (B->queue >= 0 && B->nqitems > 0) {
+        const axk::QShape &Q = tableq()[size_t(B->queue)];
+        hipLaunchKernelGGL(Q.fn[B->fma ? 1 : 0][B->xu], dim3(B->qgrid), dim3(Q.ntb), B->qlds,
+                           stream(), static_cast<const axk::AxArgs *>(B->d_qargs),
+                           static_cast<const int *>(B->d_qnx),
+                           static_cast<const axk::QItem *>(B->d_qitems), B->nqitems);
+        NB_HIP(hipGetLastError());
+    }
     for (auto &G : B->groups) {
+        if (!G.fn) continue;
         hipLaunchKernelGGL(G.fn, dim3(G.nblk_ax), dim3(G.threads), 0, stream(),
                            static_cast<const axk::AxArgs *>(G.d_ax),
                            static_cast<const int *>(G.d_axstart), int(G.members.size()));
         NB_HIP(hipGetLastError());
-        if (int rc = launch_gs_cg_batch(G.d_gs, G.d_gsstart, int(G.members.size()), G.nblk_gs))
-            return rc;
     }
+    if (int rc = launch_gs_cg_batch(B->d_gsall, B->d_gsstart, np, B->nblk_gs)) return rc;
     return launch_cg_batch_tail(B->d_args, B->d_ustart, np, B->nupd_total, precond, B->fma,
                                 !B->xu, B->xu);
 }
 
 int ensure_graph(nekbone_batch *B, int precond, int chunk)
 {
-    if (B->exec && B->exec_precond == precond && B->exec_chunk == chunk && B->exec_xu == B->xu)
+    if (B->exec && B->exec_precond == precond && B->exec_chunk == chunk &&
+        B->exec_shapes == B->shapes)
         return NEKBONE_OK;
     drop_graph(B);
     hipGraph_t g = nullptr;
@@ -172,7 +303,7 @@ int ensure_graph(nekbone_batch *B, int precond, int chunk)
     NB_HIP(e);
     B->exec_precond = precond;
     B->exec_chunk = chunk;
-    B->exec_xu = B->xu;
+    B->exec_shapes = B->shapes;
     return NEKBONE_OK;
 }
 
@@ -259,9 +390,10 @@ extern "C" int nekbone_batch_create(nekbone_batch **out, nekbone_problem *const 
         }
         B->groups[size_t(it->second)].members.push_back(k);
     }
-    // every member runs the group's shape, in the batch and alone
-    for (auto &G : B->groups)
-        for (int k : G.members) B->P[size_t(k)]->axv = AxVariant{G.v.epb, G.v.pf, B->xu};
+    // per-group launches at the orders' default shapes (measured faster
+    // than the work queue on the batched workload; the autotuner compares
+    // both); every member runs the path's shape, in the batch and alone
+    apply_shapes(B);
     *out = B;
     return NEKBONE_OK;
 }
@@ -274,9 +406,12 @@ extern "C" int nekbone_batch_destroy(nekbone_batch *B)
     for (auto &G : B->groups) {
         dfree(G.d_ax);
         dfree(G.d_axstart);
-        dfree(G.d_gs);
-        dfree(G.d_gsstart);
     }
+    dfree(B->d_qargs);
+    dfree(B->d_qnx);
+    dfree(B->d_qitems);
+    dfree(B->d_gsall);
+    dfree(B->d_gsstart);
     dfree(B->d_args);
     dfree(B->d_ustart);
     delete B;
@@ -297,8 +432,26 @@ extern "C" int nekbone_batch_set_xu(nekbone_batch *B, int xu)
 {
     if (!B) return fail(NEKBONE_ERR_ARG, "batch: null");
     B->xu = xu ? 1 : 0;
-    for (auto &G : B->groups)
-        for (int k : G.members) B->P[size_t(k)]->axv.xu = B->xu;
+    apply_shapes(B);
+    return NEKBONE_OK;
+}
+
+extern "C" int nekbone_batch_set_queue(nekbone_batch *B, int tileset)
+{
+    if (!B) return fail(NEKBONE_ERR_ARG, "batch: null");
+    if (tileset < -1 || tileset >= int(tableq().size()))
+        return fail(NEKBONE_ERR_ARG, "batch: tile set out of range");
+    B->queue = tileset;
+    apply_shapes(B);
+    return NEKBONE_OK;
+}
+
+extern "C" int nekbone_batch_get_path(const nekbone_batch *B, int *tileset, int *xu, int *ntilesets)
+{
+    if (!B) return fail(NEKBONE_ERR_ARG, "batch: null");
+    if (tileset) *tileset = B->queue;
+    if (xu) *xu = B->xu;
+    if (ntilesets) *ntilesets = int(tableq().size());
     return NEKBONE_OK;
 }
 
@@ -371,9 +524,11 @@ extern "C" int nekbone_batch_cg_tol_d(nekbone_batch *B, double *const *x, const 
     return NEKBONE_OK;
 }
 
-// Batch autotuner: the element kernels of every order group with the x
-// update carried by the element kernel or by the vector update, timed over
-// batched iterations; keeps the faster.
+// Batch autotuner: times batched iterations of every element-kernel path
+// (per-group launches at the orders' default shapes and at doubled ones,
+// the work-queue kernel with each compiled tile set) with the x update
+// carried by the element kernel or by the vector update; keeps the fastest
+// and gives the members its shapes.
 extern "C" int nekbone_batch_autotune(nekbone_batch *B, int precond, double *best_ms)
 {
     if (!B) return fail(NEKBONE_ERR_ARG, "batch: null");
@@ -387,13 +542,40 @@ extern "C" int nekbone_batch_autotune(nekbone_batch *B, int precond, double *bes
             return fail(NEKBONE_ERR_STATE, "batch: Jacobi not set up on every problem");
         mbit[k] = P->mask_mode == NEKBONE_MASK_DIRICHLET ? 1 : 2;
     }
+    struct Cand {
+        int queue, dbl, xu;
+    };
+    std::vector<Cand> cands;
+    bool any_q = false;
+    for (auto &G : B->groups) any_q = any_q || qslot(G.nx1) >= 0;
+    for (int xu = 1; xu >= 0; --xu) {
+        cands.push_back(Cand{-1, 0, xu});
+        cands.push_back(Cand{-1, 1, xu});
+        if (any_q)
+            for (int q = 0; q < int(tableq().size()); ++q) cands.push_back(Cand{q, 0, xu});
+    }
+    // per-group shape of a candidate: the order's default, or twice as many
+    // elements per workgroup where that batched kernel is compiled
+    auto set_cand = [&](const Cand &c) {
+        B->queue = c.queue;
+        B->xu = c.xu;
+        for (auto &G : B->groups) {
+            const int e = axk::epb_for(G.nx1);
+            const bool has2 =
+                tableb().count(axk::Key(G.nx1, 2 * e, B->fma ? 1 : 0, 1, 0, c.xu)) != 0;
+            G.v = AxVariant{c.dbl && has2 ? 2 * e : e, 0, 1};
+        }
+        apply_shapes(B);
+    };
     hipEvent_t e0, e1;
     NB_HIP(hipEventCreate(&e0));
     NB_HIP(hipEventCreate(&e1));
     double best = 1e300;
-    int win = B->xu, rc = NEKBONE_OK;
-    for (int xu = 0; xu <= 1 && !rc; ++xu) {
-        nekbone_batch_set_xu(B, xu);
+    Cand win{B->queue, 0, B->xu};
+    bool have_win = false;
+    int rc = NEKBONE_OK;
+    for (const Cand &c : cands) {
+        set_cand(c);
         for (size_t k = 0; k < np && !rc; ++k) {
             nekbone_problem *P = B->P[k];
             rc = launch_rand(P->d_f, P->n, 12345 + k);
@@ -409,14 +591,16 @@ extern "C" int nekbone_batch_autotune(nekbone_batch *B, int precond, double *bes
         if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = NEKBONE_ERR_HIP;
         float ms = 0.f;
         if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = NEKBONE_ERR_HIP;
-        if (!rc && ms / 6 < best) {
+        if (rc) break;
+        if (ms / 6 < best) {
             best = ms / 6;
-            win = xu;
+            win = c;
+            have_win = true;
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    nekbone_batch_set_xu(B, win);
+    if (have_win) set_cand(win);
     drop_graph(B);
     if (rc) return rc == NEKBONE_ERR_HIP ? fail(rc, "batch autotune: HIP error") : rc;
     if (best_ms) *best_ms = best;

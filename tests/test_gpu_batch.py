@@ -31,12 +31,19 @@ SPECS = [(12, 6, True, 1), (27, 8, False, 1), (7, 10, True, 1), (20, 12, True, 1
          (64, 6, False, 1), (9, 8, True, 0), (30, 10, False, 1), (5, 12, False, 1)]
 
 
-@pytest.mark.parametrize("precond,xu", [(1, 1), (1, 0), (0, 1)])
-def test_batch_matches_single_solves(nb, oracle, precond, xu):
-    _case("batch precond=%d xu=%d" % (precond, xu))
-    probs = _problems(nb, oracle, SPECS)
+@pytest.mark.parametrize("precond,xu,path", [(1, 1, 0), (1, 0, 0), (0, 1, 0), (1, 1, -1),
+                                             (1, 1, 1), (1, 1, 2), (1, 0, 3)])
+def test_batch_matches_single_solves(nb, oracle, precond, xu, path):
+    """Every element-kernel path (per-group launches, the work-queue kernel
+    with each tile set; an order outside the queue's, nx1 = 5, rides along
+    per group) and both x-update placements."""
+    _case("batch precond=%d xu=%d path=%d" % (precond, xu, path))
+    probs = _problems(nb, oracle, SPECS + [(10, 5, True, 1)])
     B = nb.Batch([p for p, _ in probs])
+    assert B.path()[0] == -1  # per-group launches by default
+    B.set_queue(path)
     B.set_xu(xu)
+    assert B.path()[:2] == (path, xu)
     tol = 1e-8
     fs, xs, rs = [], [], []
     for P, f in probs:
@@ -82,6 +89,19 @@ def test_batch_repeat_and_maxiter(nb, oracle):
             assert np.array_equal(xs[k].download(), x1.download())
     ms = B.autotune()
     assert ms > 0
+    tileset, xu, nsets = B.path()
+    assert -1 <= tileset < nsets and xu in (0, 1)
+    # after tuning, the chosen shapes are on the problems: still the values
+    # of their single solves
+    its, _ = B.cg_tol_d(xs, fs, [None] * 4, 1e-8, 400)
+    for k, (P, f) in enumerate(probs):
+        P.reset_mask()
+        x1, r1 = P.vec(), P.vec()
+        it1, _ = P.cg_tol_d(x1, fs[k], r1, 1e-8, 400, nb.PRECOND_JACOBI)
+        assert its[k] == it1
+        assert np.array_equal(xs[k].download(), x1.download())
+    with pytest.raises(nb.NekboneError):
+        B.set_queue(nsets)
     B.close()
 
 

@@ -30,3 +30,26 @@ def test_semhat_matches_reference(golden, nel, nx1):
     np.testing.assert_allclose(a.reshape(m, m, order="F"), D.T @ np.diag(b) @ D, rtol=0,
                                atol=1e-13 * np.abs(a).max())
     np.testing.assert_allclose(c.reshape(m, m, order="F"), np.diag(b) @ D, rtol=0, atol=0)
+
+
+def test_tuning_file_roundtrip(tmp_path):
+    """nekbone_tuning_load takes the entries whose shape is compiled for
+    its order and skips the others; nekbone_tuning_save writes them back
+    (host code only, no GPU)."""
+    import nekbone_amd as nb
+    src = tmp_path / "in.json"
+    src.write_text('{"format": "nekbone_amd tuning 1", "element_kernel": [\n'
+                   '  {"nx1": 10, "precond": 1, "arith": 0, "epb": 1, "pf": 0, "xu": 1},\n'
+                   '  {"nx1": 10, "precond": -1, "arith": -1, "epb": 2, "pf": 1, "xu": 1},\n'
+                   '  {"nx1": 10, "precond": 0, "arith": 0, "epb": 7, "pf": 0, "xu": 1}\n]}\n')
+    assert nb.tuning_load(str(src)) == 2
+    out = tmp_path / "out.json"
+    nb.tuning_save(str(out))
+    text = out.read_text()
+    assert '{"nx1": 10, "precond": 1, "arith": 0, "epb": 1, "pf": 0, "xu": 1}' in text
+    assert '{"nx1": 10, "precond": -1, "arith": -1, "epb": 2, "pf": 1, "xu": 1}' in text
+    assert '"epb": 7' not in text
+    bad = tmp_path / "bad.json"
+    bad.write_text("{}")
+    with pytest.raises(nb.NekboneError):
+        nb.tuning_load(str(bad))
