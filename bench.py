@@ -324,7 +324,7 @@ def tol_workload(nb, nel, nx1, delta):
     return out
 
 
-def hsmg_workload(nb, tol=1e-10):
+def hsmg_workload(nb, tol=1e-10, graph=True):
     """HSMG-preconditioned FP64 CG on a 16x16x16-element box at N=7,
     deformed +-10% (seed 1), Dirichlet faces, RHS U[-1,1) seed 0, to a
     1e-10 relative residual: with the FP64 smoother (the reference's
@@ -335,6 +335,7 @@ def hsmg_workload(nb, tol=1e-10):
     P = nb.Problem(16, 16, 16, 8)
     P.set_mask_mode(nb.MASK_DIRICHLET)
     P.set_geometry_deformed(0.1, 1)
+    P.set_graph(graph)
     t0 = time.perf_counter()
     P.setup_hsmg()
     setup_ms = (time.perf_counter() - t0) * 1e3
@@ -430,7 +431,7 @@ def main():
     if args.only:
         part = {"tol": lambda: tol_workload(nb, nel, nx1, args.delta),
                 "batch": lambda: batched_workload(nb, args.batch, graph=not args.no_graph),
-                "hsmg": lambda: hsmg_workload(nb)}[args.only]()
+                "hsmg": lambda: hsmg_workload(nb, graph=not args.no_graph)}[args.only]()
         print(json.dumps({args.only: part}))
         return
     P = nb.Problem(*nel, nx1)
@@ -501,7 +502,7 @@ def main():
         extra["time_to_1e-10"] = tol_workload(nb, nel, nx1, args.delta)
     if args.hsmg:
         print("bench: hsmg", file=sys.stderr, flush=True)
-        extra["hsmg"] = hsmg_workload(nb)
+        extra["hsmg"] = hsmg_workload(nb, graph=not args.no_graph)
 
     if args.batch > 0:
         print("bench: batched solves", file=sys.stderr, flush=True)

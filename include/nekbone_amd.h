@@ -268,7 +268,9 @@ int nekbone_ax_local_f32(nekbone_problem *P, float *w, const float *u);
  * (scalars, vector update), replayed from a captured HIP graph in chunks;
  * each problem's values are those of its own nekbone_cg_tol_d solve with
  * the batch's element-kernel shape.  The batch sets that shape on its
- * problems. */
+ * problems.  The problems must outlive the batch; each solve re-reads
+ * their device buffers into the batch's tables, which the captured graphs
+ * read, so set-up calls on a problem between solves are seen. */
 typedef struct nekbone_batch nekbone_batch;
 int nekbone_batch_create(nekbone_batch **B, nekbone_problem *const *probs, int nprob);
 int nekbone_batch_destroy(nekbone_batch *B);

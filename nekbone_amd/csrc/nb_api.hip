@@ -389,6 +389,7 @@ static int ensure_hist(nekbone_problem *P, int niter)
 {
     const int need = 1 + 3 * std::max(niter, 0) + 3;
     if (P->hist_cap >= need) return NEKBONE_OK;
+    drop_problem_graph(P);
     dfree(P->d_hist);
     if (int rc = dalloc(&P->d_hist, need)) return rc;
     P->hist_cap = need;
@@ -419,9 +420,13 @@ static int cg_step(nekbone_problem *P, double *x, double *r, int precond, int mb
     return launch_cg_iteration(P, x, r, precond, mbit);
 }
 
-static void drop_graph(nekbone_problem *P)
+void drop_problem_graph(nekbone_problem *P)
 {
-    if (P->graph) (void)hipGraphExecDestroy(P->graph);
+    if (P->graph) {
+        // no replay of it may still be running
+        (void)hipStreamSynchronize(stream());
+        (void)hipGraphExecDestroy(P->graph);
+    }
     P->graph = nullptr;
     P->graph_key.clear();
 }
@@ -445,7 +450,7 @@ static int run_iterations(nekbone_problem *P, double *x, double *r, int precond,
             uintptr_t(P->fused), uintptr_t(P->mg), uintptr_t(P->d_dinv),
             uintptr_t(P->hsmg_fp32), uintptr_t(P->hsmg_coarse)};
         if (!P->graph || P->graph_key != key) {
-            drop_graph(P);
+            drop_problem_graph(P);
             hipGraph_t g = nullptr;
             NB_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
             int rc = NEKBONE_OK;
@@ -613,6 +618,7 @@ extern "C" int nekbone_problem_destroy(nekbone_problem *P)
 {
     if (!P) return NEKBONE_OK;
     if (g_stream) (void)hipStreamSynchronize(g_stream);
+    drop_problem_graph(P);
     dfree(P->d_D);
     dfree(P->d_Drow);
     dfree(P->d_g);
@@ -632,7 +638,6 @@ extern "C" int nekbone_problem_destroy(nekbone_problem *P)
     dfree(P->d_scal);
     free_mixed(P);
     hsmg_free(P);
-    drop_graph(P);
     (void)timing_resolve(P);
     for (hipEvent_t e : P->timing.pool) (void)hipEventDestroy(e);
     if (P->timing.cur) (void)hipEventDestroy(P->timing.cur);
@@ -888,7 +893,7 @@ extern "C" int nekbone_set_graph(nekbone_problem *P, int on)
 {
     if (int rc = check(P)) return rc;
     P->use_graph = on != 0;
-    if (!P->use_graph) drop_graph(P);
+    if (!P->use_graph) drop_problem_graph(P);
     return NEKBONE_OK;
 }
 

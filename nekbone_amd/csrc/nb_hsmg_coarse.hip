@@ -294,6 +294,9 @@ extern "C" int nekbone_set_hsmg_coarse(nekbone_problem *P, int mode)
     if (mode != 0 && mode != 1) return fail(NEKBONE_ERR_ARG, "hsmg coarse: mode 0 or 1");
     if (mode == 1) {
         if (!P->mg) return fail(NEKBONE_ERR_STATE, "hsmg coarse: nekbone_setup_hsmg not called");
+        // the coarse buffers are rebuilt: a graph captured over the old ones
+        // must not replay
+        drop_problem_graph(P);
         if (int rc = hsmg_coarse_build(P)) {
             hsmg_coarse_free(P->mg);
             return rc;

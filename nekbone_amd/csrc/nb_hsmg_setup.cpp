@@ -262,6 +262,7 @@ void ghost_ext(int f, int t, int nh, int &I, int &J, int &K)
 void hsmg_free(nekbone_problem *P)
 {
     if (!P->mg) return;
+    drop_problem_graph(P);
     Hsmg *M = P->mg;
     for (int l = 1; l <= 3; ++l) {
         HsmgLevel &L = M->L[l];

@@ -529,6 +529,11 @@ int launch_gs_map(const GsMap &M, double *u);
 int gs_map_upload(GsMap &G, const std::vector<int32_t> &off, const std::vector<int32_t> &idx,
                   const int32_t *pos, int n3);
 void gs_map_free(GsMap &G);
+// Destroys the problem's captured CG-iteration graph (after the stream has
+// drained).  Every routine that frees or replaces a device buffer that the
+// iteration's kernels read calls it first: the graph's key holds buffer
+// addresses, and a new buffer can come back at a freed one's address.
+void drop_problem_graph(nekbone_problem *P);
 // HSMG (nb_hsmg_setup.cpp, nb_hsmg.hip)
 int hsmg_build(nekbone_problem *P);
 void hsmg_free(nekbone_problem *P);

@@ -293,3 +293,34 @@ def test_hsmg_direct_coarse_errors(nb):
     P.setup_hsmg()  # a new set-up returns to the reference's coarse step
     z, _ = P.hsmg_solve(np.ones(P.n))
     assert np.all(np.isfinite(z))
+
+
+def test_hsmg_resetup_between_graph_solves(nb):
+    """A new HSMG set-up, and a rebuilt direct coarse solve, between two
+    graph-replayed solves: the captured iteration graph holds the old
+    buffers' addresses, so the set-up must retire it; every solve gives
+    the first solve's iterates bit for bit."""
+    _case("hsmg re-setup between graph solves")
+    P = nb.Problem(4, 4, 4, 8)
+    P.set_mask_mode(nb.MASK_DIRICHLET)
+    P.set_geometry_deformed(0.1, 1)
+    P.set_graph(True)
+    P.setup_hsmg()
+    f, x, r = P.vec(), P.vec(), P.vec()
+    P.rand_uniform_d(f, 0)
+    P.rhs_from_raw_d(f)
+    it0, _ = P.cg_tol_d(x, f, r, 1e-10, 2000, nb.PRECOND_HSMG)
+    x0 = x.download()
+    assert it0 >= 20  # at least two replayed chunks of ten
+    for _ in range(2):
+        P.setup_hsmg()
+        it, _ = P.cg_tol_d(x, f, r, 1e-10, 2000, nb.PRECOND_HSMG)
+        assert it == it0 and np.array_equal(x.download(), x0)
+    P.set_hsmg_coarse(1)
+    it1, _ = P.cg_tol_d(x, f, r, 1e-10, 2000, nb.PRECOND_HSMG)
+    x1 = x.download()
+    P.set_hsmg_coarse(1)  # rebuilt coarse solve, same mode
+    it, _ = P.cg_tol_d(x, f, r, 1e-10, 2000, nb.PRECOND_HSMG)
+    assert it == it1 and np.array_equal(x.download(), x1)
+    for v in (f, x, r):
+        v.free()
