@@ -59,4 +59,34 @@ __device__ __forceinline__ void block_sum2(double &v0, double &v1)
     v1 = b;
 }
 
+// Members of group g of a class of size S: the first nrun groups are in
+// run form (runs of R; group t of a run has the run's first members + t),
+// the rest in the member list.
+template <int S>
+__device__ __forceinline__ void gs_members(const int32_t *__restrict__ list,
+                                           const int32_t *__restrict__ runs, int R, int64_t nrun,
+                                           int64_t g, int (&m)[S])
+{
+    const int32_t *mem;
+    int t = 0;
+    if (g < nrun) {
+        const int64_t run = g / R;
+        t = int(g - run * R);
+        mem = runs + S * run;
+    } else {
+        mem = list + S * (g - nrun);
+    }
+    if constexpr (S == 2) {
+        const int2 v = *reinterpret_cast<const int2 *>(mem);
+        m[0] = v.x + t, m[1] = v.y + t;
+    } else {
+#pragma unroll
+        for (int h = 0; h < S / 4; ++h) {
+            const int4 v = reinterpret_cast<const int4 *>(mem)[h];
+            m[4 * h] = v.x + t, m[4 * h + 1] = v.y + t, m[4 * h + 2] = v.z + t,
+                  m[4 * h + 3] = v.w + t;
+        }
+    }
+}
+
 }  // namespace nb

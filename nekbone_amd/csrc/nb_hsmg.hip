@@ -9,10 +9,10 @@
 //             :1071-1076 delivers it), local solve e = S D^-1 S^T r with
 //             S = S_z x S_y x S_x (h1mg_do_fast, :1297-1320); writes the
 //             own part and the ghost faces of e
-//   k_ovl     add the neighbours' ghost results onto the first interior
-//             layer (:1081-1085)
-//   gs        dssum on the level (:1089)
-//   k_mask_w  mask, then the square-rooted Schwarz weights (:1090, :1144-1178)
+//   k_swz_ovl add the neighbours' ghost results onto the first interior
+//   k_swz_gs  layer (:1081-1085), dssum on the level (:1089), mask, then
+//             the square-rooted Schwarz weights (:1090, :1144-1178): one
+//             pass over the points, one over the level's gs groups
 // Restriction k_rstr (h1mg_rstr, :1340-1357) and interpolation-add
 // k_intp (h1mg_intp, :1436-1443) are per-element tensor products through
 // LDS.  Every sum keeps the reference's order (mxm: left to right from the
@@ -25,6 +25,8 @@ namespace nb {
 namespace {
 
 constexpr int kT = 256;
+// threads per element of the restriction and interpolation kernels
+constexpr int kTE = 128;
 
 struct FdmArgs {
     double *r;
@@ -73,6 +75,10 @@ __device__ void tz(double *out, const double *in, const double *Ct, int n12, int
 template <int NH>
 struct FdmShape {
     static constexpr int NL = NH + 2, L2 = NL * NL, L3 = L2 * NL;
+    // block layout in LDS: point (i, j, k) at i + P1 j + P2 k, the j stride
+    // odd so that the lanes of an x-line pass (one line (j, k) each) fall in
+    // distinct banks
+    static constexpr int P1 = NL | 1, P2 = P1 * NL, LB = P2 * NL;
     static constexpr int EPB = L2 >= 256 ? 1 : 256 / L2;
     static constexpr int NT = (EPB * L2 + 63) / 64 * 64;
     static constexpr int OPS = 2 * L2 + NL;  // S, S^T (column-major), eigenvalues
@@ -80,7 +86,7 @@ struct FdmShape {
 
 // v <- C v for a column-major NL x NL matrix C in LDS: out_i is summed
 // left to right from the first product, C(i,0) v_0 + C(i,1) v_1 + ...
-// (the mxm order); FP64 rounds every product and sum, FP32 fuses them.
+// (the mxm order), each step one fused multiply-add.
 template <typename T, int NL>
 __device__ __forceinline__ void line_apply(T (&v)[NL], const T *__restrict__ C)
 {
@@ -94,7 +100,7 @@ __device__ __forceinline__ void line_apply(T (&v)[NL], const T *__restrict__ C)
             if constexpr (sizeof(T) == 4)
                 o[i] = __builtin_fmaf(C[i + NL * m], v[m], o[i]);
             else
-                o[i] = o[i] + C[i + NL * m] * v[m];
+                o[i] = __builtin_fma(C[i + NL * m], v[m], o[i]);
         }
 #pragma unroll
     for (int i = 0; i < NL; ++i) v[i] = o[i];
@@ -114,26 +120,27 @@ __device__ __forceinline__ void line_store(const T (&v)[NL], T *b, int base, int
     for (int m = 0; m < NL; ++m) b[base + stride * m] = v[m];
 }
 
-// Extended-block position of ghost slot t of face f (x-, x+, y-, y+, z-,
-// z+): the face's (a, b) = (t % NH + 1, t / NH + 1) in its two tangential
-// directions, the normal coordinate 0 or NL-1.
+// Extended-block position (in the LDS layout of FdmShape) of ghost slot t
+// of face f (x-, x+, y-, y+, z-, z+): the face's (a, b) = (t % NH + 1,
+// t / NH + 1) in its two tangential directions, the normal coordinate 0 or
+// NL-1.
 template <int NH>
 __device__ __forceinline__ int ghost_ext(int f, int t)
 {
-    constexpr int NL = NH + 2;
+    constexpr int NL = NH + 2, P1 = NL | 1, P2 = P1 * NL;
     const int a = t % NH + 1, b = t / NH + 1, side = (f & 1) ? NL - 1 : 0;
-    if (f < 2) return side + NL * a + NL * NL * b;
-    if (f < 4) return a + NL * side + NL * NL * b;
-    return a + NL * b + NL * NL * side;
+    if (f < 2) return side + P1 * a + P2 * b;
+    if (f < 4) return a + P1 * side + P2 * b;
+    return a + P1 * b + P2 * side;
 }
 
 // Extended-block position of own point p = i + NH j + NH^2 k.
 template <int NH>
 __device__ __forceinline__ int own_ext(int p)
 {
-    constexpr int NL = NH + 2;
+    constexpr int NL = NH + 2, P1 = NL | 1, P2 = P1 * NL;
     const int i = p % NH, j = (p / NH) % NH, k = p / (NH * NH);
-    return (i + 1) + NL * (j + 1) + NL * NL * (k + 1);
+    return (i + 1) + P1 * (j + 1) + P2 * (k + 1);
 }
 
 // Local solves of one Schwarz level (h1mg_schwarz_part1, src/hsmg.f:1052-1079):
@@ -154,43 +161,41 @@ __global__ void __launch_bounds__(FdmShape<NH>::NT)
     k_fdm(FdmArgs a, const T *__restrict__ ops, const T *__restrict__ epsv)
 {
     using S = FdmShape<NH>;
-    constexpr int NL = S::NL, L2 = S::L2, L3 = S::L3, EPB = S::EPB, OPS = S::OPS;
+    constexpr int NL = S::NL, L2 = S::L2, EPB = S::EPB, OPS = S::OPS;
+    constexpr int P1 = S::P1, P2 = S::P2, L3 = S::LB;
     constexpr int H2 = NH * NH, N3 = H2 * NH, G2 = 6 * H2;
     __shared__ T blk[EPB][L3];
     __shared__ T op[EPB][3][OPS];
     const int e0 = blockIdx.x * EPB, tid = threadIdx.x, nelt = a.nelt;
-    {
-        constexpr int IOP = (EPB * 3 * OPS + S::NT - 1) / S::NT;
-        T vop[IOP];
+    // Every global load (operators, own points, ghost sources) is issued
+    // before the first barrier, so they are all in flight together; the
+    // LDS is cleared meanwhile (edges and corners of the extended block
+    // stay zero).  Dirichlet points read as zero; r itself is masked at
+    // the end.
+    constexpr int IOP = (EPB * 3 * OPS + S::NT - 1) / S::NT;
+    T vop[IOP];
 #pragma unroll
-        for (int it = 0; it < IOP; ++it) {
-            const int q = tid + it * S::NT, el = q / (3 * OPS), w = q - el * (3 * OPS),
-                      d = w / OPS;
-            vop[it] = T(0);
-            if (q < EPB * 3 * OPS && e0 + el < nelt)
-                vop[it] = ops[int64_t(a.eop[3 * (e0 + el) + d]) * OPS + (w - d * OPS)];
-        }
-#pragma unroll
-        for (int it = 0; it < IOP; ++it) {
-            const int q = tid + it * S::NT;
-            if (q < EPB * 3 * OPS) (&op[0][0][0])[q] = vop[it];
-        }
+    for (int it = 0; it < IOP; ++it) {
+        const int q = tid + it * S::NT, el = q / (3 * OPS), w = q - el * (3 * OPS), d = w / OPS;
+        vop[it] = T(0);
+        if (q < EPB * 3 * OPS && e0 + el < nelt)
+            vop[it] = ops[int64_t(a.eop[3 * (e0 + el) + d]) * OPS + (w - d * OPS)];
     }
-    // edges and corners of the extended block stay zero
-    for (int q = tid; q < EPB * L3; q += S::NT) (&blk[0][0])[q] = T(0);
-    __syncthreads();
-    // Gathers first (unrolled, no stores in between, so every thread's
-    // loads are in flight together), then the LDS writes.  Dirichlet
-    // points read as zero here; r itself is masked at the end.
     constexpr int IO = (EPB * N3 + S::NT - 1) / S::NT, IG = (EPB * G2 + S::NT - 1) / S::NT;
+    // own points: memory index and mask kept for the stores at the end
     double vown[IO], vgh[IG];
+    int64_t mo[IO];
+    uint8_t bo[IO];
 #pragma unroll
     for (int it = 0; it < IO; ++it) {
         const int q = tid + it * S::NT, el = q / N3, e = e0 + el;
+        mo[it] = -1;
+        bo[it] = 0;
         vown[it] = 0.0;
         if (q < EPB * N3 && e < nelt) {
-            const int64_t m = int64_t(e) * N3 + a.pos[q - el * N3];
-            vown[it] = a.bnd[m] ? 0.0 : a.r[m];
+            mo[it] = int64_t(e) * N3 + a.pos[q - el * N3];
+            bo[it] = a.bnd[mo[it]];
+            vown[it] = a.r[mo[it]];
         }
     }
 #pragma unroll
@@ -202,12 +207,24 @@ __global__ void __launch_bounds__(FdmShape<NH>::NT)
             const int32_t sn = a.gsrc[slot];
             if (sn >= 0) {
                 const int32_t o = a.gin[slot];
-                const double vo = a.bnd[o] ? 0.0 : a.r[o];
-                const double vs = a.bnd[sn] ? 0.0 : a.r[sn];
+                const uint8_t bs = a.bnd[sn], bi = a.bnd[o];
+                const double ro = a.r[o], rs = a.r[sn];
+                const double vo = bi ? 0.0 : ro;
+                const double vs = bs ? 0.0 : rs;
                 vgh[it] = (vo + vs) - vo;
             }
         }
     }
+#pragma unroll
+    for (int it = 0; it < IO; ++it)
+        if (bo[it]) vown[it] = 0.0;
+    for (int q = tid; q < EPB * L3; q += S::NT) (&blk[0][0])[q] = T(0);
+#pragma unroll
+    for (int it = 0; it < IOP; ++it) {
+        const int q = tid + it * S::NT;
+        if (q < EPB * 3 * OPS) (&op[0][0][0])[q] = vop[it];
+    }
+    __syncthreads();
 #pragma unroll
     for (int it = 0; it < IO; ++it) {
         const int q = tid + it * S::NT, el = q / N3;
@@ -229,19 +246,19 @@ __global__ void __launch_bounds__(FdmShape<NH>::NT)
     T v[NL];
     // S^T r: x lines (la, lb) = (j, k), y lines (i, k), z lines (i, j)
     if (act) {
-        line_load(v, b, NL * l, 1);
+        line_load(v, b, P1 * la + P2 * lb, 1);
         line_apply<T, NL>(v, ox + L2);
-        line_store(v, b, NL * l, 1);
+        line_store(v, b, P1 * la + P2 * lb, 1);
     }
     __syncthreads();
     if (act) {
-        line_load(v, b, la + L2 * lb, NL);
+        line_load(v, b, la + P2 * lb, P1);
         line_apply<T, NL>(v, oy + L2);
-        line_store(v, b, la + L2 * lb, NL);
+        line_store(v, b, la + P2 * lb, P1);
     }
     __syncthreads();
     if (act) {
-        line_load(v, b, l, L2);
+        line_load(v, b, la + P1 * lb, P2);
         line_apply<T, NL>(v, oz + L2);
         const T eps = epsv[e0 + el], lxy = ox[2 * L2 + la] + oy[2 * L2 + lb];
 #pragma unroll
@@ -250,47 +267,33 @@ __global__ void __launch_bounds__(FdmShape<NH>::NT)
             const T d = (diag > eps) ? T(1) / diag : T(0);
             v[k] = d * v[k];
         }
-        line_store(v, b, l, L2);
+        line_store(v, b, la + P1 * lb, P2);
     }
     __syncthreads();
     if (act) {
-        line_load(v, b, NL * l, 1);
+        line_load(v, b, P1 * la + P2 * lb, 1);
         line_apply<T, NL>(v, ox);
-        line_store(v, b, NL * l, 1);
+        line_store(v, b, P1 * la + P2 * lb, 1);
     }
     __syncthreads();
     if (act) {
-        line_load(v, b, la + L2 * lb, NL);
+        line_load(v, b, la + P2 * lb, P1);
         line_apply<T, NL>(v, oy);
-        line_store(v, b, la + L2 * lb, NL);
+        line_store(v, b, la + P2 * lb, P1);
     }
     __syncthreads();
     if (act) {
-        line_load(v, b, l, L2);
+        line_load(v, b, la + P1 * lb, P2);
         line_apply<T, NL>(v, oz);
-        line_store(v, b, l, L2);
+        line_store(v, b, la + P1 * lb, P2);
     }
     __syncthreads();
-    {
-        int64_t mo[IO];
-        uint8_t bo[IO];
 #pragma unroll
-        for (int it = 0; it < IO; ++it) {
-            const int q = tid + it * S::NT, el2 = q / N3, e = e0 + el2;
-            mo[it] = -1;
-            bo[it] = 0;
-            if (q < EPB * N3 && e < nelt) {
-                mo[it] = int64_t(e) * N3 + a.pos[q - el2 * N3];
-                bo[it] = a.bnd[mo[it]];
-            }
-        }
-#pragma unroll
-        for (int it = 0; it < IO; ++it) {
-            const int q = tid + it * S::NT, el2 = q / N3;
-            if (mo[it] < 0) continue;
-            a.eown[mo[it]] = double(blk[el2][own_ext<NH>(q - el2 * N3)]);
-            if (bo[it]) a.r[mo[it]] = 0.0;  // the caller's r leaves masked (src/hsmg.f:1067)
-        }
+    for (int it = 0; it < IO; ++it) {
+        const int q = tid + it * S::NT, el2 = q / N3;
+        if (mo[it] < 0) continue;
+        a.eown[mo[it]] = double(blk[el2][own_ext<NH>(q - el2 * N3)]);
+        if (bo[it]) a.r[mo[it]] = 0.0;  // the caller's r leaves masked (src/hsmg.f:1067)
     }
     for (int q = tid; q < EPB * G2; q += S::NT) {
         const int el2 = q / G2, e = e0 + el2;
@@ -332,41 +335,191 @@ int launch_fdm(const FdmArgs &a, const T *ops, const T *eps, int nh)
     }
 }
 
-// e(first interior layer) += the neighbour's ghost result, x faces, then
-// y, then z (src/hsmg.f:1081-1085); in place on eown.
-__global__ void k_ovl(double *__restrict__ eown, const double *__restrict__ ghost,
-                      const int32_t *__restrict__ gpart, const int32_t *__restrict__ pos, int nh,
-                      int64_t n)
+// The rest of a Schwarz level after the local solves, in two passes: the
+// overlap sum (src/hsmg.f:1081-1085), dssum on the level (:1089), the
+// mask (:1090) and the square-rooted weights (h1mg_schwarz_wt3d2,
+// :1144-1178):
+//   v    = e_own + the neighbours' ghost results on the first interior
+//          layer, x faces, then y, then z, each added as (own + theirs) - own
+//          (the exchange of the (n+2)^3 ghost layer, :1071-1085)
+//   t    = v at an unshared point; at a shared one v(member 0) + v(member 1)
+//          + ... in the level numbering's member order, for every member
+//   out  = (masked ? 0 : t) * sqrt(weight)
+// k_swz_ovl computes v at every point, finishes the unshared points and
+// leaves v in e_own at the shared ones; k_swz_gs sums the groups and
+// finishes their members.  Every value rounds as in the sequence of the
+// three steps.
+struct SwzArgs {
+    double *out, *eown;
+    const double *ghost, *sw;
+    const int32_t *gpart;
+    const uint8_t *bnd, *shr;
+    int shrbit;  // shr[m] & shrbit: point m is in a gs group
+    int n;
+    int b2, b4, b8, br;  // workgroups of each group class
+    GsArgs G;
+};
+
+constexpr int kSwzPts = 4;  // points per lane in k_swz_ovl
+
+// Ghost slots whose neighbour results point m receives, in the order they
+// are added: direction x, y, z; within a direction the low side first
+// (both sides only when NH = 3, where layer 1 is also layer NH-2); -1: none.
+template <int NH, bool FINE>
+__device__ __forceinline__ void swz_slots(int m, int (&slot)[NH == 3 ? 6 : 3])
 {
-    const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (q >= n) return;
-    const int n3 = nh * nh * nh, h2 = nh * nh, g2 = 6 * h2;
-    const int64_t e = q / n3;
-    const int p = int(q - e * n3);
-    const int i = p % nh, j = (p / nh) % nh, k = p / h2;
-    const int64_t m = e * n3 + pos[p];
-    double v = eown[m];
+    constexpr int N3 = NH * NH * NH, H2 = NH * NH, G2 = 6 * H2;
+    const int e = m / N3, lm = m - e * N3;
+    // the fine level is in the device point order, the others in the
+    // reference order
+    const int p = FINE ? surface_order_ref(NH, lm) : lm;
+    const int i = p % NH, j = (p / NH) % NH, k = p / H2;
     const int c[3] = {i, j, k};
-    const int t[3] = {j + nh * k, i + nh * k, i + nh * j};
-    for (int d = 0; d < 3; ++d)
-        for (int side = 0; side < 2; ++side) {
-            if (c[d] != (side ? nh - 2 : 1)) continue;
-            const int64_t slot = e * g2 + (2 * d + side) * h2 + t[d];
-            const int32_t g = gpart[slot];
-            if (g < 0) continue;
-            const double own = ghost[slot];
-            v = v + ((own + ghost[g]) - own);
+    const int t[3] = {j + NH * k, i + NH * k, i + NH * j};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if constexpr (NH == 3) {
+            const bool on = c[d] == 1;
+            slot[2 * d] = on ? e * G2 + (2 * d) * H2 + t[d] : -1;
+            slot[2 * d + 1] = on ? e * G2 + (2 * d + 1) * H2 + t[d] : -1;
+        } else {
+            const int side = c[d] == 1 ? 0 : c[d] == NH - 2 ? 1 : -1;
+            slot[d] = side < 0 ? -1 : e * G2 + (2 * d + side) * H2 + t[d];
         }
-    eown[m] = v;
+    }
 }
 
-__global__ void k_mask_w(double *__restrict__ out, const double *__restrict__ in,
-                         const uint8_t *__restrict__ bnd, const double *__restrict__ sw, int64_t n)
+// kSwzPts points per lane (lane + u * kRedThreads of the workgroup's
+// range), every load of all of them issued before the neighbour gathers
+// and those before the first store.
+template <int NH, bool FINE>
+__global__ void __launch_bounds__(kRedThreads) k_swz_ovl(SwzArgs a)
 {
-    const int64_t m = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (m >= n) return;
-    const double v = bnd[m] ? 0.0 : in[m];
-    out[m] = v * sw[m];
+    constexpr int NC = NH == 3 ? 6 : 3;
+    const int base = blockIdx.x * (kRedThreads * kSwzPts) + threadIdx.x;
+    int slot[kSwzPts][NC], gp[kSwzPts][NC];
+    double ev[kSwzPts], go[kSwzPts][NC], gv[kSwzPts][NC];
+    uint8_t sh[kSwzPts];
+#pragma unroll
+    for (int u = 0; u < kSwzPts; ++u) {
+        const int m = base + u * kRedThreads;
+        const bool ok = m < a.n;
+        swz_slots<NH, FINE>(ok ? m : 0, slot[u]);
+        ev[u] = ok ? a.eown[m] : 0.0;
+        sh[u] = ok ? a.shr[m] : 0;
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+            if (!ok) slot[u][q] = -1;
+            gp[u][q] = slot[u][q] >= 0 ? a.gpart[slot[u][q]] : -1;
+            go[u][q] = slot[u][q] >= 0 ? a.ghost[slot[u][q]] : 0.0;
+        }
+    }
+    // the mask and weight of the unshared points load with the gathers
+    uint8_t bd[kSwzPts];
+    double wt[kSwzPts];
+#pragma unroll
+    for (int u = 0; u < kSwzPts; ++u) {
+        const int m = base + u * kRedThreads;
+        const bool fin = m < a.n && !(sh[u] & a.shrbit);
+        bd[u] = fin ? a.bnd[m] : 0;
+        wt[u] = fin ? a.sw[m] : 0.0;
+#pragma unroll
+        for (int q = 0; q < NC; ++q) gv[u][q] = gp[u][q] >= 0 ? a.ghost[gp[u][q]] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kSwzPts; ++u) {
+        const int m = base + u * kRedThreads;
+        if (m >= a.n) continue;
+        double v = ev[u];
+#pragma unroll
+        for (int q = 0; q < NC; ++q)
+            if (gp[u][q] >= 0) v = v + ((go[u][q] + gv[u][q]) - go[u][q]);
+        if (sh[u] & a.shrbit) {
+            a.eown[m] = v;
+        } else {
+            const double w = bd[u] ? 0.0 : v;
+            a.out[m] = w * wt[u];
+        }
+    }
+}
+
+__device__ __forceinline__ void swz_store(const SwzArgs &a, int m, double t)
+{
+    const double v = a.bnd[m] ? 0.0 : t;
+    a.out[m] = v * a.sw[m];
+}
+
+// Sum of one group and its members finished; every load before the
+// first store.
+template <int S>
+__device__ __forceinline__ void swz_group(const SwzArgs &a, const int (&m)[S])
+{
+    double v[S], wt[S];
+    uint8_t bd[S];
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+        v[q] = a.eown[m[q]];
+        bd[q] = a.bnd[m[q]];
+        wt[q] = a.sw[m[q]];
+    }
+    double t = v[0];
+#pragma unroll
+    for (int q = 1; q < S; ++q) t += v[q];
+#pragma unroll
+    for (int q = 0; q < S; ++q) a.out[m[q]] = (bd[q] ? 0.0 : t) * wt[q];
+}
+
+// The level's gather-scatter classes (pairs, quads, octets, the rest),
+// one group per lane, on the v that k_swz_ovl left at the shared points.
+__global__ void __launch_bounds__(kRedThreads) k_swz_gs(SwzArgs a)
+{
+    const GsArgs &G = a.G;
+    int blk = blockIdx.x;
+    const int64_t g0 = int64_t(threadIdx.x);
+    if (blk < a.b2) {
+        const int64_t g = int64_t(blk) * kRedThreads + g0;
+        if (g >= G.n2) return;
+        int m[2];
+        gs_members<2>(G.m2, G.m2r, G.r2, G.n2r, g, m);
+        swz_group<2>(a, m);
+    } else if ((blk -= a.b2) < a.b4) {
+        const int64_t g = int64_t(blk) * kRedThreads + g0;
+        if (g >= G.n4) return;
+        int m[4];
+        gs_members<4>(G.m4, G.m4r, G.r4, G.n4r, g, m);
+        swz_group<4>(a, m);
+    } else if ((blk -= a.b4) < a.b8) {
+        const int64_t g = int64_t(blk) * kRedThreads + g0;
+        if (g >= G.n8) return;
+        int m[8];
+        gs_members<8>(G.m8, nullptr, 1, 0, g, m);
+        swz_group<8>(a, m);
+    } else {
+        blk -= a.b8;
+        const int64_t g = int64_t(blk) * kRedThreads + g0;
+        if (g >= G.nr) return;
+        const int lo = G.roff[g], hi = G.roff[g + 1];
+        double t = a.eown[G.ridx[lo]];
+        for (int q = lo + 1; q < hi; ++q) t += a.eown[G.ridx[q]];
+        for (int q = lo; q < hi; ++q) swz_store(a, G.ridx[q], t);
+    }
+}
+
+template <int NH>
+int launch_swz_nh(const SwzArgs &a, bool fine)
+{
+    const int nbo = int((int64_t(a.n) + kRedThreads * kSwzPts - 1) / (kRedThreads * kSwzPts));
+    if (fine)
+        hipLaunchKernelGGL((k_swz_ovl<NH, true>), dim3(nbo), dim3(kRedThreads), 0, stream(), a);
+    else
+        hipLaunchKernelGGL((k_swz_ovl<NH, false>), dim3(nbo), dim3(kRedThreads), 0, stream(), a);
+    NB_HIP(hipGetLastError());
+    const int nbg = a.b2 + a.b4 + a.b8 + a.br;
+    if (nbg > 0) {
+        hipLaunchKernelGGL(k_swz_gs, dim3(nbg), dim3(kRedThreads), 0, stream(), a);
+        NB_HIP(hipGetLastError());
+    }
+    return NEKBONE_OK;
 }
 
 // Restriction of one element from the finer level F to the coarser C:
@@ -380,18 +533,22 @@ __global__ void __launch_bounds__(kT)
 {
     extern __shared__ double sm[];
     const int f3 = nf * nf * nf, c3 = nc * nc * nc;
-    double *U = sm, *W1 = sm + f3, *W2 = W1 + nc * nf * nf;
+    double *U = sm, *W1 = sm + f3, *W2 = W1 + nc * nf * nf, *J = W2 + nc * nc * nf, *Jt = J + nc * nf;
     const int64_t e = blockIdx.x;
     for (int p = threadIdx.x; p < f3; p += blockDim.x) {
         const int64_t m = e * f3 + posf[p];
         U[p] = rf[m] * cf[m];
     }
+    for (int p = threadIdx.x; p < nc * nf; p += blockDim.x) {
+        J[p] = jh[p];
+        Jt[p] = jht[p];
+    }
     __syncthreads();
-    tx(W1, U, jht, nc, nf, nf * nf);         // A = jht (nc x nf)
+    tx(W1, U, Jt, nc, nf, nf * nf);          // A = jht (nc x nf)
     __syncthreads();
-    ty(W2, W1, jh, nc, nc, nf, nf);           // Bt = jh (nf x nc)
+    ty(W2, W1, J, nc, nc, nf, nf);            // Bt = jh (nf x nc)
     __syncthreads();
-    tz(U, W2, jh, nc * nc, nc, nf);           // Ct = jh
+    tz(U, W2, J, nc * nc, nc, nf);            // Ct = jh
     __syncthreads();
     for (int p = threadIdx.x; p < c3; p += blockDim.x) {
         const int64_t m = e * c3 + p;
@@ -410,18 +567,22 @@ __global__ void __launch_bounds__(kT)
 {
     extern __shared__ double sm[];
     const int f3 = nf * nf * nf, c3 = nc * nc * nc;
-    double *U = sm, *W1 = sm + f3, *W2 = W1 + nf * nc * nc;
+    double *U = sm, *W1 = sm + f3, *W2 = W1 + nf * nc * nc, *J = W2 + nf * nf * nc, *Jt = J + nc * nf;
     const int64_t e = blockIdx.x;
     for (int p = threadIdx.x; p < c3; p += blockDim.x) {
         const int64_t m = e * c3 + p;
         U[p] = (MASK && bndc[m]) ? 0.0 : uc[m];
     }
+    for (int p = threadIdx.x; p < nc * nf; p += blockDim.x) {
+        J[p] = jh[p];
+        Jt[p] = jht[p];
+    }
     __syncthreads();
-    tx(W1, U, jh, nf, nc, nc * nc);           // A = jh (nf x nc)
+    tx(W1, U, J, nf, nc, nc * nc);            // A = jh (nf x nc)
     __syncthreads();
-    ty(W2, W1, jht, nf, nf, nc, nc);          // Bt = jht (nc x nf)
+    ty(W2, W1, Jt, nf, nf, nc, nc);           // Bt = jht (nc x nf)
     __syncthreads();
-    tz(U, W2, jht, nf * nf, nf, nc);          // Ct = jht
+    tz(U, W2, Jt, nf * nf, nf, nc);           // Ct = jht
     __syncthreads();
     for (int p = threadIdx.x; p < f3; p += blockDim.x) {
         const int64_t m = e * f3 + posf[p];
@@ -438,37 +599,64 @@ int schwarz(nekbone_problem *P, HsmgLevel &L, double *out, double *r)
     if (int rc = P->hsmg_fp32 ? launch_fdm<float>(a, L.d_ops32, L.d_eps32, L.nh)
                               : launch_fdm<double>(a, L.d_ops, L.d_eps, L.nh))
         return rc;
-    hipLaunchKernelGGL(k_ovl, dim3(grid(L.n)), dim3(kT), 0, stream(), L.d_eown,
-                       static_cast<const double *>(L.d_ghost),
-                       static_cast<const int32_t *>(L.d_gpart),
-                       static_cast<const int32_t *>(L.d_pos), L.nh, L.n);
-    NB_HIP(hipGetLastError());
+    if (int64_t(6) * L.nh * L.nh * P->nelt > INT32_MAX || L.n > INT32_MAX)
+        return fail(NEKBONE_ERR_UNSUPPORTED, "hsmg: level above 2^31 points or ghost slots");
+    SwzArgs s{};
+    s.out = out;
+    s.eown = L.d_eown;
+    s.ghost = L.d_ghost;
+    s.sw = L.d_sw;
+    s.gpart = L.d_gpart;
+    s.bnd = L.d_bnd;
+    s.n = int(L.n);
     if (L.fine) {
-        if (int rc = launch_gs(P, L.d_eown, nullptr)) return rc;
-    } else if (int rc = launch_gs_map(L.gs, L.d_eown)) {
-        return rc;
+        // the fine level is the problem's own numbering (device order)
+        s.G = gs_args(P);
+        s.shr = P->d_flag;
+        s.shrbit = kFlagShared;
+    } else {
+        s.G = gs_args_map(L.gs);
+        s.shr = L.d_shr;
+        s.shrbit = 1;
     }
-    hipLaunchKernelGGL(k_mask_w, dim3(grid(L.n)), dim3(kT), 0, stream(), out,
-                       static_cast<const double *>(L.d_eown),
-                       static_cast<const uint8_t *>(L.d_bnd),
-                       static_cast<const double *>(L.d_sw), L.n);
-    NB_HIP(hipGetLastError());
-    return NEKBONE_OK;
+    auto nb = [](int64_t k) { return int((k + kRedThreads - 1) / kRedThreads); };
+    s.b2 = nb(s.G.n2);
+    s.b4 = nb(s.G.n4);
+    s.b8 = nb(s.G.n8);
+    s.br = nb(s.G.nr);
+    switch (L.nh) {
+    case 3: return launch_swz_nh<3>(s, L.fine);
+    case 4: return launch_swz_nh<4>(s, L.fine);
+    case 5: return launch_swz_nh<5>(s, L.fine);
+    case 6: return launch_swz_nh<6>(s, L.fine);
+    case 7: return launch_swz_nh<7>(s, L.fine);
+    case 8: return launch_swz_nh<8>(s, L.fine);
+    case 9: return launch_swz_nh<9>(s, L.fine);
+    case 10: return launch_swz_nh<10>(s, L.fine);
+    case 11: return launch_swz_nh<11>(s, L.fine);
+    case 12: return launch_swz_nh<12>(s, L.fine);
+    case 13: return launch_swz_nh<13>(s, L.fine);
+    case 14: return launch_swz_nh<14>(s, L.fine);
+    case 15: return launch_swz_nh<15>(s, L.fine);
+    case 16: return launch_swz_nh<16>(s, L.fine);
+    default: return fail(NEKBONE_ERR_UNSUPPORTED, "hsmg: level size outside 3..16");
+    }
 }
 
 int rstr(nekbone_problem *P, HsmgLevel &C, HsmgLevel &F, double *rc, const double *rf, bool mask)
 {
     const int nc = C.nh, nf = F.nh;
-    const size_t lds = sizeof(double) * size_t(nf * nf * nf + nc * nf * nf + nc * nc * nf);
+    const size_t lds =
+        sizeof(double) * size_t(nf * nf * nf + nc * nf * nf + nc * nc * nf + 2 * nc * nf);
     if (mask)
-        hipLaunchKernelGGL(k_rstr<true>, dim3(P->nelt), dim3(kT), lds, stream(), rc, rf,
+        hipLaunchKernelGGL(k_rstr<true>, dim3(P->nelt), dim3(kTE), lds, stream(), rc, rf,
                            static_cast<const double *>(F.d_c),
                            static_cast<const int32_t *>(F.d_pos),
                            static_cast<const double *>(C.d_jh),
                            static_cast<const double *>(C.d_jht),
                            static_cast<const uint8_t *>(C.d_bnd), nc, nf);
     else
-        hipLaunchKernelGGL(k_rstr<false>, dim3(P->nelt), dim3(kT), lds, stream(), rc, rf,
+        hipLaunchKernelGGL(k_rstr<false>, dim3(P->nelt), dim3(kTE), lds, stream(), rc, rf,
                            static_cast<const double *>(F.d_c),
                            static_cast<const int32_t *>(F.d_pos),
                            static_cast<const double *>(C.d_jh),
@@ -481,15 +669,16 @@ int rstr(nekbone_problem *P, HsmgLevel &C, HsmgLevel &F, double *rc, const doubl
 int intp(nekbone_problem *P, HsmgLevel &C, HsmgLevel &F, double *uf, const double *uc, bool mask)
 {
     const int nc = C.nh, nf = F.nh;
-    const size_t lds = sizeof(double) * size_t(nf * nf * nf + nf * nc * nc + nf * nf * nc);
+    const size_t lds =
+        sizeof(double) * size_t(nf * nf * nf + nf * nc * nc + nf * nf * nc + 2 * nc * nf);
     if (mask)
-        hipLaunchKernelGGL(k_intp<true>, dim3(P->nelt), dim3(kT), lds, stream(), uf,
+        hipLaunchKernelGGL(k_intp<true>, dim3(P->nelt), dim3(kTE), lds, stream(), uf,
                            static_cast<const int32_t *>(F.d_pos), uc,
                            static_cast<const uint8_t *>(C.d_bnd),
                            static_cast<const double *>(C.d_jh),
                            static_cast<const double *>(C.d_jht), nc, nf);
     else
-        hipLaunchKernelGGL(k_intp<false>, dim3(P->nelt), dim3(kT), lds, stream(), uf,
+        hipLaunchKernelGGL(k_intp<false>, dim3(P->nelt), dim3(kTE), lds, stream(), uf,
                            static_cast<const int32_t *>(F.d_pos), uc,
                            static_cast<const uint8_t *>(C.d_bnd),
                            static_cast<const double *>(C.d_jh),

@@ -268,6 +268,7 @@ void hsmg_free(nekbone_problem *P)
         HsmgLevel &L = M->L[l];
         gs_map_free(L.gs);
         dfree(L.d_pos);
+        dfree(L.d_shr);
         dfree(L.d_bnd);
         dfree(L.d_c);
         dfree(L.d_sw);
@@ -351,9 +352,16 @@ int hsmg_build(nekbone_problem *P)
         std::vector<int32_t> pos(static_cast<size_t>(n3));
         for (int p = 0; p < n3; ++p) pos[size_t(p)] = L.fine ? P->perm[size_t(p)] : p;
         if (int rc = upload(&L.d_pos, pos)) return rc;
-        if (!L.fine)
+        if (!L.fine) {
             if (int rc = gs_map_upload(L.gs, hgs[size_t(l)].off, hgs[size_t(l)].idx, nullptr, n3))
                 return rc;
+            const HostGs &H = hgs[size_t(l)];
+            std::vector<uint8_t> shr(size_t(L.n), 0);
+            for (size_t g = 0; g + 1 < H.off.size(); ++g)
+                if (H.off[g + 1] - H.off[g] >= 2)
+                    for (int32_t q = H.off[g]; q < H.off[g + 1]; ++q) shr[size_t(H.idx[size_t(q)])] = 1;
+            if (int rc = upload(&L.d_shr, shr)) return rc;
+        }
         auto mem = [&](int64_t ref) -> int64_t {
             const int64_t e = ref / n3;
             return e * n3 + pos[size_t(ref - e * n3)];

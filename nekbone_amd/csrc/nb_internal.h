@@ -300,6 +300,10 @@ struct GsArgs {
     const double *pre = nullptr;
     int64_t npre = 0;
 };
+// Device view of a group map (the problem's own: gs_args); the workgroup
+// counts b2..br are those of the gather-scatter kernel.
+GsArgs gs_args_map(const GsMap &M);
+GsArgs gs_args(const nekbone_problem *P);
 // Arguments of the fused CG gather-scatter of one problem.
 struct GsCgArgs {
     double *w;
@@ -322,6 +326,7 @@ struct HsmgLevel {
     bool fine = false;
     GsMap gs;                    // coarser levels: own numbering
     int32_t *d_pos = nullptr;    // memory offset of reference point p within an element (nh^3)
+    uint8_t *d_shr = nullptr;    // coarser levels: 1 where the point is shared (in a gs group)
     uint8_t *d_bnd = nullptr;    // 1: zeroed by this level's Dirichlet mask
     double *d_c = nullptr;       // inverse multiplicity (restriction weight)
     double *d_sw = nullptr;      // square root of the Schwarz weight (1 away from faces)

@@ -48,36 +48,6 @@ __device__ __forceinline__ void gs_sum(T *__restrict__ w, const T *__restrict__ 
     for (int q = 0; q < S; ++q) gs_finish<T, CG>(w, p, flag, mbit, m[q], t, c, acc);
 }
 
-// Members of group g of a class of size S: the first nrun groups are in
-// run form (runs of R; group t of a run has the run's first members + t),
-// the rest in the member list.
-template <int S>
-__device__ __forceinline__ void gs_members(const int32_t *__restrict__ list,
-                                           const int32_t *__restrict__ runs, int R, int64_t nrun,
-                                           int64_t g, int (&m)[S])
-{
-    const int32_t *mem;
-    int t = 0;
-    if (g < nrun) {
-        const int64_t run = g / R;
-        t = int(g - run * R);
-        mem = runs + S * run;
-    } else {
-        mem = list + S * (g - nrun);
-    }
-    if constexpr (S == 2) {
-        const int2 v = *reinterpret_cast<const int2 *>(mem);
-        m[0] = v.x + t, m[1] = v.y + t;
-    } else {
-#pragma unroll
-        for (int h = 0; h < S / 4; ++h) {
-            const int4 v = reinterpret_cast<const int4 *>(mem)[h];
-            m[4 * h] = v.x + t, m[4 * h + 1] = v.y + t, m[4 * h + 2] = v.z + t,
-                  m[4 * h + 3] = v.w + t;
-        }
-    }
-}
-
 template <typename T, int S, bool CG>
 __device__ __forceinline__ void gs_group(T *__restrict__ w, const T *__restrict__ p,
                                          const uint8_t *__restrict__ flag, int mbit,
@@ -94,7 +64,7 @@ __device__ __forceinline__ void gs_group(T *__restrict__ w, const T *__restrict_
 // range): every load of all of them is issued before the first store,
 // since the compiler may not move a load of a later pair above a store of
 // an earlier one.  Each pair is summed and finished as gs_sum does.  Two
-// measured best at 32^3 elements, N=9 (four: 1.5% slower in FP64).
+// measured best at 32^3 elements, N=9 (one and four: within 2%, FP64 and FP32).
 constexpr int kPairs = 2;
 template <typename T, bool CG>
 __device__ __forceinline__ void gs_pairs(T *__restrict__ w, const T *__restrict__ p,
@@ -211,19 +181,7 @@ __global__ void __launch_bounds__(kRedThreads)
     gs_body<double, true>(A.w, A.p, A.flag, A.mbit, A.G, A.part, A.scal, b - start[lo]);
 }
 
-GsArgs gs_args_map(const GsMap &M)
-{
-    GsArgs G{M.d_m2, M.d_m4, M.d_m8, M.d_roff, M.d_ridx, M.n2, M.n4, M.n8, M.nr, 0, 0, 0, 0, 0,
-             M.d_m2r, M.d_m4r, M.r2, M.r4, M.n2r, M.n4r};
-    auto nb = [](int64_t n) { return int((n + kRedThreads - 1) / kRedThreads); };
-    G.b2 = int((M.n2 + kPairs * kRedThreads - 1) / (kPairs * kRedThreads));  // kPairs per thread
-    G.b4 = nb(M.n4);
-    G.b8 = nb(M.n8);
-    G.br = nb(M.nr);
-    return G;
-}
 
-GsArgs gs_args(const nekbone_problem *P) { return gs_args_map(P->gs); }
 
 // ---------------------------------------------------------------------
 // After dssum: w = w + 0.1 p (add2s2, src/cg.f:118), then the Dirichlet
@@ -462,6 +420,20 @@ __global__ void k_aos_soa(const double *__restrict__ src, double *__restrict__ d
 int grid1d(int64_t n, int threads) { return int((n + threads - 1) / threads); }
 
 }  // namespace
+
+GsArgs gs_args_map(const GsMap &M)
+{
+    GsArgs G{M.d_m2, M.d_m4, M.d_m8, M.d_roff, M.d_ridx, M.n2, M.n4, M.n8, M.nr, 0, 0, 0, 0, 0,
+             M.d_m2r, M.d_m4r, M.r2, M.r4, M.n2r, M.n4r};
+    auto nb = [](int64_t n) { return int((n + kRedThreads - 1) / kRedThreads); };
+    G.b2 = int((M.n2 + kPairs * kRedThreads - 1) / (kPairs * kRedThreads));  // kPairs per thread
+    G.b4 = nb(M.n4);
+    G.b8 = nb(M.n8);
+    G.br = nb(M.nr);
+    return G;
+}
+
+GsArgs gs_args(const nekbone_problem *P) { return gs_args_map(P->gs); }
 
 int reduce_blocks(int64_t n)
 {
