@@ -113,6 +113,19 @@ int nekbone_set_arith(nekbone_problem *P, int arith);
  * separate passes.  Values agree bit for bit between the two; only the
  * dot products' summation trees differ. */
 int nekbone_set_fused(nekbone_problem *P, int on);
+/* Form of the fused CG iteration.  on = 1 (default): where the problem's
+ * shared-point map has the entity form of a hex mesh (every group's
+ * members at the same offset within matching vertices, edges or faces)
+ * and the full Dirichlet mask applies, the vector-update kernel sums the
+ * copies of each shared point itself (in the reference's member order)
+ * and the element kernel forms pap from its own copies, so an iteration
+ * runs two vector kernels instead of three; on = 0: the separate
+ * gather-scatter kernel; on = -1: leave the setting.  *active (if not
+ * null) receives 1 when the gather form applies to the next solve.  Both
+ * forms compute w with the same operations in the same order; only the
+ * pap sum (hence alpha) is accumulated in another order.  No counterpart
+ * in the reference, whose dssum is a separate pass (src/cg.f:116). */
+int nekbone_set_gather(nekbone_problem *P, int on, int *active);
 /* 1 (default): CG iterations are replayed in chunks of 10 from a HIP graph
  * captured on first use (re-captured when buffers or settings change; not
  * while per-kernel timing is enabled); 0: every kernel launched directly.

@@ -54,6 +54,7 @@ _SIGS = {
     "nekbone_reset_mask": (ctypes.c_int, [_vp]),
     "nekbone_set_arith": (ctypes.c_int, [_vp, ctypes.c_int]),
     "nekbone_set_fused": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "nekbone_set_gather": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "nekbone_setup_jacobi": (ctypes.c_int, [_vp]),
     "nekbone_setup_hsmg": (ctypes.c_int, [_vp]),
     "nekbone_hsmg_solve": (ctypes.c_int, [_vp, _dp, _dp]),
@@ -395,6 +396,14 @@ class Problem:
 
     def set_fused(self, on):
         _chk(lib().nekbone_set_fused(self.handle, int(bool(on))))
+
+    def set_gather(self, on=None):
+        """Gather form of the fused CG (True/False; None keeps the setting).
+        Returns whether it applies to the next solve."""
+        act = ctypes.c_int(0)
+        _chk(lib().nekbone_set_gather(self.handle, -1 if on is None else int(bool(on)),
+                                      ctypes.byref(act)))
+        return bool(act.value)
 
     def setup_jacobi(self):
         _chk(lib().nekbone_setup_jacobi(self.handle))

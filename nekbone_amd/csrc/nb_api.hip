@@ -212,6 +212,89 @@ int download_ref(nekbone_problem *P, double *dst, const double *src)
     return NEKBONE_OK;
 }
 
+// Entity table of the gather form of the fused CG (GxView): from the
+// groups (members as reference local indices, in the reference's member
+// order) and the device-order flags.  False when some group does not have
+// the entity form (all members at the same offset within an entity of the
+// same kind, one slot list per entity) or some entity is only partly on
+// the Dirichlet list; the fused CG then keeps the gather-scatter kernel.
+static bool build_gx_table(const nekbone_problem *P, const std::vector<int32_t> &off,
+                           const std::vector<int32_t> &idx,
+                           const std::vector<uint8_t> &flag_dev, std::vector<int32_t> &tab)
+{
+    const int nx = P->nx1, I = nx - 2, n3 = nx * nx * nx;
+    const int64_t nelt = P->nelt;
+    tab.assign(size_t(nelt) * kGxStride, -1);
+    for (int64_t e = 0; e < nelt; ++e)
+        for (int k = kGxMask; k < kGxStride; ++k) tab[size_t(e * kGxStride + k)] = 0;
+    struct Ent {
+        int sb, K, o, bit;
+    };
+    // entity of device position d within an element (false: interior)
+    auto ent = [&](int d, Ent &x) -> bool {
+        if (d < 8) {
+            x = Ent{8 * d, 8, 0, d};
+            return true;
+        }
+        if (I > 0 && d < 8 + 12 * I) {
+            const int k = (d - 8) / I;
+            x = Ent{64 + 4 * k, 4, (d - 8) - k * I, 8 + k};
+            return true;
+        }
+        if (I > 0 && d < 8 + 12 * I + 6 * I * I) {
+            const int f = (d - 8 - 12 * I) / (I * I);
+            x = Ent{112 + 2 * f, 2, (d - 8 - 12 * I) - f * I * I, 20 + f};
+            return true;
+        }
+        return false;
+    };
+    std::vector<int64_t> pos;
+    for (size_t g = 0; g + 1 < off.size(); ++g) {
+        const int S = off[g + 1] - off[g];
+        pos.clear();
+        int o0 = -1;
+        for (int32_t q = off[g]; q < off[g + 1]; ++q) {
+            const int64_t D = dev_index(P, idx[size_t(q)]);
+            const int64_t e = D / n3;
+            Ent x;
+            if (!ent(int(D - e * n3), x) || S > x.K) return false;
+            if (q == off[g])
+                o0 = x.o;
+            else if (x.o != o0)
+                return false;
+            pos.push_back(D);
+        }
+        for (int k = 0; k < S; ++k) {
+            const int64_t e = pos[size_t(k)] / n3;
+            Ent x;
+            ent(int(pos[size_t(k)] - e * n3), x);
+            int32_t *sl = &tab[size_t(e * kGxStride + x.sb)];
+            if (sl[0] == -1) {
+                for (int m = 0; m < S; ++m) sl[m] = int32_t(pos[size_t(m)] - o0);
+            } else {
+                for (int m = 0; m < x.K; ++m)
+                    if (sl[m] != (m < S ? int32_t(pos[size_t(m)] - o0) : -1)) return false;
+            }
+        }
+    }
+    int8_t seen[26];
+    for (int64_t e = 0; e < nelt; ++e) {
+        std::fill(seen, seen + 26, int8_t(-1));
+        for (int d = 0; d < n3; ++d) {
+            Ent x;
+            if (!ent(d, x)) continue;
+            const int8_t m = (flag_dev[size_t(e * n3 + d)] & kFlagFull) ? 1 : 0;
+            if (seen[x.bit] < 0) {
+                seen[x.bit] = m;
+                if (m) tab[size_t(e * kGxStride + kGxMask)] |= int32_t(1u << x.bit);
+            } else if (seen[x.bit] != m) {
+                return false;
+            }
+        }
+    }
+    return true;
+}
+
 int set_c_host(nekbone_problem *P, const double *c)
 {
     std::vector<int32_t> off, idx;
@@ -367,6 +450,15 @@ static int finish_problem(nekbone_problem *P, const double *c_host)
     for (int64_t l = 0; l < n; ++l) flag_dev[size_t(dev_index(P, l))] = flag[size_t(l)];
     if (int rc = dalloc(&P->d_flag, n)) return rc;
     if (int rc = h2d(P->d_flag, flag_dev.data(), size_t(n))) return rc;
+    {
+        std::vector<int32_t> tab;
+        if (build_gx_table(P, off, idx, flag_dev, tab)) {
+            if (int rc = dalloc(&P->d_gx, int64_t(tab.size()))) return rc;
+            if (int rc = h2d(P->d_gx, tab.data(), sizeof(int32_t) * tab.size())) return rc;
+            if (int rc = dalloc(&P->d_gxbad, 1)) return rc;
+            P->gx_ok = true;
+        }
+    }
 
     if (c_host) {
         if (int rc = upload_dev(P, P->d_c, c_host)) return rc;
@@ -447,7 +539,7 @@ static int run_iterations(nekbone_problem *P, double *x, double *r, int precond,
             uintptr_t(x), uintptr_t(r), uintptr_t(precond), uintptr_t(mbit),
             uintptr_t(P->d_hist), uintptr_t(P->hist_cap), uintptr_t(P->axv.epb),
             uintptr_t(P->axv.pf), uintptr_t(P->axv.xu), uintptr_t(P->arith),
-            uintptr_t(P->fused), uintptr_t(P->mg), uintptr_t(P->d_dinv),
+            uintptr_t(P->fused), uintptr_t(cg_gather(P)), uintptr_t(P->mg), uintptr_t(P->d_dinv),
             uintptr_t(P->hsmg_fp32), uintptr_t(P->hsmg_coarse)};
         if (!P->graph || P->graph_key != key) {
             drop_problem_graph(P);
@@ -481,6 +573,7 @@ static int run_cg(nekbone_problem *P, double *x, const double *f, double *r, int
     if (int rc = check_precond(P, precond)) return rc;
     if (niter < 0) return fail(NEKBONE_ERR_ARG, "cg: niter < 0");
     if (int rc = ensure_hist(P, niter)) return rc;
+    if (int rc = cg_gather_check(P, f)) return rc;
     const int mbit0 = next_mask_bit(P);  // maskit(r), src/cg.f:40
     if (int rc = launch_cg_init(P, x, r, f, mbit0, precond, tol2, cg_zform(P, precond)))
         return rc;
@@ -625,6 +718,8 @@ extern "C" int nekbone_problem_destroy(nekbone_problem *P)
     dfree(P->d_c);
     dfree(P->d_dinv);
     dfree(P->d_flag);
+    dfree(P->d_gx);
+    dfree(P->d_gxbad);
     dfree(P->d_perm);
     dfree(P->d_iref);
     dfree(P->gs.d_m2);
@@ -784,6 +879,13 @@ extern "C" int nekbone_set_variant(nekbone_problem *P, int epb, int pf, int xu)
     return fail(NEKBONE_ERR_ARG, "set_variant: no such kernel shape for this order");
 }
 
+int nb::tune_rhs(nekbone_problem *P, uint64_t seed)
+{
+    if (int rc = launch_rand(P->d_f, P->n, seed)) return rc;
+    if (int rc = launch_gs(P, P->d_f, nullptr)) return rc;
+    return cg_gather_check(P, P->d_f);
+}
+
 extern "C" int nekbone_autotune(nekbone_problem *P, int precond, double *best_ms)
 {
     if (int rc = check(P)) return rc;
@@ -797,7 +899,7 @@ extern "C" int nekbone_autotune(nekbone_problem *P, int precond, double *best_ms
     const int mbit = P->mask_mode == NEKBONE_MASK_DIRICHLET ? 1 : 2;
     const int warm = 2, reps = 6;
     if (int rc = ensure_hist(P, warm + reps)) return rc;
-    if (int rc = launch_rand(P->d_f, P->n, 12345)) return rc;
+    if (int rc = tune_rhs(P, 12345)) return rc;
     hipEvent_t e0, e1;
     NB_HIP(hipEventCreate(&e0));
     NB_HIP(hipEventCreate(&e1));
@@ -901,6 +1003,15 @@ extern "C" int nekbone_set_fused(nekbone_problem *P, int on)
 {
     if (int rc = check(P)) return rc;
     P->fused = on != 0;
+    return NEKBONE_OK;
+}
+
+extern "C" int nekbone_set_gather(nekbone_problem *P, int on, int *active)
+{
+    if (int rc = check(P)) return rc;
+    if (on == 0 || on == 1) P->gather = on;
+    else if (on != -1) return fail(NEKBONE_ERR_ARG, "set_gather: on must be 0, 1 or -1");
+    if (active) *active = cg_gather(P) ? 1 : 0;
     return NEKBONE_OK;
 }
 

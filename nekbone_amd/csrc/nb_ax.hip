@@ -92,7 +92,7 @@ int launch_ax_local(nekbone_problem *P, double *w, const double *u)
     return timed_end(P, 0);
 }
 
-int launch_ax_cg(nekbone_problem *P, double *x, const double *z, int mbit, int rev)
+int launch_ax_cg(nekbone_problem *P, double *x, const double *z, int mbit, int rev, int gx)
 {
     AxFn fn = lookup(P->nx1, P->axv.epb, P->arith == NEKBONE_ARITH_FMA, 1, P->axv.pf,
                      P->axv.xu);
@@ -113,6 +113,7 @@ int launch_ax_cg(nekbone_problem *P, double *x, const double *z, int mbit, int r
     A.part = P->d_part;
     A.nelt = P->nelt;
     A.rev = rev;
+    A.gx = gx;
     const int threads = round64(P->nx1 * P->nx1 * P->axv.epb);
     if (int rc = timed_begin(P, 0)) return rc;
     hipLaunchKernelGGL(fn, dim3(ax_blocks(P)), dim3(threads), 0, stream(), A);

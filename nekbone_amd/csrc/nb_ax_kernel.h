@@ -50,7 +50,10 @@
 //             the element operator: w = w + 0.1 p (src/cg.f:118), mask
 //             (src/cg.f:119), and their part of pap = sum (w c) p
 //             (src/cg.f:59) with c = 1 there.  Shared points are left
-//             for the gather-scatter kernel, which finishes them.
+//             for the gather-scatter kernel, which finishes them, or in
+//             the gather form (gx) for the vector-update kernel, which
+//             sums their copies itself; then this kernel's pap parts
+//             cover the shared points as well.
 // The variant (EPB, PF, XU) is chosen per order by the autotuner.
 #pragma once
 
@@ -60,7 +63,7 @@
 
 #include <hip/hip_fp16.h>
 
-#include "nb_device.h"
+#include "nb_cg_device.h"
 
 namespace nb {
 namespace axk {
@@ -215,6 +218,7 @@ struct AxArgsT {
     int nelt;
     int64_t gcs;         // distance between factor planes (local points n)
     int rev;             // walk the elements downward (see sweep_block)
+    int gx;              // CG, gather form: the pap parts cover shared points too (below)
     // MODE 3 (first pass of the FP64 residual refresh of the mixed-
     // precision CG): x = x + the FP32 increment x32 (+ alpha p32 when
     // pending), x32 = 0, then w = A_L x at every point as in MODE 0.
@@ -554,7 +558,18 @@ __device__ __forceinline__ void ax_core(const AxArgsT<T> &A_, const int blk, T *
             } else {
                 const uint8_t fl = FLPRE ? uint8_t(flp[m / 4] >> (8 * (m % 4))) : A_.flag[q];
                 if (fl & kFlagShared) {
-                    A_.w[q] = wl;  // finished by the gather-scatter kernel
+                    A_.w[q] = wl;  // finished by the gather-scatter or update kernel
+                    if (A_.gx && !(fl & A_.mbit)) {
+                        // gather form: pap = sum_i c_i p_i w_i over the
+                        // copies of a shared group G equals
+                        // sum_{j in G} (p_j w_L,j + 0.1 c_j p_j^2), since
+                        // the copies of p agree, sum_G c = 1, and p = 0
+                        // where w is masked; so every element adds its own
+                        // copies' terms and no kernel after the sum needs
+                        // to form pap
+                        const double pq = double(pd[m]);
+                        acc = acc + pq * (double(wl) + 0.1 * inv_mult(fl) * pq);
+                    }
                 } else {
                     T v = wl + T(0.1) * pd[m];
                     if (fl & A_.mbit) v = T(0);

@@ -99,7 +99,11 @@ void apply_shapes(nekbone_batch *B)
 {
     for (auto &G : B->groups) {
         const int epb = in_queue(B, G.nx1) ? tableq()[size_t(B->queue)].e[qslot(G.nx1)] : G.v.epb;
-        for (int k : G.members) B->P[size_t(k)]->axv = AxVariant{epb, 0, B->xu};
+        for (int k : G.members) {
+            B->P[size_t(k)]->axv = AxVariant{epb, 0, B->xu};
+            // the batch runs the gather-scatter form of the iteration
+            B->P[size_t(k)]->gather = 0;
+        }
     }
     ++B->shapes;
 }

@@ -177,6 +177,112 @@ __global__ void __launch_bounds__(kRedThreads)
                               sweep_block(blockIdx.x, nblk, rev));
 }
 
+// Gather-form update, one workgroup per G.eb consecutive elements (their
+// table rows staged in LDS): for shared points w is formed here as the
+// sum of the copies in member order + 0.1 p, masked (what the
+// gather-scatter kernel stores), then the update of update_body.  The
+// member loads depend only on the point's position and the LDS table, so
+// they are issued with the points' own loads.
+template <int PRE, bool FMA, bool XP>
+__global__ void __launch_bounds__(kRedThreads)
+    k_update_gx(double *__restrict__ r, const double *__restrict__ w,
+                const double *__restrict__ dinv, const uint8_t *__restrict__ flag,
+                double *__restrict__ x, const double *__restrict__ p, int nelt,
+                double *__restrict__ part, int nblk, const CgScalars *__restrict__ s, int rev,
+                GxView G)
+{
+    __shared__ int32_t tabs[kGxMaxEb * kGxStride];
+    const int blk = sweep_block(blockIdx.x, nblk, rev);
+    if (s->done) return;
+    const int64_t e0 = int64_t(blk) * G.eb;
+    const int ne = int(nelt - e0 < G.eb ? nelt - e0 : G.eb);
+    gx_rows<kRedThreads>(tabs, G, e0, ne);
+    __shared__ double vsum[kGxMaxEb * 8];
+    gx_vertices<double, kRedThreads>(vsum, w, tabs, ne);
+    const int np = ne * G.n3;
+    const int64_t beg = e0 * G.n3;
+    const double alpha = s->alpha, alphm = -alpha;
+    double a0 = 0.0, a1 = 0.0;
+    constexpr int U = kGxU;
+    for (int j0 = threadIdx.x; j0 < np; j0 += U * kRedThreads) {
+        double rv[U], wv[U], dv[U], xv[U], pv[U], ps[U];
+        uint8_t fv[U];
+        bool sh[U], gm[U];
+        double gv[U][4];
+        int gc[U], go[U], gs[U], ge[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = j0 + u * kRedThreads;
+            sh[u] = false;
+            if (j < np) {
+                const int64_t i = beg + j;
+                rv[u] = r[i];
+                wv[u] = w[i];
+                fv[u] = flag[i];
+                if constexpr (pre_jacobi(PRE)) dv[u] = dinv[i];
+                if constexpr (XP) {
+                    xv[u] = x[i];
+                    pv[u] = p[i];
+                }
+                const int el = gx_divf(j, G.n3, G.inv_n3f);
+                ge[u] = el;
+                sh[u] = gx_issue(gv[u], gc[u], go[u], gs[u], gm[u], w, tabs + el * kGxStride, G,
+                                 j - el * G.n3, i);
+                if (sh[u]) ps[u] = XP ? pv[u] : p[i];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (sh[u]) {
+                double v = gx_finish(gv[u], gc[u], go[u], gs[u], vsum + 8 * ge[u],
+                                     wv[u]) +
+                           0.1 * ps[u];
+                if (gm[u]) v = 0.0;
+                wv[u] = v;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = j0 + u * kRedThreads;
+            if (j >= np) continue;
+            const int64_t i = beg + j;
+            if constexpr (XP) {
+                if constexpr (FMA)
+                    x[i] = __builtin_fma(alpha, pv[u], xv[u]);
+                else
+                    x[i] = xv[u] + alpha * pv[u];
+            }
+            if constexpr (PRE == kPreJacobiZ) {
+                double zi;
+                if constexpr (FMA)
+                    zi = __builtin_fma(alphm, dv[u] * wv[u], rv[u]);
+                else
+                    zi = rv[u] + alphm * (dv[u] * wv[u]);
+                r[i] = zi;
+                const double ri = zi / dv[u];
+                const double rc = ri * inv_mult(fv[u]);
+                a0 = a0 + rc * ri;
+                a1 = a1 + rc * zi;
+            } else {
+                double ri;
+                if constexpr (FMA)
+                    ri = __builtin_fma(alphm, wv[u], rv[u]);
+                else
+                    ri = rv[u] + alphm * wv[u];
+                r[i] = ri;
+                const double rc = ri * inv_mult(fv[u]);
+                a0 = a0 + rc * ri;
+                if constexpr (PRE == NEKBONE_PRECOND_JACOBI) a1 = a1 + rc * (ri * dv[u]);
+            }
+        }
+    }
+    block_sum2(a0, a1);
+    if (threadIdx.x == 0) {
+        part[blk] = a0;
+        part[nblk + blk] = a1;
+    }
+}
+
 template <int PRE>
 __global__ void __launch_bounds__(kFinalThreads)
     k_update_final(const double *__restrict__ part, int nblk, CgScalars *__restrict__ s,
@@ -356,8 +462,11 @@ int launch_update_final(nekbone_problem *P, const double *part, int nu, int prec
 
 int64_t cg_partials_needed(const nekbone_problem *P)
 {
-    // element-kernel parts: at most one per element (smallest workgroup)
-    return std::max<int64_t>(P->nelt + n_gs(P) + 2 * n_upd(P), 2 * int64_t(kMaxRedBlocks));
+    // element-kernel parts: at most one per element (smallest workgroup);
+    // update parts: the generic update's or the gather form's workgroups
+    const int eb = std::min(gx_eb(P->nx1, kGxPts64), gx_eb(P->nx1, kGxPts32));
+    const int64_t nux = std::max<int64_t>(n_upd(P), (P->nelt + eb - 1) / eb);
+    return std::max<int64_t>(P->nelt + n_gs(P) + 2 * nux, 2 * int64_t(kMaxRedBlocks));
 }
 
 bool cg_zform(const nekbone_problem *P, int precond)
@@ -407,11 +516,73 @@ int launch_cg_init(nekbone_problem *P, double *x, double *r, const double *f, in
         }                                                            \
     } while (0)
 
+// Gather form of the fused iteration (cg_gather): element kernel (pap
+// parts over all points) -> pap/alpha -> update kernel that also sums the
+// shared points' copies -> rtr/beta.  Two vector passes and two scalar
+// kernels; w, and with it x and z, bit-identical to the gather-scatter
+// form, pap summed in another order.  The element kernel walks upward and
+// the update downward, so each starts where the other ended.
+static int iteration_gather(nekbone_problem *P, double *x, double *r, int precond, int mbit)
+{
+    const bool jac = precond == NEKBONE_PRECOND_JACOBI;
+    const bool fma = P->arith == NEKBONE_ARITH_FMA;
+    const int64_t na = n_ax(P), ng = n_gs(P);
+    const GxView G = gx_view(P, kGxPts64);
+    const int nu = int((P->nelt + G.eb - 1) / G.eb);
+    double *part_upd = P->d_part + na + ng;
+    if (int rc = launch_ax_cg(P, x, r, mbit, 0, 1)) return rc;
+    if (int rc = timed_begin(P, 4)) return rc;
+    hipLaunchKernelGGL(k_pap_final, dim3(1), dim3(kFinalThreads), 0, stream(), P->d_part, na,
+                       P->d_scal, P->d_hist, P->hist_cap);
+    NB_HIP(hipGetLastError());
+    if (int rc = timed_end(P, 4)) return rc;
+    if (int rc = timed_begin(P, 3)) return rc;
+    const bool xp = !P->axv.xu;
+#define NB_UGX2(PRE, F, XP)                                                                    \
+    hipLaunchKernelGGL((k_update_gx<PRE, F, XP>), dim3(nu), dim3(kRedThreads), 0, stream(), r, \
+                       P->d_w, P->d_dinv, P->d_flag, x, P->d_p, P->nelt, part_upd, nu,         \
+                       P->d_scal, 1, G)
+#define NB_UGX(PRE, F)              \
+    do {                            \
+        if (xp)                     \
+            NB_UGX2(PRE, F, true);  \
+        else                        \
+            NB_UGX2(PRE, F, false); \
+    } while (0)
+    if (jac) {
+        if (fma)
+            NB_UGX(kPreJacobiZ, true);
+        else
+            NB_UGX(kPreJacobiZ, false);
+    } else {
+        if (fma)
+            NB_UGX(NEKBONE_PRECOND_IDENTITY, true);
+        else
+            NB_UGX(NEKBONE_PRECOND_IDENTITY, false);
+    }
+#undef NB_UGX
+#undef NB_UGX2
+    NB_HIP(hipGetLastError());
+    if (int rc = timed_end(P, 3)) return rc;
+    if (int rc = timed_begin(P, 4)) return rc;
+    if (jac)
+        hipLaunchKernelGGL(k_update_final<NEKBONE_PRECOND_JACOBI>, dim3(1), dim3(kFinalThreads),
+                           0, stream(), part_upd, nu, P->d_scal, P->d_hist, P->hist_cap,
+                           P->axv.xu);
+    else
+        hipLaunchKernelGGL(k_update_final<NEKBONE_PRECOND_IDENTITY>, dim3(1),
+                           dim3(kFinalThreads), 0, stream(), part_upd, nu, P->d_scal, P->d_hist,
+                           P->hist_cap, P->axv.xu);
+    NB_HIP(hipGetLastError());
+    return timed_end(P, 4);
+}
+
 // zin: the vector the element kernel reads in place of r (an explicit
 // z = M^-1 r from the HSMG preconditioner); null: r itself.
 static int iteration_fused(nekbone_problem *P, double *x, double *r, int precond, int mbit,
                            const double *zin = nullptr)
 {
+    if (!zin && cg_gather(P)) return iteration_gather(P, x, r, precond, mbit);
     const bool jac = precond == NEKBONE_PRECOND_JACOBI;
     const bool fma = P->arith == NEKBONE_ARITH_FMA;
     const int64_t na = n_ax(P), ng = n_gs(P);
@@ -668,6 +839,75 @@ int launch_cg_batch_tail(const CgBatchArgs *d_args, const int *d_ustart, int npr
 int cg_ax_blocks(const nekbone_problem *P) { return int(n_ax(P)); }
 bool cg_fold_partials(const nekbone_problem *P) { return n_gs(P) * kRedThreads >= n_ax(P); }
 int cg_gs_blocks(const nekbone_problem *P) { return int(n_gs(P)); }
+
+static bool gather_applies(const nekbone_problem *P)
+{
+    return P->gx_ok && P->gather && P->c_is_mult && P->fused &&
+           P->mask_mode == NEKBONE_MASK_DIRICHLET;
+}
+
+bool cg_gather(const nekbone_problem *P) { return gather_applies(P) && P->gx_cont; }
+
+namespace {
+// Continuity of f at the unmasked shared points: every member of a
+// point's group holds the same value (else *bad = 1).
+__global__ void __launch_bounds__(kRedThreads)
+    k_gx_check(const double *__restrict__ f, int nelt, GxView G, int *__restrict__ bad)
+{
+    __shared__ int32_t tabs[kGxMaxEb * kGxStride];
+    const int64_t e0 = int64_t(blockIdx.x) * G.eb;
+    const int ne = int(nelt - e0 < G.eb ? nelt - e0 : G.eb);
+    gx_rows<kRedThreads>(tabs, G, e0, ne);
+    const int np = ne * G.n3;
+    const int64_t beg = e0 * G.n3;
+    bool ok = true;
+    for (int j = threadIdx.x; j < np; j += kRedThreads) {
+        const int el = gx_divf(j, G.n3, G.inv_n3f), d = j - el * G.n3;
+        int sb, K, o, bit;
+        if (!gx_entity(d, G, sb, K, o, bit)) continue;
+        const int32_t *tb = tabs + el * kGxStride;
+        if (tb[sb] < 0 || ((tb[kGxMask] >> bit) & 1)) continue;
+        const double v = f[beg + j];
+        for (int m = 0; m < K; ++m) {
+            const int s = tb[sb + m];
+            if (s < 0) break;
+            ok = ok && f[s + o] == v;
+        }
+    }
+    if (!ok) *bad = 1;
+}
+}  // namespace
+
+int cg_gather_check(nekbone_problem *P, const double *f)
+{
+    P->gx_cont = true;
+    if (!gather_applies(P)) return NEKBONE_OK;
+    if (int rc = dzero(P->d_gxbad, sizeof(int))) return rc;
+    const GxView G = gx_view(P, kGxPts64);
+    const int nb = int((P->nelt + G.eb - 1) / G.eb);
+    hipLaunchKernelGGL(k_gx_check, dim3(nb), dim3(kRedThreads), 0, stream(), f, P->nelt, G,
+                       P->d_gxbad);
+    NB_HIP(hipGetLastError());
+    int bad = 0;
+    if (int rc = d2h(&bad, P->d_gxbad, sizeof(int))) return rc;
+    P->gx_cont = bad == 0;
+    return NEKBONE_OK;
+}
+
+GxView gx_view(const nekbone_problem *P, int pts)
+{
+    GxView G{};
+    G.tab = P->d_gx;
+    G.nx = P->nx1;
+    G.I = P->nx1 - 2;
+    G.n3 = P->nx1 * P->nx1 * P->nx1;
+    G.inv_n3 = 1.0 / double(G.n3);
+    G.inv_n3f = 1.0f / float(G.n3);
+    G.inv_I = G.I > 0 ? 1.0f / float(G.I) : 0.0f;
+    G.inv_I2 = G.I > 0 ? 1.0f / float(G.I * G.I) : 0.0f;
+    G.eb = gx_eb(P->nx1, pts);
+    return G;
+}
 
 // z form: r = z / dinv in place.
 __global__ void k_z_to_r(double *__restrict__ r, const double *__restrict__ dinv, int64_t n)

@@ -10,6 +10,10 @@ namespace nb {
 
 constexpr int kFinalThreads = 1024;
 
+// Points per thread whose loads the gather-form update kernels keep in
+// flight together (measured best of 1, 2, 4, 8 for FP64 and FP32).
+constexpr int kGxU = 2;
+
 // Template value of PRE for the fused Jacobi CG in z form: the vector the
 // iterations carry in place of r is z = M^-1 r (updated as z -= alpha
 // M^-1 w), so the element kernel reads z alone; r = z / dinv where the dot
@@ -23,6 +27,124 @@ __device__ __forceinline__ double inv_mult(uint8_t fl)
 {
     const int m = fl >> kMultShift;
     return m == 1 ? 1.0 : m == 2 ? 0.5 : m == 4 ? 0.25 : m == 8 ? 0.125 : 1. / double(m);
+}
+
+// x / D for 0 <= x < 2^24 and D >= 1, inv = 1/D rounded to float: the
+// float quotient is within one of the true one, which the remainder
+// corrects.
+__device__ __forceinline__ int gx_divf(int x, int D, float inv)
+{
+    int q = int(float(x) * inv);
+    const int rem = x - q * D;
+    if (rem < 0)
+        --q;
+    else if (rem >= D)
+        ++q;
+    return q;
+}
+
+// Entity of in-element device position d (GxView layout): slot base sb,
+// slots K, offset o within the entity, mask bit; false for the interior.
+__device__ __forceinline__ bool gx_entity(int d, const GxView &G, int &sb, int &K, int &o,
+                                          int &bit)
+{
+    const int I = G.I;
+    if (d < 8) {
+        sb = 8 * d, K = 8, o = 0, bit = d;
+        return true;
+    }
+    if (d < 8 + 12 * I) {
+        const int x = d - 8, k = gx_divf(x, I, G.inv_I);
+        o = x - k * I, sb = 64 + 4 * k, K = 4, bit = 8 + k;
+        return true;
+    }
+    const int x = d - 8 - 12 * I, I2 = I * I;
+    if (x >= 6 * I2) return false;
+    const int f = gx_divf(x, I2, G.inv_I2);
+    o = x - f * I2, sb = 112 + 2 * f, K = 2, bit = 20 + f;
+    return true;
+}
+
+// Per-element gather-form update kernels: the member sum of a shared
+// point, in two phases so that the loads of all of a thread's points are
+// in flight together with the points' own loads.  gx_issue decides from
+// the entity table alone whether in-element position d is shared (a
+// surface entity with a slot list), loads the members of a face or edge
+// point (at most four; the point's own value is not loaded again) into v,
+// and for a vertex point (up to eight members) leaves the loads to
+// gx_finish, which sums in member order.  tb: the LDS row of the point's
+// element.  The state is plain scalars and arrays per point so that it
+// stays in registers.
+template <typename T>
+__device__ __forceinline__ bool gx_issue(T (&v)[4], int &cnt, int &own, int &sb, bool &masked,
+                                         const T *__restrict__ w, const int32_t *tb,
+                                         const GxView &G, int d, int64_t i)
+{
+    int K, o, bit;
+    if (!gx_entity(d, G, sb, K, o, bit)) return false;
+    if (tb[sb] < 0) return false;  // no group: not shared
+    masked = (tb[kGxMask] >> bit) & 1;
+    cnt = 0;
+    own = 0;
+    if (K == 8) return true;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const int s = m < K ? tb[sb + m] : -1;
+        if (s >= 0) {
+            ++cnt;
+            if (int64_t(s) + o == i)
+                own |= 1 << m;
+            else
+                v[m] = w[s + o];
+        }
+    }
+    return true;
+}
+
+// vsum: the member sums of the workgroup's vertex points (gx_vertices).
+template <typename T>
+__device__ __forceinline__ T gx_finish(const T (&v)[4], int cnt, int own, int sb,
+                                       const T *vsum, T self)
+{
+    if (cnt > 0) {
+        T t = (own & 1) ? self : v[0];
+#pragma unroll
+        for (int m = 1; m < 4; ++m)
+            if (m < cnt) t += ((own >> m) & 1) ? self : v[m];
+        return t;
+    }
+    return vsum[sb >> 3];
+}
+
+// Member sums of the shared vertex points of the workgroup's ne elements
+// (vsum[8 el + v], in member order), one thread per vertex with all its
+// member loads in flight; tabs: the elements' table rows in LDS.  Ends
+// with a barrier.
+template <typename T, int NT>
+__device__ __forceinline__ void gx_vertices(T *vsum, const T *__restrict__ w, const int32_t *tabs,
+                                            int ne)
+{
+    for (int t = threadIdx.x; t < 8 * ne; t += NT) {
+        const int32_t *sl = tabs + (t >> 3) * kGxStride + 8 * (t & 7);
+        T v[8];
+#pragma unroll
+        for (int m = 0; m < 8; ++m)
+            if (sl[m] >= 0) v[m] = w[sl[m]];
+        T s = v[0];
+#pragma unroll
+        for (int m = 1; m < 8; ++m)
+            if (sl[m] >= 0) s += v[m];
+        vsum[t] = s;
+    }
+    __syncthreads();
+}
+
+// Table rows of elements e0 .. e0+ne-1 into LDS (the workgroup's threads).
+template <int NT>
+__device__ __forceinline__ void gx_rows(int32_t *tabs, const GxView &G, int64_t e0, int ne)
+{
+    for (int t = threadIdx.x; t < ne * kGxStride; t += NT) tabs[t] = G.tab[e0 * kGxStride + t];
+    __syncthreads();
 }
 
 // Fixed-order sum of a strided slice with sixteen interleaved accumulators

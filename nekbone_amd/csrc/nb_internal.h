@@ -284,6 +284,40 @@ struct CgBatchArgs {
     int hcap;
 };
 
+// Entity table of the gather form of the fused CG (the vector update sums
+// the copies of each shared point itself instead of a gather-scatter
+// kernel).  Per element kGxStride int32: the element's surface entities in
+// device order (8 vertices, 12 edges, 6 faces), each with the device
+// position of its first point in every member element of its groups, in
+// the groups' member order (the reference's summation order), -1 after
+// the last member:
+//   vertex v: [8v, 8v+8)   edge k: [64+4k, 68+4k)   face f: [112+2f, 114+2f)
+//   [kGxMask]: bit v / 8+k / 20+f set where the entity is on the full
+//              Dirichlet list (uniform over the entity's points)
+// Point t of an entity has the members slot + t.  Built only when every
+// group of the map has this form (box meshes in the device point order).
+constexpr int kGxStride = 128;
+constexpr int kGxMask = 124;
+struct GxView {
+    const int32_t *tab;
+    int nx, I, n3;
+    double inv_n3;   // 1 / n3 (element of a device position)
+    float inv_n3f, inv_I, inv_I2;
+    int eb;          // elements per workgroup of the per-element update kernels
+};
+// Elements per workgroup of the gather-form update kernels: about pts
+// points per workgroup (measured at 32^3 elements, N=9, on MI355X: 2048
+// for FP64, 4096 for FP32), at most kGxMaxEb elements for the table rows
+// in LDS.
+constexpr int kGxMaxEb = 32;
+constexpr int kGxPts64 = 2048, kGxPts32 = 4096;
+inline int gx_eb(int nx, int pts)
+{
+    const int n3 = nx * nx * nx;
+    int eb = pts / n3;
+    return eb < 1 ? 1 : eb > kGxMaxEb ? kGxMaxEb : eb;
+}
+
 // Kernel view of a gather-scatter map: the class member lists and the
 // number of workgroups each class takes.
 struct GsArgs {
@@ -418,6 +452,11 @@ struct nekbone_problem {
     int64_t part_cap = 0;
     bool c_is_mult = false;    // c == 1/multiplicity everywhere (fused CG path allowed)
     bool fused = true;         // use the fused CG path when c_is_mult
+    int32_t *d_gx = nullptr;   // entity table of the gather form (kGxStride per element)
+    int *d_gxbad = nullptr;    // continuity check result (device)
+    bool gx_ok = false;        // the map has the entity form (d_gx built)
+    int gather = 1;            // fused CG: gather form where available (0: gather-scatter kernel)
+    bool gx_cont = true;       // the current solve's right-hand side is continuous (checked)
     nb::AxVariant axv;         // element-kernel shape
     nb::CgScalars *d_scal = nullptr;
     double *d_hist = nullptr;
@@ -457,7 +496,10 @@ std::vector<AxVariant> ax_variants(int nx1);
 int launch_ax_local(nekbone_problem *P, double *w, const double *u);
 // element kernel of the fused CG iteration; z is the vector the CG carries
 // (r with the identity, M^-1 r with Jacobi in z form)
-int launch_ax_cg(nekbone_problem *P, double *x, const double *z, int mbit, int rev = 0);
+// gx: gather form (pap parts over every point, shared ones by the identity
+// in nb_ax_kernel.h; the update kernel then sums the shared copies)
+int launch_ax_cg(nekbone_problem *P, double *x, const double *z, int mbit, int rev = 0,
+                 int gx = 0);
 int launch_gs(nekbone_problem *P, double *u, const CgScalars *guard);
 // Fused CG variant: finishes shared points (+0.1p, mask) and writes one
 // pap part per workgroup (gs_blocks of them) to part.
@@ -517,6 +559,23 @@ int launch_pap_final(nekbone_problem *P, const double *part, int64_t npart);
 int launch_update_final(nekbone_problem *P, const double *part, int nu, int precond,
                         int set_xpending);
 int cg_update_blocks(const nekbone_problem *P);
+// Whether the fused iterations run in gather form: the entity table exists,
+// the caller has not switched it off, the full Dirichlet mask applies
+// (the element kernel's pap identity needs p zero wherever w is masked)
+// and c = 1/multiplicity with the fused path on.
+bool cg_gather(const nekbone_problem *P);
+// pts: points per workgroup of the update kernel (kGxPts64 / kGxPts32)
+GxView gx_view(const nekbone_problem *P, int pts);
+// Before a solve: whether the masked right-hand side f (device order) has
+// equal values at all copies of every shared point, which the gather
+// form's pap identity needs (p stays continuous then); sets gx_cont.
+// Synchronises the stream.  A no-op returning OK when the gather form
+// does not apply anyway.
+int cg_gather_check(nekbone_problem *P, const double *f);
+// Right-hand side for the autotuners' timed iterations: uniform random,
+// summed over shared copies (continuous, so the gather form applies where
+// it would in a solve).
+int tune_rhs(nekbone_problem *P, uint64_t seed);
 // batched fused CG (nb_batch.hip drives these)
 GsCgArgs gs_cg_args(nekbone_problem *P, double *w, const double *p, int mbit, double *part);
 int launch_gs_cg_batch(const GsCgArgs *d_args, const int *d_start, int nprob, int nblocks);

@@ -231,6 +231,106 @@ __global__ void __launch_bounds__(kRedThreads)
     }
 }
 
+// Gather form of k_mx_update (see cg_gather), one workgroup per G.eb
+// consecutive elements with their entity-table rows in LDS: the shared
+// points' w is formed here as the sum of their copies in member order,
+// + 0.1 p, masked (the mask bit of the entity), exactly as the
+// gather-scatter kernel stores it; the member loads are issued with the
+// points' own loads.
+template <int PRE, bool XP, bool MB, bool ZF>
+__global__ void __launch_bounds__(kRedThreads)
+    k_mx_update_gx(float *__restrict__ r, const float *__restrict__ w,
+                   const uint16_t *__restrict__ dinv, const uint8_t *__restrict__ flag,
+                   float *__restrict__ x, const float *__restrict__ p, int nelt,
+                   double *__restrict__ part, int nblk, const CgScalars *__restrict__ s,
+                   GxView G)
+{
+    __shared__ int32_t tabs[kGxMaxEb * kGxStride];
+    const int blk = blockIdx.x;
+    if (s->done) return;
+    const int64_t e0 = int64_t(blk) * G.eb;
+    const int ne = int(nelt - e0 < G.eb ? nelt - e0 : G.eb);
+    gx_rows<kRedThreads>(tabs, G, e0, ne);
+    __shared__ float vsum[kGxMaxEb * 8];
+    gx_vertices<float, kRedThreads>(vsum, w, tabs, ne);
+    const int np = ne * G.n3;
+    const int64_t beg = e0 * G.n3;
+    const float alpha = float(s->alpha);
+    double a0 = 0.0, a1 = 0.0;
+    constexpr int U = kGxU;
+    for (int j0 = threadIdx.x; j0 < np; j0 += U * kRedThreads) {
+        float rv[U], wv[U], dv[U], xv[U], pv[U], ps[U];
+        uint8_t fv[U];
+        bool sh[U], gm[U];
+        float gv[U][4];
+        int gc[U], go[U], gs[U], ge[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = j0 + u * kRedThreads;
+            sh[u] = false;
+            if (j < np) {
+                const int64_t i = beg + j;
+                rv[u] = r[i];
+                wv[u] = w[i];
+                if constexpr (MB) {
+                    const uint32_t h = dinv[i];
+                    dv[u] = __uint_as_float(h << 16);
+                    fv[u] = uint8_t(1u << (h & 3u)) << kMultShift;
+                } else {
+                    fv[u] = flag[i];
+                    if constexpr (PRE == NEKBONE_PRECOND_JACOBI) dv[u] = axk::dinv_at(dinv, i);
+                }
+                if constexpr (XP) {
+                    xv[u] = x[i];
+                    pv[u] = p[i];
+                }
+                const int el = gx_divf(j, G.n3, G.inv_n3f);
+                ge[u] = el;
+                sh[u] = gx_issue(gv[u], gc[u], go[u], gs[u], gm[u], w, tabs + el * kGxStride, G,
+                                 j - el * G.n3, i);
+                if (sh[u]) ps[u] = XP ? pv[u] : p[i];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (sh[u]) {
+                float v = gx_finish(gv[u], gc[u], go[u], gs[u], vsum + 8 * ge[u],
+                                    wv[u]) +
+                          0.1f * ps[u];
+                if (gm[u]) v = 0.0f;
+                wv[u] = v;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = j0 + u * kRedThreads;
+            if (j >= np) continue;
+            const int64_t i = beg + j;
+            if constexpr (XP) x[i] = __builtin_fmaf(alpha, pv[u], xv[u]);
+            if constexpr (ZF) {
+                const float zi = __builtin_fmaf(-alpha, dv[u] * wv[u], rv[u]);
+                r[i] = zi;
+                const double rd = double(zi / dv[u]);
+                const double rc = rd * inv_mult(fv[u]);
+                a0 = a0 + rc * rd;
+                a1 = a1 + rc * double(zi);
+            } else {
+                const float ri = __builtin_fmaf(-alpha, wv[u], rv[u]);
+                r[i] = ri;
+                const double rd = double(ri);
+                const double rc = rd * inv_mult(fv[u]);
+                a0 = a0 + rc * rd;
+                if constexpr (PRE == NEKBONE_PRECOND_JACOBI) a1 = a1 + rc * double(ri * dv[u]);
+            }
+        }
+    }
+    block_sum2(a0, a1);
+    if (threadIdx.x == 0) {
+        part[blk] = a0;
+        part[nblk + blk] = a1;
+    }
+}
+
 // Reliable update, first step: apply a pending x32 += alpha p32 (the
 // element kernel defers it by one iteration), then x64 += x32, x32 = 0.
 __global__ void k_mx_xflush(double *__restrict__ x64, float *__restrict__ x32,
@@ -392,7 +492,7 @@ int ensure_mixed(nekbone_problem *P, int precond)
     return NEKBONE_OK;
 }
 
-int launch_ax32(nekbone_problem *P, int mode, int mbit, float *w, const float *u)
+int launch_ax32(nekbone_problem *P, int mode, int mbit, float *w, const float *u, int gx = 0)
 {
     MixedState *M = P->mx;
     AxFn32 fn = lookup32(P->nx1, M->axv.epb, mode, M->axv.pf, M->axv.xu, M->g16);
@@ -415,6 +515,7 @@ int launch_ax32(nekbone_problem *P, int mode, int mbit, float *w, const float *u
     A.nelt = P->nelt;
     A.gh = reinterpret_cast<const __half *>(M->gh);
     A.gsc = M->gsc;
+    A.gx = gx;
     const int threads = axk::round64(P->nx1 * P->nx1 * M->axv.epb);
     if (int rc = timed_begin(P, 0)) return rc;
     hipLaunchKernelGGL(fn, dim3(ax32_blocks(P)), dim3(threads), 0, stream(), A);
@@ -422,8 +523,50 @@ int launch_ax32(nekbone_problem *P, int mode, int mbit, float *w, const float *u
     return timed_end(P, 0);
 }
 
+// Inner iteration in the gather form (cg_gather): element kernel with pap
+// parts over all points, pap/alpha, the update kernel summing the shared
+// points' copies, rtr/beta.
+int iteration32_gather(nekbone_problem *P, int precond, int mbit)
+{
+    MixedState *M = P->mx;
+    const bool jac = precond == NEKBONE_PRECOND_JACOBI;
+    const int64_t na = ax32_blocks(P), ng = gs_blocks(P);
+    const GxView G = gx_view(P, kGxPts32);
+    const int nu = int((P->nelt + G.eb - 1) / G.eb);
+    double *part_upd = P->d_part + na + ng;
+    if (int rc = launch_ax32(P, 1, mbit, M->w, nullptr, 1)) return rc;
+    if (int rc = launch_pap_final(P, P->d_part, na)) return rc;
+    if (int rc = timed_begin(P, 3)) return rc;
+    const bool xp = !M->axv.xu;
+#define NB_MXG(PRE, XP, MB, ZF)                                                                 \
+    hipLaunchKernelGGL((k_mx_update_gx<PRE, XP, MB, ZF>), dim3(nu), dim3(kRedThreads), 0,        \
+                       stream(), M->r, M->w, M->dinv, P->d_flag, M->x, M->p, P->nelt, part_upd, \
+                       nu, P->d_scal, G)
+    if (jac && M->mult_in_dinv) {
+        if (xp)
+            NB_MXG(NEKBONE_PRECOND_JACOBI, true, true, true);
+        else
+            NB_MXG(NEKBONE_PRECOND_JACOBI, false, true, true);
+    } else if (jac) {
+        if (xp)
+            NB_MXG(NEKBONE_PRECOND_JACOBI, true, false, true);
+        else
+            NB_MXG(NEKBONE_PRECOND_JACOBI, false, false, true);
+    } else {
+        if (xp)
+            NB_MXG(NEKBONE_PRECOND_IDENTITY, true, false, false);
+        else
+            NB_MXG(NEKBONE_PRECOND_IDENTITY, false, false, false);
+    }
+#undef NB_MXG
+    NB_HIP(hipGetLastError());
+    if (int rc = timed_end(P, 3)) return rc;
+    return launch_update_final(P, part_upd, nu, precond, M->axv.xu);
+}
+
 int iteration32(nekbone_problem *P, int precond, int mbit)
 {
+    if (cg_gather(P)) return iteration32_gather(P, precond, mbit);
     MixedState *M = P->mx;
     const bool jac = precond == NEKBONE_PRECOND_JACOBI;
     const int64_t na = ax32_blocks(P), ng = gs_blocks(P);
@@ -552,6 +695,7 @@ extern "C" int nekbone_cg_mixed_d(nekbone_problem *P, double *x, const double *f
                     "cg_mixed: needs c = 1/multiplicity (the inner kernels derive c from it)");
     if (int rc = ensure_mixed(P, precond)) return rc;
     if (int rc = launch_to_dev(P, P->d_f, f)) return rc;
+    if (int rc = cg_gather_check(P, P->d_f)) return rc;
     const int mbit0 = next_mask_bit(P);
     if (int rc = launch_cg_init(P, P->d_x, P->d_r, P->d_f, mbit0, precond, tol * tol)) return rc;
     const int mbit = next_mask_bit(P);
@@ -669,7 +813,7 @@ extern "C" int nekbone_autotune_mixed(nekbone_problem *P, int precond, double *b
     const AxVariant keep = M->axv;
     const int mbit = P->mask_mode == NEKBONE_MASK_DIRICHLET ? 1 : 2;
     const int warm = 2, reps = 6;
-    if (int rc = launch_rand(P->d_f, P->n, 12345)) return rc;
+    if (int rc = tune_rhs(P, 12345)) return rc;
     hipEvent_t e0, e1;
     NB_HIP(hipEventCreate(&e0));
     NB_HIP(hipEventCreate(&e1));

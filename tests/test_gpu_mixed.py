@@ -282,3 +282,34 @@ def test_cg_mixed_g16_meets_fp64_criterion(nb, oracle, nel, nx1, precond):
     err = np.abs(x - xs).max() / np.abs(xs).max()
     print("g16 mixed: max error vs converged FP64 solution %.2e of max|x|" % err)
     assert err <= 1e-8
+
+
+@pytest.mark.parametrize("inner", ["f32", "g16"])
+@pytest.mark.parametrize("precond", [1, 0])
+def test_cg_mixed_gather_and_gs_forms(nb, oracle, inner, precond):
+    """Inner iterations in the gather form (the update kernel sums the
+    shared copies; the default on box meshes with the Dirichlet mask) and
+    with the gather-scatter kernel: both meet the FP64 criterion with r
+    recomputed by the oracle, and their inner iteration counts agree
+    within 2 (the forms differ only in the rounding of the pap sum)."""
+    _case("cg_mixed forms %s precond=%d" % (inner, precond))
+    tol = 1e-10
+    P, O = _pair(nb, oracle, (3, 3, 3), 10, True, jacobi=bool(precond))
+    if inner == "g16":
+        P.set_mixed_inner(nb.MIXED_G16)
+    f = O.rhs_from_raw(splitmix_uniform(O.n, 2))
+    c = O.c()
+    f0 = O.maskit(f)
+    rr0 = float(np.sum(f0 * c * f0))
+    its = []
+    for on in (True, False):
+        assert P.set_gather(on) == on
+        x, r, it, nupd, rtr = P.cg_mixed(f, tol, 5000, precond)
+        print("gather=%s: %d inner iterations, %d FP64 updates" % (on, it, nupd), flush=True)
+        assert nupd >= 1 and 0 < it < 5000
+        rtrue = f0 - O.ax(x)
+        rr = float(np.sum(rtrue * c * rtrue))
+        assert rr <= tol ** 2 * rr0 * 1.0001
+        assert abs(rtr - rr) <= 1e-6 * rr
+        its.append(it)
+    assert abs(its[0] - its[1]) <= 2

@@ -210,6 +210,80 @@ def test_fused_and_generic_sequences_agree(nb):
     np.testing.assert_allclose(x1, x2, rtol=0, atol=1e-13 * np.abs(x2).max())
 
 
+def test_gather_form_activation(nb):
+    """The gather form of the fused CG (the update kernel sums the shared
+    points' copies) applies to box meshes at every order class - vertices
+    only (nx1 = 2), one point per edge and face (3), and larger - with the
+    full Dirichlet mask, and not with the reference's first-call mask list;
+    the switch turns it off and on."""
+    _case("gather form activation")
+    for nel, nx1 in [((2, 2, 2), 2), ((3, 2, 2), 3), ((2, 3, 4), 10), ((2, 1, 2), 16)]:
+        P = nb.Problem(*nel, nx1)
+        assert not P.set_gather()
+        P.set_mask_mode(nb.MASK_DIRICHLET)
+        assert P.set_gather()
+        assert not P.set_gather(False)
+        assert P.set_gather(True)
+
+
+@pytest.mark.parametrize("nel,nx1,precond,deformed,niter", [
+    ((4, 4, 3), 10, 1, False, 50), ((3, 4, 5), 3, 0, True, 30), ((5, 4, 4), 2, 1, True, 8),
+    ((3, 2, 2), 12, 1, True, 40), ((2, 3, 3), 7, 0, False, 40)])
+def test_cg_gather_form_vs_oracle_and_gs_form(nb, oracle, nel, nx1, precond, deformed, niter):
+    """Fused CG in the gather form against the oracle (history 1e-12,
+    x and r 1e-12), and against the same problem run with the
+    gather-scatter kernel (history 1e-13): w is formed by the same
+    operations in both; only the order of the pap sum differs."""
+    _case("cg gather form %s %d precond=%d" % (nel, nx1, precond))
+    P = nb.Problem(*nel, nx1)
+    O = oracle.Problem(*nel, nx1)
+    P.set_mask_mode(nb.MASK_DIRICHLET)
+    O.set_mask_mode(oracle.MASK_DIRICHLET)
+    if deformed:
+        P.set_geometry_deformed(0.1, 11)
+        O.deformed_g(0.1, 11)
+    if precond:
+        P.setup_jacobi()
+        O.setup_jacobi()
+    f = O.rhs_from_raw(splitmix_uniform(O.n, 5))
+    assert P.set_gather()
+    xo, ro, ho = O.cg(f, niter, precond)
+    x1, r1, h1 = P.cg(f, niter, precond)
+    np.testing.assert_allclose(h1, ho, rtol=HIST_RTOL, atol=0)
+    np.testing.assert_allclose(x1, xo, rtol=0, atol=1e-12 * np.abs(xo).max())
+    np.testing.assert_allclose(r1, ro, rtol=0, atol=1e-12 * np.abs(f).max())
+    assert not P.set_gather(False)
+    x2, r2, h2 = P.cg(f, niter, precond)
+    np.testing.assert_allclose(h1, h2, rtol=1e-13, atol=0)
+    np.testing.assert_allclose(x1, x2, rtol=0, atol=1e-13 * np.abs(x2).max())
+
+
+@pytest.mark.parametrize("precond", [0, 1])
+def test_cg_discontinuous_rhs_keeps_reference_sequence(nb, oracle, precond):
+    """A right-hand side that differs between the copies of shared points
+    (raw U[-1,1), not summed): the gather form's pap identity does not hold
+    for it, so the solve must detect it and run the gather-scatter form;
+    the history follows the oracle's (the reference's arithmetic on the
+    same input)."""
+    _case("cg discontinuous rhs precond=%d" % precond)
+    nel, nx1 = (3, 3, 2), 7
+    P = nb.Problem(*nel, nx1)
+    O = oracle.Problem(*nel, nx1)
+    P.set_mask_mode(nb.MASK_DIRICHLET)
+    O.set_mask_mode(oracle.MASK_DIRICHLET)
+    P.set_geometry_deformed(0.1, 4)
+    O.deformed_g(0.1, 4)
+    if precond:
+        P.setup_jacobi()
+        O.setup_jacobi()
+    f = splitmix_uniform(O.n, 9)
+    assert P.set_gather()
+    xo, ro, ho = O.cg(f, 30, precond)
+    x, r, h = P.cg(f, 30, precond)
+    np.testing.assert_allclose(h, ho, rtol=HIST_RTOL, atol=0)
+    np.testing.assert_allclose(x, xo, rtol=0, atol=1e-12 * np.abs(xo).max())
+
+
 def test_fma_arithmetic_close_to_exact(nb):
     _case("fma arithmetic")
     P = nb.Problem(4, 4, 4, 10)
