@@ -467,8 +467,14 @@ def main():
     # direction update fused in (Jacobi in z form: the CG carries
     # z = M^-1 r).  Compulsory streams per local point: reads z, p (8 B
     # each), g (48 B), the point flag byte (1 B); writes p, w (8 B each);
-    # with the deferred x update also reads and writes x (16 B).
+    # with the deferred x update also reads and writes x (16 B).  With the
+    # x-ring (the gather form's default) x is updated from the ring in
+    # the update kernel instead and the element kernel is the variant
+    # without it.
     epb, pf, xu = P.variant()
+    gather, xring = P.cg_form()
+    if xring:
+        xu = 0
     bpp = 81 + (16 if xu else 0)
     kname = "k_ax<%d,%d,false,1,%d,%d>" % (nx1, epb, pf, xu)
     ax_avg_s = ax_ms / max(launches, 1) / 1e3
@@ -479,7 +485,9 @@ def main():
                 "traffic": None, "kernel": kname,
                 "bytes_per_point": bpp, "avg_us": ax_avg_s * 1e6, "launches": launches,
                 "gflops": flops_ax / ax_avg_s / 1e9,
-                "fused": "ax_e + Jacobi p update (z form)" + (" + deferred x update" if xu else "")}
+                "fused": "ax_e + Jacobi p update (z form)" + (" + deferred x update" if xu else "")
+                + (" + pap of shared copies (gather form)" if gather else ""),
+                "cg_form": {"gather": gather, "x_ring": xring}}
     roofline["traffic"] = args.traffic if args.traffic is not None else \
         measured_traffic(kname, "%dx%dx%d" % nel)
 

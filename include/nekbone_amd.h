@@ -126,6 +126,20 @@ int nekbone_set_fused(nekbone_problem *P, int on);
  * pap sum (hence alpha) is accumulated in another order.  No counterpart
  * in the reference, whose dssum is a separate pass (src/cg.f:116). */
 int nekbone_set_gather(nekbone_problem *P, int on, int *active);
+/* x update of the gather form.  on = 1 (default): the directions of the
+ * last 10 iterations are kept in a ring of 10 vectors (allocated on the
+ * first such solve when the device has room for them, else this setting
+ * has no effect) and x receives their alpha p once every 10 iterations, in
+ * iteration order and with the same operations, so x is bit-identical to
+ * the per-iteration update while the element kernel neither reads nor
+ * writes x; on = 0: x is updated every iteration (and the ring freed).
+ * Applies to the FP64 CG (the mixed solver's FP32 inner iterations update
+ * x every iteration).  No counterpart in the reference (src/cg.f:63
+ * updates x each iteration). */
+int nekbone_set_xring(nekbone_problem *P, int on);
+/* The forms the last FP64 solve of P ran in: *gather = 1 for the gather
+ * form, *xring = 1 when it used the x-ring (either pointer may be null). */
+int nekbone_get_cg_form(nekbone_problem *P, int *gather, int *xring);
 /* 1 (default): CG iterations are replayed in chunks of 10 from a HIP graph
  * captured on first use (re-captured when buffers or settings change; not
  * while per-kernel timing is enabled); 0: every kernel launched directly.

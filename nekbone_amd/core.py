@@ -55,6 +55,9 @@ _SIGS = {
     "nekbone_set_arith": (ctypes.c_int, [_vp, ctypes.c_int]),
     "nekbone_set_fused": (ctypes.c_int, [_vp, ctypes.c_int]),
     "nekbone_set_gather": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    "nekbone_set_xring": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "nekbone_get_cg_form": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int),
+                                           ctypes.POINTER(ctypes.c_int)]),
     "nekbone_setup_jacobi": (ctypes.c_int, [_vp]),
     "nekbone_setup_hsmg": (ctypes.c_int, [_vp]),
     "nekbone_hsmg_solve": (ctypes.c_int, [_vp, _dp, _dp]),
@@ -396,6 +399,15 @@ class Problem:
 
     def set_fused(self, on):
         _chk(lib().nekbone_set_fused(self.handle, int(bool(on))))
+
+    def set_xring(self, on):
+        _chk(lib().nekbone_set_xring(self.handle, int(bool(on))))
+
+    def cg_form(self):
+        """(gather form, x-ring) of the last FP64 solve."""
+        g, x = ctypes.c_int(0), ctypes.c_int(0)
+        _chk(lib().nekbone_get_cg_form(self.handle, ctypes.byref(g), ctypes.byref(x)))
+        return bool(g.value), bool(x.value)
 
     def set_gather(self, on=None):
         """Gather form of the fused CG (True/False; None keeps the setting).

@@ -503,13 +503,13 @@ static int check_precond(nekbone_problem *P, int precond)
 // One iteration: solveM (src/cg.f:86-94) is fused into the iteration for
 // the identity and Jacobi; HSMG runs h1mg_solve first (which masks r, as
 // the reference's does) and the iteration takes its z.
-static int cg_step(nekbone_problem *P, double *x, double *r, int precond, int mbit)
+static int cg_step(nekbone_problem *P, double *x, double *r, int precond, int mbit, int k)
 {
     if (precond == NEKBONE_PRECOND_HSMG) {
         if (int rc = launch_hsmg_apply(P, P->mg->d_z, r)) return rc;
         return launch_cg_iteration_z(P, x, r, P->mg->d_z, mbit);
     }
-    return launch_cg_iteration(P, x, r, precond, mbit);
+    return launch_cg_iteration(P, x, r, precond, mbit, k);
 }
 
 void drop_problem_graph(nekbone_problem *P)
@@ -525,14 +525,17 @@ void drop_problem_graph(nekbone_problem *P)
 
 // Iterations it0..it0+count-1, from a captured graph of kGraphChunk
 // iterations when possible.  Launch arguments of an iteration depend only
-// on the problem's buffers and settings (all scalars live on the device),
-// so a chunk captured once replays any later chunk of any solve with the
-// same key; event timing is not captured (it needs host bookkeeping).
+// on the problem's buffers and settings (all scalars live on the device)
+// and, with the x-ring, on the iteration's index modulo kRing, so a chunk
+// captured once replays any later chunk that starts at a multiple of
+// kGraphChunk, of any solve with the same key; event timing is not
+// captured (it needs host bookkeeping).
 constexpr int kGraphChunk = 10;
+static_assert(kGraphChunk % kRing == 0, "a captured chunk covers whole x-ring turns");
 static int run_iterations(nekbone_problem *P, double *x, double *r, int precond, int mbit,
-                          int count)
+                          int count, int it0 = 0)
 {
-    const bool graphable = P->use_graph && P->timing.mask == 0;
+    const bool graphable = P->use_graph && P->timing.mask == 0 && it0 % kGraphChunk == 0;
     int done = 0;
     if (graphable && count >= kGraphChunk) {
         const std::vector<uintptr_t> key = {
@@ -540,13 +543,14 @@ static int run_iterations(nekbone_problem *P, double *x, double *r, int precond,
             uintptr_t(P->d_hist), uintptr_t(P->hist_cap), uintptr_t(P->axv.epb),
             uintptr_t(P->axv.pf), uintptr_t(P->axv.xu), uintptr_t(P->arith),
             uintptr_t(P->fused), uintptr_t(cg_gather(P)), uintptr_t(P->mg), uintptr_t(P->d_dinv),
-            uintptr_t(P->hsmg_fp32), uintptr_t(P->hsmg_coarse)};
+            uintptr_t(P->hsmg_fp32), uintptr_t(P->hsmg_coarse), uintptr_t(P->ring_on),
+            uintptr_t(P->d_pring)};
         if (!P->graph || P->graph_key != key) {
             drop_problem_graph(P);
             hipGraph_t g = nullptr;
             NB_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
             int rc = NEKBONE_OK;
-            for (int i = 0; i < kGraphChunk && !rc; ++i) rc = cg_step(P, x, r, precond, mbit);
+            for (int i = 0; i < kGraphChunk && !rc; ++i) rc = cg_step(P, x, r, precond, mbit, i);
             hipError_t e = hipStreamEndCapture(stream(), &g);
             if (rc) {
                 if (g) (void)hipGraphDestroy(g);
@@ -562,7 +566,7 @@ static int run_iterations(nekbone_problem *P, double *x, double *r, int precond,
             NB_HIP(hipGraphLaunch(P->graph, stream()));
     }
     for (; done < count; ++done)
-        if (int rc = cg_step(P, x, r, precond, mbit)) return rc;
+        if (int rc = cg_step(P, x, r, precond, mbit, it0 + done)) return rc;
     return NEKBONE_OK;
 }
 
@@ -573,7 +577,6 @@ static int run_cg(nekbone_problem *P, double *x, const double *f, double *r, int
     if (int rc = check_precond(P, precond)) return rc;
     if (niter < 0) return fail(NEKBONE_ERR_ARG, "cg: niter < 0");
     if (int rc = ensure_hist(P, niter)) return rc;
-    if (int rc = cg_gather_check(P, f)) return rc;
     const int mbit0 = next_mask_bit(P);  // maskit(r), src/cg.f:40
     if (int rc = launch_cg_init(P, x, r, f, mbit0, precond, tol2, cg_zform(P, precond)))
         return rc;
@@ -720,6 +723,7 @@ extern "C" int nekbone_problem_destroy(nekbone_problem *P)
     dfree(P->d_flag);
     dfree(P->d_gx);
     dfree(P->d_gxbad);
+    dfree(P->d_pring);
     dfree(P->d_perm);
     dfree(P->d_iref);
     dfree(P->gs.d_m2);
@@ -882,8 +886,7 @@ extern "C" int nekbone_set_variant(nekbone_problem *P, int epb, int pf, int xu)
 int nb::tune_rhs(nekbone_problem *P, uint64_t seed)
 {
     if (int rc = launch_rand(P->d_f, P->n, seed)) return rc;
-    if (int rc = launch_gs(P, P->d_f, nullptr)) return rc;
-    return cg_gather_check(P, P->d_f);
+    return launch_gs(P, P->d_f, nullptr);
 }
 
 extern "C" int nekbone_autotune(nekbone_problem *P, int precond, double *best_ms)
@@ -909,9 +912,12 @@ extern "C" int nekbone_autotune(nekbone_problem *P, int precond, double *best_ms
     for (const AxVariant &v : cands) {
         P->axv = v;
         rc = launch_cg_init(P, P->d_x, P->d_r, P->d_f, 1, precond, 0.0, cg_zform(P, precond));
-        for (int i = 0; i < warm && !rc; ++i) rc = launch_cg_iteration(P, P->d_x, P->d_r, precond, mbit);
+        int k = 0;
+        for (int i = 0; i < warm && !rc; ++i)
+            rc = launch_cg_iteration(P, P->d_x, P->d_r, precond, mbit, k++);
         if (!rc && hipEventRecord(e0, stream()) != hipSuccess) rc = NEKBONE_ERR_HIP;
-        for (int i = 0; i < reps && !rc; ++i) rc = launch_cg_iteration(P, P->d_x, P->d_r, precond, mbit);
+        for (int i = 0; i < reps && !rc; ++i)
+            rc = launch_cg_iteration(P, P->d_x, P->d_r, precond, mbit, k++);
         if (!rc && hipEventRecord(e1, stream()) != hipSuccess) rc = NEKBONE_ERR_HIP;
         if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = NEKBONE_ERR_HIP;
         float ms = 0.f;
@@ -1003,6 +1009,27 @@ extern "C" int nekbone_set_fused(nekbone_problem *P, int on)
 {
     if (int rc = check(P)) return rc;
     P->fused = on != 0;
+    return NEKBONE_OK;
+}
+
+extern "C" int nekbone_set_xring(nekbone_problem *P, int on)
+{
+    if (int rc = check(P)) return rc;
+    if (on != 0 && on != 1) return fail(NEKBONE_ERR_ARG, "set_xring: on must be 0 or 1");
+    drop_problem_graph(P);
+    P->xring = on;
+    if (!on) {
+        dfree(P->d_pring);
+        P->ring_on = false;
+    }
+    return NEKBONE_OK;
+}
+
+extern "C" int nekbone_get_cg_form(nekbone_problem *P, int *gather, int *xring)
+{
+    if (int rc = check(P)) return rc;
+    if (gather) *gather = cg_gather(P) ? 1 : 0;
+    if (xring) *xring = P->ring_on ? 1 : 0;
     return NEKBONE_OK;
 }
 
@@ -1180,7 +1207,7 @@ extern "C" int nekbone_cg_tol_d(nekbone_problem *P, double *x, const double *f, 
     CgScalars s{};
     for (int it = 0; it < maxiter; it += chunk) {
         if (int rc = run_iterations(P, P->d_x, P->d_r, precond, mbit,
-                                    std::min(maxiter, it + chunk) - it))
+                                    std::min(maxiter, it + chunk) - it, it))
             return rc;
         NB_HIP(hipMemcpyAsync(&s, P->d_scal, sizeof s, hipMemcpyDeviceToHost, stream()));
         NB_HIP(hipStreamSynchronize(stream()));

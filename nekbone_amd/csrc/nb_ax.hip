@@ -92,14 +92,17 @@ int launch_ax_local(nekbone_problem *P, double *w, const double *u)
     return timed_end(P, 0);
 }
 
-int launch_ax_cg(nekbone_problem *P, double *x, const double *z, int mbit, int rev, int gx)
+int launch_ax_cg(nekbone_problem *P, double *x, const double *z, int mbit, int rev, int gx,
+                 double *pin, double *pout)
 {
+    // x-ring (pout set): the variant without the x update
     AxFn fn = lookup(P->nx1, P->axv.epb, P->arith == NEKBONE_ARITH_FMA, 1, P->axv.pf,
-                     P->axv.xu);
+                     pout ? 0 : P->axv.xu);
     if (!fn) return fail(NEKBONE_ERR_UNSUPPORTED, "ax: no kernel for this order/variant");
     AxArgs A{};
     A.w = P->d_w;
-    A.p = P->d_p;
+    A.p = pout ? pin : P->d_p;
+    A.pout = pout;
     A.r = z;
     A.x = x;
     A.g = P->d_g;

@@ -41,7 +41,9 @@
 //
 // CG mode (MODE 1) fuses, per point of the thread's t-line:
 //   prologue  [XU] x += alpha_prev p_prev     (add2s2, src/cg.f:63, one
-//                                              iteration late)
+//                                              iteration late; with the
+//                                              x-ring, pout set, x is left
+//                                              to the update kernel)
 //             p = beta p + z                  (add2s1, src/cg.f:56; z is
 //                                              the vector the CG carries:
 //                                              r with the identity, M^-1 r
@@ -202,7 +204,8 @@ template <typename T>
 struct AxArgsT {
     T *w;
     const T *u;     // MODE 0 input
-    T *p;           // CG: direction, updated in place
+    T *p;           // CG: direction, updated in place (read only when pout is set)
+    T *pout;        // CG, x-ring: where the new direction goes (null: in place)
     const T *r;     // CG: z (r with the identity, M^-1 r with Jacobi)
     T *x;           // CG: solution, updated one iteration late (XU)
     const T *g;     // FP64: planes g[c * gcs + e * N3 + p]; FP32: g[6 * (e * N3 + p) + c]
@@ -354,11 +357,14 @@ __device__ __forceinline__ void ax_core(const AxArgsT<T> &A_, const int blk, T *
             }
         } else {
             const T beta = T(A_.scal->beta);
+            // x-ring: the first iteration has no previous direction (its
+            // ring slot may hold anything), beta = 0 there
+            const bool pnone = A_.pout && A_.scal->iter == 0;
             T po[NX], z[NX];
 #pragma unroll
             for (int m = 0; m < NX; ++m) {
                 const int64_t q = base + ab + N2 * m;
-                po[m] = A_.p[q];
+                po[m] = pnone ? T(0) : A_.p[q];
                 z[m] = A_.r[q];
             }
             if constexpr (XU) {
@@ -381,7 +387,7 @@ __device__ __forceinline__ void ax_core(const AxArgsT<T> &A_, const int blk, T *
                     pn = fma_t(beta, po[m], z[m]);
                 else
                     pn = beta * po[m] + z[m];
-                A_.p[base + ab + N2 * m] = pn;
+                (A_.pout ? A_.pout : A_.p)[base + ab + N2 * m] = pn;
                 pd[m] = pn;
                 A[ref[m]] = pn;
             }

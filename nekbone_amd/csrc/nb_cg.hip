@@ -70,6 +70,7 @@ __global__ void __launch_bounds__(kFinalThreads)
         s->done = (tol2 > 0.0 && rtr == 0.0) ? 1 : 0;
         s->status = 0;
         s->xpending = 0;
+        s->xdone = 0;
         s->tol2 = tol2;
         if (hcap > 0) hist[0] = rtr;
         prepare_next(s, hist, hcap);
@@ -183,13 +184,17 @@ __global__ void __launch_bounds__(kRedThreads)
 // gather-scatter kernel stores), then the update of update_body.  The
 // member loads depend only on the point's position and the LDS table, so
 // they are issued with the points' own loads.
-template <int PRE, bool FMA, bool XP>
+//
+// [XF] x-ring flush (iterations k with k % kRing == kRing - 1): x receives
+// alpha_j p_j of the iterations j in [xdone, k] from the ring (rs apart),
+// in iteration order.
+template <int PRE, bool FMA, bool XP, bool XF = false>
 __global__ void __launch_bounds__(kRedThreads)
     k_update_gx(double *__restrict__ r, const double *__restrict__ w,
                 const double *__restrict__ dinv, const uint8_t *__restrict__ flag,
                 double *__restrict__ x, const double *__restrict__ p, int nelt,
                 double *__restrict__ part, int nblk, const CgScalars *__restrict__ s, int rev,
-                GxView G)
+                GxView G, const double *__restrict__ ring = nullptr, int64_t rs = 0)
 {
     __shared__ int32_t tabs[kGxMaxEb * kGxStride];
     const int blk = sweep_block(blockIdx.x, nblk, rev);
@@ -229,6 +234,17 @@ __global__ void __launch_bounds__(kRedThreads)
                 sh[u] = gx_issue(gv[u], gc[u], go[u], gs[u], gm[u], w, tabs + el * kGxStride, G,
                                  j - el * G.n3, i);
                 if (sh[u]) ps[u] = XP ? pv[u] : p[i];
+            }
+        }
+        if constexpr (XF) {
+            const int k = s->iter, lo = s->xdone > k - kRing + 1 ? s->xdone : k - kRing + 1;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int j = j0 + u * kRedThreads;
+                if (j < np) {
+                    const int64_t i = beg + j;
+                    x[i] = ring_fold<double, FMA>(x[i], ring, rs, i, s, lo, k + 1);
+                }
             }
         }
 #pragma unroll
@@ -283,12 +299,27 @@ __global__ void __launch_bounds__(kRedThreads)
     }
 }
 
+// x-ring, end of a solve: x receives alpha_j p_j of the iterations j in
+// [xdone, iter) that no flush has folded in yet; k_ring_mark then records
+// them as folded.
+template <typename T, bool FMA>
+__global__ void k_ring_tail(T *__restrict__ x, const T *__restrict__ ring, int64_t n,
+                            const CgScalars *__restrict__ s)
+{
+    const int lo = s->xdone, hi = s->iter;
+    if (hi <= lo) return;
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = ring_fold<T, FMA>(x[i], ring, n, i, s, lo, hi);
+}
+
+__global__ void k_ring_mark(CgScalars *__restrict__ s) { s->xdone = s->iter; }
+
 template <int PRE>
 __global__ void __launch_bounds__(kFinalThreads)
     k_update_final(const double *__restrict__ part, int nblk, CgScalars *__restrict__ s,
-                   double *__restrict__ hist, int hcap, int set_xpending)
+                   double *__restrict__ hist, int hcap, int set_xpending, int xmark = 0)
 {
-    update_final_body<PRE, kFinalThreads>(part, nblk, s, hist, hcap, set_xpending);
+    update_final_body<PRE, kFinalThreads>(part, nblk, s, hist, hcap, set_xpending, xmark);
 }
 
 // ---- batched forms: problem q owns workgroups [start[q], start[q+1]) of
@@ -446,16 +477,16 @@ int launch_pap_final(nekbone_problem *P, const double *part, int64_t npart)
 }
 
 int launch_update_final(nekbone_problem *P, const double *part, int nu, int precond,
-                        int set_xpending)
+                        int set_xpending, int xmark)
 {
     if (int rc = timed_begin(P, 4)) return rc;
     if (precond == NEKBONE_PRECOND_JACOBI)
         hipLaunchKernelGGL(k_update_final<NEKBONE_PRECOND_JACOBI>, dim3(1), dim3(kFinalThreads),
-                           0, stream(), part, nu, P->d_scal, P->d_hist, 0, set_xpending);
+                           0, stream(), part, nu, P->d_scal, P->d_hist, 0, set_xpending, xmark);
     else
         hipLaunchKernelGGL(k_update_final<NEKBONE_PRECOND_IDENTITY>, dim3(1),
                            dim3(kFinalThreads), 0, stream(), part, nu, P->d_scal, P->d_hist, 0,
-                           set_xpending);
+                           set_xpending, xmark);
     NB_HIP(hipGetLastError());
     return timed_end(P, 4);
 }
@@ -477,6 +508,10 @@ bool cg_zform(const nekbone_problem *P, int precond)
 int launch_cg_init(nekbone_problem *P, double *x, double *r, const double *f, int mbit,
                    int precond, double tol2, bool zform)
 {
+    // every solve decides afresh whether its right-hand side allows the
+    // gather form, and with it the x-ring
+    if (int rc = cg_gather_check(P, f)) return rc;
+    P->ring_on = cg_gather(P) && precond != NEKBONE_PRECOND_HSMG && ensure_ring(P);
     const int nb = reduce_blocks(P->n);
     if (precond == NEKBONE_PRECOND_JACOBI && zform) {
         hipLaunchKernelGGL(k_cg_init<kPreJacobiZ>, dim3(nb), dim3(kRedThreads), 0, stream(), x,
@@ -522,7 +557,13 @@ int launch_cg_init(nekbone_problem *P, double *x, double *r, const double *f, in
 // kernels; w, and with it x and z, bit-identical to the gather-scatter
 // form, pap summed in another order.  The element kernel walks upward and
 // the update downward, so each starts where the other ended.
-static int iteration_gather(nekbone_problem *P, double *x, double *r, int precond, int mbit)
+//
+// With the x-ring (P->ring_on) the element kernel takes the previous
+// direction from ring slot (k-1) % kRing and stores the new one in slot
+// k % kRing, without touching x; the update kernel of every kRing-th
+// iteration folds the ring's alpha p into x.
+static int iteration_gather(nekbone_problem *P, double *x, double *r, int precond, int mbit,
+                            int k)
 {
     const bool jac = precond == NEKBONE_PRECOND_JACOBI;
     const bool fma = P->arith == NEKBONE_ARITH_FMA;
@@ -530,24 +571,33 @@ static int iteration_gather(nekbone_problem *P, double *x, double *r, int precon
     const GxView G = gx_view(P, kGxPts64);
     const int nu = int((P->nelt + G.eb - 1) / G.eb);
     double *part_upd = P->d_part + na + ng;
-    if (int rc = launch_ax_cg(P, x, r, mbit, 0, 1)) return rc;
+    const bool ring = P->ring_on;
+    const int64_t n = P->n;
+    double *pcur = ring ? P->d_pring + int64_t(k % kRing) * n : P->d_p;
+    double *pprev = ring ? P->d_pring + int64_t((k + kRing - 1) % kRing) * n : P->d_p;
+    const bool xf = ring && k % kRing == kRing - 1;
+    if (int rc = launch_ax_cg(P, x, r, mbit, 0, 1, ring ? pprev : nullptr, ring ? pcur : nullptr))
+        return rc;
     if (int rc = timed_begin(P, 4)) return rc;
     hipLaunchKernelGGL(k_pap_final, dim3(1), dim3(kFinalThreads), 0, stream(), P->d_part, na,
                        P->d_scal, P->d_hist, P->hist_cap);
     NB_HIP(hipGetLastError());
     if (int rc = timed_end(P, 4)) return rc;
     if (int rc = timed_begin(P, 3)) return rc;
-    const bool xp = !P->axv.xu;
-#define NB_UGX2(PRE, F, XP)                                                                    \
-    hipLaunchKernelGGL((k_update_gx<PRE, F, XP>), dim3(nu), dim3(kRedThreads), 0, stream(), r, \
-                       P->d_w, P->d_dinv, P->d_flag, x, P->d_p, P->nelt, part_upd, nu,         \
-                       P->d_scal, 1, G)
-#define NB_UGX(PRE, F)              \
-    do {                            \
-        if (xp)                     \
-            NB_UGX2(PRE, F, true);  \
-        else                        \
-            NB_UGX2(PRE, F, false); \
+    const bool xp = !ring && !P->axv.xu;
+#define NB_UGX3(PRE, F, XP, XF)                                                                   \
+    hipLaunchKernelGGL((k_update_gx<PRE, F, XP, XF>), dim3(nu), dim3(kRedThreads), 0, stream(),  \
+                       r, P->d_w, P->d_dinv, P->d_flag, x, static_cast<const double *>(pcur),    \
+                       P->nelt, part_upd, nu, P->d_scal, 1, G,                                    \
+                       static_cast<const double *>(P->d_pring), n)
+#define NB_UGX(PRE, F)                         \
+    do {                                       \
+        if (xf)                                \
+            NB_UGX3(PRE, F, false, true);      \
+        else if (xp)                           \
+            NB_UGX3(PRE, F, true, false);      \
+        else                                   \
+            NB_UGX3(PRE, F, false, false);     \
     } while (0)
     if (jac) {
         if (fma)
@@ -561,18 +611,19 @@ static int iteration_gather(nekbone_problem *P, double *x, double *r, int precon
             NB_UGX(NEKBONE_PRECOND_IDENTITY, false);
     }
 #undef NB_UGX
-#undef NB_UGX2
+#undef NB_UGX3
     NB_HIP(hipGetLastError());
     if (int rc = timed_end(P, 3)) return rc;
     if (int rc = timed_begin(P, 4)) return rc;
+    const int xpend = ring ? 0 : P->axv.xu;
     if (jac)
         hipLaunchKernelGGL(k_update_final<NEKBONE_PRECOND_JACOBI>, dim3(1), dim3(kFinalThreads),
-                           0, stream(), part_upd, nu, P->d_scal, P->d_hist, P->hist_cap,
-                           P->axv.xu);
+                           0, stream(), part_upd, nu, P->d_scal, P->d_hist, P->hist_cap, xpend,
+                           xf ? 1 : 0);
     else
         hipLaunchKernelGGL(k_update_final<NEKBONE_PRECOND_IDENTITY>, dim3(1),
                            dim3(kFinalThreads), 0, stream(), part_upd, nu, P->d_scal, P->d_hist,
-                           P->hist_cap, P->axv.xu);
+                           P->hist_cap, xpend, xf ? 1 : 0);
     NB_HIP(hipGetLastError());
     return timed_end(P, 4);
 }
@@ -580,9 +631,9 @@ static int iteration_gather(nekbone_problem *P, double *x, double *r, int precon
 // zin: the vector the element kernel reads in place of r (an explicit
 // z = M^-1 r from the HSMG preconditioner); null: r itself.
 static int iteration_fused(nekbone_problem *P, double *x, double *r, int precond, int mbit,
-                           const double *zin = nullptr)
+                           const double *zin = nullptr, int k = 0)
 {
-    if (!zin && cg_gather(P)) return iteration_gather(P, x, r, precond, mbit);
+    if (!zin && cg_gather(P)) return iteration_gather(P, x, r, precond, mbit, k);
     const bool jac = precond == NEKBONE_PRECOND_JACOBI;
     const bool fma = P->arith == NEKBONE_ARITH_FMA;
     const int64_t na = n_ax(P), ng = n_gs(P);
@@ -773,10 +824,11 @@ int launch_cg_iteration_z(nekbone_problem *P, double *x, double *r, const double
     return NEKBONE_OK;
 }
 
-// One PCG iteration (src/cg.f:48-75), no host round trip.
-int launch_cg_iteration(nekbone_problem *P, double *x, double *r, int precond, int mbit)
+// One PCG iteration (src/cg.f:48-75), no host round trip; k is its index
+// within the solve.
+int launch_cg_iteration(nekbone_problem *P, double *x, double *r, int precond, int mbit, int k)
 {
-    if (P->c_is_mult && P->fused) return iteration_fused(P, x, r, precond, mbit);
+    if (P->c_is_mult && P->fused) return iteration_fused(P, x, r, precond, mbit, nullptr, k);
     return iteration_generic(P, x, r, precond, mbit);
 }
 
@@ -916,6 +968,37 @@ __global__ void k_z_to_r(double *__restrict__ r, const double *__restrict__ dinv
     if (i < n) r[i] = r[i] / dinv[i];
 }
 
+int launch_ring_tail(nekbone_problem *P, double *x)
+{
+    if (P->arith == NEKBONE_ARITH_FMA)
+        hipLaunchKernelGGL((k_ring_tail<double, true>), dim3(grid1d(P->n, 256)), dim3(256), 0,
+                           stream(), x, static_cast<const double *>(P->d_pring), P->n,
+                           static_cast<const CgScalars *>(P->d_scal));
+    else
+        hipLaunchKernelGGL((k_ring_tail<double, false>), dim3(grid1d(P->n, 256)), dim3(256), 0,
+                           stream(), x, static_cast<const double *>(P->d_pring), P->n,
+                           static_cast<const CgScalars *>(P->d_scal));
+    hipLaunchKernelGGL(k_ring_mark, dim3(1), dim3(1), 0, stream(), P->d_scal);
+    NB_HIP(hipGetLastError());
+    return NEKBONE_OK;
+}
+
+bool ensure_ring(nekbone_problem *P)
+{
+    if (!P->xring) return false;
+    if (P->d_pring) return true;
+    // keep the ring only where it leaves room: it takes kRing vectors
+    size_t freeb = 0, total = 0;
+    if (hipMemGetInfo(&freeb, &total) != hipSuccess) return false;
+    const size_t need = sizeof(double) * size_t(kRing) * size_t(P->n);
+    if (need + (size_t(4) << 30) > freeb) return false;
+    if (dalloc(&P->d_pring, int64_t(kRing) * P->n) != NEKBONE_OK) {
+        P->d_pring = nullptr;
+        return false;
+    }
+    return true;
+}
+
 int launch_cg_finish(nekbone_problem *P, double *x, double *r, bool zform)
 {
     if (zform) {
@@ -923,6 +1006,7 @@ int launch_cg_finish(nekbone_problem *P, double *x, double *r, bool zform)
                            static_cast<const double *>(P->d_dinv), P->n);
         NB_HIP(hipGetLastError());
     }
+    if (P->ring_on) return launch_ring_tail(P, x);
     if (!(P->c_is_mult && P->fused) || !P->axv.xu) return NEKBONE_OK;
     if (P->arith == NEKBONE_ARITH_FMA)
         hipLaunchKernelGGL(k_x_finish<true>, dim3(grid1d(P->n, 256)), dim3(256), 0, stream(), x,

@@ -139,6 +139,37 @@ __device__ __forceinline__ void gx_vertices(T *vsum, const T *__restrict__ w, co
     __syncthreads();
 }
 
+// x-ring: x += alpha_j p_j for the iterations j in [lo, hi) (at most
+// kRing, the directions in ring slots j % kRing, rs apart), in iteration
+// order and with the operations of the per-iteration update, so x comes
+// out as it would have step by step.
+__device__ __forceinline__ float ring_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double ring_fma(double a, double b, double c)
+{
+    return __builtin_fma(a, b, c);
+}
+
+template <typename T, bool FMA>
+__device__ __forceinline__ T ring_fold(T xv, const T *__restrict__ ring, int64_t rs, int64_t i,
+                                       const CgScalars *__restrict__ s, int lo, int hi)
+{
+    T pv[kRing];
+#pragma unroll
+    for (int t = 0; t < kRing; ++t)
+        if (lo + t < hi) pv[t] = ring[int64_t((lo + t) % kRing) * rs + i];
+#pragma unroll
+    for (int t = 0; t < kRing; ++t) {
+        if (lo + t < hi) {
+            const T a = T(s->aring[(lo + t) % kRing]);
+            if constexpr (FMA)
+                xv = ring_fma(a, pv[t], xv);
+            else
+                xv = xv + a * pv[t];
+        }
+    }
+    return xv;
+}
+
 // Table rows of elements e0 .. e0+ne-1 into LDS (the workgroup's threads).
 template <int NT>
 __device__ __forceinline__ void gx_rows(int32_t *tabs, const GxView &G, int64_t e0, int ne)
@@ -204,6 +235,7 @@ __device__ __forceinline__ void pap_final_body(const double *__restrict__ part, 
     if (threadIdx.x == 0) {
         s->pap = pap;
         s->alpha = s->rtz1 / pap;
+        s->aring[s->iter % kRing] = s->alpha;
         s->xpending = 0;  // the element kernel of this iteration applied it
         if (!(pap > 0.0) || !isfinite(pap)) {
             s->status = 1;
@@ -214,11 +246,12 @@ __device__ __forceinline__ void pap_final_body(const double *__restrict__ part, 
     }
 }
 
+// xmark: this iteration's update kernel folded the x-ring into x
 template <int PRE, int NT>
 __device__ __forceinline__ void update_final_body(const double *__restrict__ part, int nblk,
                                                   CgScalars *__restrict__ s,
                                                   double *__restrict__ hist, int hcap,
-                                                  int set_xpending)
+                                                  int set_xpending, int xmark = 0)
 {
     if (s->done) return;
     double rtr, rtz;
@@ -228,6 +261,7 @@ __device__ __forceinline__ void update_final_body(const double *__restrict__ par
         const int h = 3 + 3 * s->iter;
         if (h < hcap) hist[h] = rtr;
         s->iter += 1;
+        if (xmark) s->xdone = s->iter;
         if (set_xpending) s->xpending = 1;
         s->rtz_next = pre_jacobi(PRE) ? rtz : rtr;
         if (s->tol2 > 0.0 && rtr <= s->tol2 * s->rtr0) s->done = 1;

@@ -256,6 +256,11 @@ struct Timing {
     hipEvent_t cur = nullptr;
 };
 
+// Directions kept for the x update of the gather form (x-ring): the
+// iteration k stores p_k in ring slot k % kRing and x receives the
+// alpha_j p_j of kRing iterations at once (one captured graph chunk).
+constexpr int kRing = 10;
+
 // Scalars of a CG solve, resident on the device; the finalize kernels
 // update them so that no host round trip is needed per iteration.
 struct CgScalars {
@@ -265,6 +270,10 @@ struct CgScalars {
     int status;   // sticky: 1 = non-finite or non-positive pap / rtz
     int xpending; // x += alpha p of the last completed iteration not yet applied
     double rtr_upd;  // mixed precision: FP64 rtr at the last reliable update
+    // x-ring: iterations whose alpha p is in x, and alpha of iteration j
+    // in slot j % kRing
+    int xdone;
+    double aring[kRing];
 };
 
 // Per-problem arguments of the batched fused CG kernels.
@@ -457,6 +466,9 @@ struct nekbone_problem {
     bool gx_ok = false;        // the map has the entity form (d_gx built)
     int gather = 1;            // fused CG: gather form where available (0: gather-scatter kernel)
     bool gx_cont = true;       // the current solve's right-hand side is continuous (checked)
+    double *d_pring = nullptr; // x-ring: kRing direction vectors (n each)
+    int xring = 1;             // use the x-ring with the gather form when memory allows
+    bool ring_on = false;      // the current solve uses it (set by launch_cg_init)
     nb::AxVariant axv;         // element-kernel shape
     nb::CgScalars *d_scal = nullptr;
     double *d_hist = nullptr;
@@ -498,8 +510,13 @@ int launch_ax_local(nekbone_problem *P, double *w, const double *u);
 // (r with the identity, M^-1 r with Jacobi in z form)
 // gx: gather form (pap parts over every point, shared ones by the identity
 // in nb_ax_kernel.h; the update kernel then sums the shared copies)
+// pin/pout (x-ring): the previous direction is read from pin and the new
+// one written to pout, and x is not updated; null: P->d_p in place
 int launch_ax_cg(nekbone_problem *P, double *x, const double *z, int mbit, int rev = 0,
-                 int gx = 0);
+                 int gx = 0, double *pin = nullptr, double *pout = nullptr);
+// x-ring tail before a result is read: x += alpha_j p_j of the iterations
+// no flush has folded in yet
+int launch_ring_tail(nekbone_problem *P, double *x);
 int launch_gs(nekbone_problem *P, double *u, const CgScalars *guard);
 // Fused CG variant: finishes shared points (+0.1p, mask) and writes one
 // pap part per workgroup (gs_blocks of them) to part.
@@ -548,7 +565,11 @@ int launch_cg_init(nekbone_problem *P, double *x, double *r, const double *f, in
 // whether the fused Jacobi CG runs in z form (the vector r holds z = M^-1 r
 // between launch_cg_init and launch_cg_finish)
 bool cg_zform(const nekbone_problem *P, int precond);
-int launch_cg_iteration(nekbone_problem *P, double *x, double *r, int precond, int mbit);
+// k: the iteration's index within the solve (its x-ring slot)
+int launch_cg_iteration(nekbone_problem *P, double *x, double *r, int precond, int mbit, int k);
+// x-ring buffers of the problem (FP64) allocated when the device has room
+// for them; false: the solve keeps the per-iteration x update
+bool ensure_ring(nekbone_problem *P);
 // after the last iteration: the pending x update, and r = z / dinv in z form
 int launch_cg_finish(nekbone_problem *P, double *x, double *r, bool zform);
 int64_t cg_partials_needed(const nekbone_problem *P);
@@ -557,7 +578,7 @@ int64_t cg_partials_needed(const nekbone_problem *P);
 // 2*nu parts (rtr parts, then rtz parts).
 int launch_pap_final(nekbone_problem *P, const double *part, int64_t npart);
 int launch_update_final(nekbone_problem *P, const double *part, int nu, int precond,
-                        int set_xpending);
+                        int set_xpending, int xmark = 0);
 int cg_update_blocks(const nekbone_problem *P);
 // Whether the fused iterations run in gather form: the entity table exists,
 // the caller has not switched it off, the full Dirichlet mask applies
@@ -566,11 +587,11 @@ int cg_update_blocks(const nekbone_problem *P);
 bool cg_gather(const nekbone_problem *P);
 // pts: points per workgroup of the update kernel (kGxPts64 / kGxPts32)
 GxView gx_view(const nekbone_problem *P, int pts);
-// Before a solve: whether the masked right-hand side f (device order) has
-// equal values at all copies of every shared point, which the gather
-// form's pap identity needs (p stays continuous then); sets gx_cont.
-// Synchronises the stream.  A no-op returning OK when the gather form
-// does not apply anyway.
+// Before a solve (launch_cg_init calls it): whether the masked
+// right-hand side f (device order) has equal values at all copies of
+// every shared point, which the gather form's pap identity needs (p stays
+// continuous then); sets gx_cont.  Synchronises the stream.  A no-op
+// returning OK when the gather form does not apply anyway.
 int cg_gather_check(nekbone_problem *P, const double *f);
 // Right-hand side for the autotuners' timed iterations: uniform random,
 // summed over shared copies (continuous, so the gather form applies where

@@ -258,6 +258,40 @@ def test_cg_gather_form_vs_oracle_and_gs_form(nb, oracle, nel, nx1, precond, def
     np.testing.assert_allclose(x1, x2, rtol=0, atol=1e-13 * np.abs(x2).max())
 
 
+@pytest.mark.parametrize("niter", [37, 40])
+def test_xring_bit_identical_to_per_iteration_update(nb, oracle, niter):
+    """x-ring (x += alpha p of ten iterations at once, in order) against the
+    per-iteration x update: the same x, r and history bit for bit, for
+    fixed iteration counts that are and are not multiples of the ring, and
+    for a tolerance solve; and against the oracle."""
+    _case("xring niter=%d" % niter)
+    nel, nx1 = (4, 3, 3), 9
+    P = nb.Problem(*nel, nx1)
+    O = oracle.Problem(*nel, nx1)
+    P.set_mask_mode(nb.MASK_DIRICHLET)
+    O.set_mask_mode(oracle.MASK_DIRICHLET)
+    P.set_geometry_deformed(0.1, 2)
+    O.deformed_g(0.1, 2)
+    P.setup_jacobi()
+    O.setup_jacobi()
+    f = O.rhs_from_raw(splitmix_uniform(O.n, 8))
+    xo, ro, ho = O.cg(f, niter, oracle.PRECOND_JACOBI)
+    x1, r1, h1 = P.cg(f, niter, nb.PRECOND_JACOBI)
+    np.testing.assert_allclose(h1, ho, rtol=HIST_RTOL, atol=0)
+    np.testing.assert_allclose(x1, xo, rtol=0, atol=1e-12 * np.abs(xo).max())
+    fv = P.vec()
+    fv.upload(f)
+    xa, ra, xb, rb = P.vec(), P.vec(), P.vec(), P.vec()
+    ita = P.cg_tol_d(xa, fv, ra, 1e-7, 500, nb.PRECOND_JACOBI)
+    P.set_xring(False)
+    x2, r2, h2 = P.cg(f, niter, nb.PRECOND_JACOBI)
+    itb = P.cg_tol_d(xb, fv, rb, 1e-7, 500, nb.PRECOND_JACOBI)
+    assert np.array_equal(x1, x2) and np.array_equal(r1, r2) and np.array_equal(h1, h2)
+    assert ita == itb and ita[0] % 10 != 0
+    assert np.array_equal(xa.download(), xb.download())
+    assert np.array_equal(ra.download(), rb.download())
+
+
 @pytest.mark.parametrize("precond", [0, 1])
 def test_cg_discontinuous_rhs_keeps_reference_sequence(nb, oracle, precond):
     """A right-hand side that differs between the copies of shared points
@@ -282,6 +316,21 @@ def test_cg_discontinuous_rhs_keeps_reference_sequence(nb, oracle, precond):
     x, r, h = P.cg(f, 30, precond)
     np.testing.assert_allclose(h, ho, rtol=HIST_RTOL, atol=0)
     np.testing.assert_allclose(x, xo, rtol=0, atol=1e-12 * np.abs(xo).max())
+    # tolerance solves decide per solve too: a continuous right-hand side
+    # first (gather form), then the discontinuous one, whose iterates must
+    # be those of the gather-scatter form
+    fc, fd = P.vec(), P.vec()
+    fc.upload(O.rhs_from_raw(splitmix_uniform(O.n, 3)))
+    fd.upload(f)
+    xs = [P.vec(), P.vec(), P.vec()]
+    rs = [P.vec(), P.vec(), P.vec()]
+    P.cg_tol_d(xs[0], fc, rs[0], 1e-9, 500, precond)
+    it1, rtr1 = P.cg_tol_d(xs[1], fd, rs[1], 1e-9, 500, precond)
+    assert not P.set_gather(False)
+    it2, rtr2 = P.cg_tol_d(xs[2], fd, rs[2], 1e-9, 500, precond)
+    assert it1 == it2
+    np.testing.assert_allclose(xs[1].download(), xs[2].download(), rtol=0,
+                               atol=1e-12 * np.abs(xs[2].download()).max())
 
 
 def test_fma_arithmetic_close_to_exact(nb):
