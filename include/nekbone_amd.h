@@ -320,6 +320,11 @@ int nekbone_batch_autotune(nekbone_batch *B, int precond, double *best_ms);
  * solve with the same shape (the batch sets it on the problem). */
 int nekbone_batch_set_queue(nekbone_batch *B, int tileset);
 int nekbone_batch_get_path(const nekbone_batch *B, int *tileset, int *xu, int *ntilesets);
+/* Whether the last batched solve ran in the gather form (every member
+ * eligible: box map in entity form, full Dirichlet mask, continuous
+ * right-hand side); the members keep that form (and the gather form
+ * without the x-ring), so their single solves give the batch's values. */
+int nekbone_batch_get_form(const nekbone_batch *B, int *gather);
 /* CG to a relative residual for every problem k: x[k], f[k], r[k] as for
  * nekbone_cg_tol_d (x, r may be NULL); stops when every problem has met
  * tol or after maxiter iterations.  iters[k], final_rtr[k] may be NULL.

@@ -112,6 +112,7 @@ _SIGS = {
     "nekbone_batch_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     "nekbone_batch_set_queue": (ctypes.c_int, [_vp, ctypes.c_int]),
     "nekbone_batch_get_path": (ctypes.c_int, [_vp] + [ctypes.POINTER(ctypes.c_int)] * 3),
+    "nekbone_batch_get_form": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
     "nekbone_batch_cg_tol_d": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                               ctypes.POINTER(_vp), ctypes.c_double, ctypes.c_int,
                                               ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp]),
@@ -248,6 +249,12 @@ class Batch:
         v = [ctypes.c_int() for _ in range(3)]
         _chk(lib().nekbone_batch_get_path(self.handle, *[ctypes.byref(x) for x in v]))
         return tuple(x.value for x in v)
+
+    def gather_form(self):
+        """Whether the last batched solve ran in the gather form."""
+        g = ctypes.c_int(0)
+        _chk(lib().nekbone_batch_get_form(self.handle, ctypes.byref(g)))
+        return bool(g.value)
 
     def autotune(self, precond=None):
         precond = PRECOND_JACOBI if precond is None else precond

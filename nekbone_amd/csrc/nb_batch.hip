@@ -5,9 +5,11 @@
 //     (problem, element tile) items for the orders 6, 8, 10, 12 (the work-
 //     queue kernel, k_ax_queue), or one launch per order group; other
 //     orders always launch per group;
-//   * one gather-scatter launch for all problems;
+//   * one gather-scatter launch for all problems, or none in the gather
+//     form (cg_gather), which the batch takes when every member can;
 //   * one launch each for all problems of pap/alpha, the vector update with
-//     rtr and rtz, and rtr/beta.
+//     rtr and rtz (in the gather form also the sums of the shared copies),
+//     and rtr/beta.
 // The autotuner picks the element-kernel path, the tile shapes per order
 // and the x-update placement.  Each problem keeps its own workgroups,
 // partial sums and device scalars, so every value is the one its single-
@@ -56,6 +58,10 @@ struct nekbone_batch {
     int exec_precond = -1, exec_chunk = 0, exec_shapes = -1;
     int shapes = 0;
     int use_graph = 1;  // replay chunks of iterations from a captured graph
+    // gather form for every member (the iteration has no gather-scatter
+    // kernel); decided per solve, see decide_form
+    bool gx = false;
+    int exec_gx = -1;
 };
 
 namespace nb {
@@ -99,11 +105,7 @@ void apply_shapes(nekbone_batch *B)
 {
     for (auto &G : B->groups) {
         const int epb = in_queue(B, G.nx1) ? tableq()[size_t(B->queue)].e[qslot(G.nx1)] : G.v.epb;
-        for (int k : G.members) {
-            B->P[size_t(k)]->axv = AxVariant{epb, 0, B->xu};
-            // the batch runs the gather-scatter form of the iteration
-            B->P[size_t(k)]->gather = 0;
-        }
+        for (int k : G.members) B->P[size_t(k)]->axv = AxVariant{epb, 0, B->xu};
     }
     ++B->shapes;
 }
@@ -119,6 +121,33 @@ int upload(T **d, const std::vector<T> &h)
     // stream-ordered: kernels of an earlier replay that read the table
     // finish before it is rewritten
     return h2d(*d, h.data(), sizeof(T) * h.size());
+}
+
+// Before the members' CG init: every member may take the gather form
+// (without the x-ring, so that a member's single solve runs the kernels
+// the batch runs).
+void offer_gather(nekbone_batch *B)
+{
+    for (nekbone_problem *P : B->P) {
+        P->gather = 1;
+        if (P->xring) {
+            P->xring = 0;
+            drop_problem_graph(P);
+        }
+    }
+}
+
+// After the members' CG init (which checks each right-hand side): the
+// batch runs the gather form when every member can; otherwise the
+// gather-scatter form, and the members keep that setting, so that their
+// single solves give the batch's values.
+void decide_form(nekbone_batch *B)
+{
+    bool all = true;
+    for (nekbone_problem *P : B->P) all = all && cg_gather(P);
+    B->gx = all;
+    if (!all)
+        for (nekbone_problem *P : B->P) P->gather = 0;
 }
 
 void drop_graph(nekbone_batch *B)
@@ -151,6 +180,7 @@ axk::AxArgs ax_args(nekbone_batch *B, int k, int mbit)
     A.scal = P->d_scal;
     A.part = P->d_part;
     A.nelt = P->nelt;
+    A.gx = B->gx ? 1 : 0;
     return A;
 }
 
@@ -240,7 +270,7 @@ int prepare(nekbone_batch *B, int precond, const std::vector<int> &mbit)
     }
     std::vector<GsCgArgs> gs;
     std::vector<int> gss{0};
-    for (size_t k = 0; k < B->P.size(); ++k) {
+    for (size_t k = 0; k < B->P.size() && !B->gx; ++k) {
         nekbone_problem *P = B->P[k];
         gs.push_back(gs_cg_args(P, P->d_w, P->d_p, mbit[k], P->d_part + cg_ax_blocks(P)));
         if (cg_fold_partials(P)) {  // as in the single-problem iteration
@@ -250,14 +280,17 @@ int prepare(nekbone_batch *B, int precond, const std::vector<int> &mbit)
         gss.push_back(gss.back() + cg_gs_blocks(P));
     }
     B->nblk_gs = gss.back();
-    if (int rc = upload(&B->d_gsall, gs)) return rc;
-    if (int rc = upload(&B->d_gsstart, gss)) return rc;
+    if (!B->gx) {
+        if (int rc = upload(&B->d_gsall, gs)) return rc;
+        if (int rc = upload(&B->d_gsstart, gss)) return rc;
+    }
     std::vector<CgBatchArgs> args;
     std::vector<int> us{0};
     for (nekbone_problem *P : B->P) {
-        args.push_back(cg_batch_args(P, P->d_x, P->d_r));
+        args.push_back(cg_batch_args(P, P->d_x, P->d_r, B->gx));
         us.push_back(us.back() + args.back().nupd);
     }
+
     B->nupd_total = us.back();
     if (int rc = upload(&B->d_args, args)) return rc;
     return upload(&B->d_ustart, us);
@@ -281,15 +314,16 @@ int launch_iteration(nekbone_batch *B, int precond)
                            static_cast<const int *>(G.d_axstart), int(G.members.size()));
         NB_HIP(hipGetLastError());
     }
-    if (int rc = launch_gs_cg_batch(B->d_gsall, B->d_gsstart, np, B->nblk_gs)) return rc;
+    if (!B->gx)
+        if (int rc = launch_gs_cg_batch(B->d_gsall, B->d_gsstart, np, B->nblk_gs)) return rc;
     return launch_cg_batch_tail(B->d_args, B->d_ustart, np, B->nupd_total, precond, B->fma,
-                                !B->xu, B->xu);
+                                !B->xu, B->xu, B->gx);
 }
 
 int ensure_graph(nekbone_batch *B, int precond, int chunk)
 {
     if (B->exec && B->exec_precond == precond && B->exec_chunk == chunk &&
-        B->exec_shapes == B->shapes)
+        B->exec_shapes == B->shapes && B->exec_gx == int(B->gx))
         return NEKBONE_OK;
     drop_graph(B);
     hipGraph_t g = nullptr;
@@ -308,6 +342,7 @@ int ensure_graph(nekbone_batch *B, int precond, int chunk)
     B->exec_precond = precond;
     B->exec_chunk = chunk;
     B->exec_shapes = B->shapes;
+    B->exec_gx = int(B->gx);
     return NEKBONE_OK;
 }
 
@@ -459,6 +494,13 @@ extern "C" int nekbone_batch_get_path(const nekbone_batch *B, int *tileset, int 
     return NEKBONE_OK;
 }
 
+extern "C" int nekbone_batch_get_form(const nekbone_batch *B, int *gather)
+{
+    if (!B) return fail(NEKBONE_ERR_ARG, "batch: null");
+    if (gather) *gather = B->gx ? 1 : 0;
+    return NEKBONE_OK;
+}
+
 extern "C" int nekbone_batch_cg_tol_d(nekbone_batch *B, double *const *x, const double *const *f,
                                       double *const *r, double tol, int maxiter, int precond,
                                       int *iters, double *final_rtr)
@@ -470,6 +512,7 @@ extern "C" int nekbone_batch_cg_tol_d(nekbone_batch *B, double *const *x, const 
         return fail(NEKBONE_ERR_UNSUPPORTED, "batch: identity or Jacobi preconditioner");
     const size_t np = B->P.size();
     std::vector<int> mbit(np);
+    offer_gather(B);
     for (size_t k = 0; k < np; ++k) {
         nekbone_problem *P = B->P[k];
         if (precond == NEKBONE_PRECOND_JACOBI && !P->jacobi)
@@ -482,6 +525,7 @@ extern "C" int nekbone_batch_cg_tol_d(nekbone_batch *B, double *const *x, const 
             return rc;
         mbit[k] = next_mask_bit(P);
     }
+    decide_form(B);
     if (int rc = prepare(B, precond, mbit)) return rc;
     const int chunk = 10;
     if (B->use_graph)
@@ -580,13 +624,17 @@ extern "C" int nekbone_batch_autotune(nekbone_batch *B, int precond, double *bes
     int rc = NEKBONE_OK;
     for (const Cand &c : cands) {
         set_cand(c);
+        offer_gather(B);
         for (size_t k = 0; k < np && !rc; ++k) {
             nekbone_problem *P = B->P[k];
-            rc = launch_rand(P->d_f, P->n, 12345 + k);
+            // continuous right-hand sides, as a solve's (the form is timed
+            // that the solves will run)
+            rc = tune_rhs(P, 12345 + k);
             if (!rc)
                 rc = launch_cg_init(P, P->d_x, P->d_r, P->d_f, 1, precond, 0.0,
                                     precond == NEKBONE_PRECOND_JACOBI);
         }
+        decide_form(B);
         if (!rc) rc = prepare(B, precond, mbit);
         for (int i = 0; i < 2 && !rc; ++i) rc = launch_iteration(B, precond);
         if (!rc && hipEventRecord(e0, stream()) != hipSuccess) rc = NEKBONE_ERR_HIP;

@@ -188,16 +188,18 @@ __global__ void __launch_bounds__(kRedThreads)
 // [XF] x-ring flush (iterations k with k % kRing == kRing - 1): x receives
 // alpha_j p_j of the iterations j in [xdone, k] from the ring (rs apart),
 // in iteration order.
-template <int PRE, bool FMA, bool XP, bool XF = false>
-__global__ void __launch_bounds__(kRedThreads)
-    k_update_gx(double *__restrict__ r, const double *__restrict__ w,
-                const double *__restrict__ dinv, const uint8_t *__restrict__ flag,
-                double *__restrict__ x, const double *__restrict__ p, int nelt,
-                double *__restrict__ part, int nblk, const CgScalars *__restrict__ s, int rev,
-                GxView G, const double *__restrict__ ring = nullptr, int64_t rs = 0)
+template <int PRE, bool FMA, bool XP, bool XF>
+__device__ __forceinline__ void update_gx_body(double *__restrict__ r, const double *__restrict__ w,
+                                               const double *__restrict__ dinv,
+                                               const uint8_t *__restrict__ flag,
+                                               double *__restrict__ x,
+                                               const double *__restrict__ p, int nelt,
+                                               double *__restrict__ part, int nblk,
+                                               const CgScalars *__restrict__ s, const int blk,
+                                               const GxView &G, const double *__restrict__ ring,
+                                               int64_t rs)
 {
     __shared__ int32_t tabs[kGxMaxEb * kGxStride];
-    const int blk = sweep_block(blockIdx.x, nblk, rev);
     if (s->done) return;
     const int64_t e0 = int64_t(blk) * G.eb;
     const int ne = int(nelt - e0 < G.eb ? nelt - e0 : G.eb);
@@ -299,6 +301,18 @@ __global__ void __launch_bounds__(kRedThreads)
     }
 }
 
+template <int PRE, bool FMA, bool XP, bool XF = false>
+__global__ void __launch_bounds__(kRedThreads)
+    k_update_gx(double *__restrict__ r, const double *__restrict__ w,
+                const double *__restrict__ dinv, const uint8_t *__restrict__ flag,
+                double *__restrict__ x, const double *__restrict__ p, int nelt,
+                double *__restrict__ part, int nblk, const CgScalars *__restrict__ s, int rev,
+                GxView G, const double *__restrict__ ring = nullptr, int64_t rs = 0)
+{
+    update_gx_body<PRE, FMA, XP, XF>(r, w, dinv, flag, x, p, nelt, part, nblk, s,
+                                     sweep_block(blockIdx.x, nblk, rev), G, ring, rs);
+}
+
 // x-ring, end of a solve: x receives alpha_j p_j of the iterations j in
 // [xdone, iter) that no flush has folded in yet; k_ring_mark then records
 // them as folded.
@@ -345,6 +359,19 @@ __global__ void __launch_bounds__(kRedThreads)
     const CgBatchArgs A = args[q];
     update_body<PRE, FMA, XP>(A.r, A.w, A.dinv, A.flag, A.x, A.p, A.n, A.part_upd, A.nupd,
                               A.scal, blockIdx.x - start[q]);
+}
+
+// Batched gather-form update: problem q owns workgroups
+// [start[q], start[q+1]), each the workgroup of its single-problem launch.
+template <int PRE, bool FMA, bool XP>
+__global__ void __launch_bounds__(kRedThreads)
+    k_update_gx_batch(const CgBatchArgs *__restrict__ args, const int *__restrict__ start,
+                      int nprob)
+{
+    const int q = owner(start, nprob, blockIdx.x);
+    const CgBatchArgs &A = args[q];
+    update_gx_body<PRE, FMA, XP, false>(A.r, A.w, A.dinv, A.flag, A.x, A.p, A.nelt, A.part_upd,
+                                        A.nupd, A.scal, blockIdx.x - start[q], A.G, nullptr, 0);
 }
 
 __global__ void __launch_bounds__(kFinalThreads) k_pap_final_batch(const CgBatchArgs *__restrict__ args)
@@ -832,9 +859,30 @@ int launch_cg_iteration(nekbone_problem *P, double *x, double *r, int precond, i
     return iteration_generic(P, x, r, precond, mbit);
 }
 
-CgBatchArgs cg_batch_args(nekbone_problem *P, double *x, double *r)
+CgBatchArgs cg_batch_args(nekbone_problem *P, double *x, double *r, bool gx)
 {
     CgBatchArgs A{};
+    if (gx) {
+        // gather form: the element kernel's parts alone make pap; the
+        // update parts come from the gather-form update's workgroups
+        A.r = r;
+        A.w = P->d_w;
+        A.dinv = P->d_dinv;
+        A.flag = P->d_flag;
+        A.x = x;
+        A.p = P->d_p;
+        A.n = P->n;
+        A.part = P->d_part;
+        A.npap = n_ax(P);
+        A.part_upd = P->d_part + n_ax(P) + n_gs(P);
+        A.G = gx_view(P, kGxPts64);
+        A.nupd = int((P->nelt + A.G.eb - 1) / A.G.eb);
+        A.nelt = P->nelt;
+        A.scal = P->d_scal;
+        A.hist = P->d_hist;
+        A.hcap = P->hist_cap;
+        return A;
+    }
     A.r = r;
     A.w = P->d_w;
     A.dinv = P->d_dinv;
@@ -857,9 +905,38 @@ CgBatchArgs cg_batch_args(nekbone_problem *P, double *x, double *r)
 // of problems (the element and gather-scatter kernels are launched per
 // order group before): pap/alpha, r update with rtr and rtz, rtr/beta.
 int launch_cg_batch_tail(const CgBatchArgs *d_args, const int *d_ustart, int nprob, int nupd_total,
-                         int precond, bool fma, bool xp, int set_xpending)
+                         int precond, bool fma, bool xp, int set_xpending, bool gx)
 {
     hipLaunchKernelGGL(k_pap_final_batch, dim3(nprob), dim3(kFinalThreads), 0, stream(), d_args);
+    if (gx) {
+#define NB_UGB(PRE, F, X)                                                                     \
+    hipLaunchKernelGGL((k_update_gx_batch<PRE, F, X>), dim3(nupd_total), dim3(kRedThreads), 0, \
+                       stream(), d_args, d_ustart, nprob)
+        const bool j = precond == NEKBONE_PRECOND_JACOBI;
+        if (j) {
+            if (fma) {
+                if (xp) NB_UGB(kPreJacobiZ, true, true); else NB_UGB(kPreJacobiZ, true, false);
+            } else {
+                if (xp) NB_UGB(kPreJacobiZ, false, true); else NB_UGB(kPreJacobiZ, false, false);
+            }
+        } else {
+            if (fma) {
+                if (xp) NB_UGB(NEKBONE_PRECOND_IDENTITY, true, true); else NB_UGB(NEKBONE_PRECOND_IDENTITY, true, false);
+            } else {
+                if (xp) NB_UGB(NEKBONE_PRECOND_IDENTITY, false, true); else NB_UGB(NEKBONE_PRECOND_IDENTITY, false, false);
+            }
+        }
+#undef NB_UGB
+        NB_HIP(hipGetLastError());
+        if (j)
+            hipLaunchKernelGGL(k_update_final_batch<NEKBONE_PRECOND_JACOBI>, dim3(nprob),
+                               dim3(kFinalThreads), 0, stream(), d_args, set_xpending);
+        else
+            hipLaunchKernelGGL(k_update_final_batch<NEKBONE_PRECOND_IDENTITY>, dim3(nprob),
+                               dim3(kFinalThreads), 0, stream(), d_args, set_xpending);
+        NB_HIP(hipGetLastError());
+        return NEKBONE_OK;
+    }
 #define NB_UB(PRE, F, X)                                                                     \
     hipLaunchKernelGGL((k_update_batch<PRE, F, X>), dim3(nupd_total), dim3(kRedThreads), 0, \
                        stream(), d_args, d_ustart, nprob)

@@ -276,23 +276,6 @@ struct CgScalars {
     double aring[kRing];
 };
 
-// Per-problem arguments of the batched fused CG kernels.
-struct CgBatchArgs {
-    double *r;
-    const double *w, *dinv;
-    const uint8_t *flag;
-    double *x;
-    const double *p;
-    int64_t n;
-    double *part;       // element + gather-scatter pap parts
-    int64_t npap;
-    double *part_upd;   // update parts (rtr, then rtz)
-    int nupd;
-    CgScalars *scal;
-    double *hist;
-    int hcap;
-};
-
 // Entity table of the gather form of the fused CG (the vector update sums
 // the copies of each shared point itself instead of a gather-scatter
 // kernel).  Per element kGxStride int32: the element's surface entities in
@@ -326,6 +309,26 @@ inline int gx_eb(int nx, int pts)
     int eb = pts / n3;
     return eb < 1 ? 1 : eb > kGxMaxEb ? kGxMaxEb : eb;
 }
+
+// Per-problem arguments of the batched fused CG kernels.
+struct CgBatchArgs {
+    double *r;
+    const double *w, *dinv;
+    const uint8_t *flag;
+    double *x;
+    const double *p;
+    int64_t n;
+    double *part;       // element + gather-scatter pap parts
+    int64_t npap;
+    double *part_upd;   // update parts (rtr, then rtz)
+    int nupd;
+    CgScalars *scal;
+    double *hist;
+    int hcap;
+    // gather form: the problem's entity table view and element count
+    GxView G;
+    int nelt;
+};
 
 // Kernel view of a gather-scatter map: the class member lists and the
 // number of workgroups each class takes.
@@ -600,9 +603,10 @@ int tune_rhs(nekbone_problem *P, uint64_t seed);
 // batched fused CG (nb_batch.hip drives these)
 GsCgArgs gs_cg_args(nekbone_problem *P, double *w, const double *p, int mbit, double *part);
 int launch_gs_cg_batch(const GsCgArgs *d_args, const int *d_start, int nprob, int nblocks);
-CgBatchArgs cg_batch_args(nekbone_problem *P, double *x, double *r);
+// gx: the gather form (no gather-scatter kernel; see cg_gather)
+CgBatchArgs cg_batch_args(nekbone_problem *P, double *x, double *r, bool gx = false);
 int launch_cg_batch_tail(const CgBatchArgs *d_args, const int *d_ustart, int nprob, int nupd_total,
-                         int precond, bool fma, bool xp, int set_xpending);
+                         int precond, bool fma, bool xp, int set_xpending, bool gx = false);
 int cg_ax_blocks(const nekbone_problem *P);
 // whether a fused iteration folds the element kernel's partial sums into
 // the gather-scatter kernel's (it has workgroups enough)

@@ -63,6 +63,56 @@ def test_batch_matches_single_solves(nb, oracle, precond, xu, path):
     B.close()
 
 
+SPECS_GX = [(12, 6, True, 1), (27, 8, False, 1), (7, 10, True, 1), (20, 12, True, 1),
+            (64, 6, False, 1), (9, 8, True, 1), (30, 10, False, 1), (5, 12, False, 1),
+            (10, 5, True, 1)]
+
+
+@pytest.mark.parametrize("precond,xu,path", [(1, 1, -1), (1, 0, -1), (0, 1, -1), (1, 1, 0),
+                                             (1, 0, 2)])
+def test_batch_gather_form_matches_single_solves(nb, oracle, precond, xu, path):
+    """Every member with the full Dirichlet mask and a continuous
+    right-hand side: the batch runs the gather form (no gather-scatter
+    launch; the update kernel sums the shared copies), on every element-
+    kernel path, and each member's values are those of its own single
+    solve bit for bit; with one member on the reference's mask list the
+    batch falls back to the gather-scatter form."""
+    _case("batch gather form precond=%d xu=%d path=%d" % (precond, xu, path))
+    probs = _problems(nb, oracle, SPECS_GX)
+    B = nb.Batch([p for p, _ in probs])
+    B.set_queue(path)
+    B.set_xu(xu)
+    tol = 1e-8
+    fs, xs, rs = [], [], []
+    for P, f in probs:
+        fv, xv, rv = P.vec(), P.vec(), P.vec()
+        fv.upload(f)
+        fs.append(fv), xs.append(xv), rs.append(rv)
+    its, rtrs = B.cg_tol_d(xs, fs, rs, tol, 400, precond)
+    assert B.gather_form()
+    for k, (P, f) in enumerate(probs):
+        x1, r1 = P.vec(), P.vec()
+        it1, rtr1 = P.cg_tol_d(x1, fs[k], r1, tol, 400, precond)
+        assert P.cg_form() == (True, False)
+        assert its[k] == it1, k
+        assert rtrs[k] == rtr1, k
+        assert np.array_equal(xs[k].download(), x1.download()), k
+        assert np.array_equal(rs[k].download(), r1.download()), k
+        assert 0 < its[k] < 400
+    B.close()
+    # one member on the reference's mask list: gather-scatter form
+    probs = _problems(nb, oracle, SPECS_GX[:3] + [(9, 8, True, 0)])
+    B = nb.Batch([p for p, _ in probs])
+    fs = []
+    for P, f in probs:
+        v = P.vec()
+        v.upload(f)
+        fs.append(v)
+    B.cg_tol_d([P.vec() for P, _ in probs], fs, [None] * 4, tol, 400, precond)
+    assert not B.gather_form()
+    B.close()
+
+
 def test_batch_repeat_and_maxiter(nb, oracle):
     """A second solve reuses the captured graph; a maxiter below the
     chunk size and one that is not a multiple of it."""
