@@ -148,7 +148,8 @@ int nekbone_set_graph(nekbone_problem *P, int on);
 
 /* Element-kernel shape: elements per workgroup; pf = 0 geometric factors
  * loaded where they are used, 1 prefetched at kernel entry, 2 as 0 but
- * compiled for 5 waves per SIMD; x update carried by the element kernel
+ * compiled for 5 waves per SIMD, 3 loaded one t-step of the third phase
+ * ahead of their use; x update carried by the element kernel
  * (0/1).  nekbone_autotune times every shape compiled for this order on
  * this problem (a few fused CG iterations each), keeps the fastest and
  * remembers it for later problems of the same order, preconditioner and

@@ -40,9 +40,10 @@ AxVariant ax_default_variant(int nx1)
 {
     AxVariant v;
     // measured fastest at N=9 on MI355X: one element per workgroup at this
-    // order, factors loaded in the geometric phase, x update fused
+    // order, factors loaded one t-step ahead of their use where that shape
+    // is compiled (else in the geometric phase), x update fused
     v.epb = axk::epb_for(nx1);
-    v.pf = 0;
+    v.pf = lookup(nx1, v.epb, false, 1, 3, 1) ? 3 : 0;
     v.xu = 1;
     return v;
 }
@@ -127,7 +128,8 @@ int launch_ax_cg(nekbone_problem *P, double *x, const double *z, int mbit, int r
 int launch_ax_flush(nekbone_problem *P, float *x32, const float *p32)
 {
     const int epb = axk::epb_for(P->nx1);
-    AxFn fn = lookup(P->nx1, epb, false, 3, 0, 0);
+    AxFn fn = lookup(P->nx1, epb, false, 3, 3, 0);
+    if (!fn) fn = lookup(P->nx1, epb, false, 3, 0, 0);
     if (!fn || P->arith == NEKBONE_ARITH_FMA) return NEKBONE_ERR_UNSUPPORTED;
     AxArgs A{};
     A.w = P->d_w;

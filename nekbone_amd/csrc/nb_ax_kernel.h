@@ -37,7 +37,8 @@
 //    VGPRs); PF=0 issues them in the geometric phase (higher occupancy);
 //    PF=2 is PF=0 compiled for 5 waves per SIMD in FP64 (at most 96
 //    VGPRs, the most that 16 KB of LDS per element allows at one element
-//    per workgroup) and for 8 in FP32 (at most 64 VGPRs).
+//    per workgroup) and for 8 in FP32 (at most 64 VGPRs).  PF=3 loads
+//    them one t-step ahead of their use (software pipelining).
 //
 // CG mode (MODE 1) fuses, per point of the thread's t-line:
 //   prologue  [XU] x += alpha_prev p_prev     (add2s2, src/cg.f:63, one
@@ -478,6 +479,16 @@ __device__ __forceinline__ void ax_core(const AxArgsT<T> &A_, const int blk, T *
         }
 
         T lr[NX], ls[NX];
+        // PF=3: the factors are loaded one t-step ahead of their use, the
+        // first step's before the r and s contractions, so a load round trip
+        // overlaps a step's arithmetic instead of stalling it (the empty asm
+        // that keeps D in scalar registers is a scheduling barrier, so the
+        // compiler would not hoist them itself)
+        constexpr bool PIPE = PF == 3;
+        T gnx[PIPE ? 6 : 1];
+        if constexpr (PIPE) {
+            if (act) gload6(a + NX * b, gnx);
+        }
         if (act) {
 #pragma unroll
             for (int m = 0; m < NX; ++m) lr[m] = A[m + NX * a + N2 * b];  // r-line (j,k)=(a,b)
@@ -499,12 +510,17 @@ __device__ __forceinline__ void ax_core(const AxArgsT<T> &A_, const int blk, T *
             for (int k = 0; k < NX; ++k) {
                 const int q = a + NX * b + N2 * k;
                 const T ur = B[q], us = A[q];
-                const T ut = dot_row<NX, FMA>(Drow, k, uc);
                 T gq[6];
+                if constexpr (PIPE) {
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) gq[c] = gnx[c];
+                    if (k + 1 < NX) gload6(q + N2, gnx);
+                }
+                const T ut = dot_row<NX, FMA>(Drow, k, uc);
                 if constexpr (PRE) {
 #pragma unroll
                     for (int c = 0; c < 6; ++c) gq[c] = gv[k][c];
-                } else {
+                } else if constexpr (!PIPE) {
                     gload6(q, gq);
                 }
                 const T g1 = gq[0], g2 = gq[1], g3 = gq[2], g4 = gq[3], g5 = gq[4], g6 = gq[5];
@@ -729,12 +745,14 @@ void reg_modes(Table &t)
 }
 
 // x flush + FP64 operator of the mixed-precision CG's residual refresh
-// (MODE 3): the default shape, exact arithmetic.  Key (nx, epb, 0, 3, 0, 0).
+// (MODE 3): the default shape, exact arithmetic.  Key (nx, epb, 0, 3, pf, 0),
+// pf 0, and 3 (factors one t-step ahead) for the orders with every shape.
 template <int NX>
-void reg_refresh(Table &t)
+void reg_refresh(Table &t, bool full)
 {
     constexpr int E = epb_for(NX);
     t[Key(NX, E, 0, 3, 0, 0)] = k_ax<double, NX, E, false, 3, 0, false>;
+    if (full) t[Key(NX, E, 0, 3, 3, 0)] = k_ax<double, NX, E, false, 3, 3, false>;
 }
 
 // Orders the benchmark configurations use get the full variant space;
@@ -743,10 +761,11 @@ template <int NX>
 void reg_order(Table &t, bool full)
 {
     constexpr int E = epb_for(NX);
-    reg_refresh<NX>(t);
+    reg_refresh<NX>(t, full);
     reg_modes<NX, E, 0>(t);
     if (full) {
         reg_modes<NX, E, 1>(t);
+        reg_modes<NX, E, 3>(t);
         reg_modes<NX, 2 * E, 0>(t);
         reg_modes<NX, 2 * E, 1>(t);
         if constexpr (E == 1) reg_modes<NX, E, 2>(t);
@@ -772,8 +791,12 @@ void regb_order(TableB &t)
     constexpr int E = epb_for(NX);
     regb_modes<NX, E, 0>(t);
     // the batched workload's orders also get the doubled shape (the
-    // single-problem table has it too), for the batch autotuner
-    if constexpr (NX == 6 || NX == 8 || NX == 10 || NX == 12) regb_modes<NX, 2 * E, 0>(t);
+    // single-problem table has it too), for the batch autotuner, and the
+    // default shape with the factors loaded one t-step ahead
+    if constexpr (NX == 6 || NX == 8 || NX == 10 || NX == 12) {
+        regb_modes<NX, 2 * E, 0>(t);
+        regb_modes<NX, E, 3>(t);
+    }
 }
 
 // FP32 kernels (inner iterations of the mixed-precision CG), always with
@@ -800,8 +823,10 @@ void reg32_order(Table32 &t, bool full)
     reg32_modes<NX, E, 0>(t);
     if (full) {
         reg32_modes<NX, E, 1>(t);
+        reg32_modes<NX, E, 3>(t);
         reg32_modes<NX, 2 * E, 0>(t);
         reg32_modes<NX, 2 * E, 1>(t);
+        reg32_modes<NX, 2 * E, 3>(t);
         reg32_modes<NX, E, 2>(t);
         reg32_modes<NX, 2 * E, 2>(t);
     }

@@ -105,7 +105,7 @@ void apply_shapes(nekbone_batch *B)
 {
     for (auto &G : B->groups) {
         const int epb = in_queue(B, G.nx1) ? tableq()[size_t(B->queue)].e[qslot(G.nx1)] : G.v.epb;
-        for (int k : G.members) B->P[size_t(k)]->axv = AxVariant{epb, 0, B->xu};
+        for (int k : G.members) B->P[size_t(k)]->axv = AxVariant{epb, in_queue(B, G.nx1) ? 0 : G.v.pf, B->xu};
     }
     ++B->shapes;
 }
@@ -591,27 +591,33 @@ extern "C" int nekbone_batch_autotune(nekbone_batch *B, int precond, double *bes
         mbit[k] = P->mask_mode == NEKBONE_MASK_DIRICHLET ? 1 : 2;
     }
     struct Cand {
-        int queue, dbl, xu;
+        int queue, dbl, xu, pf;
     };
     std::vector<Cand> cands;
     bool any_q = false;
     for (auto &G : B->groups) any_q = any_q || qslot(G.nx1) >= 0;
     for (int xu = 1; xu >= 0; --xu) {
-        cands.push_back(Cand{-1, 0, xu});
-        cands.push_back(Cand{-1, 1, xu});
+        cands.push_back(Cand{-1, 0, xu, 0});
+        cands.push_back(Cand{-1, 0, xu, 3});
+        cands.push_back(Cand{-1, 1, xu, 0});
         if (any_q)
-            for (int q = 0; q < int(tableq().size()); ++q) cands.push_back(Cand{q, 0, xu});
+            for (int q = 0; q < int(tableq().size()); ++q) cands.push_back(Cand{q, 0, xu, 0});
     }
-    // per-group shape of a candidate: the order's default, or twice as many
-    // elements per workgroup where that batched kernel is compiled
+    // per-group shape of a candidate: the order's default, the default with
+    // the factors loaded one t-step ahead, or twice as many elements per
+    // workgroup, where that batched kernel is compiled
     auto set_cand = [&](const Cand &c) {
         B->queue = c.queue;
         B->xu = c.xu;
+        const int fm = B->fma ? 1 : 0;
         for (auto &G : B->groups) {
             const int e = axk::epb_for(G.nx1);
-            const bool has2 =
-                tableb().count(axk::Key(G.nx1, 2 * e, B->fma ? 1 : 0, 1, 0, c.xu)) != 0;
-            G.v = AxVariant{c.dbl && has2 ? 2 * e : e, 0, 1};
+            const bool has2 = tableb().count(axk::Key(G.nx1, 2 * e, fm, 1, 0, c.xu)) != 0;
+            const bool has3 = tableb().count(axk::Key(G.nx1, e, fm, 1, 3, c.xu)) != 0;
+            if (c.dbl && has2)
+                G.v = AxVariant{2 * e, 0, 1};
+            else
+                G.v = AxVariant{e, c.pf == 3 && has3 ? 3 : 0, 1};
         }
         apply_shapes(B);
     };
@@ -619,7 +625,7 @@ extern "C" int nekbone_batch_autotune(nekbone_batch *B, int precond, double *bes
     NB_HIP(hipEventCreate(&e0));
     NB_HIP(hipEventCreate(&e1));
     double best = 1e300;
-    Cand win{B->queue, 0, B->xu};
+    Cand win{B->queue, 0, B->xu, 0};
     bool have_win = false;
     int rc = NEKBONE_OK;
     for (const Cand &c : cands) {

@@ -67,7 +67,7 @@ def sweep_order(nx1, points, reps):
     u64 = np.random.default_rng(nx1).uniform(-1.0, 1.0, n)
     u32 = u64.astype(np.float32)
     e0 = _epb_default(nx1)
-    shapes = [(e, pf) for e in (e0, 2 * e0) for pf in (0, 1, 2)]
+    shapes = [(e, pf) for e in (e0, 2 * e0) for pf in (0, 1, 2, 3)]
 
     def row(kind, epb, pf, ms):
         us = ms * 1e3
