@@ -64,6 +64,38 @@ AxVariant tuned_or_default(int nx1)
 
 hipStream_t stream() { return g_stream; }
 
+namespace {
+CgScalars *g_hscal = nullptr;  // pinned: two slots of posted scalars
+hipEvent_t g_sev[2] = {nullptr, nullptr};
+int g_sev_dev = -1;
+}  // namespace
+
+int scal_post(const nekbone_problem *P, int slot)
+{
+    if (g_sev_dev != g_device) {
+        if (!g_hscal)
+            NB_HIP(hipHostMalloc(reinterpret_cast<void **>(&g_hscal), 2 * sizeof(CgScalars),
+                                 hipHostMallocDefault));
+        for (hipEvent_t &e : g_sev) {
+            if (e) (void)hipEventDestroy(e);
+            e = nullptr;
+            NB_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        }
+        g_sev_dev = g_device;
+    }
+    NB_HIP(hipMemcpyAsync(&g_hscal[slot], P->d_scal, sizeof(CgScalars), hipMemcpyDeviceToHost,
+                          stream()));
+    NB_HIP(hipEventRecord(g_sev[slot], stream()));
+    return NEKBONE_OK;
+}
+
+int scal_wait(int slot, CgScalars &s)
+{
+    NB_HIP(hipEventSynchronize(g_sev[slot]));
+    s = g_hscal[slot];
+    return NEKBONE_OK;
+}
+
 int ensure_init()
 {
     if (g_stream) return NEKBONE_OK;
@@ -1201,17 +1233,28 @@ extern "C" int nekbone_cg_tol_d(nekbone_problem *P, double *x, const double *f, 
                                 cg_zform(P, precond)))
         return rc;
     const int mbit = next_mask_bit(P);
-    // Iterations are queued in chunks; kernels of an already converged
-    // solve return at once, so only one host check per chunk is needed.
+    // Iterations are queued in chunks, each followed by a copy of the
+    // scalars into pinned memory; the next chunk is queued before the host
+    // waits for the previous one's copy, so the GPU never waits for the
+    // host check.  Kernels of a converged solve return at once, so the
+    // chunk queued past convergence costs only its launches.
     const int chunk = 10;
     CgScalars s{};
-    for (int it = 0; it < maxiter; it += chunk) {
-        if (int rc = run_iterations(P, P->d_x, P->d_r, precond, mbit,
-                                    std::min(maxiter, it + chunk) - it, it))
-            return rc;
-        NB_HIP(hipMemcpyAsync(&s, P->d_scal, sizeof s, hipMemcpyDeviceToHost, stream()));
-        NB_HIP(hipStreamSynchronize(stream()));
-        if (s.done) break;
+    int queued = 0, posted = 0, seen = 0;
+    auto queue = [&]() -> int {
+        const int todo = std::min(chunk, maxiter - queued);
+        if (int rc = run_iterations(P, P->d_x, P->d_r, precond, mbit, todo, queued)) return rc;
+        queued += todo;
+        return scal_post(P, posted++ & 1);
+    };
+    if (maxiter > 0) {
+        if (int rc = queue()) return rc;
+        for (;;) {
+            if (queued < maxiter)
+                if (int rc = queue()) return rc;
+            if (int rc = scal_wait(seen++ & 1, s)) return rc;
+            if (s.done || s.status || seen == posted) break;
+        }
     }
     if (int rc = launch_cg_finish(P, P->d_x, P->d_r, cg_zform(P, precond))) return rc;
     if (x)

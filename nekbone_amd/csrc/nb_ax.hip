@@ -125,7 +125,7 @@ int launch_ax_cg(nekbone_problem *P, double *x, const double *z, int mbit, int r
     return timed_end(P, 0);
 }
 
-int launch_ax_flush(nekbone_problem *P, float *x32, const float *p32)
+int launch_ax_flush(nekbone_problem *P, float *x32, const float *p32, int gate)
 {
     const int epb = axk::epb_for(P->nx1);
     AxFn fn = lookup(P->nx1, epb, false, 3, 3, 0);
@@ -143,6 +143,7 @@ int launch_ax_flush(nekbone_problem *P, float *x32, const float *p32)
     A.nelt = P->nelt;
     A.x32 = x32;
     A.p32 = p32;
+    A.gate = gate;
     const int threads = round64(P->nx1 * P->nx1 * epb);
     const int blocks = (P->nelt + epb - 1) / epb;
     hipLaunchKernelGGL(fn, dim3(blocks), dim3(threads), 0, stream(), A);

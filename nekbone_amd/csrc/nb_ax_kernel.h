@@ -223,6 +223,7 @@ struct AxArgsT {
     int64_t gcs;         // distance between factor planes (local points n)
     int rev;             // walk the elements downward (see sweep_block)
     int gx;              // CG, gather form: the pap parts cover shared points too (below)
+    int gate;            // MODE 3: run only when mx_want holds (device-driven refreshes)
     // MODE 3 (first pass of the FP64 residual refresh of the mixed-
     // precision CG): x = x + the FP32 increment x32 (+ alpha p32 when
     // pending), x32 = 0, then w = A_L x at every point as in MODE 0.
@@ -304,6 +305,9 @@ __device__ __forceinline__ void ax_core(const AxArgsT<T> &A_, const int blk, T *
 
     if constexpr (MODE == 1) {
         if (A_.scal->done) return;  // converged tolerance run: nothing to do
+    }
+    if constexpr (MODE == 3) {
+        if (A_.gate && !mx_want(A_.scal)) return;  // no refresh due
     }
 
     constexpr bool PRE = PF == 1;

@@ -533,12 +533,13 @@ bool cg_zform(const nekbone_problem *P, int precond)
 }
 
 int launch_cg_init(nekbone_problem *P, double *x, double *r, const double *f, int mbit,
-                   int precond, double tol2, bool zform)
+                   int precond, double tol2, bool zform, bool fp64_iter)
 {
     // every solve decides afresh whether its right-hand side allows the
     // gather form, and with it the x-ring
     if (int rc = cg_gather_check(P, f)) return rc;
-    P->ring_on = cg_gather(P) && precond != NEKBONE_PRECOND_HSMG && ensure_ring(P);
+    P->ring_on =
+        fp64_iter && cg_gather(P) && precond != NEKBONE_PRECOND_HSMG && ensure_ring(P);
     const int nb = reduce_blocks(P->n);
     if (precond == NEKBONE_PRECOND_JACOBI && zform) {
         hipLaunchKernelGGL(k_cg_init<kPreJacobiZ>, dim3(nb), dim3(kRedThreads), 0, stream(), x,

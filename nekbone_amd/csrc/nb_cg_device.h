@@ -170,6 +170,18 @@ __device__ __forceinline__ T ring_fold(T xv, const T *__restrict__ ring, int64_t
     return xv;
 }
 
+// Mixed-precision CG with device-driven refreshes: whether the FP64
+// residual refresh queued after a chunk of inner iterations runs (inner
+// recurrence converged, residual reduced by delta since the last refresh,
+// or the iteration limit reached), until the FP64 residual has met the
+// tolerance.  The refresh kernels evaluate it on scalars that only the
+// last of them changes.
+__device__ __forceinline__ bool mx_want(const CgScalars *s)
+{
+    return !s->status && !s->fin &&
+           (s->done || s->rtr <= s->delta2 * s->rtr_upd || s->iter >= s->maxit);
+}
+
 // Table rows of elements e0 .. e0+ne-1 into LDS (the workgroup's threads).
 template <int NT>
 __device__ __forceinline__ void gx_rows(int32_t *tabs, const GxView &G, int64_t e0, int ne)

@@ -274,6 +274,13 @@ struct CgScalars {
     // in slot j % kRing
     int xdone;
     double aring[kRing];
+    // mixed precision, device-driven refreshes (set by k_mx_start): the
+    // refresh kernels queued after every chunk of inner iterations run when
+    // mx_want holds (inner convergence, residual reduction delta2 since the
+    // last refresh, or maxit iterations); fin: the FP64 residual met tol;
+    // nupd: refreshes run
+    int fin, nupd, maxit;
+    double delta2;
 };
 
 // Entity table of the gather form of the fused CG (the vector update sums
@@ -528,7 +535,8 @@ int launch_gs_cg(nekbone_problem *P, double *w, const double *p, int mbit, doubl
 // First pass of the mixed solver's FP64 residual refresh (nb_ax.hip):
 // x += x32 (+ alpha p32 when pending), x32 = 0, w = A_L x;
 // NEKBONE_ERR_UNSUPPORTED when no such kernel is compiled for the shape.
-int launch_ax_flush(nekbone_problem *P, float *x32, const float *p32);
+// gate: run only when mx_want holds (device-driven refreshes)
+int launch_ax_flush(nekbone_problem *P, float *x32, const float *p32, int gate = 0);
 // FP32 gather-scatter (mixed-precision inner solve): plain, and the fused
 // CG variant (pap parts in FP64).
 int launch_gs32(nekbone_problem *P, float *u);
@@ -561,10 +569,18 @@ int launch_to_ref(nekbone_problem *P, double *dst, const double *src);
 // Host vector (reference order) <-> device-order array, through d_t.
 int upload_dev(nekbone_problem *P, double *dst, const double *src);
 int download_ref(nekbone_problem *P, double *dst, const double *src);
+// The scalars of P queued for the host: scal_post copies them into pinned
+// slot 0 or 1 after the work queued so far, scal_wait blocks until that
+// copy has landed (so the host can check one chunk of iterations while the
+// next one runs).
+int scal_post(const nekbone_problem *P, int slot);
+int scal_wait(int slot, CgScalars &s);
 // CG building blocks
-// zform (fused Jacobi CG only, see cg_zform): r receives z = M^-1 r
+// zform (fused Jacobi CG only, see cg_zform): r receives z = M^-1 r;
+// fp64_iter: FP64 CG iterations follow (false for the mixed-precision
+// solver, whose inner iterations never use the x-ring)
 int launch_cg_init(nekbone_problem *P, double *x, double *r, const double *f, int mbit,
-                   int precond, double tol2, bool zform = false);
+                   int precond, double tol2, bool zform = false, bool fp64_iter = true);
 // whether the fused Jacobi CG runs in z form (the vector r holds z = M^-1 r
 // between launch_cg_init and launch_cg_finish)
 bool cg_zform(const nekbone_problem *P, int precond);

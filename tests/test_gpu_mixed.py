@@ -206,9 +206,11 @@ def test_cg_mixed_flush_in_operator_matches_separate_pass(nb, oracle, nel, nx1, 
     kernel against the flush as a pass of its own: the same arithmetic on
     every point (x + float(x32 + alpha p32), the operator, the residual
     kernel), so the solves agree bit for bit; and both final rtr are the
-    FP64 residual's."""
+    FP64 residual's.  (Gather-scatter form: in the gather form the fused
+    refresh is the device-driven one, tested below.)"""
     _case("cg_mixed fused refresh %s %d deformed=%s precond=%d" % (nel, nx1, deformed, precond))
     P, O = _pair(nb, oracle, nel, nx1, deformed, jacobi=bool(precond))
+    P.set_gather(False)
     f = O.rhs_from_raw(splitmix_uniform(O.n, 8))
     out = {}
     for fused in (1, 0):
@@ -313,3 +315,50 @@ def test_cg_mixed_gather_and_gs_forms(nb, oracle, inner, precond):
         assert abs(rtr - rr) <= 1e-6 * rr
         its.append(it)
     assert abs(its[0] - its[1]) <= 2
+
+
+@pytest.mark.parametrize("inner", ["f32", "g16"])
+@pytest.mark.parametrize("precond", [1, 0])
+@pytest.mark.parametrize("nel,nx1", [((3, 3, 3), 10), ((4, 2, 3), 6)])
+def test_cg_mixed_device_driven_refreshes(nb, oracle, inner, precond, nel, nx1):
+    """Gather form: the refreshes decided on the device (the refresh
+    kernels queued after every chunk run when due, the host checks one
+    chunk behind) against the host-decided loop (set_mixed_refresh(0)):
+    the same refresh rule, so the same inner iteration count within 1 and
+    the same number of refreshes within 1 (the residual is the same per
+    point, the copies summed in the same member order, but its dot
+    products are summed over other blocks); both meet the FP64 criterion
+    with r recomputed by the oracle, and each final rtr is that
+    residual's.  Then the iteration limit: a tolerance that cannot be
+    met stops after exactly maxiter inner iterations (not a multiple of
+    the chunk) with x and r of the last refresh."""
+    _case("cg_mixed device-driven %s %s %d precond=%d" % (inner, nel, nx1, precond))
+    tol = 1e-10
+    P, O = _pair(nb, oracle, nel, nx1, True, jacobi=bool(precond))
+    if inner == "g16":
+        P.set_mixed_inner(nb.MIXED_G16)
+    assert P.set_gather(True)
+    f = O.rhs_from_raw(splitmix_uniform(O.n, 3))
+    c = O.c()
+    f0 = O.maskit(f)
+    rr0 = float(np.sum(f0 * c * f0))
+    out = {}
+    for fused in (1, 0):
+        P.set_mixed_refresh(fused)
+        x, r, it, nupd, rtr = P.cg_mixed(f, tol, 5000, precond)
+        print("refresh=%d: %d inner iterations, %d FP64 updates" % (fused, it, nupd), flush=True)
+        assert nupd >= 1 and 0 < it < 5000
+        rtrue = f0 - O.ax(x)
+        rr = float(np.sum(rtrue * c * rtrue))
+        assert rr <= tol ** 2 * rr0 * 1.0001
+        assert abs(rtr - rr) <= 1e-6 * rr
+        np.testing.assert_allclose(r, rtrue, rtol=0, atol=1e-9 * np.abs(f0).max())
+        out[fused] = (it, nupd)
+    assert abs(out[1][0] - out[0][0]) <= 1 and abs(out[1][1] - out[0][1]) <= 1
+    P.set_mixed_refresh(1)
+    x, r, it, nupd, rtr = P.cg_mixed(f, 1e-30, 21, precond, maxiter_ok=True)
+    assert it == 21 and nupd >= 1
+    rtrue = f0 - O.ax(x)
+    assert abs(rtr - float(np.sum(rtrue * c * rtrue))) <= 1e-6 * rtr
+    with pytest.raises(nb.NotConverged):
+        P.cg_mixed(f, 1e-30, 21, precond)
