@@ -538,7 +538,8 @@ static int check_precond(nekbone_problem *P, int precond)
 static int cg_step(nekbone_problem *P, double *x, double *r, int precond, int mbit, int k)
 {
     if (precond == NEKBONE_PRECOND_HSMG) {
-        if (int rc = launch_hsmg_apply(P, P->mg->d_z, r)) return rc;
+        // applies queued past convergence return at once
+        if (int rc = launch_hsmg_apply(P, P->mg->d_z, r, P->d_scal)) return rc;
         return launch_cg_iteration_z(P, x, r, P->mg->d_z, mbit);
     }
     return launch_cg_iteration(P, x, r, precond, mbit, k);
@@ -1236,9 +1237,13 @@ extern "C" int nekbone_cg_tol_d(nekbone_problem *P, double *x, const double *f, 
     // Iterations are queued in chunks, each followed by a copy of the
     // scalars into pinned memory; the next chunk is queued before the host
     // waits for the previous one's copy, so the GPU never waits for the
-    // host check.  Kernels of a converged solve return at once, so the
-    // chunk queued past convergence costs only its launches.
+    // host check.  The kernels of a converged solve return at once, so the
+    // chunk queued past convergence costs only its launches; with the HSMG
+    // preconditioner those are some 25 per iteration, more than the host
+    // check costs, so there the host checks each chunk before queueing the
+    // next.
     const int chunk = 10;
+    const bool ahead = precond != NEKBONE_PRECOND_HSMG;
     CgScalars s{};
     int queued = 0, posted = 0, seen = 0;
     auto queue = [&]() -> int {
@@ -1250,10 +1255,14 @@ extern "C" int nekbone_cg_tol_d(nekbone_problem *P, double *x, const double *f, 
     if (maxiter > 0) {
         if (int rc = queue()) return rc;
         for (;;) {
-            if (queued < maxiter)
+            if (ahead && queued < maxiter)
                 if (int rc = queue()) return rc;
             if (int rc = scal_wait(seen++ & 1, s)) return rc;
-            if (s.done || s.status || seen == posted) break;
+            if (s.done || s.status) break;
+            if (seen == posted) {
+                if (queued >= maxiter) break;
+                if (int rc = queue()) return rc;
+            }
         }
     }
     if (int rc = launch_cg_finish(P, P->d_x, P->d_r, cg_zform(P, precond))) return rc;

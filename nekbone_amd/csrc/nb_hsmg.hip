@@ -35,6 +35,7 @@ struct FdmArgs {
     const int16_t *eop;
     double *eown, *ghost;
     int nelt;
+    const CgScalars *guard;  // may be null: return at once when guard->done
 };
 
 // out(i,jk) = sum_m M(i,m) in(m,jk)       (first index)
@@ -160,6 +161,7 @@ template <typename T, int NH>
 __global__ void __launch_bounds__(FdmShape<NH>::NT)
     k_fdm(FdmArgs a, const T *__restrict__ ops, const T *__restrict__ epsv)
 {
+    if (a.guard && a.guard->done) return;
     using S = FdmShape<NH>;
     constexpr int NL = S::NL, L2 = S::L2, EPB = S::EPB, OPS = S::OPS;
     constexpr int P1 = S::P1, P2 = S::P2, L3 = S::LB;
@@ -358,6 +360,7 @@ struct SwzArgs {
     int n;
     int b2, b4, b8, br;  // workgroups of each group class
     GsArgs G;
+    const CgScalars *guard;  // may be null: return at once when guard->done
 };
 
 constexpr int kSwzPts = 4;  // points per lane in k_swz_ovl
@@ -395,6 +398,7 @@ __device__ __forceinline__ void swz_slots(int m, int (&slot)[NH == 3 ? 6 : 3])
 template <int NH, bool FINE>
 __global__ void __launch_bounds__(kRedThreads) k_swz_ovl(SwzArgs a)
 {
+    if (a.guard && a.guard->done) return;
     constexpr int NC = NH == 3 ? 6 : 3;
     const int base = blockIdx.x * (kRedThreads * kSwzPts) + threadIdx.x;
     int slot[kSwzPts][NC], gp[kSwzPts][NC];
@@ -473,6 +477,7 @@ __device__ __forceinline__ void swz_group(const SwzArgs &a, const int (&m)[S])
 // one group per lane, on the v that k_swz_ovl left at the shared points.
 __global__ void __launch_bounds__(kRedThreads) k_swz_gs(SwzArgs a)
 {
+    if (a.guard && a.guard->done) return;
     const GsArgs &G = a.G;
     int blk = blockIdx.x;
     const int64_t g0 = int64_t(threadIdx.x);
@@ -529,8 +534,10 @@ template <bool MASK>
 __global__ void __launch_bounds__(kT)
     k_rstr(double *__restrict__ rc, const double *__restrict__ rf, const double *__restrict__ cf,
            const int32_t *__restrict__ posf, const double *__restrict__ jh,
-           const double *__restrict__ jht, const uint8_t *__restrict__ bndc, int nc, int nf)
+           const double *__restrict__ jht, const uint8_t *__restrict__ bndc, int nc, int nf,
+           const CgScalars *__restrict__ guard)
 {
+    if (guard && guard->done) return;
     extern __shared__ double sm[];
     const int f3 = nf * nf * nf, c3 = nc * nc * nc;
     double *U = sm, *W1 = sm + f3, *W2 = W1 + nc * nf * nf, *J = W2 + nc * nc * nf, *Jt = J + nc * nf;
@@ -563,8 +570,10 @@ template <bool MASK>
 __global__ void __launch_bounds__(kT)
     k_intp(double *__restrict__ uf, const int32_t *__restrict__ posf,
            const double *__restrict__ uc, const uint8_t *__restrict__ bndc,
-           const double *__restrict__ jh, const double *__restrict__ jht, int nc, int nf)
+           const double *__restrict__ jh, const double *__restrict__ jht, int nc, int nf,
+           const CgScalars *__restrict__ guard)
 {
+    if (guard && guard->done) return;
     extern __shared__ double sm[];
     const int f3 = nf * nf * nf, c3 = nc * nc * nc;
     double *U = sm, *W1 = sm + f3, *W2 = W1 + nf * nc * nc, *J = W2 + nf * nf * nc, *Jt = J + nc * nf;
@@ -593,9 +602,10 @@ __global__ void __launch_bounds__(kT)
 int grid(int64_t n) { return int((n + kT - 1) / kT); }
 
 // One Schwarz level: out = W^1/2 mask(QQ^T(ovl(fdm(mask(r))))); r masked in place.
-int schwarz(nekbone_problem *P, HsmgLevel &L, double *out, double *r)
+int schwarz(nekbone_problem *P, HsmgLevel &L, double *out, double *r, const CgScalars *guard)
 {
-    const FdmArgs a{r, L.d_bnd, L.d_pos, L.d_gsrc, L.d_gin, L.d_eop, L.d_eown, L.d_ghost, P->nelt};
+    const FdmArgs a{r,       L.d_bnd,  L.d_pos,   L.d_gsrc, L.d_gin,
+                    L.d_eop, L.d_eown, L.d_ghost, P->nelt,  guard};
     if (int rc = P->hsmg_fp32 ? launch_fdm<float>(a, L.d_ops32, L.d_eps32, L.nh)
                               : launch_fdm<double>(a, L.d_ops, L.d_eps, L.nh))
         return rc;
@@ -609,6 +619,7 @@ int schwarz(nekbone_problem *P, HsmgLevel &L, double *out, double *r)
     s.gpart = L.d_gpart;
     s.bnd = L.d_bnd;
     s.n = int(L.n);
+    s.guard = guard;
     if (L.fine) {
         // the fine level is the problem's own numbering (device order)
         s.G = gs_args(P);
@@ -643,7 +654,8 @@ int schwarz(nekbone_problem *P, HsmgLevel &L, double *out, double *r)
     }
 }
 
-int rstr(nekbone_problem *P, HsmgLevel &C, HsmgLevel &F, double *rc, const double *rf, bool mask)
+int rstr(nekbone_problem *P, HsmgLevel &C, HsmgLevel &F, double *rc, const double *rf, bool mask,
+         const CgScalars *guard)
 {
     const int nc = C.nh, nf = F.nh;
     const size_t lds =
@@ -654,19 +666,20 @@ int rstr(nekbone_problem *P, HsmgLevel &C, HsmgLevel &F, double *rc, const doubl
                            static_cast<const int32_t *>(F.d_pos),
                            static_cast<const double *>(C.d_jh),
                            static_cast<const double *>(C.d_jht),
-                           static_cast<const uint8_t *>(C.d_bnd), nc, nf);
+                           static_cast<const uint8_t *>(C.d_bnd), nc, nf, guard);
     else
         hipLaunchKernelGGL(k_rstr<false>, dim3(P->nelt), dim3(kTE), lds, stream(), rc, rf,
                            static_cast<const double *>(F.d_c),
                            static_cast<const int32_t *>(F.d_pos),
                            static_cast<const double *>(C.d_jh),
                            static_cast<const double *>(C.d_jht),
-                           static_cast<const uint8_t *>(C.d_bnd), nc, nf);
+                           static_cast<const uint8_t *>(C.d_bnd), nc, nf, guard);
     NB_HIP(hipGetLastError());
     return NEKBONE_OK;
 }
 
-int intp(nekbone_problem *P, HsmgLevel &C, HsmgLevel &F, double *uf, const double *uc, bool mask)
+int intp(nekbone_problem *P, HsmgLevel &C, HsmgLevel &F, double *uf, const double *uc, bool mask,
+         const CgScalars *guard)
 {
     const int nc = C.nh, nf = F.nh;
     const size_t lds =
@@ -676,13 +689,13 @@ int intp(nekbone_problem *P, HsmgLevel &C, HsmgLevel &F, double *uf, const doubl
                            static_cast<const int32_t *>(F.d_pos), uc,
                            static_cast<const uint8_t *>(C.d_bnd),
                            static_cast<const double *>(C.d_jh),
-                           static_cast<const double *>(C.d_jht), nc, nf);
+                           static_cast<const double *>(C.d_jht), nc, nf, guard);
     else
         hipLaunchKernelGGL(k_intp<false>, dim3(P->nelt), dim3(kTE), lds, stream(), uf,
                            static_cast<const int32_t *>(F.d_pos), uc,
                            static_cast<const uint8_t *>(C.d_bnd),
                            static_cast<const double *>(C.d_jh),
-                           static_cast<const double *>(C.d_jht), nc, nf);
+                           static_cast<const double *>(C.d_jht), nc, nf, guard);
     NB_HIP(hipGetLastError());
     return NEKBONE_OK;
 }
@@ -691,30 +704,31 @@ int intp(nekbone_problem *P, HsmgLevel &C, HsmgLevel &F, double *uf, const doubl
 
 // h1mg_solve (src/hsmg.f:960-1034): z = M^-1 r on the finest level
 // (device point order); r is masked in place.
-int launch_hsmg_apply(nekbone_problem *P, double *z, double *r)
+int launch_hsmg_apply(nekbone_problem *P, double *z, double *r, const CgScalars *guard)
 {
     Hsmg *M = P->mg;
     if (!M) return fail(NEKBONE_ERR_STATE, "hsmg: not set up");
     const int lmax = M->lmax;
     HsmgLevel *L = M->L;
     if (int rc = timed_begin(P, 2)) return rc;
-    if (int rc = schwarz(P, L[lmax], z, r)) return rc;
+    if (int rc = schwarz(P, L[lmax], z, r, guard)) return rc;
     const double *rcur = r;  // r := rhs, masked by the Schwarz step
     for (int l = lmax - 1; l >= 2; --l) {
-        if (int rc = rstr(P, L[l], L[l + 1], L[l].d_r, rcur, false)) return rc;
-        if (int rc = launch_gs_map(L[l].gs, L[l].d_r)) return rc;
-        if (int rc = schwarz(P, L[l], L[l].d_e, L[l].d_r)) return rc;
+        if (int rc = rstr(P, L[l], L[l + 1], L[l].d_r, rcur, false, guard)) return rc;
+        if (int rc = launch_gs_map(L[l].gs, L[l].d_r, guard)) return rc;
+        if (int rc = schwarz(P, L[l], L[l].d_e, L[l].d_r, guard)) return rc;
         rcur = L[l].d_r;
     }
     // coarse level: e1 = mask(QQ^T mask(J^T r2)) (hsmg_coarse_solve, :945-957),
     // or with the direct coarse solve e1 = Q A_c^-1 Q^T mask(J^T r2)
-    if (int rc = rstr(P, L[1], L[2], L[1].d_e, rcur, true)) return rc;
-    if (int rc = launch_gs_map(L[1].gs, L[1].d_e)) return rc;
+    if (int rc = rstr(P, L[1], L[2], L[1].d_e, rcur, true, guard)) return rc;
+    if (int rc = launch_gs_map(L[1].gs, L[1].d_e, guard)) return rc;
     if (P->hsmg_coarse)
-        if (int rc = hsmg_coarse_apply(P, L[1].d_e)) return rc;
+        if (int rc = hsmg_coarse_apply(P, L[1].d_e, guard)) return rc;
     for (int l = 2; l <= lmax - 1; ++l)
-        if (int rc = intp(P, L[l - 1], L[l], L[l].d_e, L[l - 1].d_e, l - 1 == 1)) return rc;
-    if (int rc = intp(P, L[lmax - 1], L[lmax], z, L[lmax - 1].d_e, lmax - 1 == 1)) return rc;
+        if (int rc = intp(P, L[l - 1], L[l], L[l].d_e, L[l - 1].d_e, l - 1 == 1, guard)) return rc;
+    if (int rc = intp(P, L[lmax - 1], L[lmax], z, L[lmax - 1].d_e, lmax - 1 == 1, guard))
+        return rc;
     return timed_end(P, 2);
 }
 

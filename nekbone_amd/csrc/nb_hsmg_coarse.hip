@@ -77,8 +77,10 @@ __global__ void k_mirror_lower(double *__restrict__ a, int n)
 
 // r_g[j] = e[prim[j]]: the summed residual of each unknown.
 __global__ void k_coarse_gather(double *__restrict__ rg, const double *__restrict__ e,
-                                const int32_t *__restrict__ prim, int nc)
+                                const int32_t *__restrict__ prim, int nc,
+                                const CgScalars *__restrict__ guard)
 {
+    if (guard && guard->done) return;
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j < nc) rg[j] = e[prim[j]];
 }
@@ -90,8 +92,9 @@ __global__ void k_coarse_gather(double *__restrict__ rg, const double *__restric
 constexpr int kGemvThreads = 256;
 __global__ void __launch_bounds__(kGemvThreads)
     k_coarse_gemv(double *__restrict__ xg, const double *__restrict__ ainv,
-                  const double *__restrict__ rg, int nc)
+                  const double *__restrict__ rg, int nc, const CgScalars *__restrict__ guard)
 {
+    if (guard && guard->done) return;
     const int lane = threadIdx.x & 63;
     const int row = blockIdx.x * (kGemvThreads / 64) + (threadIdx.x >> 6);
     if (row >= nc) return;
@@ -109,8 +112,10 @@ __global__ void __launch_bounds__(kGemvThreads)
 }
 
 __global__ void k_coarse_scatter(double *__restrict__ e, const double *__restrict__ xg,
-                                 const int32_t *__restrict__ cidx, int64_t n)
+                                 const int32_t *__restrict__ cidx, int64_t n,
+                                 const CgScalars *__restrict__ guard)
 {
+    if (guard && guard->done) return;
     const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (q >= n) return;
     const int32_t j = cidx[q];
@@ -263,23 +268,24 @@ int hsmg_coarse_build(nekbone_problem *P)
 }
 
 // e <- Q A_c^-1 Q^T e on the order-1 level (e already summed by the
-// level's dssum: every copy holds Q^T e of its vertex).
-int hsmg_coarse_apply(nekbone_problem *P, double *e)
+// level's dssum: every copy holds Q^T e of its vertex).  guard (may be
+// null): the kernels return at once once the solve has converged.
+int hsmg_coarse_apply(nekbone_problem *P, double *e, const CgScalars *guard)
 {
     Hsmg *M = P->mg;
     const int nc = M->nc;
     const int rows = kGemvThreads / 64;
     hipLaunchKernelGGL(k_coarse_gather, dim3(unsigned((nc + 255) / 256)), dim3(256), 0, stream(),
                        M->d_rc, static_cast<const double *>(e),
-                       static_cast<const int32_t *>(M->d_cprim), nc);
+                       static_cast<const int32_t *>(M->d_cprim), nc, guard);
     hipLaunchKernelGGL(k_coarse_gemv, dim3(unsigned((nc + rows - 1) / rows)), dim3(kGemvThreads),
                        0, stream(), M->d_xc, static_cast<const double *>(M->d_ainv),
-                       static_cast<const double *>(M->d_rc), nc);
+                       static_cast<const double *>(M->d_rc), nc, guard);
     NB_HIP(hipGetLastError());
     const int64_t n1 = M->L[1].n;
     hipLaunchKernelGGL(k_coarse_scatter, dim3(unsigned((n1 + 255) / 256)), dim3(256), 0, stream(),
                        e, static_cast<const double *>(M->d_xc),
-                       static_cast<const int32_t *>(M->d_cidx), n1);
+                       static_cast<const int32_t *>(M->d_cidx), n1, guard);
     NB_HIP(hipGetLastError());
     return NEKBONE_OK;
 }

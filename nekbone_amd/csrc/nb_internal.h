@@ -630,7 +630,8 @@ bool cg_fold_partials(const nekbone_problem *P);
 int cg_gs_blocks(const nekbone_problem *P);
 void free_mixed(nekbone_problem *P);
 // gather-scatter with an explicit map (HSMG levels)
-int launch_gs_map(const GsMap &M, double *u);
+// guard (may be null): the kernel returns at once when guard->done is set
+int launch_gs_map(const GsMap &M, double *u, const CgScalars *guard = nullptr);
 int gs_map_upload(GsMap &G, const std::vector<int32_t> &off, const std::vector<int32_t> &idx,
                   const int32_t *pos, int n3);
 void gs_map_free(GsMap &G);
@@ -644,8 +645,11 @@ int hsmg_build(nekbone_problem *P);
 void hsmg_free(nekbone_problem *P);
 int hsmg_coarse_build(nekbone_problem *P);
 void hsmg_coarse_free(Hsmg *M);
-int hsmg_coarse_apply(nekbone_problem *P, double *e);
-int launch_hsmg_apply(nekbone_problem *P, double *z, double *r);
+int hsmg_coarse_apply(nekbone_problem *P, double *e, const CgScalars *guard = nullptr);
+// guard (may be null): every kernel of the apply returns at once when
+// guard->done is set (iterations queued past a CG's convergence)
+int launch_hsmg_apply(nekbone_problem *P, double *z, double *r,
+                      const CgScalars *guard = nullptr);
 // One CG iteration with an externally computed z = M^-1 r (r as the
 // preconditioner left it): rtz, beta, p, w = A p, pap, alpha, x, r, rtr.
 int launch_cg_iteration_z(nekbone_problem *P, double *x, double *r, const double *z, int mbit);

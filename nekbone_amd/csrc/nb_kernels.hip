@@ -493,15 +493,14 @@ int launch_gs_cg_batch(const GsCgArgs *d_args, const int *d_start, int nprob, in
     return NEKBONE_OK;
 }
 
-int launch_gs_map(const GsMap &M, double *u)
+int launch_gs_map(const GsMap &M, double *u, const CgScalars *guard)
 {
     const GsArgs G = gs_args_map(M);
     const int nb = G.b2 + G.b4 + G.b8 + G.br;
     if (nb == 0) return NEKBONE_OK;
     hipLaunchKernelGGL((k_gs_cls<double, false>), dim3(nb), dim3(kRedThreads), 0, stream(), u,
                        static_cast<const double *>(nullptr), static_cast<const uint8_t *>(nullptr),
-                       0, G, static_cast<double *>(nullptr),
-                       static_cast<const CgScalars *>(nullptr));
+                       0, G, static_cast<double *>(nullptr), guard);
     NB_HIP(hipGetLastError());
     return NEKBONE_OK;
 }
