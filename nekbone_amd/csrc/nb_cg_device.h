@@ -192,8 +192,8 @@ __device__ __forceinline__ void gx_rows(int32_t *tabs, const GxView &G, int64_t 
 
 // Fixed-order sum of a strided slice with sixteen interleaved accumulators
 // (sixteen loads in flight per thread: the single-block final kernels sum
-// up to ~60k partials, and their time is the memory latency of the
-// rounds of loads), combined pairwise in a fixed tree.
+// up to ~60k partials, and their time is the memory latency of the rounds
+// of loads), combined pairwise in a fixed tree.
 template <int NT>
 __device__ __forceinline__ double slice_sum(const double *__restrict__ a, int64_t na)
 {
@@ -214,14 +214,61 @@ __device__ __forceinline__ double slice_sum(const double *__restrict__ a, int64_
     return s[0];
 }
 
-// Sum of a partials region in a fixed order by one block.
+// The same for two slices of equal length, the loads of both in flight
+// together (sixteen per slice).
+template <int NT>
+__device__ __forceinline__ void slice_sum_pair(const double *__restrict__ a,
+                                               const double *__restrict__ b, int64_t n,
+                                               double &sa, double &sb)
+{
+    constexpr int U = 16;
+    double s[U], t[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) s[u] = t[u] = 0.0;
+    int64_t i = threadIdx.x;
+    for (; i + (U - 1) * NT < n; i += U * NT) {
+        double va[U], vb[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            va[u] = a[i + u * NT];
+            vb[u] = b[i + u * NT];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            s[u] = s[u] + va[u];
+            t[u] = t[u] + vb[u];
+        }
+    }
+    for (int u = 0; i < n; i += NT, ++u) {
+        s[u] = s[u] + a[i];
+        t[u] = t[u] + b[i];
+    }
+#pragma unroll
+    for (int h = U / 2; h > 0; h >>= 1)
+#pragma unroll
+        for (int u = 0; u < h; ++u) {
+            s[u] = s[u] + s[u + h];
+            t[u] = t[u] + t[u + h];
+        }
+    sa = s[0];
+    sb = t[0];
+}
+
+// Sum of a partials region in a fixed order by one block (two regions of
+// equal length: their loads interleaved).  (Thirty-two loads in flight for
+// one region ran the single-block kernel at 22 instead of 9 us.)
 template <int NT>
 __device__ __forceinline__ void sum_region2(const double *__restrict__ a, int64_t na,
                                             const double *__restrict__ b, int64_t nb, double &s0,
                                             double &s1)
 {
-    double a0 = slice_sum<NT>(a, na);
-    double a1 = b ? slice_sum<NT>(b, nb) : 0.0;
+    double a0, a1;
+    if (b && na == nb) {
+        slice_sum_pair<NT>(a, b, na, a0, a1);
+    } else {
+        a0 = slice_sum<NT>(a, na);
+        a1 = b ? slice_sum<NT>(b, nb) : 0.0;
+    }
     block_sum2<NT>(a0, a1);
     s0 = a0;
     s1 = a1;
