@@ -441,6 +441,34 @@ def test_cg_jacobi_history_full_size_vs_oracle(nb, oracle_c1):
     assert h[-1] < 1e-12 * h[0]
 
 
+@pytest.mark.parametrize("nx1", list(range(2, 17)))
+def test_cg_every_order_vs_oracle(nb, oracle, nx1):
+    """The fused Jacobi CG at every order, in the form and element-kernel
+    shape a solve picks by default (gather form with the x-ring on these
+    box meshes; the default shape of each order, factors loaded one t-step
+    ahead where that kernel is built): 23 iterations on a deformed mesh
+    (two ring flushes and a tail of three), history within 1e-12 of the
+    oracle's, x and r within 1e-12."""
+    _case("cg order %d" % nx1)
+    # enough unknowns that 23 iterations stay well above roundoff
+    nel = (8, 6, 5) if nx1 == 2 else (6, 5, 4) if nx1 <= 4 else (3, 2, 2) if nx1 <= 10 else (2, 2, 1)
+    P = nb.Problem(*nel, nx1)
+    O = oracle.Problem(*nel, nx1)
+    P.set_mask_mode(nb.MASK_DIRICHLET)
+    O.set_mask_mode(oracle.MASK_DIRICHLET)
+    P.set_geometry_deformed(0.1, 7)
+    O.deformed_g(0.1, 7)
+    P.setup_jacobi()
+    O.setup_jacobi()
+    f = O.rhs_from_raw(splitmix_uniform(O.n, nx1))
+    xo, ro, ho = O.cg(f, 23, oracle.PRECOND_JACOBI)
+    x, r, h = P.cg(f, 23, nb.PRECOND_JACOBI)
+    assert P.cg_form() == (True, True)
+    np.testing.assert_allclose(h, ho, rtol=HIST_RTOL, atol=0)
+    np.testing.assert_allclose(x, xo, rtol=0, atol=1e-12 * np.abs(xo).max())
+    np.testing.assert_allclose(r, ro, rtol=0, atol=1e-12 * np.abs(f).max())
+
+
 def test_cg_c0_jacobi_history_vs_oracle(nb, oracle):
     """The CPU-runnable configuration: 8x8x8 elements at N=7, brick,
     Dirichlet mask, Jacobi, RHS c * dssum(U[-1,1) seed 0), a fixed 100
