@@ -362,3 +362,29 @@ def test_cg_mixed_device_driven_refreshes(nb, oracle, inner, precond, nel, nx1):
     assert abs(rtr - float(np.sum(rtrue * c * rtrue))) <= 1e-6 * rtr
     with pytest.raises(nb.NotConverged):
         P.cg_mixed(f, 1e-30, 21, precond)
+
+
+@pytest.mark.parametrize("inner", ["f32", "g16"])
+@pytest.mark.parametrize("nx1", list(range(2, 17)))
+def test_cg_mixed_every_order(nb, oracle, inner, nx1):
+    """The mixed-precision CG at every order, FP32 and FP16 factors, in
+    the gather form with device-driven refreshes (the default on these
+    box meshes), Jacobi, deformed: the returned x meets the FP64 criterion
+    with r recomputed by the oracle, before maxiter, and the reported final
+    rtr is that residual's."""
+    _case("cg_mixed order %d %s" % (nx1, inner))
+    tol = 1e-10
+    nel = (6, 5, 4) if nx1 <= 4 else (3, 2, 2) if nx1 <= 10 else (2, 2, 1)
+    P, O = _pair(nb, oracle, nel, nx1, True, gseed=9, jacobi=True)
+    if inner == "g16":
+        P.set_mixed_inner(nb.MIXED_G16)
+    f = O.rhs_from_raw(splitmix_uniform(O.n, 100 + nx1))
+    x, r, it, nupd, rtr = P.cg_mixed(f, tol, 3000, 1)
+    assert nupd >= 1 and 0 < it < 3000
+    c = O.c()
+    f0 = O.maskit(f)
+    rtrue = f0 - O.ax(x)
+    rr = float(np.sum(rtrue * c * rtrue))
+    rr0 = float(np.sum(f0 * c * f0))
+    assert rr <= tol ** 2 * rr0 * 1.0001
+    assert abs(rtr - rr) <= 1e-6 * rr
