@@ -540,7 +540,7 @@ static int cg_step(nekbone_problem *P, double *x, double *r, int precond, int mb
     if (precond == NEKBONE_PRECOND_HSMG) {
         // applies queued past convergence return at once
         if (int rc = launch_hsmg_apply(P, P->mg->d_z, r, P->d_scal)) return rc;
-        return launch_cg_iteration_z(P, x, r, P->mg->d_z, mbit);
+        return launch_cg_iteration_z(P, x, r, P->mg->d_z, mbit, k);
     }
     return launch_cg_iteration(P, x, r, precond, mbit, k);
 }

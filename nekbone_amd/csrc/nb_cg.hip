@@ -538,8 +538,7 @@ int launch_cg_init(nekbone_problem *P, double *x, double *r, const double *f, in
     // every solve decides afresh whether its right-hand side allows the
     // gather form, and with it the x-ring
     if (int rc = cg_gather_check(P, f)) return rc;
-    P->ring_on =
-        fp64_iter && cg_gather(P) && precond != NEKBONE_PRECOND_HSMG && ensure_ring(P);
+    P->ring_on = fp64_iter && cg_gather(P) && ensure_ring(P);
     const int nb = reduce_blocks(P->n);
     if (precond == NEKBONE_PRECOND_JACOBI && zform) {
         hipLaunchKernelGGL(k_cg_init<kPreJacobiZ>, dim3(nb), dim3(kRedThreads), 0, stream(), x,
@@ -589,9 +588,10 @@ int launch_cg_init(nekbone_problem *P, double *x, double *r, const double *f, in
 // With the x-ring (P->ring_on) the element kernel takes the previous
 // direction from ring slot (k-1) % kRing and stores the new one in slot
 // k % kRing, without touching x; the update kernel of every kRing-th
-// iteration folds the ring's alpha p into x.
+// iteration folds the ring's alpha p into x.  zin (HSMG): the element
+// kernel reads the preconditioner's z, the update takes r itself.
 static int iteration_gather(nekbone_problem *P, double *x, double *r, int precond, int mbit,
-                            int k)
+                            int k, const double *zin = nullptr)
 {
     const bool jac = precond == NEKBONE_PRECOND_JACOBI;
     const bool fma = P->arith == NEKBONE_ARITH_FMA;
@@ -604,7 +604,8 @@ static int iteration_gather(nekbone_problem *P, double *x, double *r, int precon
     double *pcur = ring ? P->d_pring + int64_t(k % kRing) * n : P->d_p;
     double *pprev = ring ? P->d_pring + int64_t((k + kRing - 1) % kRing) * n : P->d_p;
     const bool xf = ring && k % kRing == kRing - 1;
-    if (int rc = launch_ax_cg(P, x, r, mbit, 0, 1, ring ? pprev : nullptr, ring ? pcur : nullptr))
+    if (int rc = launch_ax_cg(P, x, zin ? zin : r, mbit, 0, 1, ring ? pprev : nullptr,
+                              ring ? pcur : nullptr))
         return rc;
     if (int rc = timed_begin(P, 4)) return rc;
     hipLaunchKernelGGL(k_pap_final, dim3(1), dim3(kFinalThreads), 0, stream(), P->d_part, na,
@@ -661,7 +662,7 @@ static int iteration_gather(nekbone_problem *P, double *x, double *r, int precon
 static int iteration_fused(nekbone_problem *P, double *x, double *r, int precond, int mbit,
                            const double *zin = nullptr, int k = 0)
 {
-    if (!zin && cg_gather(P)) return iteration_gather(P, x, r, precond, mbit, k);
+    if (cg_gather(P)) return iteration_gather(P, x, r, precond, mbit, k, zin);
     const bool jac = precond == NEKBONE_PRECOND_JACOBI;
     const bool fma = P->arith == NEKBONE_ARITH_FMA;
     const int64_t na = n_ax(P), ng = n_gs(P);
@@ -810,7 +811,8 @@ __global__ void k_pupdate_z(double *__restrict__ p, const double *__restrict__ z
 
 }  // namespace
 
-int launch_cg_iteration_z(nekbone_problem *P, double *x, double *r, const double *z, int mbit)
+int launch_cg_iteration_z(nekbone_problem *P, double *x, double *r, const double *z, int mbit,
+                          int k)
 {
     const bool fma = P->arith == NEKBONE_ARITH_FMA;
     const int nb = reduce_blocks(P->n);
@@ -824,7 +826,7 @@ int launch_cg_iteration_z(nekbone_problem *P, double *x, double *r, const double
     // with c = 1/multiplicity the rest is the fused identity iteration with
     // the element kernel reading z (p = z + beta p fused into it)
     if (P->c_is_mult && P->fused)
-        return iteration_fused(P, x, r, NEKBONE_PRECOND_IDENTITY, mbit, z);
+        return iteration_fused(P, x, r, NEKBONE_PRECOND_IDENTITY, mbit, z, k);
     if (fma)
         hipLaunchKernelGGL(k_pupdate_z<true>, dim3(grid1d(P->n, 256)), dim3(256), 0, stream(),
                            P->d_p, z, P->n, static_cast<const CgScalars *>(P->d_scal));

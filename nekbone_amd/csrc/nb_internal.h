@@ -651,8 +651,10 @@ int hsmg_coarse_apply(nekbone_problem *P, double *e, const CgScalars *guard = nu
 int launch_hsmg_apply(nekbone_problem *P, double *z, double *r,
                       const CgScalars *guard = nullptr);
 // One CG iteration with an externally computed z = M^-1 r (r as the
-// preconditioner left it): rtz, beta, p, w = A p, pap, alpha, x, r, rtr.
-int launch_cg_iteration_z(nekbone_problem *P, double *x, double *r, const double *z, int mbit);
+// preconditioner left it): rtz, beta, p, w = A p, pap, alpha, x, r, rtr;
+// k: the iteration's index within the solve (its x-ring slot).
+int launch_cg_iteration_z(nekbone_problem *P, double *x, double *r, const double *z, int mbit,
+                          int k);
 int set_c_host(nekbone_problem *P, const double *c);
 int timed_begin(nekbone_problem *P, int kclass);
 int timed_end(nekbone_problem *P, int kclass);
