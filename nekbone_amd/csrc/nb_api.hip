@@ -538,9 +538,10 @@ static int check_precond(nekbone_problem *P, int precond)
 static int cg_step(nekbone_problem *P, double *x, double *r, int precond, int mbit, int k)
 {
     if (precond == NEKBONE_PRECOND_HSMG) {
-        // applies queued past convergence return at once
-        if (int rc = launch_hsmg_apply(P, P->mg->d_z, r, P->d_scal)) return rc;
-        return launch_cg_iteration_z(P, x, r, P->mg->d_z, mbit, k);
+        // applies queued past convergence return at once; the finest
+        // interpolation leaves the rtz parts, one per element
+        if (int rc = launch_hsmg_apply(P, P->mg->d_z, r, P->d_scal, P->d_part)) return rc;
+        return launch_cg_iteration_z(P, x, r, P->mg->d_z, mbit, k, P->nelt);
     }
     return launch_cg_iteration(P, x, r, precond, mbit, k);
 }

@@ -812,16 +812,17 @@ __global__ void k_pupdate_z(double *__restrict__ p, const double *__restrict__ z
 }  // namespace
 
 int launch_cg_iteration_z(nekbone_problem *P, double *x, double *r, const double *z, int mbit,
-                          int k)
+                          int k, int rtz_parts)
 {
     const bool fma = P->arith == NEKBONE_ARITH_FMA;
     const int nb = reduce_blocks(P->n);
-    hipLaunchKernelGGL(k_rtz, dim3(nb), dim3(kRedThreads), 0, stream(),
-                       static_cast<const double *>(r), static_cast<const double *>(P->d_c), z,
-                       P->n, P->d_part, static_cast<const CgScalars *>(P->d_scal));
+    if (rtz_parts <= 0)
+        hipLaunchKernelGGL(k_rtz, dim3(nb), dim3(kRedThreads), 0, stream(),
+                           static_cast<const double *>(r), static_cast<const double *>(P->d_c),
+                           z, P->n, P->d_part, static_cast<const CgScalars *>(P->d_scal));
     hipLaunchKernelGGL(k_rtz_final, dim3(1), dim3(kFinalThreads), 0, stream(),
-                       static_cast<const double *>(P->d_part), nb, P->d_scal, P->d_hist,
-                       P->hist_cap);
+                       static_cast<const double *>(P->d_part), rtz_parts > 0 ? rtz_parts : nb,
+                       P->d_scal, P->d_hist, P->hist_cap);
     NB_HIP(hipGetLastError());
     // with c = 1/multiplicity the rest is the fused identity iteration with
     // the element kernel reading z (p = z + beta p fused into it)

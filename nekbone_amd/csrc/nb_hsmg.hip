@@ -566,12 +566,16 @@ __global__ void __launch_bounds__(kT)
 // Interpolation-add of one element: uf += [J x J x J] uc
 // (h1mg_intp then the add of h1mg_solve); with MASK the coarse input is
 // masked first (the coarse solve's final mask, :1011).
-template <bool MASK>
+// RTZ (the finest level's interpolation, which completes z inside a CG):
+// also the element's part of rtz = glsc3(r, c, z) (src/cg.f:52) into
+// part[e], so no separate pass reads r, c and z again.
+template <bool MASK, bool RTZ = false>
 __global__ void __launch_bounds__(kT)
     k_intp(double *__restrict__ uf, const int32_t *__restrict__ posf,
            const double *__restrict__ uc, const uint8_t *__restrict__ bndc,
            const double *__restrict__ jh, const double *__restrict__ jht, int nc, int nf,
-           const CgScalars *__restrict__ guard)
+           const CgScalars *__restrict__ guard, const double *__restrict__ rr = nullptr,
+           const double *__restrict__ cc = nullptr, double *__restrict__ part = nullptr)
 {
     if (guard && guard->done) return;
     extern __shared__ double sm[];
@@ -593,9 +597,17 @@ __global__ void __launch_bounds__(kT)
     __syncthreads();
     tz(U, W2, Jt, nf * nf, nf, nc);           // Ct = jht
     __syncthreads();
+    double acc = 0.0;
     for (int p = threadIdx.x; p < f3; p += blockDim.x) {
         const int64_t m = e * f3 + posf[p];
-        uf[m] = uf[m] + U[p];
+        const double v = uf[m] + U[p];
+        uf[m] = v;
+        if constexpr (RTZ) acc = acc + rr[m] * cc[m] * v;
+    }
+    if constexpr (RTZ) {
+        double dummy = 0.0;
+        block_sum2<kTE>(acc, dummy);
+        if (threadIdx.x == 0) part[e] = acc;
     }
 }
 
@@ -678,24 +690,29 @@ int rstr(nekbone_problem *P, HsmgLevel &C, HsmgLevel &F, double *rc, const doubl
     return NEKBONE_OK;
 }
 
+// rtz_part (may be null): k_intp's RTZ form, the parts of glsc3(r, c, uf)
 int intp(nekbone_problem *P, HsmgLevel &C, HsmgLevel &F, double *uf, const double *uc, bool mask,
-         const CgScalars *guard)
+         const CgScalars *guard, double *rtz_part = nullptr, const double *r = nullptr)
 {
     const int nc = C.nh, nf = F.nh;
     const size_t lds =
         sizeof(double) * size_t(nf * nf * nf + nf * nc * nc + nf * nf * nc + 2 * nc * nf);
-    if (mask)
-        hipLaunchKernelGGL(k_intp<true>, dim3(P->nelt), dim3(kTE), lds, stream(), uf,
-                           static_cast<const int32_t *>(F.d_pos), uc,
-                           static_cast<const uint8_t *>(C.d_bnd),
-                           static_cast<const double *>(C.d_jh),
-                           static_cast<const double *>(C.d_jht), nc, nf, guard);
+#define NB_INTP(MASK, RTZ)                                                                  \
+    hipLaunchKernelGGL((k_intp<MASK, RTZ>), dim3(P->nelt), dim3(kTE), lds, stream(), uf,      \
+                       static_cast<const int32_t *>(F.d_pos), uc,                            \
+                       static_cast<const uint8_t *>(C.d_bnd),                                \
+                       static_cast<const double *>(C.d_jh),                                  \
+                       static_cast<const double *>(C.d_jht), nc, nf, guard, r,               \
+                       static_cast<const double *>(P->d_c), rtz_part)
+    if (mask && rtz_part)
+        NB_INTP(true, true);
+    else if (mask)
+        NB_INTP(true, false);
+    else if (rtz_part)
+        NB_INTP(false, true);
     else
-        hipLaunchKernelGGL(k_intp<false>, dim3(P->nelt), dim3(kTE), lds, stream(), uf,
-                           static_cast<const int32_t *>(F.d_pos), uc,
-                           static_cast<const uint8_t *>(C.d_bnd),
-                           static_cast<const double *>(C.d_jh),
-                           static_cast<const double *>(C.d_jht), nc, nf, guard);
+        NB_INTP(false, false);
+#undef NB_INTP
     NB_HIP(hipGetLastError());
     return NEKBONE_OK;
 }
@@ -704,7 +721,8 @@ int intp(nekbone_problem *P, HsmgLevel &C, HsmgLevel &F, double *uf, const doubl
 
 // h1mg_solve (src/hsmg.f:960-1034): z = M^-1 r on the finest level
 // (device point order); r is masked in place.
-int launch_hsmg_apply(nekbone_problem *P, double *z, double *r, const CgScalars *guard)
+int launch_hsmg_apply(nekbone_problem *P, double *z, double *r, const CgScalars *guard,
+                      double *rtz_part)
 {
     Hsmg *M = P->mg;
     if (!M) return fail(NEKBONE_ERR_STATE, "hsmg: not set up");
@@ -727,7 +745,8 @@ int launch_hsmg_apply(nekbone_problem *P, double *z, double *r, const CgScalars 
         if (int rc = hsmg_coarse_apply(P, L[1].d_e, guard)) return rc;
     for (int l = 2; l <= lmax - 1; ++l)
         if (int rc = intp(P, L[l - 1], L[l], L[l].d_e, L[l - 1].d_e, l - 1 == 1, guard)) return rc;
-    if (int rc = intp(P, L[lmax - 1], L[lmax], z, L[lmax - 1].d_e, lmax - 1 == 1, guard))
+    if (int rc = intp(P, L[lmax - 1], L[lmax], z, L[lmax - 1].d_e, lmax - 1 == 1, guard,
+                      rtz_part, r))
         return rc;
     return timed_end(P, 2);
 }

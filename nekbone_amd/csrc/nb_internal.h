@@ -647,14 +647,18 @@ int hsmg_coarse_build(nekbone_problem *P);
 void hsmg_coarse_free(Hsmg *M);
 int hsmg_coarse_apply(nekbone_problem *P, double *e, const CgScalars *guard = nullptr);
 // guard (may be null): every kernel of the apply returns at once when
-// guard->done is set (iterations queued past a CG's convergence)
+// guard->done is set (iterations queued past a CG's convergence);
+// rtz_part (may be null): the finest interpolation also writes the
+// per-element parts of glsc3(r, c, z), nelt of them
 int launch_hsmg_apply(nekbone_problem *P, double *z, double *r,
-                      const CgScalars *guard = nullptr);
+                      const CgScalars *guard = nullptr, double *rtz_part = nullptr);
 // One CG iteration with an externally computed z = M^-1 r (r as the
 // preconditioner left it): rtz, beta, p, w = A p, pap, alpha, x, r, rtr;
-// k: the iteration's index within the solve (its x-ring slot).
+// k: the iteration's index within the solve (its x-ring slot);
+// rtz_parts > 0: the rtz parts are already in P->d_part (that many, from
+// the preconditioner's last kernel), else they are formed here.
 int launch_cg_iteration_z(nekbone_problem *P, double *x, double *r, const double *z, int mbit,
-                          int k);
+                          int k, int rtz_parts = 0);
 int set_c_host(nekbone_problem *P, const double *c);
 int timed_begin(nekbone_problem *P, int kclass);
 int timed_end(nekbone_problem *P, int kclass);
