@@ -19,8 +19,8 @@
 // lower triangle is mirrored.  Apply, replacing the coarse step of
 // h1mg_solve: after the level's dssum every copy of a vertex holds the
 // summed residual, so
-//   r_g[j] = e[prim[j]],  x_g = A_c^-1 r_g  (one wave per row, rows of the
-//   symmetric inverse read as contiguous columns; the inverse, 91 MB for
+//   r_g[j] = e[prim[j]],  x_g = A_c^-1 r_g  (one workgroup per row, rows of
+//   the symmetric inverse read as contiguous columns; the inverse, 91 MB for
 //   the 15^3 interior vertices of a 16^3 mesh, is read once per apply),
 //   e[q] = x_g[cidx[q]] or 0 on the mask.
 // Every sum has a fixed order, so results are reproducible run to run.
@@ -85,30 +85,36 @@ __global__ void k_coarse_gather(double *__restrict__ rg, const double *__restric
     if (j < nc) rg[j] = e[prim[j]];
 }
 
-// x_g = A^-1 r_g, one wave per row (a contiguous column of the symmetric
-// inverse); four interleaved partial sums per lane keep four loads in
-// flight, combined as (s0 + s1) + (s2 + s3), then the wave's fixed
-// butterfly.
+// x_g = A^-1 r_g, one 256-thread workgroup per row (a contiguous column
+// of the symmetric inverse): every lane's loads of the row (up to
+// kGemvU) in flight together, its partial sums added in a fixed order,
+// then the workgroup's fixed tree.  (One wave per row with four or sixteen
+// loads in flight per lane left the ~3 waves per SIMD that 3,375 rows
+// give latency-bound at 3.7 TB/s or below; this form reaches 4.0 TB/s,
+// 23 us at 16^3 N=7.)
 constexpr int kGemvThreads = 256;
+constexpr int kGemvU = 16;  // row length up to kGemvThreads * kGemvU per pass
 __global__ void __launch_bounds__(kGemvThreads)
     k_coarse_gemv(double *__restrict__ xg, const double *__restrict__ ainv,
                   const double *__restrict__ rg, int nc, const CgScalars *__restrict__ guard)
 {
     if (guard && guard->done) return;
-    const int lane = threadIdx.x & 63;
-    const int row = blockIdx.x * (kGemvThreads / 64) + (threadIdx.x >> 6);
-    if (row >= nc) return;
+    const int row = blockIdx.x;
     const double *col = ainv + int64_t(row) * nc;
-    double s[4] = {0.0, 0.0, 0.0, 0.0};
-    int i = lane;
-    for (; i + 192 < nc; i += 256) {
+    double t = 0.0;
+    for (int i0 = threadIdx.x; i0 < nc; i0 += kGemvThreads * kGemvU) {
+        double v[kGemvU];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) s[u] = s[u] + col[i + 64 * u] * rg[i + 64 * u];
+        for (int u = 0; u < kGemvU; ++u) {
+            const int i = i0 + kGemvThreads * u;
+            v[u] = i < nc ? col[i] * rg[i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kGemvU; ++u) t = t + v[u];
     }
-    for (int u = 0; i < nc; i += 64, ++u) s[u] = s[u] + col[i] * rg[i];
-    double t = (s[0] + s[1]) + (s[2] + s[3]);
-    t = wave_sum(t);
-    if (lane == 0) xg[row] = t;
+    double dummy = 0.0;
+    block_sum2<kGemvThreads>(t, dummy);
+    if (threadIdx.x == 0) xg[row] = t;
 }
 
 __global__ void k_coarse_scatter(double *__restrict__ e, const double *__restrict__ xg,
@@ -274,11 +280,10 @@ int hsmg_coarse_apply(nekbone_problem *P, double *e, const CgScalars *guard)
 {
     Hsmg *M = P->mg;
     const int nc = M->nc;
-    const int rows = kGemvThreads / 64;
     hipLaunchKernelGGL(k_coarse_gather, dim3(unsigned((nc + 255) / 256)), dim3(256), 0, stream(),
                        M->d_rc, static_cast<const double *>(e),
                        static_cast<const int32_t *>(M->d_cprim), nc, guard);
-    hipLaunchKernelGGL(k_coarse_gemv, dim3(unsigned((nc + rows - 1) / rows)), dim3(kGemvThreads),
+    hipLaunchKernelGGL(k_coarse_gemv, dim3(unsigned(nc)), dim3(kGemvThreads),
                        0, stream(), M->d_xc, static_cast<const double *>(M->d_ainv),
                        static_cast<const double *>(M->d_rc), nc, guard);
     NB_HIP(hipGetLastError());
