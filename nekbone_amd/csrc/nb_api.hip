@@ -25,6 +25,11 @@ int fail(int code, const std::string &msg)
 namespace {
 int g_device = -1;
 hipStream_t g_stream = nullptr;
+// second stream for work that runs beside the library stream's (created
+// and ordered like it: non-blocking, default priority), and the events
+// that fork work to it and join it back
+hipStream_t g_side = nullptr;
+hipEvent_t g_fork = nullptr, g_join = nullptr;
 
 __global__ void k_fill(double *a, double v, int64_t n)
 {
@@ -63,6 +68,21 @@ AxVariant tuned_or_default(int nx1)
 }  // namespace
 
 hipStream_t stream() { return g_stream; }
+hipStream_t side_stream() { return g_side; }
+
+int fork_side()
+{
+    NB_HIP(hipEventRecord(g_fork, g_stream));
+    NB_HIP(hipStreamWaitEvent(g_side, g_fork, 0));
+    return NEKBONE_OK;
+}
+
+int join_side()
+{
+    NB_HIP(hipEventRecord(g_join, g_side));
+    NB_HIP(hipStreamWaitEvent(g_stream, g_join, 0));
+    return NEKBONE_OK;
+}
 
 namespace {
 CgScalars *g_hscal = nullptr;  // pinned: two slots of posted scalars
@@ -638,7 +658,13 @@ extern "C" int nekbone_init(int device)
     NB_HIP(hipSetDevice(device));
     if (int rc = noalloc_setup(device)) return rc;
     if (g_stream) NB_HIP(hipStreamDestroy(g_stream));
+    if (g_side) NB_HIP(hipStreamDestroy(g_side));
+    if (g_fork) NB_HIP(hipEventDestroy(g_fork));
+    if (g_join) NB_HIP(hipEventDestroy(g_join));
     NB_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    NB_HIP(hipStreamCreateWithFlags(&g_side, hipStreamNonBlocking));
+    NB_HIP(hipEventCreateWithFlags(&g_fork, hipEventDisableTiming));
+    NB_HIP(hipEventCreateWithFlags(&g_join, hipEventDisableTiming));
     g_device = device;
     return NEKBONE_OK;
 }
@@ -649,7 +675,14 @@ extern "C" int nekbone_finalize(void)
         NB_HIP(hipStreamSynchronize(g_stream));
         NB_HIP(hipStreamDestroy(g_stream));
     }
-    g_stream = nullptr;
+    if (g_side) {
+        NB_HIP(hipStreamSynchronize(g_side));
+        NB_HIP(hipStreamDestroy(g_side));
+    }
+    if (g_fork) NB_HIP(hipEventDestroy(g_fork));
+    if (g_join) NB_HIP(hipEventDestroy(g_join));
+    g_stream = g_side = nullptr;
+    g_fork = g_join = nullptr;
     return NEKBONE_OK;
 }
 

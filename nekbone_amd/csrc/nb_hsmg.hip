@@ -306,33 +306,33 @@ __global__ void __launch_bounds__(FdmShape<NH>::NT)
 }
 
 template <typename T, int NH>
-int launch_fdm_nh(const FdmArgs &a, const T *ops, const T *eps)
+int launch_fdm_nh(const FdmArgs &a, const T *ops, const T *eps, hipStream_t st)
 {
     using S = FdmShape<NH>;
-    hipLaunchKernelGGL((k_fdm<T, NH>), dim3((a.nelt + S::EPB - 1) / S::EPB), dim3(S::NT), 0,
-                       stream(), a, ops, eps);
+    hipLaunchKernelGGL((k_fdm<T, NH>), dim3((a.nelt + S::EPB - 1) / S::EPB), dim3(S::NT), 0, st,
+                       a, ops, eps);
     NB_HIP(hipGetLastError());
     return NEKBONE_OK;
 }
 
 template <typename T>
-int launch_fdm(const FdmArgs &a, const T *ops, const T *eps, int nh)
+int launch_fdm(const FdmArgs &a, const T *ops, const T *eps, int nh, hipStream_t st)
 {
     switch (nh) {
-    case 3: return launch_fdm_nh<T, 3>(a, ops, eps);
-    case 4: return launch_fdm_nh<T, 4>(a, ops, eps);
-    case 5: return launch_fdm_nh<T, 5>(a, ops, eps);
-    case 6: return launch_fdm_nh<T, 6>(a, ops, eps);
-    case 7: return launch_fdm_nh<T, 7>(a, ops, eps);
-    case 8: return launch_fdm_nh<T, 8>(a, ops, eps);
-    case 9: return launch_fdm_nh<T, 9>(a, ops, eps);
-    case 10: return launch_fdm_nh<T, 10>(a, ops, eps);
-    case 11: return launch_fdm_nh<T, 11>(a, ops, eps);
-    case 12: return launch_fdm_nh<T, 12>(a, ops, eps);
-    case 13: return launch_fdm_nh<T, 13>(a, ops, eps);
-    case 14: return launch_fdm_nh<T, 14>(a, ops, eps);
-    case 15: return launch_fdm_nh<T, 15>(a, ops, eps);
-    case 16: return launch_fdm_nh<T, 16>(a, ops, eps);
+    case 3: return launch_fdm_nh<T, 3>(a, ops, eps, st);
+    case 4: return launch_fdm_nh<T, 4>(a, ops, eps, st);
+    case 5: return launch_fdm_nh<T, 5>(a, ops, eps, st);
+    case 6: return launch_fdm_nh<T, 6>(a, ops, eps, st);
+    case 7: return launch_fdm_nh<T, 7>(a, ops, eps, st);
+    case 8: return launch_fdm_nh<T, 8>(a, ops, eps, st);
+    case 9: return launch_fdm_nh<T, 9>(a, ops, eps, st);
+    case 10: return launch_fdm_nh<T, 10>(a, ops, eps, st);
+    case 11: return launch_fdm_nh<T, 11>(a, ops, eps, st);
+    case 12: return launch_fdm_nh<T, 12>(a, ops, eps, st);
+    case 13: return launch_fdm_nh<T, 13>(a, ops, eps, st);
+    case 14: return launch_fdm_nh<T, 14>(a, ops, eps, st);
+    case 15: return launch_fdm_nh<T, 15>(a, ops, eps, st);
+    case 16: return launch_fdm_nh<T, 16>(a, ops, eps, st);
     default: return fail(NEKBONE_ERR_UNSUPPORTED, "hsmg: level size outside 3..16");
     }
 }
@@ -511,17 +511,17 @@ __global__ void __launch_bounds__(kRedThreads) k_swz_gs(SwzArgs a)
 }
 
 template <int NH>
-int launch_swz_nh(const SwzArgs &a, bool fine)
+int launch_swz_nh(const SwzArgs &a, bool fine, hipStream_t st)
 {
     const int nbo = int((int64_t(a.n) + kRedThreads * kSwzPts - 1) / (kRedThreads * kSwzPts));
     if (fine)
-        hipLaunchKernelGGL((k_swz_ovl<NH, true>), dim3(nbo), dim3(kRedThreads), 0, stream(), a);
+        hipLaunchKernelGGL((k_swz_ovl<NH, true>), dim3(nbo), dim3(kRedThreads), 0, st, a);
     else
-        hipLaunchKernelGGL((k_swz_ovl<NH, false>), dim3(nbo), dim3(kRedThreads), 0, stream(), a);
+        hipLaunchKernelGGL((k_swz_ovl<NH, false>), dim3(nbo), dim3(kRedThreads), 0, st, a);
     NB_HIP(hipGetLastError());
     const int nbg = a.b2 + a.b4 + a.b8 + a.br;
     if (nbg > 0) {
-        hipLaunchKernelGGL(k_swz_gs, dim3(nbg), dim3(kRedThreads), 0, stream(), a);
+        hipLaunchKernelGGL(k_swz_gs, dim3(nbg), dim3(kRedThreads), 0, st, a);
         NB_HIP(hipGetLastError());
     }
     return NEKBONE_OK;
@@ -529,13 +529,16 @@ int launch_swz_nh(const SwzArgs &a, bool fine)
 
 // Restriction of one element from the finer level F to the coarser C:
 // rc = [J^T x J^T x J^T] (c_F .* rf) (h1mg_do_wt then h1mg_tnsr1_3d);
-// with MASK the coarse result is written masked.
-template <bool MASK>
+// with MASK the coarse result is written masked; with MIN the fine input
+// is read masked by bndf (the finest level's mask, which that level's
+// Schwarz step applies to rf in place while this kernel runs beside it:
+// either value read gives the masked one).
+template <bool MASK, bool MIN = false>
 __global__ void __launch_bounds__(kT)
     k_rstr(double *__restrict__ rc, const double *__restrict__ rf, const double *__restrict__ cf,
            const int32_t *__restrict__ posf, const double *__restrict__ jh,
            const double *__restrict__ jht, const uint8_t *__restrict__ bndc, int nc, int nf,
-           const CgScalars *__restrict__ guard)
+           const CgScalars *__restrict__ guard, const uint8_t *__restrict__ bndf)
 {
     if (guard && guard->done) return;
     extern __shared__ double sm[];
@@ -544,7 +547,8 @@ __global__ void __launch_bounds__(kT)
     const int64_t e = blockIdx.x;
     for (int p = threadIdx.x; p < f3; p += blockDim.x) {
         const int64_t m = e * f3 + posf[p];
-        U[p] = rf[m] * cf[m];
+        const double v = MIN && bndf[m] ? 0.0 : rf[m];
+        U[p] = v * cf[m];
     }
     for (int p = threadIdx.x; p < nc * nf; p += blockDim.x) {
         J[p] = jh[p];
@@ -614,12 +618,13 @@ __global__ void __launch_bounds__(kT)
 int grid(int64_t n) { return int((n + kT - 1) / kT); }
 
 // One Schwarz level: out = W^1/2 mask(QQ^T(ovl(fdm(mask(r))))); r masked in place.
-int schwarz(nekbone_problem *P, HsmgLevel &L, double *out, double *r, const CgScalars *guard)
+int schwarz(nekbone_problem *P, HsmgLevel &L, double *out, double *r, const CgScalars *guard,
+            hipStream_t st)
 {
     const FdmArgs a{r,       L.d_bnd,  L.d_pos,   L.d_gsrc, L.d_gin,
                     L.d_eop, L.d_eown, L.d_ghost, P->nelt,  guard};
-    if (int rc = P->hsmg_fp32 ? launch_fdm<float>(a, L.d_ops32, L.d_eps32, L.nh)
-                              : launch_fdm<double>(a, L.d_ops, L.d_eps, L.nh))
+    if (int rc = P->hsmg_fp32 ? launch_fdm<float>(a, L.d_ops32, L.d_eps32, L.nh, st)
+                              : launch_fdm<double>(a, L.d_ops, L.d_eps, L.nh, st))
         return rc;
     if (int64_t(6) * L.nh * L.nh * P->nelt > INT32_MAX || L.n > INT32_MAX)
         return fail(NEKBONE_ERR_UNSUPPORTED, "hsmg: level above 2^31 points or ghost slots");
@@ -648,44 +653,47 @@ int schwarz(nekbone_problem *P, HsmgLevel &L, double *out, double *r, const CgSc
     s.b8 = nb(s.G.n8);
     s.br = nb(s.G.nr);
     switch (L.nh) {
-    case 3: return launch_swz_nh<3>(s, L.fine);
-    case 4: return launch_swz_nh<4>(s, L.fine);
-    case 5: return launch_swz_nh<5>(s, L.fine);
-    case 6: return launch_swz_nh<6>(s, L.fine);
-    case 7: return launch_swz_nh<7>(s, L.fine);
-    case 8: return launch_swz_nh<8>(s, L.fine);
-    case 9: return launch_swz_nh<9>(s, L.fine);
-    case 10: return launch_swz_nh<10>(s, L.fine);
-    case 11: return launch_swz_nh<11>(s, L.fine);
-    case 12: return launch_swz_nh<12>(s, L.fine);
-    case 13: return launch_swz_nh<13>(s, L.fine);
-    case 14: return launch_swz_nh<14>(s, L.fine);
-    case 15: return launch_swz_nh<15>(s, L.fine);
-    case 16: return launch_swz_nh<16>(s, L.fine);
+    case 3: return launch_swz_nh<3>(s, L.fine, st);
+    case 4: return launch_swz_nh<4>(s, L.fine, st);
+    case 5: return launch_swz_nh<5>(s, L.fine, st);
+    case 6: return launch_swz_nh<6>(s, L.fine, st);
+    case 7: return launch_swz_nh<7>(s, L.fine, st);
+    case 8: return launch_swz_nh<8>(s, L.fine, st);
+    case 9: return launch_swz_nh<9>(s, L.fine, st);
+    case 10: return launch_swz_nh<10>(s, L.fine, st);
+    case 11: return launch_swz_nh<11>(s, L.fine, st);
+    case 12: return launch_swz_nh<12>(s, L.fine, st);
+    case 13: return launch_swz_nh<13>(s, L.fine, st);
+    case 14: return launch_swz_nh<14>(s, L.fine, st);
+    case 15: return launch_swz_nh<15>(s, L.fine, st);
+    case 16: return launch_swz_nh<16>(s, L.fine, st);
     default: return fail(NEKBONE_ERR_UNSUPPORTED, "hsmg: level size outside 3..16");
     }
 }
 
 int rstr(nekbone_problem *P, HsmgLevel &C, HsmgLevel &F, double *rc, const double *rf, bool mask,
-         const CgScalars *guard)
+         const CgScalars *guard, bool maskin = false)
 {
     const int nc = C.nh, nf = F.nh;
     const size_t lds =
         sizeof(double) * size_t(nf * nf * nf + nc * nf * nf + nc * nc * nf + 2 * nc * nf);
-    if (mask)
-        hipLaunchKernelGGL(k_rstr<true>, dim3(P->nelt), dim3(kTE), lds, stream(), rc, rf,
-                           static_cast<const double *>(F.d_c),
-                           static_cast<const int32_t *>(F.d_pos),
-                           static_cast<const double *>(C.d_jh),
-                           static_cast<const double *>(C.d_jht),
-                           static_cast<const uint8_t *>(C.d_bnd), nc, nf, guard);
+#define NB_RSTR(MASK, MIN)                                                                  \
+    hipLaunchKernelGGL((k_rstr<MASK, MIN>), dim3(P->nelt), dim3(kTE), lds, stream(), rc, rf, \
+                       static_cast<const double *>(F.d_c),                                   \
+                       static_cast<const int32_t *>(F.d_pos),                                \
+                       static_cast<const double *>(C.d_jh),                                  \
+                       static_cast<const double *>(C.d_jht),                                 \
+                       static_cast<const uint8_t *>(C.d_bnd), nc, nf, guard,                 \
+                       static_cast<const uint8_t *>(F.d_bnd))
+    if (mask && maskin)
+        NB_RSTR(true, true);
+    else if (mask)
+        NB_RSTR(true, false);
+    else if (maskin)
+        NB_RSTR(false, true);
     else
-        hipLaunchKernelGGL(k_rstr<false>, dim3(P->nelt), dim3(kTE), lds, stream(), rc, rf,
-                           static_cast<const double *>(F.d_c),
-                           static_cast<const int32_t *>(F.d_pos),
-                           static_cast<const double *>(C.d_jh),
-                           static_cast<const double *>(C.d_jht),
-                           static_cast<const uint8_t *>(C.d_bnd), nc, nf, guard);
+        NB_RSTR(false, false);
+#undef NB_RSTR
     NB_HIP(hipGetLastError());
     return NEKBONE_OK;
 }
@@ -729,22 +737,41 @@ int launch_hsmg_apply(nekbone_problem *P, double *z, double *r, const CgScalars 
     const int lmax = M->lmax;
     HsmgLevel *L = M->L;
     if (int rc = timed_begin(P, 2)) return rc;
-    if (int rc = schwarz(P, L[lmax], z, r, guard)) return rc;
+    // The finest level's Schwarz step (z) and the rest of the V-cycle (the
+    // restrictions, the other levels, the coarse solve) depend only on r:
+    // the finest step runs on the side stream beside them, joined before
+    // the last interpolation adds into z.  It masks r in place as it reads
+    // it (the reference's side effect), so the first restriction masks as
+    // it reads as well (maskin) and can run first: the side stream starts
+    // after it, so that the small kernels of the chain are not queued
+    // behind the finest level's local solves.
     const double *rcur = r;  // r := rhs, masked by the Schwarz step
+    bool forked = false;
     for (int l = lmax - 1; l >= 2; --l) {
-        if (int rc = rstr(P, L[l], L[l + 1], L[l].d_r, rcur, false, guard)) return rc;
+        if (int rc = rstr(P, L[l], L[l + 1], L[l].d_r, rcur, false, guard, l == lmax - 1))
+            return rc;
+        if (!forked) {
+            if (int rc = fork_side()) return rc;
+            if (int rc = schwarz(P, L[lmax], z, r, guard, side_stream())) return rc;
+            forked = true;
+        }
         if (int rc = launch_gs_map(L[l].gs, L[l].d_r, guard)) return rc;
-        if (int rc = schwarz(P, L[l], L[l].d_e, L[l].d_r, guard)) return rc;
+        if (int rc = schwarz(P, L[l], L[l].d_e, L[l].d_r, guard, stream())) return rc;
         rcur = L[l].d_r;
     }
     // coarse level: e1 = mask(QQ^T mask(J^T r2)) (hsmg_coarse_solve, :945-957),
     // or with the direct coarse solve e1 = Q A_c^-1 Q^T mask(J^T r2)
-    if (int rc = rstr(P, L[1], L[2], L[1].d_e, rcur, true, guard)) return rc;
+    if (int rc = rstr(P, L[1], L[2], L[1].d_e, rcur, true, guard, lmax == 2)) return rc;
+    if (!forked) {  // two levels: the coarse restriction was the first
+        if (int rc = fork_side()) return rc;
+        if (int rc = schwarz(P, L[lmax], z, r, guard, side_stream())) return rc;
+    }
     if (int rc = launch_gs_map(L[1].gs, L[1].d_e, guard)) return rc;
     if (P->hsmg_coarse)
         if (int rc = hsmg_coarse_apply(P, L[1].d_e, guard)) return rc;
     for (int l = 2; l <= lmax - 1; ++l)
         if (int rc = intp(P, L[l - 1], L[l], L[l].d_e, L[l - 1].d_e, l - 1 == 1, guard)) return rc;
+    if (int rc = join_side()) return rc;
     if (int rc = intp(P, L[lmax - 1], L[lmax], z, L[lmax - 1].d_e, lmax - 1 == 1, guard,
                       rtz_part, r))
         return rc;

@@ -41,6 +41,14 @@ int fail(int code, const std::string &msg);
 // it: a null-stream fill of a new vector may land after a later copy into
 // that vector on this stream.
 hipStream_t stream();
+// A second stream for work that may run beside the library stream's
+// (non-blocking, default priority).  fork_side: work queued on it from now
+// on follows what stream() has queued so far; join_side: what stream()
+// queues from now on follows what the side stream has queued.  Both are
+// recorded into a graph when stream() is being captured.
+hipStream_t side_stream();
+int fork_side();
+int join_side();
 
 // Stream-ordered transfers and fills on stream(); each returns once the
 // operation has completed (the host buffer may be reused at once).
