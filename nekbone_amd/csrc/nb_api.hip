@@ -25,11 +25,12 @@ int fail(int code, const std::string &msg)
 namespace {
 int g_device = -1;
 hipStream_t g_stream = nullptr;
-// second stream for work that runs beside the library stream's (created
-// and ordered like it: non-blocking, default priority), and the events
-// that fork work to it and join it back
-hipStream_t g_side = nullptr;
-hipEvent_t g_fork = nullptr, g_join = nullptr;
+// two more streams for work that runs beside the library stream's
+// (created like it: non-blocking, default priority), and the events that
+// fork work to them and join it back
+constexpr int kSides = 2;
+hipStream_t g_side[kSides] = {nullptr, nullptr};
+hipEvent_t g_fork[kSides] = {nullptr, nullptr}, g_join[kSides] = {nullptr, nullptr};
 
 __global__ void k_fill(double *a, double v, int64_t n)
 {
@@ -68,19 +69,19 @@ AxVariant tuned_or_default(int nx1)
 }  // namespace
 
 hipStream_t stream() { return g_stream; }
-hipStream_t side_stream() { return g_side; }
+hipStream_t side_stream(int k) { return g_side[k]; }
 
-int fork_side()
+int fork_side(int k)
 {
-    NB_HIP(hipEventRecord(g_fork, g_stream));
-    NB_HIP(hipStreamWaitEvent(g_side, g_fork, 0));
+    NB_HIP(hipEventRecord(g_fork[k], g_stream));
+    NB_HIP(hipStreamWaitEvent(g_side[k], g_fork[k], 0));
     return NEKBONE_OK;
 }
 
-int join_side()
+int join_side(int k)
 {
-    NB_HIP(hipEventRecord(g_join, g_side));
-    NB_HIP(hipStreamWaitEvent(g_stream, g_join, 0));
+    NB_HIP(hipEventRecord(g_join[k], g_side[k]));
+    NB_HIP(hipStreamWaitEvent(g_stream, g_join[k], 0));
     return NEKBONE_OK;
 }
 
@@ -658,13 +659,17 @@ extern "C" int nekbone_init(int device)
     NB_HIP(hipSetDevice(device));
     if (int rc = noalloc_setup(device)) return rc;
     if (g_stream) NB_HIP(hipStreamDestroy(g_stream));
-    if (g_side) NB_HIP(hipStreamDestroy(g_side));
-    if (g_fork) NB_HIP(hipEventDestroy(g_fork));
-    if (g_join) NB_HIP(hipEventDestroy(g_join));
+    for (int k = 0; k < kSides; ++k) {
+        if (g_side[k]) NB_HIP(hipStreamDestroy(g_side[k]));
+        if (g_fork[k]) NB_HIP(hipEventDestroy(g_fork[k]));
+        if (g_join[k]) NB_HIP(hipEventDestroy(g_join[k]));
+    }
     NB_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
-    NB_HIP(hipStreamCreateWithFlags(&g_side, hipStreamNonBlocking));
-    NB_HIP(hipEventCreateWithFlags(&g_fork, hipEventDisableTiming));
-    NB_HIP(hipEventCreateWithFlags(&g_join, hipEventDisableTiming));
+    for (int k = 0; k < kSides; ++k) {
+        NB_HIP(hipStreamCreateWithFlags(&g_side[k], hipStreamNonBlocking));
+        NB_HIP(hipEventCreateWithFlags(&g_fork[k], hipEventDisableTiming));
+        NB_HIP(hipEventCreateWithFlags(&g_join[k], hipEventDisableTiming));
+    }
     g_device = device;
     return NEKBONE_OK;
 }
@@ -675,14 +680,17 @@ extern "C" int nekbone_finalize(void)
         NB_HIP(hipStreamSynchronize(g_stream));
         NB_HIP(hipStreamDestroy(g_stream));
     }
-    if (g_side) {
-        NB_HIP(hipStreamSynchronize(g_side));
-        NB_HIP(hipStreamDestroy(g_side));
+    for (int k = 0; k < kSides; ++k) {
+        if (g_side[k]) {
+            NB_HIP(hipStreamSynchronize(g_side[k]));
+            NB_HIP(hipStreamDestroy(g_side[k]));
+        }
+        if (g_fork[k]) NB_HIP(hipEventDestroy(g_fork[k]));
+        if (g_join[k]) NB_HIP(hipEventDestroy(g_join[k]));
+        g_side[k] = nullptr;
+        g_fork[k] = g_join[k] = nullptr;
     }
-    if (g_fork) NB_HIP(hipEventDestroy(g_fork));
-    if (g_join) NB_HIP(hipEventDestroy(g_join));
-    g_stream = g_side = nullptr;
-    g_fork = g_join = nullptr;
+    g_stream = nullptr;
     return NEKBONE_OK;
 }
 

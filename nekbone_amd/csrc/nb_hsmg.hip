@@ -737,41 +737,51 @@ int launch_hsmg_apply(nekbone_problem *P, double *z, double *r, const CgScalars 
     const int lmax = M->lmax;
     HsmgLevel *L = M->L;
     if (int rc = timed_begin(P, 2)) return rc;
-    // The finest level's Schwarz step (z) and the rest of the V-cycle (the
-    // restrictions, the other levels, the coarse solve) depend only on r:
-    // the finest step runs on the side stream beside them, joined before
-    // the last interpolation adds into z.  It masks r in place as it reads
-    // it (the reference's side effect), so the first restriction masks as
-    // it reads as well (maskin) and can run first: the side stream starts
-    // after it, so that the small kernels of the chain are not queued
-    // behind the finest level's local solves.
+    // Additive Schwarz: each level's Schwarz step depends only on that
+    // level's r, and the coarser levels only on the restricted r.  The
+    // finest level's step runs on side stream 0, forked after the first
+    // restriction (so that the small kernels of the chain are not queued
+    // behind its local solves), the last middle level's on side stream 1,
+    // forked after that level's r is summed; each is joined before the
+    // interpolation that adds into its output.  A Schwarz step masks its
+    // r in place as it reads it (the reference's side effect), so the
+    // restriction that reads that r beside it masks as it reads (maskin):
+    // every value is the serial order's.
     const double *rcur = r;  // r := rhs, masked by the Schwarz step
-    bool forked = false;
+    bool fine_forked = false, mid_forked = false;
+    auto fork_fine = [&]() -> int {
+        if (fine_forked) return NEKBONE_OK;
+        fine_forked = true;
+        if (int rc = fork_side(0)) return rc;
+        return schwarz(P, L[lmax], z, r, guard, side_stream(0));
+    };
     for (int l = lmax - 1; l >= 2; --l) {
         if (int rc = rstr(P, L[l], L[l + 1], L[l].d_r, rcur, false, guard, l == lmax - 1))
             return rc;
-        if (!forked) {
-            if (int rc = fork_side()) return rc;
-            if (int rc = schwarz(P, L[lmax], z, r, guard, side_stream())) return rc;
-            forked = true;
-        }
+        if (int rc = fork_fine()) return rc;
         if (int rc = launch_gs_map(L[l].gs, L[l].d_r, guard)) return rc;
-        if (int rc = schwarz(P, L[l], L[l].d_e, L[l].d_r, guard, stream())) return rc;
+        if (l == 2) {
+            if (int rc = fork_side(1)) return rc;
+            if (int rc = schwarz(P, L[l], L[l].d_e, L[l].d_r, guard, side_stream(1))) return rc;
+            mid_forked = true;
+        } else if (int rc = schwarz(P, L[l], L[l].d_e, L[l].d_r, guard, stream())) {
+            return rc;
+        }
         rcur = L[l].d_r;
     }
     // coarse level: e1 = mask(QQ^T mask(J^T r2)) (hsmg_coarse_solve, :945-957),
     // or with the direct coarse solve e1 = Q A_c^-1 Q^T mask(J^T r2)
-    if (int rc = rstr(P, L[1], L[2], L[1].d_e, rcur, true, guard, lmax == 2)) return rc;
-    if (!forked) {  // two levels: the coarse restriction was the first
-        if (int rc = fork_side()) return rc;
-        if (int rc = schwarz(P, L[lmax], z, r, guard, side_stream())) return rc;
-    }
+    if (int rc = rstr(P, L[1], L[2], L[1].d_e, rcur, true, guard, true)) return rc;
+    if (int rc = fork_fine()) return rc;  // two levels: the coarse restriction was the first
     if (int rc = launch_gs_map(L[1].gs, L[1].d_e, guard)) return rc;
     if (P->hsmg_coarse)
         if (int rc = hsmg_coarse_apply(P, L[1].d_e, guard)) return rc;
-    for (int l = 2; l <= lmax - 1; ++l)
+    for (int l = 2; l <= lmax - 1; ++l) {
+        if (l == 2 && mid_forked)
+            if (int rc = join_side(1)) return rc;
         if (int rc = intp(P, L[l - 1], L[l], L[l].d_e, L[l - 1].d_e, l - 1 == 1, guard)) return rc;
-    if (int rc = join_side()) return rc;
+    }
+    if (int rc = join_side(0)) return rc;
     if (int rc = intp(P, L[lmax - 1], L[lmax], z, L[lmax - 1].d_e, lmax - 1 == 1, guard,
                       rtz_part, r))
         return rc;

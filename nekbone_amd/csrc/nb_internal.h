@@ -41,14 +41,15 @@ int fail(int code, const std::string &msg);
 // it: a null-stream fill of a new vector may land after a later copy into
 // that vector on this stream.
 hipStream_t stream();
-// A second stream for work that may run beside the library stream's
-// (non-blocking, default priority).  fork_side: work queued on it from now
-// on follows what stream() has queued so far; join_side: what stream()
-// queues from now on follows what the side stream has queued.  Both are
-// recorded into a graph when stream() is being captured.
-hipStream_t side_stream();
-int fork_side();
-int join_side();
+// Two more streams (k = 0, 1) for work that may run beside the library
+// stream's (non-blocking, default priority).  fork_side(k): work queued on
+// side stream k from now on follows what stream() has queued so far;
+// join_side(k): what stream() queues from now on follows what side stream
+// k has queued.  Both are recorded into a graph when stream() is being
+// captured.
+hipStream_t side_stream(int k);
+int fork_side(int k);
+int join_side(int k);
 
 // Stream-ordered transfers and fills on stream(); each returns once the
 // operation has completed (the host buffer may be reused at once).
