@@ -307,13 +307,26 @@ int launch_iteration(nekbone_batch *B, int precond)
                            static_cast<const axk::QItem *>(B->d_qitems), B->nqitems);
         NB_HIP(hipGetLastError());
     }
+    // the order groups' element kernels write disjoint problems: they run
+    // on the library stream and the two side streams at once, so one
+    // group's tail overlaps another's launch
+    int ng = 0;
+    for (auto &G : B->groups) ng += G.fn ? 1 : 0;
+    const int nside = std::min(ng - 1, 2);
+    for (int k = 0; k < nside; ++k)
+        if (int rc = fork_side(k)) return rc;
+    int gi = 0;
     for (auto &G : B->groups) {
         if (!G.fn) continue;
-        hipLaunchKernelGGL(G.fn, dim3(G.nblk_ax), dim3(G.threads), 0, stream(),
+        const int lane = gi++ % (nside + 1);
+        hipStream_t st = lane == 0 ? stream() : side_stream(lane - 1);
+        hipLaunchKernelGGL(G.fn, dim3(G.nblk_ax), dim3(G.threads), 0, st,
                            static_cast<const axk::AxArgs *>(G.d_ax),
                            static_cast<const int *>(G.d_axstart), int(G.members.size()));
         NB_HIP(hipGetLastError());
     }
+    for (int k = 0; k < nside; ++k)
+        if (int rc = join_side(k)) return rc;
     if (!B->gx)
         if (int rc = launch_gs_cg_batch(B->d_gsall, B->d_gsstart, np, B->nblk_gs)) return rc;
     return launch_cg_batch_tail(B->d_args, B->d_ustart, np, B->nupd_total, precond, B->fma,
