@@ -309,16 +309,27 @@ int launch_iteration(nekbone_batch *B, int precond)
     }
     // the order groups' element kernels write disjoint problems: they run
     // on the library stream and the two side streams at once, so one
-    // group's tail overlaps another's launch
-    int ng = 0;
-    for (auto &G : B->groups) ng += G.fn ? 1 : 0;
-    const int nside = std::min(ng - 1, 2);
+    // group's tail overlaps another's launch; each group goes, largest
+    // first, to the stream with the fewest points queued so far
+    std::vector<std::pair<int64_t, size_t>> order;
+    for (size_t g = 0; g < B->groups.size(); ++g) {
+        const auto &G = B->groups[g];
+        if (!G.fn) continue;
+        int64_t pts = 0;
+        for (int k : G.members) pts += B->P[size_t(k)]->n;
+        order.emplace_back(-pts, g);
+    }
+    std::sort(order.begin(), order.end());
+    const int nside = std::min(int(order.size()) - 1, 2);
     for (int k = 0; k < nside; ++k)
         if (int rc = fork_side(k)) return rc;
-    int gi = 0;
-    for (auto &G : B->groups) {
-        if (!G.fn) continue;
-        const int lane = gi++ % (nside + 1);
+    int64_t load[3] = {0, 0, 0};
+    for (const auto &og : order) {
+        int lane = 0;
+        for (int l = 1; l <= nside; ++l)
+            if (load[l] < load[lane]) lane = l;
+        load[lane] -= og.first;
+        const auto &G = B->groups[og.second];
         hipStream_t st = lane == 0 ? stream() : side_stream(lane - 1);
         hipLaunchKernelGGL(G.fn, dim3(G.nblk_ax), dim3(G.threads), 0, st,
                            static_cast<const axk::AxArgs *>(G.d_ax),
