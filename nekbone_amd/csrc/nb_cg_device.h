@@ -256,7 +256,9 @@ __device__ __forceinline__ void slice_sum_pair(const double *__restrict__ a,
 
 // Sum of a partials region in a fixed order by one block (two regions of
 // equal length: their loads interleaved).  (Thirty-two loads in flight for
-// one region ran the single-block kernel at 22 instead of 9 us.)
+// one region ran the single-block kernel at 22 instead of 9 us, and so did
+// each thread summing a contiguous chunk of 16 or 32, whose loads no
+// longer coalesce.)
 template <int NT>
 __device__ __forceinline__ void sum_region2(const double *__restrict__ a, int64_t na,
                                             const double *__restrict__ b, int64_t nb, double &s0,
