@@ -196,6 +196,10 @@ int nekbone_set_hsmg_coarse(nekbone_problem *P, int mode);
 /* ---------------------------------------------------------------------- */
 /* Operators (host arrays; *_d: device pointers on the library stream)     */
 /* ---------------------------------------------------------------------- */
+/* Every vector argument, host or device, holds the reference's point order
+ * u(i,j,k,e).  The host-array calls return with their results written; the
+ * *_d operator calls return with their work queued on the library stream
+ * (nekbone_sync before the host reads the result by other means). */
 /* w = mask(QQ^T A_L u + 0.1 u): ax, src/cg.f:96-125 */
 int nekbone_ax(nekbone_problem *P, double *w, const double *u);
 int nekbone_ax_d(nekbone_problem *P, double *w, const double *u);
